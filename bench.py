@@ -1,0 +1,188 @@
+"""Headline benchmark: GCUPS of the batched engine on one MI355X.
+
+Workload (N=1): protein semi-global, blosum62, gap-open 3, gap-extend 1,
+ends-free FFTT, one 350-aa query against 2,000,000 synthetic FASTA targets
+(log-normal lengths, median 350, sigma 0.6, clipped to [30, 5000]; about
+0.84 G residues; ishlib workload generator in ish_amd/csrc/workload.cpp).
+A step is one pass of the hot path over the whole device-resident target
+database: the persistent batched kernel scores every target and writes
+score / end_ref / end_query per target. Inputs are resident in HBM before the
+timed region starts.
+
+Also reported on the same line: the plain one-launch-per-batch int32 kernel
+on the same targets, end-to-end rates, the integer-VALU roofline of the
+dominant kernel, parity against the CPU oracle on a sample, and the oracle
+timed on this host's cores as the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCUPS on 1 MI355X (bit-exact scores), batched vs plain; p50/p99 query latency"
+PEAK_INT16_TOPS = 78.6  # 256 CU x 2.4 GHz x 128 packed int16 lane-ops/clk
+PEAK_INT32_TOPS = 39.3  # 256 CU x 2.4 GHz x 64 int32 lane-ops/clk
+OPS_PER_CELL_SG = 8     # semi-global Gotoh: 1 add + 2 max (H), 2 sub + 1 max (E), 1 sub + 1 max (F)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--targets", type=int, default=2_000_000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=100_000,
+                    help="targets in the CPU-baseline / parity sample")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this engine has no multi-GPU path in this build "
+              "(targets shard independently; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import ish_amd
+    from ish_amd import workloads
+
+    t0 = time.perf_counter()
+    w = workloads.generate(2, args.targets, seed=args.seed)
+    log(f"workload: {w.targets.n} targets, {int(w.targets.offsets[-1])} residues, "
+        f"hash {w.content_hash()}, {time.perf_counter() - t0:.1f}s")
+    Q = len(w.query)
+    cells = w.cells
+    flags = "FFTT"
+
+    eng = ish_amd.Engine(0)
+    eng.set_engine("batched")
+    eng.set_query(w.query, "blosum62", 3, 1, flags)
+    t0 = time.perf_counter()
+    db = eng.load_db(w.targets)
+    load_s = time.perf_counter() - t0
+    log(f"db load (H2D + sort + pack): {load_s * 1e3:.1f} ms")
+
+    for _ in range(args.warmup):
+        eng.align_db_async(db)
+        eng.sync()
+    eng.sync()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.align_db_async(db)
+        eng.sync()
+        ms, launches = eng.last_timing()
+        kernel_ms.append(ms)
+    wall = time.perf_counter() - t0
+    s, er, eq, st = eng.fetch(db)
+    step_s = wall / args.steps
+    gcups = cells / step_s / 1e9
+    kern_s = float(np.mean(kernel_ms)) / 1e3
+    gcups_kernel = cells / kern_s / 1e9
+    widened = int((st & ish_amd.STATUS_WIDENED != 0).sum())
+    log(f"batched: {step_s * 1e3:.2f} ms/step, {gcups:.1f} GCUPS (kernel {gcups_kernel:.1f}), "
+        f"widened targets {widened}")
+
+    # end-to-end: upload + pack + score + fetch, for one query
+    t0 = time.perf_counter()
+    db2 = eng.load_db(w.targets)
+    eng.align_db_async(db2)
+    eng.fetch(db2)
+    e2e_s = time.perf_counter() - t0
+    db2.close()
+
+    plain = None
+    if not args.no_plain:
+        eng.set_engine("plain")
+        eng.set_query(w.query, "blosum62", 3, 1, flags)
+        t0 = time.perf_counter()
+        ps, per, peq, _ = eng.align(w.targets)
+        p_e2e = time.perf_counter() - t0
+        p_ms, _ = eng.last_timing()
+        same = bool(np.array_equal(ps, s) and np.array_equal(per, er) and np.array_equal(peq, eq))
+        plain = {"gcups_kernel": round(cells / (p_ms / 1e3) / 1e9, 2),
+                 "gcups_e2e": round(cells / p_e2e / 1e9, 2),
+                 "kernel_ms": round(p_ms, 3), "identical_to_batched": same,
+                 "batched_over_plain_kernel": round((p_ms / 1e3) / kern_s, 2)}
+        log(f"plain: kernel {p_ms:.1f} ms, {plain['gcups_kernel']} GCUPS, identical={same}")
+        eng.set_engine("batched")
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        n_s = min(args.cpu_sample, w.targets.n)
+        idx = np.linspace(0, w.targets.n - 1, n_s).astype(np.int64)
+        lens = (w.targets.offsets[idx + 1] - w.targets.offsets[idx]).astype(np.int64)
+        off = np.zeros(n_s + 1, np.uint64)
+        np.cumsum(lens, out=off[1:])
+        res = np.concatenate([w.targets.residues[int(w.targets.offsets[i]):int(w.targets.offsets[i + 1])]
+                              for i in idx])
+        m = O.Matrix("blosum62")
+        t0 = time.perf_counter()
+        os_, oer, oeq, used = O.batch(0, w.query, res, off, m, 3, 1, O.flags_from_str(flags),
+                                     nthreads=min(16, os.cpu_count() or 1))
+        cpu_s = time.perf_counter() - t0
+        sample_cells = Q * int(off[-1])
+        cpu = {"value": round(sample_cells / cpu_s / 1e9, 3), "unit": "GCUPS", "cores": int(used),
+               "kind": "port",
+               "sample": f"{n_s} of the {w.targets.n} targets (every {w.targets.n // n_s}th), "
+                         f"{sample_cells / 1e9:.2f} Gcells, scalar int32 oracle (oracle/oracle.c), "
+                         f"{os.cpu_count()} host threads"}
+        parity = bool(np.array_equal(os_, s[idx]) and np.array_equal(oer, er[idx]) and np.array_equal(oeq, eq[idx]))
+        ms_ = int(m.values.reshape(24, 24)[w.query, w.query].sum())
+        emit_gpu = ish_amd.emit("striped-semi-global", s[idx], er[idx], eq[idx], ms_, 0.8)
+        emit_ref = np.array([O.sg_emit(v, ms_, 0.8) for v in os_])
+        parity = parity and bool(np.array_equal(emit_gpu, emit_ref))
+        log(f"cpu oracle: {cpu['value']} GCUPS on {used} threads; parity on sample: {parity}")
+
+    achieved = cells * OPS_PER_CELL_SG / kern_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic",
+        "config": {"workload": "protein semi-global, blosum62, go 3 ge 1, ends-free FFTT, "
+                               "1 x 350 aa query vs 2M log-normal targets (median 350, sigma 0.6, [30,5000])",
+                   "targets": w.targets.n, "residues": int(w.targets.offsets[-1]), "query_len": Q,
+                   "cells": cells, "seed": args.seed, "content_hash": w.content_hash(),
+                   "engine": "batched persistent, packed int16 (int32 for wide groups)"},
+        "gcups_kernel": round(gcups_kernel, 2),
+        "gcups_e2e": round(cells / e2e_s / 1e9, 2),
+        "db_load_ms": round(load_s * 1e3, 1),
+        "widened_targets": widened,
+        "plain": plain,
+        "parity_sample": parity,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
+                     "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
+                     "traffic": None,
+                     "ops_per_cell": OPS_PER_CELL_SG, "kernel_ms": round(kern_s * 1e3, 3)},
+        "cpu_baseline": cpu,
+    }
+    db.close()
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+/*
+ * ish_hip: MI355X (gfx950) alignment-scoring engine for ish, C ABI.
+ *
+ * Plain C types only, so that ish's Mojo host can bind it with
+ * external_call / DLHandle (the same mechanism ish uses for libz,
+ * ishlib/vendor/zlib.mojo:61-100).
+ *
+ * What it replaces in the reference (BioRadOpenSource/ish):
+ *   - the GPU kernel entry GpuMatcher.batch_match_coarse
+ *     (ishlib/matcher/__init__.mojo:96-121, implemented in
+ *     ishlib/matcher/striped_semi_global_matcher.mojo:165-298 and
+ *     ishlib/gpu/kernels/semi_global.mojo:20-115);
+ *   - the device manager SearcherDevice (ishlib/gpu/searcher_device.mojo:
+ *     98-519: create_devices, set_host_inputs, copy_inputs_to_device,
+ *     launch_kernel, copy_outputs_to_host, synchronize);
+ *   - the per-target CPU kernels behind Matcher.first_match for the data-
+ *     parallel batch executor (ishlib/cpu/process_batch.mojo:14-34):
+ *     semi_global_aln (semi_global_aln/striped.mojo:252-498) and ssw_align
+ *     (local_aln/striped.mojo:142-269).
+ *
+ * Conventions
+ *   - Return value 0 = OK, < 0 = error code (ISH_E_*). No exceptions cross
+ *     the ABI. ish_hip_last_error(ctx) describes the last failure on ctx.
+ *   - The caller owns every host array; arrays are only read (or written,
+ *     for outputs) during the call.
+ *   - A context owns its device memory, stream and tables. A context is
+ *     single-thread-affine; distinct contexts may be used concurrently.
+ *   - End coordinates are 0-based inclusive, like the reference's CPU
+ *     striped kernels (AlignmentEnd, striped_utils.mojo:101-105). A
+ *     MatchResult(start, end+1) is formed by the caller.
+ *   - Residues are encoded codes (ish_hip_encoding), one byte each.
+ */
+#ifndef ISH_HIP_H
+#define ISH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISH_HIP_VERSION 10000 /* 1.0.0 */
+
+/* error codes */
+#define ISH_OK 0
+#define ISH_E_INVALID (-1)   /* bad argument */
+#define ISH_E_HIP (-2)       /* HIP runtime error */
+#define ISH_E_NOMEM (-3)     /* allocation failed */
+#define ISH_E_NO_QUERY (-4)  /* ish_hip_set_query not called */
+#define ISH_E_RANGE (-5)     /* size beyond what the engine supports */
+#define ISH_E_NO_DEVICE (-6) /* no gfx950 device */
+
+/* matrix kinds, MatrixKind (ishlib/matcher/alignment/scoring_matrix.mojo:
+ * 176-254) */
+#define ISH_MATRIX_ASCII 0
+#define ISH_MATRIX_ACTGN 1
+#define ISH_MATRIX_ACTGN0 2
+#define ISH_MATRIX_BLOSUM62 3
+
+/* --match-algo (ishlib/searcher_settings.mojo:58-193) */
+#define ISH_ALGO_SEMI_GLOBAL 0 /* striped-semi-global */
+#define ISH_ALGO_LOCAL 1       /* striped-local */
+
+/* --sg-ends-free letters in order (searcher_settings.mojo:404-407) */
+#define ISH_FREE_QUERY_START 1u
+#define ISH_FREE_QUERY_END 2u
+#define ISH_FREE_TARGET_START 4u
+#define ISH_FREE_TARGET_END 8u
+
+/* engines */
+#define ISH_ENGINE_BATCHED 0 /* default: length-bucketed persistent kernel,
+                                packed int16 lanes, int32 for targets whose
+                                range bound does not fit int16 */
+#define ISH_ENGINE_PLAIN 1   /* one launch per batch, int32 cells, input
+                                order, one target per lane */
+
+/* per-target status byte: cell width used, plus a flag when the target was
+ * routed to int32 because its proven score range exceeds int16. */
+#define ISH_STATUS_I16 16u
+#define ISH_STATUS_I32 32u
+#define ISH_STATUS_WIDENED 0x80u
+
+typedef struct ish_hip_ctx ish_hip_ctx;
+typedef struct ish_hip_db ish_hip_db;
+
+int32_t ish_hip_version(void);
+/* Number of visible gfx950 devices (0 when none). */
+int32_t ish_hip_device_count(void);
+
+/* SearcherDevice.create_devices (searcher_device.mojo:152-174), one device
+ * per context. */
+int32_t ish_hip_create(int32_t device, ish_hip_ctx **out);
+void ish_hip_destroy(ish_hip_ctx *ctx);
+const char *ish_hip_last_error(const ish_hip_ctx *ctx);
+
+/* Tables (host only, no device needed). ScoringMatrix values and
+ * ascii_to_encoding / encoding_to_ascii (scoring_matrix.mojo:9-172,
+ * :304-358). ish_hip_matrix writes A*A bytes (65536 for ascii) and returns
+ * A; ish_hip_encoding writes 256 bytes; ish_hip_decoding writes A bytes and
+ * returns A. */
+int32_t ish_hip_matrix(int32_t matrix_kind, int8_t *out, int32_t cap);
+int32_t ish_hip_encoding(int32_t matrix_kind, uint8_t *out256);
+int32_t ish_hip_decoding(int32_t matrix_kind, uint8_t *out, int32_t cap);
+/* convert_ascii_to_encoding_and_score (scoring_matrix.mojo:403-416):
+ * encodes n bytes into out and returns the optimal score, the sum of the
+ * matrix diagonal over the encoded bytes (the --score normaliser). */
+int64_t ish_hip_encode_and_score(int32_t matrix_kind, const uint8_t *ascii,
+                                 int64_t n, uint8_t *out);
+
+/* Query setup: StripedSemiGlobalMatcher.__init__ /
+ * StripedLocalMatcher.__init__ (striped_semi_global_matcher.mojo:41-75,
+ * striped_local_matcher.mojo:33-59). `matrix` is the A*A int8 matrix of
+ * matrix_kind (NULL selects the built-in one). gap_open / gap_extend are
+ * the absolute values of the CLI flags (searcher_settings.mojo:218-219).
+ * ends_free is only used by ISH_ALGO_SEMI_GLOBAL. */
+int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *enc_query,
+                          int32_t qlen, const int8_t *matrix, int32_t alphabet,
+                          int32_t matrix_kind, int32_t gap_open,
+                          int32_t gap_extend, uint32_t ends_free,
+                          int32_t algo);
+int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine);
+
+/* One synchronous batch (the batch_match_coarse + copy_outputs_to_host
+ * step). Targets are residues[offsets[t] .. offsets[t+1]) for t < n.
+ * Outputs (n entries each): score, 0-based end_ref / end_query, status.
+ * status may be NULL. */
+int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
+                            const uint64_t *offsets, int64_t n, int32_t *score,
+                            int32_t *end_ref, int32_t *end_query,
+                            uint8_t *status);
+
+/* Device-resident target database: uploaded once, sorted by length and
+ * packed into coalesced groups on the device, then scored against any
+ * number of queries. The packing replaces SearcherDevice.set_host_inputs
+ * (searcher_device.mojo:419-450). */
+int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
+                        const uint64_t *offsets, int64_t n, ish_hip_db **out);
+void ish_hip_db_free(ish_hip_db *db);
+int32_t ish_hip_db_size(const ish_hip_db *db, int64_t *n_targets,
+                        int64_t *n_residues);
+/* Scores every target of db against the context's query; synchronous,
+ * results copied to the host arrays (n_targets entries, original order). */
+int32_t ish_hip_align_db(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
+                         int32_t *end_ref, int32_t *end_query,
+                         uint8_t *status);
+/* Asynchronous form: enqueue on the context's stream; results stay on the
+ * device until ish_hip_db_fetch. ish_hip_sync waits for the stream. */
+int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db);
+int32_t ish_hip_sync(ish_hip_ctx *ctx);
+int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
+                         int32_t *end_ref, int32_t *end_query,
+                         uint8_t *status);
+
+/* Timing of the last align call on ctx, measured with HIP events on the
+ * context's stream around each kernel launch: total milliseconds and the
+ * number of kernel launches it covered. */
+int32_t ish_hip_last_timing(const ish_hip_ctx *ctx, float *kernel_ms,
+                            int32_t *launches);
+
+/* Emission rule of the matchers, float32 throughout:
+ *  semi-global: f32(score) >= f32(max_score)*thr, then
+ *               f32(score)/f32(max_score) >= thr
+ *               (striped_semi_global_matcher.mojo:77-107);
+ *  local:       f32(score) > thr*f32(max_score), end_ref > 0, end_query > 0,
+ *               then f32(score)/f32(max_score) >= thr
+ *               (striped_local_matcher.mojo:61-90, local_aln/striped.mojo:
+ *               213-230).
+ * Writes emit[t] = 0/1 for n targets. */
+int32_t ish_hip_emit(int32_t algo, const int32_t *score,
+                     const int32_t *end_ref, const int32_t *end_query,
+                     int64_t n, int64_t max_score, float threshold,
+                     uint8_t *emit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,267 @@
+"""ish_amd: MI355X-native alignment-scoring engine for ish.
+
+Python binding of the C ABI in include/ish_hip.h (libish_hip.so). The engine
+itself is the HIP library; this module only marshals arrays. There is no
+CPU fallback: every scoring call goes to the GPU and fails loudly when the
+library or the device is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+
+ISH_OK = 0
+ERRORS = {-1: "ISH_E_INVALID", -2: "ISH_E_HIP", -3: "ISH_E_NOMEM", -4: "ISH_E_NO_QUERY",
+          -5: "ISH_E_RANGE", -6: "ISH_E_NO_DEVICE"}
+MATRIX = {"ascii": 0, "actgn": 1, "actgn0": 2, "blosum62": 3}
+ALGO = {"striped-semi-global": 0, "semi-global": 0, "striped-local": 1, "local": 1}
+ENGINE = {"batched": 0, "plain": 1}
+FREE_QUERY_START, FREE_QUERY_END, FREE_TARGET_START, FREE_TARGET_END = 1, 2, 4, 8
+STATUS_I16, STATUS_I32, STATUS_WIDENED = 16, 32, 0x80
+
+# every symbol the public header declares
+ABI_SYMBOLS = [
+    "ish_hip_version", "ish_hip_device_count", "ish_hip_create", "ish_hip_destroy",
+    "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
+    "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
+    "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
+    "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
+    "ish_hip_last_timing", "ish_hip_emit",
+]
+
+
+class IshError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib_path() -> str:
+    return os.path.join(LIB_DIR, "libish_hip.so")
+
+
+def lib():
+    """Loads libish_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise IshError(f"{path} is missing: run `python -m ish_amd.build` first")
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    u8p, i8p, i32p, u64p = P(ctypes.c_uint8), P(ctypes.c_int8), P(ctypes.c_int32), P(ctypes.c_uint64)
+    i32, u32, i64, vp = ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_void_p
+    sigs = {
+        "ish_hip_version": ([], i32),
+        "ish_hip_device_count": ([], i32),
+        "ish_hip_create": ([i32, P(vp)], i32),
+        "ish_hip_destroy": ([vp], None),
+        "ish_hip_last_error": ([vp], ctypes.c_char_p),
+        "ish_hip_matrix": ([i32, i8p, i32], i32),
+        "ish_hip_encoding": ([i32, u8p], i32),
+        "ish_hip_decoding": ([i32, u8p, i32], i32),
+        "ish_hip_encode_and_score": ([i32, u8p, i64, u8p], i64),
+        "ish_hip_set_query": ([vp, u8p, i32, i8p, i32, i32, i32, i32, u32, i32], i32),
+        "ish_hip_set_engine": ([vp, i32], i32),
+        "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
+        "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
+        "ish_hip_db_free": ([vp], None),
+        "ish_hip_db_size": ([vp, P(i64), P(i64)], i32),
+        "ish_hip_align_db": ([vp, vp, i32p, i32p, i32p, u8p], i32),
+        "ish_hip_align_db_async": ([vp, vp], i32),
+        "ish_hip_sync": ([vp], i32),
+        "ish_hip_db_fetch": ([vp, vp, i32p, i32p, i32p, u8p], i32),
+        "ish_hip_last_timing": ([vp, P(ctypes.c_float), i32p], i32),
+        "ish_hip_emit": ([i32, i32p, i32p, i32p, i64, i64, ctypes.c_float, u8p], i32),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def device_count() -> int:
+    return int(lib().ish_hip_device_count())
+
+
+def matrix(kind: str) -> np.ndarray:
+    out = np.zeros(65536, np.int8)
+    A = lib().ish_hip_matrix(MATRIX[kind], _p(out, ctypes.c_int8), 65536)
+    if A < 0:
+        raise IshError(f"unknown matrix {kind}")
+    return out[: A * A].reshape(A, A).copy()
+
+
+def encoding(kind: str) -> np.ndarray:
+    out = np.zeros(256, np.uint8)
+    if lib().ish_hip_encoding(MATRIX[kind], _p(out, ctypes.c_uint8)) != 0:
+        raise IshError(f"unknown matrix {kind}")
+    return out
+
+
+def decoding(kind: str) -> np.ndarray:
+    out = np.zeros(256, np.uint8)
+    A = lib().ish_hip_decoding(MATRIX[kind], _p(out, ctypes.c_uint8), 256)
+    if A < 0:
+        raise IshError(f"unknown matrix {kind}")
+    return out[:A].copy()
+
+
+def encode_and_score(kind: str, text) -> tuple[np.ndarray, int]:
+    b = np.frombuffer(text.encode() if isinstance(text, str) else bytes(text), np.uint8).copy()
+    out = np.zeros(max(len(b), 1), np.uint8)
+    s = lib().ish_hip_encode_and_score(MATRIX[kind], _p(b, ctypes.c_uint8), len(b), _p(out, ctypes.c_uint8))
+    return out[: len(b)], int(s)
+
+
+def emit(algo, score, end_ref, end_query, max_score: int, threshold: float) -> np.ndarray:
+    score = np.ascontiguousarray(score, np.int32)
+    end_ref = np.ascontiguousarray(end_ref, np.int32)
+    end_query = np.ascontiguousarray(end_query, np.int32)
+    out = np.zeros(max(len(score), 1), np.uint8)
+    rc = lib().ish_hip_emit(ALGO.get(algo, algo) if isinstance(algo, str) else algo,
+                            _p(score, ctypes.c_int32), _p(end_ref, ctypes.c_int32),
+                            _p(end_query, ctypes.c_int32), len(score), int(max_score),
+                            ctypes.c_float(threshold), _p(out, ctypes.c_uint8))
+    if rc != 0:
+        raise IshError(f"ish_hip_emit failed ({rc})")
+    return out[: len(score)].astype(bool)
+
+
+def flags_from_str(s: str) -> int:
+    """--sg-ends-free letters (query start, query end, target start, target end)."""
+    s = s.upper()
+    if len(s) != 4 or set(s) - {"T", "F"}:
+        raise ValueError("ends-free must be 4 letters of T/F, like FFTT")
+    return sum(1 << k for k, c in enumerate(s) if c == "T")
+
+
+class Targets:
+    """Encoded targets as one residue buffer plus offsets[n+1]."""
+
+    def __init__(self, residues: np.ndarray, offsets: np.ndarray):
+        self.residues = np.ascontiguousarray(residues, np.uint8)
+        self.offsets = np.ascontiguousarray(offsets, np.uint64)
+        if len(self.residues) == 0:
+            self.residues = np.zeros(1, np.uint8)
+
+    @classmethod
+    def from_list(cls, seqs):
+        lens = np.array([len(s) for s in seqs], np.uint64)
+        off = np.zeros(len(seqs) + 1, np.uint64)
+        np.cumsum(lens, out=off[1:])
+        res = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if len(seqs) and off[-1] else np.zeros(1, np.uint8)
+        return cls(res, off)
+
+    @property
+    def n(self) -> int:
+        return len(self.offsets) - 1
+
+
+class Engine:
+    """One ish_hip context on one device."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.ish_hip_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise IshError(f"ish_hip_create failed: {ERRORS.get(rc, rc)}")
+        self._h = h
+        self.algo = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ish_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._L.ish_hip_last_error(self._h)
+            raise IshError(f"{ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def set_engine(self, engine: str):
+        self._check(self._L.ish_hip_set_engine(self._h, ENGINE[engine]))
+
+    def set_query(self, query: np.ndarray, matrix_kind: str, gap_open: int = 3,
+                  gap_extend: int = 1, ends_free="FFTT", algo="striped-semi-global"):
+        q = np.ascontiguousarray(query, np.uint8)
+        qb = q if len(q) else np.zeros(1, np.uint8)
+        fl = flags_from_str(ends_free) if isinstance(ends_free, str) else int(ends_free)
+        self.algo = ALGO[algo] if isinstance(algo, str) else int(algo)
+        self._check(self._L.ish_hip_set_query(self._h, _p(qb, ctypes.c_uint8), len(q), None, 0,
+                                              MATRIX[matrix_kind], gap_open, gap_extend, fl,
+                                              self.algo))
+
+    def align(self, targets: Targets):
+        n = targets.n
+        out = [np.zeros(max(n, 1), np.int32) for _ in range(3)]
+        st = np.zeros(max(n, 1), np.uint8)
+        self._check(self._L.ish_hip_align_batch(
+            self._h, _p(targets.residues, ctypes.c_uint8), _p(targets.offsets, ctypes.c_uint64), n,
+            _p(out[0], ctypes.c_int32), _p(out[1], ctypes.c_int32), _p(out[2], ctypes.c_int32),
+            _p(st, ctypes.c_uint8)))
+        return out[0][:n], out[1][:n], out[2][:n], st[:n]
+
+    def load_db(self, targets: Targets) -> "Database":
+        h = ctypes.c_void_p()
+        self._check(self._L.ish_hip_db_load(self._h, _p(targets.residues, ctypes.c_uint8),
+                                            _p(targets.offsets, ctypes.c_uint64), targets.n,
+                                            ctypes.byref(h)))
+        return Database(self, h, targets.n)
+
+    def align_db(self, db: "Database"):
+        self.align_db_async(db)
+        return self.fetch(db)
+
+    def align_db_async(self, db: "Database"):
+        self._check(self._L.ish_hip_align_db_async(self._h, db._h))
+
+    def sync(self):
+        self._check(self._L.ish_hip_sync(self._h))
+
+    def fetch(self, db: "Database"):
+        n = db.n
+        out = [np.zeros(max(n, 1), np.int32) for _ in range(3)]
+        st = np.zeros(max(n, 1), np.uint8)
+        self._check(self._L.ish_hip_db_fetch(self._h, db._h, _p(out[0], ctypes.c_int32),
+                                             _p(out[1], ctypes.c_int32), _p(out[2], ctypes.c_int32),
+                                             _p(st, ctypes.c_uint8)))
+        return out[0][:n], out[1][:n], out[2][:n], st[:n]
+
+    def last_timing(self):
+        ms = ctypes.c_float()
+        k = ctypes.c_int32()
+        self._check(self._L.ish_hip_last_timing(self._h, ctypes.byref(ms), ctypes.byref(k)))
+        return float(ms.value), int(k.value)
+
+
+class Database:
+    def __init__(self, engine: Engine, handle, n: int):
+        self._engine = engine
+        self._h = handle
+        self.n = n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._engine._L.ish_hip_db_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,67 @@
+"""Builds the native libraries in-tree.
+
+  ish_amd/lib/libish_hip.so       C ABI + HIP kernels (hipcc, gfx950)
+  ish_amd/lib/libish_workload.so  synthetic workload generator (host C++)
+
+Run `python -m ish_amd.build`. Every call compiles again.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+ARCH = "gfx950"
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-Wall",
+    # no packed fp32 instructions anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+HIP_SOURCES = ["ish_hip.cpp", "sg_kernels.hip"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose: bool = True) -> None:
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(LIB, os.path.splitext(src)[0] + ".o")
+        _run([hipcc(), *HIP_FLAGS(), "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    out = os.path.join(LIB, "libish_hip.so")
+    tmp = out + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, out)
+    wl = os.path.join(LIB, "libish_workload.so")
+    _run(["g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-pthread",
+          os.path.join(CSRC, "workload.cpp"), "-o", wl + ".tmp"])
+    os.replace(wl + ".tmp", wl)
+
+
+def HIP_FLAGS():
+    return list(HIPCC_FLAGS)
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,99 @@
+// Internal declarations shared by the host side (ish_hip.cpp) and the HIP
+// kernels of the ish_hip engine. Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ish {
+
+// Targets per group of the device-resident layout: 64 lanes x 2 targets.
+constexpr int kGroup = 128;
+// Lanes per wavefront on CDNA.
+constexpr int kWave = 64;
+
+// One group of 128 length-sorted targets. Its residues are stored as
+// ncols/4 blocks of 64 uint2: block b, lane l holds columns 4b..4b+3 of
+// target `first + l` in .x and of target `first + 64 + l` in .y (little
+// endian, column 4b in the low byte). A wave reading one block issues one
+// coalesced 512-byte load.
+struct GroupDesc {
+    uint64_t res_off; // offset of the group's first block, in uint2 units
+    int32_t ncols;    // longest target of the group rounded up to 4
+    int32_t first;    // index of the group's first target in sorted arrays
+};
+
+// Everything a semi-global / local scoring launch needs.
+struct ScoreParams {
+    // query profile (staged into LDS): nrows rows of `stride` int16 scores,
+    // row r, query position j at prof[r * stride + j]; lut maps a residue
+    // code to its profile row.
+    const int16_t *prof;
+    const uint8_t *lut;
+    int32_t prof_words; // int32 words to stage for the profile
+    int32_t stride;     // int16 elements per profile row (odd dword count)
+    int32_t Q;          // query length
+    int32_t nchunks;    // ceil(Q / K)
+    int32_t go, ge;     // gap open / extend (non-negative)
+    uint32_t flags;     // free ends: qs|qe<<1|ts<<2|te<<3
+    int32_t neg_limit;  // int16 floor that seeds the last-row maximum
+
+    // device-resident targets
+    const GroupDesc *groups;
+    const uint2 *res;
+    const int32_t *lens; // sorted order
+    const int32_t *orig; // sorted order -> original index (-1 = padding)
+    int32_t g_begin, g_end;
+
+    // work queue counter (one uint32, zeroed before each launch)
+    uint32_t *counter;
+
+    // per-wave boundary rows: slot s owns bnd[s*slot_stride ...], column i,
+    // lane l at [i*64 + l]
+    uint2 *bnd;
+    int64_t slot_stride;
+
+    // outputs, original target order
+    int32_t *score, *end_ref, *end_q;
+    uint8_t *status;
+};
+
+// Plain (one launch per batch) parameters: targets in input order,
+// contiguous residues with offsets.
+struct PlainParams {
+    const int16_t *prof;
+    const uint8_t *lut;
+    int32_t prof_words, stride, Q, nchunks, go, ge;
+    uint32_t flags;
+    int32_t neg_limit;
+    const uint8_t *res;
+    const uint64_t *off;
+    int64_t n;
+    int2 *bnd; // per wave slot: [i*64 + lane]
+    int64_t slot_stride;
+    int32_t *score, *end_ref, *end_q;
+    uint8_t *status;
+};
+
+// Launchers (sg_kernels.hip). Return hipError_t of the launch.
+hipError_t launch_reset(uint32_t *counter, hipStream_t st);
+hipError_t launch_sg_batched_i16(const ScoreParams &p, int grid, int block,
+                                 size_t lds, hipStream_t st);
+hipError_t launch_sg_batched_i32(const ScoreParams &p, int grid, int block,
+                                 size_t lds, hipStream_t st);
+hipError_t launch_sg_plain(const PlainParams &p, int grid, int block,
+                           size_t lds, hipStream_t st);
+hipError_t launch_pack(const uint8_t *raw, const uint64_t *off,
+                       const int32_t *sorted_idx, const GroupDesc *groups,
+                       int32_t ngroups, int64_t n, uint2 *res,
+                       hipStream_t st);
+// Occupancy helpers: blocks per CU of each kernel at the given LDS size.
+int occupancy_sg_batched_i16(int block, size_t lds, uint32_t flags);
+int occupancy_sg_batched_i32(int block, size_t lds, uint32_t flags);
+int occupancy_sg_plain(int block, size_t lds, uint32_t flags);
+
+// Rows of the query held in registers per lane (chunk height).
+constexpr int kChunk = 32;
+constexpr int kChunkPlain = 32;
+
+} // namespace ish

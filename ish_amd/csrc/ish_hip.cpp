@@ -1,0 +1,759 @@
+// Host side of the ish_hip C ABI (include/ish_hip.h): contexts, query
+// profiles, the device-resident target database and kernel dispatch.
+#include "../../include/ish_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "tables.h"
+
+using namespace ish;
+
+namespace {
+
+constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
+constexpr int kBlock = 256;              // 4 waves per workgroup
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t c = 1;
+        while (c < bytes) c <<= 1;
+        if (c < 256) c = 256;
+        hipError_t e = hipMalloc(&p, c);
+        if (e == hipSuccess) cap = c;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+} // namespace
+
+struct ish_hip_db {
+    int device = 0;
+    int64_t n = 0, nres = 0;
+    int32_t ngroups = 0, maxcols = 0;
+    std::vector<int32_t> group_maxlen; // per group, descending
+    GroupDesc *d_groups = nullptr;
+    uint2 *d_res = nullptr;
+    int32_t *d_lens = nullptr, *d_orig = nullptr;
+    int32_t *d_score = nullptr, *d_end_ref = nullptr, *d_end_q = nullptr;
+    uint8_t *d_status = nullptr;
+};
+
+struct ish_hip_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // query
+    bool have_query = false;
+    int32_t Q = 0, algo = 0, go = 0, ge = 0, kind = 0, nchunks = 0, stride = 0, rows = 0;
+    uint32_t flags = 0;
+    int32_t neg_limit = 0, min_score = 0, max_score = 0;
+    int64_t r16max = -1; // longest target routed to the int16 kernel
+    int32_t prof_words = 0;
+    size_t lds = 0;
+    DevBuf prof_t, prof_p, lut;
+    int engine = ISH_ENGINE_BATCHED;
+    // work buffers
+    DevBuf scratch, counter, in_res, in_off, out_score, out_er, out_eq, out_st;
+    // timing
+    std::vector<hipEvent_t> ev;
+    int launches = 0;
+    float last_ms = 0.f;
+};
+
+static int32_t fail(ish_hip_ctx *ctx, int32_t code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return fail(ctx, _e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP,         \
+                        "%s failed: %s", #expr, hipGetErrorString(_e));                   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+extern "C" int32_t ish_hip_version(void) { return ISH_HIP_VERSION; }
+
+extern "C" int32_t ish_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    int ok = 0;
+    for (int d = 0; d < n; ++d) {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, d) == hipSuccess && strncmp(pr.gcnArchName, "gfx950", 6) == 0)
+            ++ok;
+    }
+    return ok;
+}
+
+static int32_t alphabet_of(int32_t kind) {
+    switch (kind) {
+    case ISH_MATRIX_ASCII: return 256;
+    case ISH_MATRIX_ACTGN:
+    case ISH_MATRIX_ACTGN0: return 5;
+    case ISH_MATRIX_BLOSUM62: return 24;
+    default: return -1;
+    }
+}
+
+extern "C" int32_t ish_hip_matrix(int32_t kind, int8_t *out, int32_t cap) {
+    const int32_t A = alphabet_of(kind);
+    if (A < 0 || !out || cap < A * A) return ISH_E_INVALID;
+    for (int a = 0; a < A; ++a)
+        for (int b = 0; b < A; ++b) {
+            int8_t v;
+            if (kind == ISH_MATRIX_ASCII) v = a == b ? 2 : -2;
+            else if (kind == ISH_MATRIX_BLOSUM62) v = kBlosum62[a * 24 + b];
+            else v = nt_score(a, b, kind == ISH_MATRIX_ACTGN);
+            out[a * A + b] = v;
+        }
+    return A;
+}
+
+extern "C" int32_t ish_hip_encoding(int32_t kind, uint8_t *out) {
+    if (!out) return ISH_E_INVALID;
+    switch (kind) {
+    case ISH_MATRIX_ASCII:
+        for (int c = 0; c < 256; ++c) out[c] = (uint8_t)c;
+        return ISH_OK;
+    case ISH_MATRIX_ACTGN:
+    case ISH_MATRIX_ACTGN0:
+        for (int c = 0; c < 256; ++c) {
+            const int u = (c >= 'a' && c <= 'z') ? c - 32 : c;
+            out[c] = u == 'A' ? 0 : u == 'C' ? 1 : u == 'G' ? 2 : (u == 'T' || u == 'U') ? 3 : 4;
+        }
+        return ISH_OK;
+    case ISH_MATRIX_BLOSUM62:
+        for (int c = 0; c < 256; ++c) {
+            const int u = (c >= 'a' && c <= 'z') ? c - 32 : c;
+            uint8_t code = 23;
+            if (u >= 'A' && u <= 'Z')
+                for (int k = 0; k < 23; ++k)
+                    if (kAminoLetters[k] == u) code = (uint8_t)k;
+            out[c] = code;
+        }
+        return ISH_OK;
+    default: return ISH_E_INVALID;
+    }
+}
+
+extern "C" int32_t ish_hip_decoding(int32_t kind, uint8_t *out, int32_t cap) {
+    const int32_t A = alphabet_of(kind);
+    if (A < 0 || !out || cap < A) return ISH_E_INVALID;
+    for (int c = 0; c < A; ++c)
+        out[c] = kind == ISH_MATRIX_ASCII ? (uint8_t)c
+                 : kind == ISH_MATRIX_BLOSUM62 ? (uint8_t)kAminoLetters[c]
+                                               : (uint8_t)kNucLetters[c];
+    return A;
+}
+
+extern "C" int64_t ish_hip_encode_and_score(int32_t kind, const uint8_t *ascii, int64_t n,
+                                            uint8_t *out) {
+    uint8_t enc[256];
+    const int32_t A = alphabet_of(kind);
+    if (A < 0 || ish_hip_encoding(kind, enc) != ISH_OK || (n > 0 && (!ascii || !out)))
+        return INT64_MIN;
+    std::vector<int8_t> m((size_t)A * A);
+    ish_hip_matrix(kind, m.data(), A * A);
+    int64_t s = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint8_t c = enc[ascii[i]];
+        out[i] = c;
+        s += m[(size_t)c * A + c];
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
+    if (!out) return ISH_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return ISH_E_NO_DEVICE;
+    if (device < 0 || device >= n) return ISH_E_INVALID;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, device) != hipSuccess) return ISH_E_HIP;
+    if (strncmp(pr.gcnArchName, "gfx950", 6) != 0) return ISH_E_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return ISH_E_HIP;
+    ish_hip_ctx *c = new ish_hip_ctx();
+    c->device = device;
+    c->cus = pr.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ISH_E_HIP;
+    }
+    *out = c;
+    return ISH_OK;
+}
+
+extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto e : ctx->ev) hipEventDestroy(e);
+    for (DevBuf *b : {&ctx->prof_t, &ctx->prof_p, &ctx->lut, &ctx->scratch, &ctx->counter,
+                      &ctx->in_res, &ctx->in_off, &ctx->out_score, &ctx->out_er, &ctx->out_eq,
+                      &ctx->out_st})
+        b->release();
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *ish_hip_last_error(const ish_hip_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine) {
+    if (!ctx) return ISH_E_INVALID;
+    if (engine != ISH_ENGINE_BATCHED && engine != ISH_ENGINE_PLAIN)
+        return fail(ctx, ISH_E_INVALID, "unknown engine %d", engine);
+    ctx->engine = engine;
+    return ISH_OK;
+}
+
+// Longest target the packed int16 kernel may score for this query: every
+// value the offset-form recurrence produces (H^, E^, F^, their sums with the
+// profile and gap terms, the tracking keys) is then proven to lie strictly
+// inside int16, so int16 arithmetic equals int32 arithmetic. The bounds:
+//   H(j,i) <= (j+1)*M                       (each row adds at most M=max S)
+//   H(j,i) >= -2go - ge*(i+j)               (gap from the boundary)
+//   H^ = H + (i+j)*ge in [-2go, Q*M + (R+Q)*ge]
+// plus one profile term (<= M + 2ge) and one gap term (>= -go - m).
+static int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s) {
+    const int64_t M = std::max(max_s, 0), m = std::max(-min_s, 0);
+    const int64_t lim = 32767 - 16;
+    if (3LL * go + 2LL * ge + m > lim) return -1;
+    int64_t best = 65534;
+    // upper: Q*M + (R+Q)*ge + M + 2ge + go <= lim
+    const int64_t up = lim - (int64_t)Q * M - M - 2LL * ge - go;
+    if (up < (int64_t)Q * ge) return -1;
+    if (ge > 0) best = std::min(best, up / ge - Q);
+    // lower (plain values, the tracking keys and the int16 floor):
+    // 2go + ge*(R+Q) + max(go, m) + 2 <= lim
+    const int64_t lo = lim - 2LL * go - std::max<int64_t>(go, m) - 2;
+    if (lo < (int64_t)Q * ge) return -1;
+    if (ge > 0) best = std::min(best, lo / ge - Q);
+    if ((int64_t)Q > 65534) return -1;
+    return best < 0 ? -1 : best;
+}
+
+extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t Q,
+                                     const int8_t *matrix, int32_t A, int32_t kind, int32_t go,
+                                     int32_t ge, uint32_t ends_free, int32_t algo) {
+    if (!ctx) return ISH_E_INVALID;
+    if (Q < 0 || (Q > 0 && !q)) return fail(ctx, ISH_E_INVALID, "bad query");
+    if (algo != ISH_ALGO_SEMI_GLOBAL && algo != ISH_ALGO_LOCAL)
+        return fail(ctx, ISH_E_INVALID, "unknown algo %d", algo);
+    if (algo == ISH_ALGO_LOCAL) return fail(ctx, ISH_E_INVALID, "local alignment not built yet");
+    if (go < 0 || ge < 0) return fail(ctx, ISH_E_INVALID, "gap penalties must be >= 0");
+    std::vector<int8_t> mat;
+    if (!matrix) {
+        A = alphabet_of(kind);
+        if (A < 0) return fail(ctx, ISH_E_INVALID, "unknown matrix kind %d", kind);
+        mat.resize((size_t)A * A);
+        ish_hip_matrix(kind, mat.data(), A * A);
+    } else {
+        if (A <= 0 || A > 256) return fail(ctx, ISH_E_INVALID, "alphabet must be 1..256");
+        mat.assign(matrix, matrix + (size_t)A * A);
+    }
+    for (int32_t j = 0; j < Q; ++j)
+        if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query code %d at %d >= alphabet %d", q[j], j, A);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+
+    int32_t mn = 0, mx = 0; // both seeded at 0 (striped.mojo:62-71)
+    for (int8_t v : mat) {
+        mn = std::min<int32_t>(mn, v);
+        mx = std::max<int32_t>(mx, v);
+    }
+    const int32_t nl = (-go < mn) ? -32768 + go + 1 : -32768 - mn + 1;
+
+    // Profile rows: target codes whose score column over the query is
+    // identical share a row (for the ascii matrix this collapses 256 codes
+    // to the distinct query bytes plus one "absent" row).
+    std::map<std::vector<int8_t>, int> row_of;
+    std::vector<std::vector<int8_t>> rows;
+    uint8_t lut[256];
+    for (int c = 0; c < 256; ++c) {
+        const int cc = c < A ? c : A - 1; // out-of-alphabet codes score as the last code
+        std::vector<int8_t> col((size_t)Q);
+        for (int32_t j = 0; j < Q; ++j) col[j] = mat[(size_t)cc * A + q[j]];
+        auto it = row_of.find(col);
+        if (it == row_of.end()) {
+            it = row_of.emplace(col, (int)rows.size()).first;
+            rows.push_back(col);
+        }
+        lut[c] = (uint8_t)it->second;
+    }
+    const int32_t nch = (Q + kChunk - 1) / kChunk;
+    const int32_t qpad = nch * kChunk;
+    const int32_t stride = qpad + 2; // odd dword count per row: rows hit distinct LDS banks
+    const int32_t nrows = (int32_t)rows.size();
+    const size_t prof_elems = (size_t)nrows * stride;
+    const size_t lds = prof_elems * 2 + 256;
+    if (lds > kLdsLimit)
+        return fail(ctx, ISH_E_RANGE, "query profile needs %zu bytes of LDS (limit %zu)", lds,
+                    kLdsLimit);
+    std::vector<int16_t> pt(prof_elems, 0), pp(prof_elems, 0);
+    for (int32_t r = 0; r < nrows; ++r)
+        for (int32_t j = 0; j < Q; ++j) {
+            pp[(size_t)r * stride + j] = rows[r][j];
+            pt[(size_t)r * stride + j] = (int16_t)(rows[r][j] + 2 * ge);
+        }
+    HIP_TRY(ctx, ctx->prof_t.ensure(prof_elems * 2));
+    HIP_TRY(ctx, ctx->prof_p.ensure(prof_elems * 2));
+    HIP_TRY(ctx, ctx->lut.ensure(256));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_t.p, pt.data(), prof_elems * 2, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_p.p, pp.data(), prof_elems * 2, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->lut.p, lut, 256, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+
+    ctx->Q = Q;
+    ctx->algo = algo;
+    ctx->go = go;
+    ctx->ge = ge;
+    ctx->kind = kind;
+    ctx->flags = ends_free & 15u;
+    ctx->nchunks = nch;
+    ctx->stride = stride;
+    ctx->rows = nrows;
+    ctx->neg_limit = (int32_t)(int16_t)nl;
+    ctx->min_score = mn;
+    ctx->max_score = mx;
+    ctx->prof_words = (int32_t)(prof_elems / 2);
+    ctx->lds = lds;
+    ctx->r16max = int16_max_len(Q, go, ge, mx, mn);
+    ctx->have_query = true;
+    return ISH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// timing helpers
+// ---------------------------------------------------------------------------
+static hipError_t timed_begin(ish_hip_ctx *ctx) {
+    const size_t need = (size_t)(ctx->launches + 1) * 2;
+    while (ctx->ev.size() < need) {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        ctx->ev.push_back(e);
+    }
+    return hipEventRecord(ctx->ev[(size_t)ctx->launches * 2], ctx->stream);
+}
+static hipError_t timed_end(ish_hip_ctx *ctx) {
+    hipError_t r = hipEventRecord(ctx->ev[(size_t)ctx->launches * 2 + 1], ctx->stream);
+    ctx->launches++;
+    return r;
+}
+static void collect_timing(ish_hip_ctx *ctx) {
+    float total = 0.f;
+    for (int k = 0; k < ctx->launches; ++k) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[(size_t)k * 2], ctx->ev[(size_t)k * 2 + 1]) == hipSuccess)
+            total += ms;
+    }
+    ctx->last_ms = total;
+}
+
+extern "C" int32_t ish_hip_last_timing(const ish_hip_ctx *ctx, float *ms, int32_t *launches) {
+    if (!ctx) return ISH_E_INVALID;
+    if (ms) *ms = ctx->last_ms;
+    if (launches) *launches = ctx->launches;
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_sync(ish_hip_ctx *ctx) {
+    if (!ctx) return ISH_E_INVALID;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timing(ctx);
+    return ISH_OK;
+}
+
+// Scratch for the boundary rows: one slot of `cols` columns x 64 lanes x 8
+// bytes per resident wave. The grid is capped so that scratch stays within
+// half of the free device memory.
+static int32_t size_grid(ish_hip_ctx *ctx, int occ_blocks, int64_t cols, int *grid_out,
+                         int64_t *slot_stride) {
+    int64_t grid = (int64_t)occ_blocks * ctx->cus;
+    const int64_t per_block = (int64_t)(kBlock / 64) * std::max<int64_t>(cols, 4) * 64 * 8;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+    const int64_t budget = std::min<int64_t>((int64_t)(free_b / 2) + (int64_t)ctx->scratch.cap,
+                                             (int64_t)64 << 30);
+    if (grid * per_block > budget) grid = std::max<int64_t>(1, budget / per_block);
+    HIP_TRY(ctx, ctx->scratch.ensure((size_t)(grid * per_block)));
+    *grid_out = (int)grid;
+    *slot_stride = std::max<int64_t>(cols, 4) * 64;
+    return ISH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident database
+// ---------------------------------------------------------------------------
+extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
+                                   const uint64_t *offsets, int64_t n, ish_hip_db **out) {
+    if (!ctx || !out || n < 0 || (n > 0 && (!offsets || !residues)))
+        return fail(ctx, ISH_E_INVALID, "bad db arguments");
+    if (n > (int64_t)INT32_MAX - kGroup) return fail(ctx, ISH_E_RANGE, "too many targets");
+    *out = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const uint64_t base = n > 0 ? offsets[0] : 0;
+    const uint64_t nres = n > 0 ? offsets[n] - base : 0;
+    int64_t maxlen = 0;
+    for (int64_t t = 0; t < n; ++t) {
+        if (offsets[t + 1] < offsets[t]) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
+        maxlen = std::max<int64_t>(maxlen, (int64_t)(offsets[t + 1] - offsets[t]));
+    }
+    if (maxlen > (int64_t)INT32_MAX / 2) return fail(ctx, ISH_E_RANGE, "target too long");
+    // counting sort by length, descending, stable in input order
+    std::vector<int64_t> hist((size_t)maxlen + 2, 0);
+    for (int64_t t = 0; t < n; ++t) hist[(size_t)(maxlen - (int64_t)(offsets[t + 1] - offsets[t]))]++;
+    int64_t acc = 0;
+    for (auto &h : hist) {
+        const int64_t v = h;
+        h = acc;
+        acc += v;
+    }
+    const int32_t ng = (int32_t)((n + kGroup - 1) / kGroup);
+    const int64_t nslots = (int64_t)ng * kGroup;
+    std::vector<int32_t> sorted((size_t)nslots, -1), lens((size_t)nslots, 0);
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t L = (int64_t)(offsets[t + 1] - offsets[t]);
+        const int64_t pos = hist[(size_t)(maxlen - L)]++;
+        sorted[(size_t)pos] = (int32_t)t;
+        lens[(size_t)pos] = (int32_t)L;
+    }
+    ish_hip_db *db = new ish_hip_db();
+    db->device = ctx->device;
+    db->n = n;
+    db->nres = (int64_t)nres;
+    db->ngroups = ng;
+    std::vector<GroupDesc> groups((size_t)ng);
+    uint64_t off = 0;
+    for (int32_t g = 0; g < ng; ++g) {
+        const int32_t L = lens[(size_t)g * kGroup];
+        groups[g].res_off = off;
+        groups[g].ncols = (L + 3) & ~3;
+        groups[g].first = g * kGroup;
+        db->group_maxlen.push_back(L);
+        db->maxcols = std::max(db->maxcols, groups[g].ncols);
+        off += (uint64_t)(groups[g].ncols / 4) * 64;
+    }
+    const uint64_t res_words = off + 64; // one spare block: empty groups read block 0
+    auto cleanup = [&](int32_t code) {
+        ish_hip_db_free(db);
+        return code;
+    };
+    hipError_t e;
+#define DB_TRY(expr)                                                                          \
+    if ((e = (expr)) != hipSuccess)                                                           \
+        return cleanup(fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "%s: %s", \
+                            #expr, hipGetErrorString(e)));
+    DB_TRY(hipMalloc((void **)&db->d_groups, sizeof(GroupDesc) * std::max(ng, 1)));
+    DB_TRY(hipMalloc((void **)&db->d_res, sizeof(uint2) * res_words));
+    DB_TRY(hipMalloc((void **)&db->d_lens, sizeof(int32_t) * std::max<int64_t>(nslots, 1)));
+    DB_TRY(hipMalloc((void **)&db->d_orig, sizeof(int32_t) * std::max<int64_t>(nslots, 1)));
+    DB_TRY(hipMalloc((void **)&db->d_score, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    DB_TRY(hipMalloc((void **)&db->d_end_ref, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    DB_TRY(hipMalloc((void **)&db->d_end_q, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    DB_TRY(hipMalloc((void **)&db->d_status, std::max<int64_t>(n, 1)));
+    DB_TRY(hipMemsetAsync(db->d_res, 0, sizeof(uint2) * res_words, ctx->stream));
+    if (ng > 0) {
+        DB_TRY(hipMemcpyAsync(db->d_groups, groups.data(), sizeof(GroupDesc) * ng, hipMemcpyHostToDevice, ctx->stream));
+        DB_TRY(hipMemcpyAsync(db->d_lens, lens.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
+        DB_TRY(hipMemcpyAsync(db->d_orig, sorted.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
+        // raw residues + offsets, rebased to 0, packed on the device
+        uint8_t *d_raw = nullptr;
+        uint64_t *d_off = nullptr;
+        std::vector<uint64_t> rebased((size_t)n + 1);
+        for (int64_t t = 0; t <= n; ++t) rebased[(size_t)t] = offsets[t] - base;
+        DB_TRY(hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1)));
+        if ((e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (n + 1))) != hipSuccess) {
+            hipFree(d_raw);
+            return cleanup(fail(ctx, ISH_E_NOMEM, "hipMalloc offsets: %s", hipGetErrorString(e)));
+        }
+        hipError_t e1 = hipMemcpyAsync(d_raw, residues + base, nres, hipMemcpyHostToDevice, ctx->stream);
+        hipError_t e2 = hipMemcpyAsync(d_off, rebased.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
+        hipError_t e3 = (e1 == hipSuccess && e2 == hipSuccess)
+                            ? launch_pack(d_raw, d_off, db->d_orig, db->d_groups, ng, n, db->d_res, ctx->stream)
+                            : (e1 != hipSuccess ? e1 : e2);
+        hipError_t e4 = hipStreamSynchronize(ctx->stream);
+        hipFree(d_raw);
+        hipFree(d_off);
+        if (e3 != hipSuccess || e4 != hipSuccess)
+            return cleanup(fail(ctx, ISH_E_HIP, "db packing failed: %s",
+                                hipGetErrorString(e3 != hipSuccess ? e3 : e4)));
+    }
+    DB_TRY(hipStreamSynchronize(ctx->stream));
+#undef DB_TRY
+    *out = db;
+    return ISH_OK;
+}
+
+extern "C" void ish_hip_db_free(ish_hip_db *db) {
+    if (!db) return;
+    hipSetDevice(db->device);
+    for (void *p : {(void *)db->d_groups, (void *)db->d_res, (void *)db->d_lens, (void *)db->d_orig,
+                    (void *)db->d_score, (void *)db->d_end_ref, (void *)db->d_end_q,
+                    (void *)db->d_status})
+        if (p) hipFree(p);
+    delete db;
+}
+
+extern "C" int32_t ish_hip_db_size(const ish_hip_db *db, int64_t *n, int64_t *nres) {
+    if (!db) return ISH_E_INVALID;
+    if (n) *n = db->n;
+    if (nres) *nres = db->nres;
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
+    if (!ctx || !db) return ISH_E_INVALID;
+    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
+    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx->launches = 0;
+    if (db->ngroups == 0) return ISH_OK;
+    HIP_TRY(ctx, ctx->counter.ensure(256));
+
+    // groups whose longest target exceeds the int16 bound come first
+    // (descending lengths); they run in the int32 kernel.
+    int32_t g16 = 0;
+    {
+        int32_t lo = 0, hi = db->ngroups;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) / 2;
+            if ((int64_t)db->group_maxlen[mid] > ctx->r16max) lo = mid + 1;
+            else hi = mid;
+        }
+        g16 = lo;
+    }
+    ScoreParams p{};
+    p.prof = (const int16_t *)ctx->prof_t.p;
+    p.lut = (const uint8_t *)ctx->lut.p;
+    p.prof_words = ctx->prof_words;
+    p.stride = ctx->stride;
+    p.Q = ctx->Q;
+    p.nchunks = ctx->nchunks;
+    p.go = ctx->go;
+    p.ge = ctx->ge;
+    p.flags = ctx->flags;
+    p.neg_limit = ctx->neg_limit;
+    p.groups = db->d_groups;
+    p.res = db->d_res;
+    p.lens = db->d_lens;
+    p.orig = db->d_orig;
+    p.counter = (uint32_t *)ctx->counter.p;
+    p.score = db->d_score;
+    p.end_ref = db->d_end_ref;
+    p.end_q = db->d_end_q;
+    p.status = db->d_status;
+
+    if (g16 > 0) {
+        int grid;
+        int64_t ss;
+        const int occ = occupancy_sg_batched_i32(kBlock, ctx->lds, ctx->flags);
+        int32_t rc = size_grid(ctx, occ, db->group_maxlen[0] + 3, &grid, &ss);
+        if (rc) return rc;
+        p.bnd = (uint2 *)ctx->scratch.p;
+        p.slot_stride = ss;
+        p.g_begin = 0;
+        p.g_end = g16;
+        HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
+        HIP_TRY(ctx, timed_begin(ctx));
+        HIP_TRY(ctx, launch_sg_batched_i32(p, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, timed_end(ctx));
+    }
+    if (g16 < db->ngroups) {
+        int grid;
+        int64_t ss;
+        const int occ = occupancy_sg_batched_i16(kBlock, ctx->lds, ctx->flags);
+        int32_t rc = size_grid(ctx, occ, db->group_maxlen[g16] + 3, &grid, &ss);
+        if (rc) return rc;
+        p.bnd = (uint2 *)ctx->scratch.p;
+        p.slot_stride = ss;
+        p.g_begin = g16;
+        p.g_end = db->ngroups;
+        HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
+        HIP_TRY(ctx, timed_begin(ctx));
+        HIP_TRY(ctx, launch_sg_batched_i16(p, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, timed_end(ctx));
+    }
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
+                                    int32_t *end_ref, int32_t *end_q, uint8_t *status) {
+    if (!ctx || !db || (db->n > 0 && (!score || !end_ref || !end_q)))
+        return ISH_E_INVALID;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int64_t n = db->n;
+    if (n > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(score, db->d_score, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(end_ref, db->d_end_ref, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(end_q, db->d_end_q, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        if (status)
+            HIP_TRY(ctx, hipMemcpyAsync(status, db->d_status, n, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timing(ctx);
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_align_db(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
+                                    int32_t *end_ref, int32_t *end_q, uint8_t *status) {
+    int32_t rc = ish_hip_align_db_async(ctx, db);
+    if (rc) return rc;
+    return ish_hip_db_fetch(ctx, db, score, end_ref, end_q, status);
+}
+
+// ---------------------------------------------------------------------------
+// one synchronous batch
+// ---------------------------------------------------------------------------
+static int32_t align_plain(ish_hip_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
+                           int64_t n, int32_t *score, int32_t *end_ref, int32_t *end_q,
+                           uint8_t *status) {
+    const uint64_t base = offsets[0];
+    const uint64_t nres = offsets[n] - base;
+    int64_t maxlen = 0;
+    std::vector<uint64_t> rebased((size_t)n + 1);
+    for (int64_t t = 0; t <= n; ++t) {
+        if (t < n && offsets[t + 1] < offsets[t]) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
+        rebased[(size_t)t] = offsets[t] - base;
+        if (t < n) maxlen = std::max<int64_t>(maxlen, (int64_t)(offsets[t + 1] - offsets[t]));
+    }
+    HIP_TRY(ctx, ctx->in_res.ensure(std::max<uint64_t>(nres, 1)));
+    HIP_TRY(ctx, ctx->in_off.ensure(sizeof(uint64_t) * (n + 1)));
+    HIP_TRY(ctx, ctx->out_score.ensure(sizeof(int32_t) * n));
+    HIP_TRY(ctx, ctx->out_er.ensure(sizeof(int32_t) * n));
+    HIP_TRY(ctx, ctx->out_eq.ensure(sizeof(int32_t) * n));
+    HIP_TRY(ctx, ctx->out_st.ensure((size_t)n));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->in_res.p, residues + base, nres, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->in_off.p, rebased.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream));
+    PlainParams p{};
+    p.prof = (const int16_t *)ctx->prof_p.p;
+    p.lut = (const uint8_t *)ctx->lut.p;
+    p.prof_words = ctx->prof_words;
+    p.stride = ctx->stride;
+    p.Q = ctx->Q;
+    p.nchunks = ctx->nchunks;
+    p.go = ctx->go;
+    p.ge = ctx->ge;
+    p.flags = ctx->flags;
+    p.neg_limit = ctx->neg_limit;
+    p.res = (const uint8_t *)ctx->in_res.p;
+    p.off = (const uint64_t *)ctx->in_off.p;
+    p.n = n;
+    p.score = (int32_t *)ctx->out_score.p;
+    p.end_ref = (int32_t *)ctx->out_er.p;
+    p.end_q = (int32_t *)ctx->out_eq.p;
+    p.status = (uint8_t *)ctx->out_st.p;
+    int grid;
+    int64_t ss;
+    const int occ = occupancy_sg_plain(kBlock, ctx->lds, ctx->flags);
+    int32_t rc = size_grid(ctx, occ, maxlen, &grid, &ss);
+    if (rc) return rc;
+    const int64_t waves_needed = (n + 63) / 64;
+    grid = (int)std::min<int64_t>(grid, (waves_needed + kBlock / 64 - 1) / (kBlock / 64));
+    p.bnd = (int2 *)ctx->scratch.p;
+    p.slot_stride = ss;
+    ctx->launches = 0;
+    HIP_TRY(ctx, timed_begin(ctx));
+    HIP_TRY(ctx, launch_sg_plain(p, grid, kBlock, ctx->lds, ctx->stream));
+    HIP_TRY(ctx, timed_end(ctx));
+    HIP_TRY(ctx, hipMemcpyAsync(score, p.score, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(end_ref, p.end_ref, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(end_q, p.end_q, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (status)
+        HIP_TRY(ctx, hipMemcpyAsync(status, p.status, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timing(ctx);
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
+                                       const uint64_t *offsets, int64_t n, int32_t *score,
+                                       int32_t *end_ref, int32_t *end_q, uint8_t *status) {
+    if (!ctx) return ISH_E_INVALID;
+    if (n < 0 || (n > 0 && (!offsets || !residues || !score || !end_ref || !end_q)))
+        return fail(ctx, ISH_E_INVALID, "bad batch arguments");
+    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
+    if (n == 0) return ISH_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (ctx->engine == ISH_ENGINE_PLAIN)
+        return align_plain(ctx, residues, offsets, n, score, end_ref, end_q, status);
+    ish_hip_db *db = nullptr;
+    int32_t rc = ish_hip_db_load(ctx, residues, offsets, n, &db);
+    if (rc) return rc;
+    rc = ish_hip_align_db(ctx, db, score, end_ref, end_q, status);
+    ish_hip_db_free(db);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// emission
+// ---------------------------------------------------------------------------
+extern "C" int32_t ish_hip_emit(int32_t algo, const int32_t *score, const int32_t *end_ref,
+                                const int32_t *end_q, int64_t n, int64_t max_score,
+                                float threshold, uint8_t *emit) {
+    if (n < 0 || (n > 0 && (!score || !emit))) return ISH_E_INVALID;
+    if (algo == ISH_ALGO_LOCAL && n > 0 && (!end_ref || !end_q)) return ISH_E_INVALID;
+    const volatile float ms = (float)max_score;
+    for (int64_t t = 0; t < n; ++t) {
+        const volatile float s = (float)score[t];
+        bool ok;
+        if (algo == ISH_ALGO_SEMI_GLOBAL) {
+            const volatile float cutoff = ms * threshold;
+            ok = !(s < cutoff);
+            if (ok) {
+                const volatile float ratio = s / ms;
+                ok = ratio >= threshold;
+            }
+        } else {
+            const volatile float cutoff = threshold * ms;
+            ok = s > cutoff && end_ref[t] > 0 && end_q[t] > 0;
+            if (ok) {
+                const volatile float ratio = s / ms;
+                ok = ratio >= threshold;
+            }
+        }
+        emit[t] = ok ? 1 : 0;
+    }
+    return ISH_OK;
+}

@@ -1,0 +1,586 @@
+// Semi-global (Gotoh, configurable free ends) scoring kernels for gfx950.
+//
+// Semantics: semi_global_aln of the reference's CPU striped path
+// (ishlib/matcher/alignment/semi_global_aln/striped.mojo:252-498):
+//   H(-1,-1)=0, H(j,-1)=qs?0:-go-ge*j, H(-1,i)=ts?0:-go-ge*i,
+//   E(j,i)=max(E(j,i-1)-ge, H(j,i-1)-go), F(j,i)=max(F(j-1,i)-ge, H(j-1,i)-go),
+//   H(j,i)=max(H(j-1,i-1)+S[r_i][q_j], E, F);
+//   end rules: first column of the strict last-row maximum seeded at the
+//   int16 floor (:436-441, :446-450), smallest row of the strict last-column
+//   maximum with the end_ref == R-1 tie rule (:453-468), corner otherwise
+//   (:471-477).
+//
+// Layout of the work: one lane owns whole targets and a chunk of K query
+// rows in registers; it streams the target's residues column by column.
+// The bottom row of a chunk (H and F of every column) goes to a per-wave
+// boundary buffer and is read back by the next chunk. There is no
+// cross-lane traffic and no data-dependent control flow in the cell loop.
+//
+// Batched kernels (persistent, one launch drains a device work queue over
+// length-sorted groups of 128 targets):
+//   - sg_batched<short2>: two targets per lane in packed int16 halves.
+//   - sg_batched<int>   : one target per lane in int32, for groups whose
+//                         proven score range does not fit int16.
+// Both use the offset form of the recurrence, H^ = H + (i+j)*ge (likewise
+// E^, F^), which removes the two gap-extension subtractions:
+//   G  = H^ + (ge-go)
+//   E^ = max(E^, G)          (next column)
+//   F^ = max(F^, G)          (next row)
+//   H^ = max(H^diag + (S+2ge), E^, F^)
+// i.e. 6 integer ops per cell instead of 8.
+//
+// Plain kernel (one launch per batch, the comparison baseline): one target
+// per lane in input order, int32, textbook 8-op recurrence, residues read
+// straight from the caller's contiguous layout.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <stdint.h>
+
+#include "engine.h"
+
+namespace ish {
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) {
+    return __builtin_elementwise_max(a, b);
+}
+__device__ __forceinline__ int vmax(int a, int b) { return a > b ? a : b; }
+
+template <typename V> __device__ __forceinline__ V splat(int v);
+template <> __device__ __forceinline__ s16x2 splat<s16x2>(int v) {
+    s16x2 r;
+    r.x = (short)v;
+    r.y = (short)v;
+    return r;
+}
+template <> __device__ __forceinline__ int splat<int>(int v) { return v; }
+
+__device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t as_u32(int v) { return (uint32_t)v; }
+template <typename V> __device__ __forceinline__ V from_u32(uint32_t v);
+template <> __device__ __forceinline__ s16x2 from_u32<s16x2>(uint32_t v) {
+    return __builtin_bit_cast(s16x2, v);
+}
+template <> __device__ __forceinline__ int from_u32<int>(uint32_t v) { return (int)v; }
+
+__device__ __forceinline__ uint32_t wave_broadcast0(uint32_t v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Stage the query profile and the residue->row table into LDS.
+__device__ __forceinline__ void stage_profile(const int16_t *prof, const uint8_t *lut,
+                                              int prof_words, uint8_t *smem) {
+    const int *src = (const int *)prof;
+    int *dst = (int *)smem;
+    for (int x = threadIdx.x; x < prof_words; x += blockDim.x) dst[x] = src[x];
+    const int *lsrc = (const int *)lut;
+    int *ldst = (int *)(smem + (size_t)prof_words * 4);
+    for (int x = threadIdx.x; x < 64; x += blockDim.x) ldst[x] = lsrc[x];
+    __syncthreads();
+}
+
+// Running (value, first index) maximum as one ordered key per target:
+// value in the high 16 bits (signed), 0xFFFF - index in the low 16 bits, so
+// the integer maximum picks the largest value and, among equals, the
+// smallest index. Valid while values fit int16 and indices < 65536, which
+// the int16 range bound guarantees for every group routed to this kernel.
+__device__ __forceinline__ int key_lo(uint32_t packed, int idx) {
+    return (int)((packed << 16) | (uint32_t)(0xFFFF - idx));
+}
+__device__ __forceinline__ int key_hi(uint32_t packed, int idx) {
+    return (int)((packed & 0xFFFF0000u) | (uint32_t)(0xFFFF - idx));
+}
+
+struct TrackI16 {
+    int te_a, te_b, qe_a, qe_b, corner_a, corner_b;
+};
+
+// Int32 tracking keeps (value, index) pairs explicitly.
+struct TrackI32 {
+    int te_v, te_i, qe_v, qe_j, corner;
+};
+
+// Final end-coordinate rules shared by every kernel (values already
+// converted back from the offset form).
+__device__ __forceinline__ void finalize(int R, int Q, bool te, bool qe, bool has_te,
+                                         int te_val, int te_idx, bool has_qe, int qe_val,
+                                         int qe_j, int corner, int neg_limit, int &score,
+                                         int &end_ref, int &end_q) {
+    if (R == 0 || Q == 0) {
+        score = 0;
+        end_ref = 0;
+        end_q = 0;
+        return;
+    }
+    score = neg_limit;
+    end_ref = -1;
+    end_q = Q - 1;
+    if (te && has_te && te_val > neg_limit) {
+        score = te_val;
+        end_ref = te_idx;
+    }
+    if (qe && has_qe) {
+        if (qe_val > score) {
+            score = qe_val;
+            end_q = qe_j;
+            end_ref = R - 1;
+        } else if (qe_val == score && end_ref == R - 1 && qe_j < end_q) {
+            end_q = qe_j;
+        }
+    }
+    if (!te && !qe) {
+        score = corner;
+        end_ref = R - 1;
+        end_q = Q - 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent kernel.
+//   V = s16x2 : lane owns targets (first + lane) and (first + 64 + lane)
+//   V = int   : work item = (group, half); lane owns one target
+// ---------------------------------------------------------------------------
+template <typename V, int K, bool TE, bool QE>
+__global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    stage_profile(p.prof, p.lut, p.prof_words, smem);
+    const int16_t *sprof = (const int16_t *)smem;
+    const uint8_t *slut = smem + (size_t)p.prof_words * 4;
+
+    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    uint2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
+    const int go = p.go, ge = p.ge, cst = ge - go, Q = p.Q;
+    const bool qs = p.flags & 1u, ts = p.flags & 4u;
+    const int stride = p.stride;
+    const V vcst = splat<V>(cst);
+    const int items = PACKED ? (p.g_end - p.g_begin) : 2 * (p.g_end - p.g_begin);
+
+    for (;;) {
+        uint32_t w = 0;
+        if (lane == 0) w = atomicAdd(p.counter, 1u);
+        w = wave_broadcast0(__shfl(w, 0));
+        if ((int)w >= items) break;
+        const int g = p.g_begin + (PACKED ? (int)w : (int)(w >> 1));
+        const int half = PACKED ? 0 : (int)(w & 1);
+        const GroupDesc gd = p.groups[g];
+        const int tA = gd.first + half * 64 + lane;
+        const int tB = gd.first + 64 + lane;
+        const int RA = p.lens[tA];
+        const int RB = PACKED ? p.lens[tB] : 0;
+        const uint2 *res = p.res + gd.res_off;
+        const int nblk = gd.ncols >> 2;
+
+        TrackI16 t16;
+        TrackI32 t32;
+        t16.te_a = t16.te_b = t16.qe_a = t16.qe_b = INT_MIN;
+        t16.corner_a = t16.corner_b = 0;
+        t32.te_v = INT_MIN;
+        t32.te_i = -1;
+        t32.qe_v = INT_MIN;
+        t32.qe_j = 0;
+        t32.corner = 0;
+
+        for (int c = 0; c < p.nchunks; ++c) {
+            const int j0 = c * K;
+            const bool first = c == 0;
+            const bool last = c == p.nchunks - 1;
+            const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
+            V H[K], E[K];
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const int j = j0 + m;
+                const int hl = qs ? (j - 1) * ge : -go - ge; // H^(j,-1)
+                H[m] = splat<V>(hl);
+                E[m] = splat<V>(hl + cst); // E^(j,0)
+            }
+            V diag_top = splat<V>(j0 == 0 ? -2 * ge : (qs ? (j0 - 2) * ge : -go - ge));
+
+            uint2 rr_next = res[lane];
+            uint2 bn_next[4];
+            if (!first) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
+            }
+            for (int b = 0; b < nblk; ++b) {
+                const uint2 rr = rr_next;
+                uint2 bn[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
+                if (b + 1 < nblk) {
+                    rr_next = res[(size_t)(b + 1) * 64 + lane];
+                    if (!first) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int i = b * 4 + t;
+                    uint32_t codeA = ((half ? rr.y : rr.x) >> (8 * t)) & 0xffu;
+                    const int16_t *pa = sprof + (int)slut[codeA] * stride + j0;
+                    const int16_t *pb = pa;
+                    if (PACKED) {
+                        uint32_t codeB = (rr.y >> (8 * t)) & 0xffu;
+                        pb = sprof + (int)slut[codeB] * stride + j0;
+                    }
+                    V htop, f;
+                    if (first) {
+                        const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
+                        htop = splat<V>(ht);
+                        f = splat<V>(ht + cst); // F^(0,i)
+                    } else {
+                        htop = from_u32<V>(bn[t].x);
+                        f = from_u32<V>(bn[t].y);
+                    }
+                    V diag = diag_top;
+#pragma unroll
+                    for (int m = 0; m < K; ++m) {
+                        V s;
+                        if constexpr (PACKED) {
+                            s.x = pa[m];
+                            s.y = pb[m];
+                        } else {
+                            s = (int)pa[m];
+                        }
+                        V h = vmax(vmax(diag + s, E[m]), f);
+                        diag = H[m];
+                        H[m] = h;
+                        const V gg = h + vcst;
+                        E[m] = vmax(E[m], gg);
+                        f = vmax(f, gg);
+                    }
+                    diag_top = htop;
+                    if (!last) {
+                        uint2 o;
+                        o.x = as_u32(H[K - 1]);
+                        o.y = as_u32(f);
+                        bnd[(size_t)i * 64 + lane] = o;
+                    } else if (TE || (!TE && !QE)) {
+                        // row Q-1 lives in register mstar of the last chunk
+                        V hl = H[0];
+#pragma unroll
+                        for (int m = 1; m < K; ++m)
+                            if (m == mstar) hl = H[m];
+                        // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
+                        const V u = hl - splat<V>(i * ge);
+                        if constexpr (PACKED) {
+                            const uint32_t uu = as_u32(u);
+                            if (TE) {
+                                if (i < RA) t16.te_a = max(t16.te_a, key_lo(uu, i));
+                                if (i < RB) t16.te_b = max(t16.te_b, key_hi(uu, i));
+                            } else {
+                                if (i == RA - 1) t16.corner_a = (int)(short)(uu & 0xffffu);
+                                if (i == RB - 1) t16.corner_b = (int)(short)(uu >> 16);
+                            }
+                        } else {
+                            const int uv = (int)as_u32(u);
+                            if (TE) {
+                                if (i < RA && uv > t32.te_v) {
+                                    t32.te_v = uv;
+                                    t32.te_i = i;
+                                }
+                            } else if (i == RA - 1) {
+                                t32.corner = uv;
+                            }
+                        }
+                    }
+                    if (QE) {
+                        const bool cap_a = i == RA - 1;
+                        const bool cap_b = PACKED && i == RB - 1;
+                        if (cap_a || cap_b) {
+#pragma unroll
+                            for (int m = 0; m < K; ++m) {
+                                if (last && m > mstar) continue;
+                                const int j = j0 + m;
+                                // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
+                                const V wv = H[m] - splat<V>(j * ge);
+                                if constexpr (PACKED) {
+                                    const uint32_t ww = as_u32(wv);
+                                    if (cap_a) t16.qe_a = max(t16.qe_a, key_lo(ww, j));
+                                    if (cap_b) t16.qe_b = max(t16.qe_b, key_hi(ww, j));
+                                } else {
+                                    const int wi = (int)as_u32(wv);
+                                    if (wi > t32.qe_v) {
+                                        t32.qe_v = wi;
+                                        t32.qe_j = j;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+
+        // results
+        const int qoff = (Q - 1) * ge;
+        if constexpr (PACKED) {
+            for (int which = 0; which < 2; ++which) {
+                const int t = which ? tB : tA;
+                const int R = which ? RB : RA;
+                const int o = p.orig[t];
+                if (o < 0) continue;
+                const int kte = which ? t16.te_b : t16.te_a;
+                const int kqe = which ? t16.qe_b : t16.qe_a;
+                const int cor = which ? t16.corner_b : t16.corner_a;
+                const int te_val = (kte >> 16) - qoff;
+                const int te_idx = 0xFFFF - (kte & 0xFFFF);
+                const int qe_val = (kqe >> 16) - (R - 1) * ge;
+                const int qe_j = 0xFFFF - (kqe & 0xFFFF);
+                int sc, er, eq;
+                finalize(R, Q, TE, QE, kte != INT_MIN, te_val, te_idx, kqe != INT_MIN, qe_val,
+                         qe_j, cor - qoff, p.neg_limit, sc, er, eq);
+                p.score[o] = sc;
+                p.end_ref[o] = er;
+                p.end_q[o] = eq;
+                p.status[o] = 16;
+            }
+        } else {
+            const int o = p.orig[tA];
+            if (o >= 0) {
+                int sc, er, eq;
+                finalize(RA, Q, TE, QE, t32.te_i >= 0, t32.te_v - qoff, t32.te_i,
+                         t32.qe_v != INT_MIN, t32.qe_v - (RA - 1) * ge, t32.qe_j,
+                         t32.corner - qoff, p.neg_limit, sc, er, eq);
+                p.score[o] = sc;
+                p.end_ref[o] = er;
+                p.end_q[o] = eq;
+                p.status[o] = 32 | 0x80;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain kernel: textbook int32 recurrence, one target per lane, input order,
+// grid-stride over targets (one launch per batch).
+// ---------------------------------------------------------------------------
+template <int K, bool TE, bool QE>
+__global__ void __launch_bounds__(256) sg_plain(PlainParams p) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    stage_profile(p.prof, p.lut, p.prof_words, smem);
+    const int16_t *sprof = (const int16_t *)smem;
+    const uint8_t *slut = smem + (size_t)p.prof_words * 4;
+
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    int2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
+    const int go = p.go, ge = p.ge, Q = p.Q, stride = p.stride;
+    const bool qs = p.flags & 1u, ts = p.flags & 4u;
+    const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+    const int64_t base_stride = nthreads - (threadIdx.x & 63);
+    (void)base_stride;
+
+    // whole waves step together so that every lane of a wave uses the same
+    // boundary slot rows
+    for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < p.n;
+         t0 += nthreads) {
+        const int64_t t = t0 + lane;
+        const bool valid = t < p.n;
+        const uint8_t *r = p.res + (valid ? p.off[t] : 0);
+        const int R = valid ? (int)(p.off[t + 1] - p.off[t]) : 0;
+        // the wave runs to its longest target
+        int Rw = R;
+        for (int d = 32; d > 0; d >>= 1) Rw = max(Rw, __shfl_xor(Rw, d));
+
+        int te_v = p.neg_limit, te_i = -1, qe_v = INT_MIN, qe_j = 0, corner = 0;
+        for (int c = 0; c < p.nchunks; ++c) {
+            const int j0 = c * K;
+            const bool first = c == 0, last = c == p.nchunks - 1;
+            const int mstar = Q - 1 - j0;
+            int H[K], E[K];
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const int j = j0 + m;
+                H[m] = qs ? 0 : -go - ge * j;
+                E[m] = H[m] - go;
+            }
+            int diag_top = j0 == 0 ? 0 : (qs ? 0 : -go - ge * (j0 - 1));
+            for (int i = 0; i < Rw; ++i) {
+                const uint32_t code = i < R ? r[i] : 0u;
+                const int16_t *pa = sprof + (int)slut[code] * stride + j0;
+                int htop, f;
+                if (first) {
+                    htop = ts ? 0 : -go - ge * i;
+                    f = htop - go;
+                } else {
+                    const int2 bv = bnd[(size_t)i * 64 + lane];
+                    htop = bv.x;
+                    f = bv.y;
+                }
+                int diag = diag_top;
+#pragma unroll
+                for (int m = 0; m < K; ++m) {
+                    int h = max(max(diag + (int)pa[m], E[m]), f);
+                    diag = H[m];
+                    H[m] = h;
+                    const int hg = h - go;
+                    E[m] = max(E[m] - ge, hg);
+                    f = max(f - ge, hg);
+                }
+                diag_top = htop;
+                if (!last) {
+                    bnd[(size_t)i * 64 + lane] = make_int2(H[K - 1], f);
+                } else {
+                    int hl = H[0];
+#pragma unroll
+                    for (int m = 1; m < K; ++m)
+                        if (m == mstar) hl = H[m];
+                    if (TE && i < R && hl > te_v) {
+                        te_v = hl;
+                        te_i = i;
+                    }
+                    if (!TE && !QE && i == R - 1) corner = hl;
+                }
+                if (QE && i == R - 1) {
+#pragma unroll
+                    for (int m = 0; m < K; ++m) {
+                        if (last && m > mstar) continue;
+                        if (H[m] > qe_v) {
+                            qe_v = H[m];
+                            qe_j = j0 + m;
+                        }
+                    }
+                }
+            }
+        }
+        if (valid) {
+            int sc, er, eq;
+            finalize(R, Q, TE, QE, te_i >= 0, te_v, te_i, qe_v != INT_MIN, qe_v, qe_j, corner,
+                     p.neg_limit, sc, er, eq);
+            p.score[t] = sc;
+            p.end_ref[t] = er;
+            p.end_q[t] = eq;
+            p.status[t] = 32;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Group packing: sorted targets -> coalesced uint2 blocks. One thread per
+// (group, block, lane); a thread gathers 4 residues of each of its two
+// targets (zero past the target's end).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) pack_groups(const uint8_t *raw, const uint64_t *off,
+                                                   const int32_t *sorted_idx,
+                                                   const GroupDesc *groups, int32_t ngroups,
+                                                   int64_t n, uint2 *res) {
+    const int g = blockIdx.y;
+    if (g >= ngroups) return;
+    const GroupDesc gd = groups[g];
+    const int nblk = gd.ncols >> 2;
+    const int lane = threadIdx.x & 63;
+    for (int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); b < nblk;
+         b += gridDim.x * (blockDim.x >> 6)) {
+        uint32_t word[2];
+        for (int h = 0; h < 2; ++h) {
+            const int64_t si = (int64_t)gd.first + h * 64 + lane;
+            uint32_t wv = 0;
+            if (si < n) {
+                const int32_t t = sorted_idx[si];
+                const uint64_t o0 = off[t], o1 = off[t + 1];
+                const uint64_t c0 = o0 + (uint64_t)b * 4;
+                for (int k = 0; k < 4; ++k)
+                    if (c0 + k < o1) wv |= (uint32_t)raw[c0 + k] << (8 * k);
+            }
+            word[h] = wv;
+        }
+        res[gd.res_off + (size_t)b * 64 + lane] = make_uint2(word[0], word[1]);
+    }
+}
+
+__global__ void reset_counter(uint32_t *c) { *c = 0u; }
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename V, int K>
+static hipError_t launch_batched(const ScoreParams &p, int grid, int block, size_t lds,
+                                 hipStream_t st) {
+    const bool te = p.flags & 8u, qe = p.flags & 2u;
+    if (te && qe)
+        hipLaunchKernelGGL((sg_batched<V, K, true, true>), dim3(grid), dim3(block), lds, st, p);
+    else if (te)
+        hipLaunchKernelGGL((sg_batched<V, K, true, false>), dim3(grid), dim3(block), lds, st, p);
+    else if (qe)
+        hipLaunchKernelGGL((sg_batched<V, K, false, true>), dim3(grid), dim3(block), lds, st, p);
+    else
+        hipLaunchKernelGGL((sg_batched<V, K, false, false>), dim3(grid), dim3(block), lds, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sg_batched_i16(const ScoreParams &p, int grid, int block, size_t lds,
+                                 hipStream_t st) {
+    return launch_batched<s16x2, kChunk>(p, grid, block, lds, st);
+}
+hipError_t launch_sg_batched_i32(const ScoreParams &p, int grid, int block, size_t lds,
+                                 hipStream_t st) {
+    return launch_batched<int, kChunk>(p, grid, block, lds, st);
+}
+
+hipError_t launch_sg_plain(const PlainParams &p, int grid, int block, size_t lds,
+                           hipStream_t st) {
+    const bool te = p.flags & 8u, qe = p.flags & 2u;
+    constexpr int K = kChunkPlain;
+    if (te && qe)
+        hipLaunchKernelGGL((sg_plain<K, true, true>), dim3(grid), dim3(block), lds, st, p);
+    else if (te)
+        hipLaunchKernelGGL((sg_plain<K, true, false>), dim3(grid), dim3(block), lds, st, p);
+    else if (qe)
+        hipLaunchKernelGGL((sg_plain<K, false, true>), dim3(grid), dim3(block), lds, st, p);
+    else
+        hipLaunchKernelGGL((sg_plain<K, false, false>), dim3(grid), dim3(block), lds, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const uint8_t *raw, const uint64_t *off, const int32_t *sorted_idx,
+                       const GroupDesc *groups, int32_t ngroups, int64_t n, uint2 *res,
+                       hipStream_t st) {
+    // grid.y = groups (<= 65535 per launch slice), grid.x covers blocks
+    for (int32_t g0 = 0; g0 < ngroups; g0 += 65535) {
+        const int32_t ng = ngroups - g0 < 65535 ? ngroups - g0 : 65535;
+        hipLaunchKernelGGL(pack_groups, dim3(8, ng), dim3(256), 0, st, raw, off, sorted_idx,
+                           groups + g0, ng, n, res);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_reset(uint32_t *counter, hipStream_t st) {
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, counter);
+    return hipGetLastError();
+}
+
+template <typename KernelT> static int occ(KernelT k, int block, size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+int occupancy_sg_batched_i16(int block, size_t lds, uint32_t flags) {
+    const bool te = flags & 8u, qe = flags & 2u;
+    if (te && qe) return occ(sg_batched<s16x2, kChunk, true, true>, block, lds);
+    if (te) return occ(sg_batched<s16x2, kChunk, true, false>, block, lds);
+    if (qe) return occ(sg_batched<s16x2, kChunk, false, true>, block, lds);
+    return occ(sg_batched<s16x2, kChunk, false, false>, block, lds);
+}
+int occupancy_sg_batched_i32(int block, size_t lds, uint32_t flags) {
+    const bool te = flags & 8u, qe = flags & 2u;
+    if (te && qe) return occ(sg_batched<int, kChunk, true, true>, block, lds);
+    if (te) return occ(sg_batched<int, kChunk, true, false>, block, lds);
+    if (qe) return occ(sg_batched<int, kChunk, false, true>, block, lds);
+    return occ(sg_batched<int, kChunk, false, false>, block, lds);
+}
+int occupancy_sg_plain(int block, size_t lds, uint32_t flags) {
+    const bool te = flags & 8u, qe = flags & 2u;
+    if (te && qe) return occ(sg_plain<kChunkPlain, true, true>, block, lds);
+    if (te) return occ(sg_plain<kChunkPlain, true, false>, block, lds);
+    if (qe) return occ(sg_plain<kChunkPlain, false, true>, block, lds);
+    return occ(sg_plain<kChunkPlain, false, false>, block, lds);
+}
+
+} // namespace ish

@@ -1,0 +1,89 @@
+"""Deterministic synthetic workloads (libish_workload.so).
+
+Each workload is reproducible from (kind, seed, n): the same bytes come out
+on any machine and with any thread count. The parameters and the choices the
+task leaves open are documented in csrc/workload.cpp.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import LIB_DIR, IshError, Targets
+
+_wl = None
+
+KIND_QUERY_LEN = {1: 150, 2: 350, 4: 1000}
+KIND_MATRIX = {1: "actgn", 2: "blosum62", 3: "blosum62", 4: "actgn0"}
+
+
+def _lib():
+    global _wl
+    if _wl is None:
+        path = os.path.join(LIB_DIR, "libish_workload.so")
+        if not os.path.exists(path):
+            raise IshError(f"{path} is missing: run `python -m ish_amd.build` first")
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER
+        L.ish_wl_query.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, P(ctypes.c_uint8)]
+        L.ish_wl_query.restype = ctypes.c_int32
+        L.ish_wl_lengths.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_int64)]
+        L.ish_wl_lengths.restype = ctypes.c_int64
+        L.ish_wl_targets.argtypes = [ctypes.c_int32, ctypes.c_uint64, P(ctypes.c_uint8), ctypes.c_int32,
+                                     ctypes.c_int64, P(ctypes.c_int64), P(ctypes.c_uint64),
+                                     P(ctypes.c_uint8), P(ctypes.c_uint8), ctypes.c_int32]
+        L.ish_wl_targets.restype = ctypes.c_int32
+        L.ish_wl_hash.argtypes = [P(ctypes.c_uint8), ctypes.c_int64]
+        L.ish_wl_hash.restype = ctypes.c_uint64
+        _wl = L
+    return _wl
+
+
+@dataclass
+class Workload:
+    kind: int
+    seed: int
+    query: np.ndarray
+    targets: Targets
+    planted: np.ndarray
+    matrix: str
+
+    @property
+    def cells(self) -> int:
+        return int(len(self.query)) * int(self.targets.offsets[-1])
+
+    def content_hash(self) -> str:
+        L = _lib()
+        h1 = L.ish_wl_hash(self.targets.residues.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                           int(self.targets.offsets[-1]))
+        h2 = L.ish_wl_hash(self.query.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(self.query))
+        return f"{h2:016x}-{h1:016x}"
+
+
+def query(kind: int, seed: int, qlen: int | None = None) -> np.ndarray:
+    qlen = KIND_QUERY_LEN.get(kind, 350) if qlen is None else qlen
+    out = np.zeros(max(qlen, 1), np.uint8)
+    _lib().ish_wl_query(kind, seed, qlen, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out[:qlen]
+
+
+def generate(kind: int, n: int, seed: int = 1, qlen: int | None = None, threads: int = 0) -> Workload:
+    """kind 1: nucleotide lines, 2: protein FASTA, 4: long reads (see workload.cpp)."""
+    L = _lib()
+    q = query(kind, seed, qlen)
+    lens = np.zeros(max(n, 1), np.int64)
+    L.ish_wl_lengths(kind, seed, n, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    lens = lens[:n]
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    res = np.zeros(max(int(off[-1]), 1), np.uint8)
+    planted = np.zeros(max(n, 1), np.uint8)
+    L.ish_wl_targets(kind, seed, q.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(q), n,
+                     lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                     off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                     res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                     planted.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), threads)
+    return Workload(kind, seed, q, Targets(res, off), planted[:n].astype(bool), KIND_MATRIX[kind])

@@ -1,0 +1,171 @@
+"""Parity of the HIP engine with the CPU oracle, through the C ABI.
+
+Bit-exact on score, end_ref and end_query for every target; identical
+emitted record sets at --score 0.8. Both engines are checked: the batched
+persistent kernel (packed int16, int32 for wide groups) and the plain
+one-launch-per-batch int32 kernel.
+"""
+import json
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+import ish_amd
+from ish_amd import Targets, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VEC = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
+ENGINES = ["batched", "plain"]
+
+
+@pytest.fixture(scope="module")
+def eng():
+    if ish_amd.device_count() == 0:
+        pytest.fail("no gfx950 device visible to the HIP library")
+    e = ish_amd.Engine(0)
+    yield e
+    e.close()
+
+
+def oracle_batch(q, targets, kind, go, ge, flags):
+    m = O.Matrix(kind)
+    s, er, eq, _ = O.batch(0, q, targets.residues, targets.offsets, m, go, ge, flags)
+    return s, er, eq
+
+
+def check(eng, engine, q, targets, kind, go, ge, flags, label=""):
+    eng.set_engine(engine)
+    eng.set_query(q, kind, go, ge, flags, "striped-semi-global")
+    s, er, eq, st = eng.align(targets)
+    os_, oer, oeq = oracle_batch(q, targets, kind, go, ge, flags)
+    bad = np.nonzero((s != os_) | (er != oer) | (eq != oeq))[0]
+    if len(bad):
+        t = bad[0]
+        raise AssertionError(
+            f"{label} engine={engine} {len(bad)}/{targets.n} differ; first t={t} "
+            f"R={targets.offsets[t + 1] - targets.offsets[t]} gpu=({s[t]},{er[t]},{eq[t]}) "
+            f"oracle=({os_[t]},{oer[t]},{oeq[t]}) status={st[t]}")
+    return s, er, eq, st
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_golden_vectors(eng, engine):
+    eng.set_engine(engine)
+    for c in VEC["semi_global"]:
+        m = O.Matrix(c["matrix"])
+        q, t = m.encode(c["query"]), m.encode(c["target"])
+        eng.set_query(q, c["matrix"], c["go"], c["ge"], c["flags"])
+        s, er, eq, _ = eng.align(Targets.from_list([t]))
+        assert s[0] == c["score"], c
+        if "end_ref" in c:
+            assert er[0] == c["end_ref"], c
+        if "end_query" in c:
+            assert eq[0] == c["end_query"], c
+        assert (s[0], er[0], eq[0]) == O.sg(q, t, m, c["go"], c["ge"], O.flags_from_str(c["flags"]))
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("kind", ["actgn", "actgn0", "blosum62", "ascii"])
+def test_random_all_flags(eng, engine, kind):
+    rng = np.random.default_rng(zlib.crc32(f"{engine}-{kind}".encode()))
+    alpha = {"actgn": 5, "actgn0": 5, "blosum62": 24, "ascii": 256}[kind]
+    for flags in range(16):
+        Q = int(rng.integers(1, 140))
+        q = rng.integers(0, alpha if kind != "ascii" else 60, Q).astype(np.uint8) + (0 if kind != "ascii" else 40)
+        seqs = []
+        for t in range(300):
+            R = int(rng.integers(0, 260))
+            r = rng.integers(0, alpha, R).astype(np.uint8)
+            if kind == "ascii":
+                r = (rng.integers(0, 60, R) + 40).astype(np.uint8)
+            if R and t % 5 == 0:  # plant part of the query
+                k = min(Q, R)
+                off = int(rng.integers(0, R - k + 1))
+                r[off:off + k] = q[:k]
+            seqs.append(r)
+        go = int(rng.integers(1, 12))
+        ge = int(rng.integers(0, go))
+        check(eng, engine, q, Targets.from_list(seqs), kind, go, ge, flags, f"flags={flags}")
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_gap_open_not_above_extend(eng, engine):
+    """go <= ge: the engine computes textbook Gotoh (the oracle)."""
+    rng = np.random.default_rng(9)
+    q = rng.integers(0, 5, 77).astype(np.uint8)
+    seqs = [rng.integers(0, 5, int(rng.integers(1, 200))).astype(np.uint8) for _ in range(400)]
+    for go, ge in ((1, 1), (2, 3), (0, 2), (0, 0)):
+        for flags in (0, 5, 10, 15):
+            check(eng, engine, q, Targets.from_list(seqs), "actgn", go, ge, flags, f"go={go} ge={ge}")
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("n", [1, 63, 127, 128, 129, 1000])
+def test_batch_sizes_and_empty_targets(eng, engine, n):
+    rng = np.random.default_rng(n)
+    q = rng.integers(0, 24, 61).astype(np.uint8)
+    seqs = [rng.integers(0, 24, int(rng.integers(0, 9)) if t % 3 == 0 else int(rng.integers(0, 400))).astype(np.uint8)
+            for t in range(n)]
+    for flags in (0, 12, 3, 15):
+        check(eng, engine, q, Targets.from_list(seqs), "blosum62", 3, 1, flags)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_empty_query(eng, engine):
+    seqs = [np.array([1, 2, 3], np.uint8), np.zeros(0, np.uint8)]
+    eng.set_engine(engine)
+    eng.set_query(np.zeros(0, np.uint8), "actgn", 3, 1, "FFTT")
+    s, er, eq, _ = eng.align(Targets.from_list(seqs))
+    assert list(s) == [0, 0] and list(er) == [0, 0] and list(eq) == [0, 0]
+
+
+def test_int32_widening(eng):
+    """Groups whose proven range exceeds int16 run in int32 and stay exact."""
+    rng = np.random.default_rng(4)
+    q = rng.integers(0, 24, 300).astype(np.uint8)
+    lens = [20000, 12000, 9000, 6000, 3000, 800, 50] + [int(x) for x in rng.integers(1, 600, 300)]
+    seqs = [rng.integers(0, 24, L).astype(np.uint8) for L in lens]
+    for flags in (12, 0, 15, 2):
+        s, er, eq, st = check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 3, 4, flags)
+        assert (st[:3] & ish_amd.STATUS_WIDENED).all()
+        assert (st == ish_amd.STATUS_I16).sum() > 100
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_nucleotide_workload(eng, engine):
+    w = workloads.generate(1, 20000, seed=5)
+    s, er, eq, _ = check(eng, engine, w.query, w.targets, "actgn", 3, 1, O.flags_from_str("FFTT"), "wl1")
+    ms = int(O.Matrix("actgn").values.reshape(5, 5)[w.query, w.query].sum())
+    emitted = ish_amd.emit("striped-semi-global", s, er, eq, ms, 0.8)
+    os_, _, _ = oracle_batch(w.query, w.targets, "actgn", 3, 1, O.flags_from_str("FFTT"))
+    ref = np.array([O.sg_emit(v, ms, 0.8) for v in os_])
+    assert np.array_equal(emitted, ref)
+    assert emitted.sum() > 0
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_protein_workload(eng, engine):
+    w = workloads.generate(2, 6000, seed=6)
+    for flags in ("FFTT", "TTTT"):
+        check(eng, engine, w.query, w.targets, "blosum62", 3, 1, O.flags_from_str(flags), "wl2")
+
+
+def test_database_reuse_across_queries(eng):
+    w = workloads.generate(2, 5000, seed=8)
+    eng.set_engine("batched")
+    db = eng.load_db(w.targets)
+    m = O.Matrix("blosum62")
+    for qlen, flags in ((350, "FFTT"), (57, "TTTT"), (1000, "FFFF"), (1, "TFTF")):
+        q = workloads.query(2, qlen + 17, qlen)
+        eng.set_query(q, "blosum62", 3, 1, flags)
+        s, er, eq, st = eng.align_db(db)
+        os_, oer, oeq, _ = O.batch(0, q, w.targets.residues, w.targets.offsets, m, 3, 1,
+                                   O.flags_from_str(flags))
+        assert np.array_equal(s, os_) and np.array_equal(er, oer) and np.array_equal(eq, oeq), (qlen, flags)
+        assert (st == ish_amd.STATUS_I16).all()
+    db.close()
