@@ -119,6 +119,12 @@ int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *enc_query,
                           int32_t gap_extend, uint32_t ends_free,
                           int32_t algo);
 int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine);
+/* Tile height of the batched kernels: query rows each lane holds in
+ * registers (16 or 32). A tuning knob only; results never depend on it.
+ * No counterpart in the reference (its GPU kernel keeps whole rows in
+ * private memory, ishlib/matcher/alignment/semi_global_aln/basic.mojo:
+ * 305-306). */
+int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane);
 
 /* One synchronous batch (the batch_match_coarse + copy_outputs_to_host
  * step). Targets are residues[offsets[t] .. offsets[t+1]) for t < n.

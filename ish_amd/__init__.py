@@ -29,7 +29,7 @@ ABI_SYMBOLS = [
     "ish_hip_version", "ish_hip_device_count", "ish_hip_create", "ish_hip_destroy",
     "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
-    "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
+    "ish_hip_set_tile", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit",
 ]
@@ -70,6 +70,7 @@ def lib():
         "ish_hip_encode_and_score": ([i32, u8p, i64, u8p], i64),
         "ish_hip_set_query": ([vp, u8p, i32, i8p, i32, i32, i32, i32, u32, i32], i32),
         "ish_hip_set_engine": ([vp, i32], i32),
+        "ish_hip_set_tile": ([vp, i32], i32),
         "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
         "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
         "ish_hip_db_free": ([vp], None),
@@ -198,6 +199,9 @@ class Engine:
 
     def set_engine(self, engine: str):
         self._check(self._L.ish_hip_set_engine(self._h, ENGINE[engine]))
+
+    def set_tile(self, rows: int):
+        self._check(self._L.ish_hip_set_tile(self._h, rows))
 
     def set_query(self, query: np.ndarray, matrix_kind: str, gap_open: int = 3,
                   gap_extend: int = 1, ends_free="FFTT", algo="striped-semi-global"):

@@ -77,10 +77,10 @@ struct PlainParams {
 
 // Launchers (sg_kernels.hip). Return hipError_t of the launch.
 hipError_t launch_reset(uint32_t *counter, hipStream_t st);
-hipError_t launch_sg_batched_i16(const ScoreParams &p, int grid, int block,
-                                 size_t lds, hipStream_t st);
-hipError_t launch_sg_batched_i32(const ScoreParams &p, int grid, int block,
-                                 size_t lds, hipStream_t st);
+hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid,
+                                 int block, size_t lds, hipStream_t st);
+hipError_t launch_sg_batched_i32(const ScoreParams &p, int chunk, int grid,
+                                 int block, size_t lds, hipStream_t st);
 hipError_t launch_sg_plain(const PlainParams &p, int grid, int block,
                            size_t lds, hipStream_t st);
 hipError_t launch_pack(const uint8_t *raw, const uint64_t *off,
@@ -88,12 +88,15 @@ hipError_t launch_pack(const uint8_t *raw, const uint64_t *off,
                        int32_t ngroups, int64_t n, uint2 *res,
                        hipStream_t st);
 // Occupancy helpers: blocks per CU of each kernel at the given LDS size.
-int occupancy_sg_batched_i16(int block, size_t lds, uint32_t flags);
-int occupancy_sg_batched_i32(int block, size_t lds, uint32_t flags);
+int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags);
+int occupancy_sg_batched_i32(int chunk, int block, size_t lds, uint32_t flags);
 int occupancy_sg_plain(int block, size_t lds, uint32_t flags);
 
-// Rows of the query held in registers per lane (chunk height).
-constexpr int kChunk = 32;
+// Rows of the query held in registers per lane (chunk height). The batched
+// kernels are compiled for 16 and 32; the profile is padded to a multiple
+// of kChunkAlign so that either fits.
+constexpr int kChunkAlign = 32;
+constexpr int kChunkDefault = 32;
 constexpr int kChunkPlain = 32;
 
 } // namespace ish

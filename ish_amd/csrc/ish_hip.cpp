@@ -75,6 +75,7 @@ struct ish_hip_ctx {
     size_t lds = 0;
     DevBuf prof_t, prof_p, lut;
     int engine = ISH_ENGINE_BATCHED;
+    int chunk = kChunkDefault; // query rows per lane in the batched kernels
     // work buffers
     DevBuf scratch, counter, in_res, in_off, out_score, out_er, out_eq, out_st;
     // timing
@@ -239,6 +240,14 @@ extern "C" const char *ish_hip_last_error(const ish_hip_ctx *ctx) {
     return ctx ? ctx->err.c_str() : "null context";
 }
 
+extern "C" int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane) {
+    if (!ctx) return ISH_E_INVALID;
+    if (query_rows_per_lane != 16 && query_rows_per_lane != 32)
+        return fail(ctx, ISH_E_INVALID, "tile height must be 16 or 32 (got %d)", query_rows_per_lane);
+    ctx->chunk = query_rows_per_lane;
+    return ISH_OK;
+}
+
 extern "C" int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine) {
     if (!ctx) return ISH_E_INVALID;
     if (engine != ISH_ENGINE_BATCHED && engine != ISH_ENGINE_PLAIN)
@@ -320,8 +329,7 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
         }
         lut[c] = (uint8_t)it->second;
     }
-    const int32_t nch = (Q + kChunk - 1) / kChunk;
-    const int32_t qpad = nch * kChunk;
+    const int32_t qpad = (Q + kChunkAlign - 1) / kChunkAlign * kChunkAlign;
     const int32_t stride = qpad + 2; // odd dword count per row: rows hit distinct LDS banks
     const int32_t nrows = (int32_t)rows.size();
     const size_t prof_elems = (size_t)nrows * stride;
@@ -349,7 +357,7 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     ctx->ge = ge;
     ctx->kind = kind;
     ctx->flags = ends_free & 15u;
-    ctx->nchunks = nch;
+    ctx->nchunks = 0; // set per launch from the tile height
     ctx->stride = stride;
     ctx->rows = nrows;
     ctx->neg_limit = (int32_t)(int16_t)nl;
@@ -570,7 +578,7 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     p.prof_words = ctx->prof_words;
     p.stride = ctx->stride;
     p.Q = ctx->Q;
-    p.nchunks = ctx->nchunks;
+    p.nchunks = (ctx->Q + ctx->chunk - 1) / ctx->chunk;
     p.go = ctx->go;
     p.ge = ctx->ge;
     p.flags = ctx->flags;
@@ -588,7 +596,7 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     if (g16 > 0) {
         int grid;
         int64_t ss;
-        const int occ = occupancy_sg_batched_i32(kBlock, ctx->lds, ctx->flags);
+        const int occ = occupancy_sg_batched_i32(ctx->chunk, kBlock, ctx->lds, ctx->flags);
         int32_t rc = size_grid(ctx, occ, db->group_maxlen[0] + 3, &grid, &ss);
         if (rc) return rc;
         p.bnd = (uint2 *)ctx->scratch.p;
@@ -597,13 +605,13 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
         p.g_end = g16;
         HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
         HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, launch_sg_batched_i32(p, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, launch_sg_batched_i32(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
         HIP_TRY(ctx, timed_end(ctx));
     }
     if (g16 < db->ngroups) {
         int grid;
         int64_t ss;
-        const int occ = occupancy_sg_batched_i16(kBlock, ctx->lds, ctx->flags);
+        const int occ = occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, ctx->flags);
         int32_t rc = size_grid(ctx, occ, db->group_maxlen[g16] + 3, &grid, &ss);
         if (rc) return rc;
         p.bnd = (uint2 *)ctx->scratch.p;
@@ -612,7 +620,7 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
         p.g_end = db->ngroups;
         HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
         HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, launch_sg_batched_i16(p, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, launch_sg_batched_i16(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
         HIP_TRY(ctx, timed_end(ctx));
     }
     return ISH_OK;
@@ -672,7 +680,7 @@ static int32_t align_plain(ish_hip_ctx *ctx, const uint8_t *residues, const uint
     p.prof_words = ctx->prof_words;
     p.stride = ctx->stride;
     p.Q = ctx->Q;
-    p.nchunks = ctx->nchunks;
+    p.nchunks = (ctx->Q + kChunkPlain - 1) / kChunkPlain;
     p.go = ctx->go;
     p.ge = ctx->ge;
     p.flags = ctx->flags;

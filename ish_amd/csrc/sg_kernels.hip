@@ -103,8 +103,8 @@ struct TrackI32 {
 
 // Final end-coordinate rules shared by every kernel (values already
 // converted back from the offset form).
-__device__ __forceinline__ void finalize(int R, int Q, bool te, bool qe, bool has_te,
-                                         int te_val, int te_idx, bool has_qe, int qe_val,
+__device__ __forceinline__ void finalize(int R, int Q, bool te, bool qe, bool has_track,
+                                         int track_val, int track_idx, bool has_qe, int qe_val,
                                          int qe_j, int corner, int neg_limit, int &score,
                                          int &end_ref, int &end_q) {
     if (R == 0 || Q == 0) {
@@ -113,12 +113,14 @@ __device__ __forceinline__ void finalize(int R, int Q, bool te, bool qe, bool ha
         end_q = 0;
         return;
     }
+    // The last-row running maximum (seeded at the int16 floor) always runs;
+    // it gives end_ref, and the score when the target end is free.
     score = neg_limit;
     end_ref = -1;
     end_q = Q - 1;
-    if (te && has_te && te_val > neg_limit) {
-        score = te_val;
-        end_ref = te_idx;
+    if (has_track && track_val > neg_limit) {
+        end_ref = track_idx;
+        if (te) score = track_val;
     }
     if (qe && has_qe) {
         if (qe_val > score) {
@@ -258,7 +260,7 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
                         o.x = as_u32(H[K - 1]);
                         o.y = as_u32(f);
                         bnd[(size_t)i * 64 + lane] = o;
-                    } else if (TE || (!TE && !QE)) {
+                    } else {
                         // row Q-1 lives in register mstar of the last chunk
                         V hl = H[0];
 #pragma unroll
@@ -268,7 +270,7 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
                         const V u = hl - splat<V>(i * ge);
                         if constexpr (PACKED) {
                             const uint32_t uu = as_u32(u);
-                            if (TE) {
+                            if (TE || QE) {
                                 if (i < RA) t16.te_a = max(t16.te_a, key_lo(uu, i));
                                 if (i < RB) t16.te_b = max(t16.te_b, key_hi(uu, i));
                             } else {
@@ -277,7 +279,7 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
                             }
                         } else {
                             const int uv = (int)as_u32(u);
-                            if (TE) {
+                            if (TE || QE) {
                                 if (i < RA && uv > t32.te_v) {
                                     t32.te_v = uv;
                                     t32.te_i = i;
@@ -429,7 +431,7 @@ __global__ void __launch_bounds__(256) sg_plain(PlainParams p) {
 #pragma unroll
                     for (int m = 1; m < K; ++m)
                         if (m == mstar) hl = H[m];
-                    if (TE && i < R && hl > te_v) {
+                    if ((TE || QE) && i < R && hl > te_v) {
                         te_v = hl;
                         te_i = i;
                     }
@@ -512,13 +514,15 @@ static hipError_t launch_batched(const ScoreParams &p, int grid, int block, size
     return hipGetLastError();
 }
 
-hipError_t launch_sg_batched_i16(const ScoreParams &p, int grid, int block, size_t lds,
-                                 hipStream_t st) {
-    return launch_batched<s16x2, kChunk>(p, grid, block, lds, st);
+hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid, int block,
+                                 size_t lds, hipStream_t st) {
+    if (chunk == 16) return launch_batched<s16x2, 16>(p, grid, block, lds, st);
+    return launch_batched<s16x2, 32>(p, grid, block, lds, st);
 }
-hipError_t launch_sg_batched_i32(const ScoreParams &p, int grid, int block, size_t lds,
-                                 hipStream_t st) {
-    return launch_batched<int, kChunk>(p, grid, block, lds, st);
+hipError_t launch_sg_batched_i32(const ScoreParams &p, int chunk, int grid, int block,
+                                 size_t lds, hipStream_t st) {
+    if (chunk == 16) return launch_batched<int, 16>(p, grid, block, lds, st);
+    return launch_batched<int, 32>(p, grid, block, lds, st);
 }
 
 hipError_t launch_sg_plain(const PlainParams &p, int grid, int block, size_t lds,
@@ -561,19 +565,20 @@ template <typename KernelT> static int occ(KernelT k, int block, size_t lds) {
     return nb > 0 ? nb : 1;
 }
 
-int occupancy_sg_batched_i16(int block, size_t lds, uint32_t flags) {
+template <typename V, int K> static int occ_batched(int block, size_t lds, uint32_t flags) {
     const bool te = flags & 8u, qe = flags & 2u;
-    if (te && qe) return occ(sg_batched<s16x2, kChunk, true, true>, block, lds);
-    if (te) return occ(sg_batched<s16x2, kChunk, true, false>, block, lds);
-    if (qe) return occ(sg_batched<s16x2, kChunk, false, true>, block, lds);
-    return occ(sg_batched<s16x2, kChunk, false, false>, block, lds);
+    if (te && qe) return occ(sg_batched<V, K, true, true>, block, lds);
+    if (te) return occ(sg_batched<V, K, true, false>, block, lds);
+    if (qe) return occ(sg_batched<V, K, false, true>, block, lds);
+    return occ(sg_batched<V, K, false, false>, block, lds);
 }
-int occupancy_sg_batched_i32(int block, size_t lds, uint32_t flags) {
-    const bool te = flags & 8u, qe = flags & 2u;
-    if (te && qe) return occ(sg_batched<int, kChunk, true, true>, block, lds);
-    if (te) return occ(sg_batched<int, kChunk, true, false>, block, lds);
-    if (qe) return occ(sg_batched<int, kChunk, false, true>, block, lds);
-    return occ(sg_batched<int, kChunk, false, false>, block, lds);
+int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags) {
+    return chunk == 16 ? occ_batched<s16x2, 16>(block, lds, flags)
+                       : occ_batched<s16x2, 32>(block, lds, flags);
+}
+int occupancy_sg_batched_i32(int chunk, int block, size_t lds, uint32_t flags) {
+    return chunk == 16 ? occ_batched<int, 16>(block, lds, flags)
+                       : occ_batched<int, 32>(block, lds, flags);
 }
 int occupancy_sg_plain(int block, size_t lds, uint32_t flags) {
     const bool te = flags & 8u, qe = flags & 2u;

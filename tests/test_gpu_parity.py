@@ -38,8 +38,9 @@ def oracle_batch(q, targets, kind, go, ge, flags):
     return s, er, eq
 
 
-def check(eng, engine, q, targets, kind, go, ge, flags, label=""):
+def check(eng, engine, q, targets, kind, go, ge, flags, label="", tile=32):
     eng.set_engine(engine)
+    eng.set_tile(tile)
     eng.set_query(q, kind, go, ge, flags, "striped-semi-global")
     s, er, eq, st = eng.align(targets)
     os_, oer, oeq = oracle_batch(q, targets, kind, go, ge, flags)
@@ -69,9 +70,9 @@ def test_golden_vectors(eng, engine):
         assert (s[0], er[0], eq[0]) == O.sg(q, t, m, c["go"], c["ge"], O.flags_from_str(c["flags"]))
 
 
-@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("engine,tile", [("batched", 32), ("batched", 16), ("plain", 32)])
 @pytest.mark.parametrize("kind", ["actgn", "actgn0", "blosum62", "ascii"])
-def test_random_all_flags(eng, engine, kind):
+def test_random_all_flags(eng, engine, tile, kind):
     rng = np.random.default_rng(zlib.crc32(f"{engine}-{kind}".encode()))
     alpha = {"actgn": 5, "actgn0": 5, "blosum62": 24, "ascii": 256}[kind]
     for flags in range(16):
@@ -90,7 +91,7 @@ def test_random_all_flags(eng, engine, kind):
             seqs.append(r)
         go = int(rng.integers(1, 12))
         ge = int(rng.integers(0, go))
-        check(eng, engine, q, Targets.from_list(seqs), kind, go, ge, flags, f"flags={flags}")
+        check(eng, engine, q, Targets.from_list(seqs), kind, go, ge, flags, f"flags={flags}", tile)
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -131,9 +132,10 @@ def test_int32_widening(eng):
     lens = [20000, 12000, 9000, 6000, 3000, 800, 50] + [int(x) for x in rng.integers(1, 600, 300)]
     seqs = [rng.integers(0, 24, L).astype(np.uint8) for L in lens]
     for flags in (12, 0, 15, 2):
-        s, er, eq, st = check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 3, 4, flags)
-        assert (st[:3] & ish_amd.STATUS_WIDENED).all()
-        assert (st == ish_amd.STATUS_I16).sum() > 100
+        for tile in (16, 32):
+            s, er, eq, st = check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 3, 4, flags, "", tile)
+            assert (st[:3] & ish_amd.STATUS_WIDENED).all()
+            assert (st == ish_amd.STATUS_I16).sum() > 100
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -148,16 +150,17 @@ def test_nucleotide_workload(eng, engine):
     assert emitted.sum() > 0
 
 
-@pytest.mark.parametrize("engine", ENGINES)
-def test_protein_workload(eng, engine):
+@pytest.mark.parametrize("engine,tile", [("batched", 32), ("batched", 16), ("plain", 32)])
+def test_protein_workload(eng, engine, tile):
     w = workloads.generate(2, 6000, seed=6)
     for flags in ("FFTT", "TTTT"):
-        check(eng, engine, w.query, w.targets, "blosum62", 3, 1, O.flags_from_str(flags), "wl2")
+        check(eng, engine, w.query, w.targets, "blosum62", 3, 1, O.flags_from_str(flags), "wl2", tile)
 
 
 def test_database_reuse_across_queries(eng):
     w = workloads.generate(2, 5000, seed=8)
     eng.set_engine("batched")
+    eng.set_tile(32)
     db = eng.load_db(w.targets)
     m = O.Matrix("blosum62")
     for qlen, flags in ((350, "FFTT"), (57, "TTTT"), (1000, "FFFF"), (1, "TFTF")):
