@@ -27,11 +27,11 @@ struct GroupDesc {
 struct ScoreParams {
     // query profile (staged into LDS): nrows rows of `stride` int16 scores,
     // row r, query position j at prof[r * stride + j]; lut maps a residue
-    // code to its profile row.
+    // code to the dword offset of its profile row (r * stride / 2).
     const int16_t *prof;
-    const uint8_t *lut;
+    const uint16_t *lut;
     int32_t prof_words; // int32 words to stage for the profile
-    int32_t stride;     // int16 elements per profile row (odd dword count)
+    int32_t stride;     // int16 elements per profile row (16-byte multiple + 16)
     int32_t Q;          // query length
     int32_t nchunks;    // ceil(Q / K)
     int32_t go, ge;     // gap open / extend (non-negative)
@@ -62,7 +62,7 @@ struct ScoreParams {
 // contiguous residues with offsets.
 struct PlainParams {
     const int16_t *prof;
-    const uint8_t *lut;
+    const uint16_t *lut;
     int32_t prof_words, stride, Q, nchunks, go, ge;
     uint32_t flags;
     int32_t neg_limit;

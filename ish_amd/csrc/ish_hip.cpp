@@ -317,7 +317,7 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     // to the distinct query bytes plus one "absent" row).
     std::map<std::vector<int8_t>, int> row_of;
     std::vector<std::vector<int8_t>> rows;
-    uint8_t lut[256];
+    uint16_t lut[256];
     for (int c = 0; c < 256; ++c) {
         const int cc = c < A ? c : A - 1; // out-of-alphabet codes score as the last code
         std::vector<int8_t> col((size_t)Q);
@@ -327,13 +327,16 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
             it = row_of.emplace(col, (int)rows.size()).first;
             rows.push_back(col);
         }
-        lut[c] = (uint8_t)it->second;
+        lut[c] = (uint16_t)it->second; // row index; scaled to an offset below
     }
     const int32_t qpad = (Q + kChunkAlign - 1) / kChunkAlign * kChunkAlign;
-    const int32_t stride = qpad + 2; // odd dword count per row: rows hit distinct LDS banks
+    // rows start 16 bytes apart modulo 128: 16-byte slices of different rows
+    // spread over the LDS banks and every slice read is 16-byte aligned
+    const int32_t stride = qpad + 8;
     const int32_t nrows = (int32_t)rows.size();
     const size_t prof_elems = (size_t)nrows * stride;
-    const size_t lds = prof_elems * 2 + 256;
+    const size_t lds = prof_elems * 2 + 512;
+    for (int c = 0; c < 256; ++c) lut[c] = (uint16_t)(lut[c] * (stride / 2));
     if (lds > kLdsLimit)
         return fail(ctx, ISH_E_RANGE, "query profile needs %zu bytes of LDS (limit %zu)", lds,
                     kLdsLimit);
@@ -345,10 +348,10 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
         }
     HIP_TRY(ctx, ctx->prof_t.ensure(prof_elems * 2));
     HIP_TRY(ctx, ctx->prof_p.ensure(prof_elems * 2));
-    HIP_TRY(ctx, ctx->lut.ensure(256));
+    HIP_TRY(ctx, ctx->lut.ensure(512));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_t.p, pt.data(), prof_elems * 2, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_p.p, pp.data(), prof_elems * 2, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->lut.p, lut, 256, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->lut.p, lut, 512, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
     ctx->Q = Q;
@@ -574,7 +577,7 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     }
     ScoreParams p{};
     p.prof = (const int16_t *)ctx->prof_t.p;
-    p.lut = (const uint8_t *)ctx->lut.p;
+    p.lut = (const uint16_t *)ctx->lut.p;
     p.prof_words = ctx->prof_words;
     p.stride = ctx->stride;
     p.Q = ctx->Q;
@@ -676,7 +679,7 @@ static int32_t align_plain(ish_hip_ctx *ctx, const uint8_t *residues, const uint
     HIP_TRY(ctx, hipMemcpyAsync(ctx->in_off.p, rebased.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream));
     PlainParams p{};
     p.prof = (const int16_t *)ctx->prof_p.p;
-    p.lut = (const uint8_t *)ctx->lut.p;
+    p.lut = (const uint16_t *)ctx->lut.p;
     p.prof_words = ctx->prof_words;
     p.stride = ctx->stride;
     p.Q = ctx->Q;

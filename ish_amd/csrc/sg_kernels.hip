@@ -68,17 +68,21 @@ __device__ __forceinline__ uint32_t wave_broadcast0(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Stage the query profile and the residue->row table into LDS.
-__device__ __forceinline__ void stage_profile(const int16_t *prof, const uint8_t *lut,
+// Stage the query profile and the residue -> profile-row table into LDS.
+// The table holds each row's offset in dwords (256 x uint16), so a residue
+// code turns into an LDS address with one shift-add.
+__device__ __forceinline__ void stage_profile(const int16_t *prof, const uint16_t *lut,
                                               int prof_words, uint8_t *smem) {
     const int *src = (const int *)prof;
     int *dst = (int *)smem;
     for (int x = threadIdx.x; x < prof_words; x += blockDim.x) dst[x] = src[x];
     const int *lsrc = (const int *)lut;
     int *ldst = (int *)(smem + (size_t)prof_words * 4);
-    for (int x = threadIdx.x; x < 64; x += blockDim.x) ldst[x] = lsrc[x];
+    for (int x = threadIdx.x; x < 128; x += blockDim.x) ldst[x] = lsrc[x];
     __syncthreads();
 }
+
+typedef short short8 __attribute__((ext_vector_type(8)));
 
 // Running (value, first index) maximum as one ordered key per target:
 // value in the high 16 bits (signed), 0xFFFF - index in the low 16 bits, so
@@ -147,16 +151,15 @@ template <typename V, int K, bool TE, bool QE>
 __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     stage_profile(p.prof, p.lut, p.prof_words, smem);
-    const int16_t *sprof = (const int16_t *)smem;
-    const uint8_t *slut = smem + (size_t)p.prof_words * 4;
+    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
 
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     uint2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
     const int go = p.go, ge = p.ge, cst = ge - go, Q = p.Q;
     const bool qs = p.flags & 1u, ts = p.flags & 4u;
-    const int stride = p.stride;
     const V vcst = splat<V>(cst);
     const int items = PACKED ? (p.g_end - p.g_begin) : 2 * (p.g_end - p.g_begin);
 
@@ -190,6 +193,11 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
             const bool first = c == 0;
             const bool last = c == p.nchunks - 1;
             const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
+            // LDS address of this chunk's rows in profile row `code`
+            const uint8_t *chunk_base = smem + 2 * j0;
+            auto row_ptr = [&](uint32_t code) -> const short8 * {
+                return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
+            };
             V H[K], E[K];
 #pragma unroll
             for (int m = 0; m < K; ++m) {
@@ -206,6 +214,17 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
             }
+            // profile slices of column 0
+            short8 sa[NS], sb[NS];
+            {
+                const short8 *na = row_ptr((half ? rr_next.y : rr_next.x) & 0xffu);
+                const short8 *nb = row_ptr(rr_next.y & 0xffu);
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    sa[k] = na[k];
+                    if (PACKED) sb[k] = nb[k];
+                }
+            }
             for (int b = 0; b < nblk; ++b) {
                 const uint2 rr = rr_next;
                 uint2 bn[4];
@@ -221,13 +240,12 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int i = b * 4 + t;
-                    uint32_t codeA = ((half ? rr.y : rr.x) >> (8 * t)) & 0xffu;
-                    const int16_t *pa = sprof + (int)slut[codeA] * stride + j0;
-                    const int16_t *pb = pa;
-                    if (PACKED) {
-                        uint32_t codeB = (rr.y >> (8 * t)) & 0xffu;
-                        pb = sprof + (int)slut[codeB] * stride + j0;
-                    }
+                    // residue codes of the next column (past the end: any code)
+                    const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
+                    const uint32_t wb = t < 3 ? rr.y : rr_next.y;
+                    const int sh = 8 * ((t + 1) & 3);
+                    const short8 *na = row_ptr((wa >> sh) & 0xffu);
+                    const short8 *nb = row_ptr((wb >> sh) & 0xffu);
                     V htop, f;
                     if (first) {
                         const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
@@ -239,20 +257,31 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
                     }
                     V diag = diag_top;
 #pragma unroll
-                    for (int m = 0; m < K; ++m) {
-                        V s;
-                        if constexpr (PACKED) {
-                            s.x = pa[m];
-                            s.y = pb[m];
-                        } else {
-                            s = (int)pa[m];
+                    for (int k = 0; k < NS; ++k) {
+#pragma unroll
+                        for (int mm = 0; mm < 8; ++mm) {
+                            const int m = k * 8 + mm;
+                            V s;
+                            if constexpr (PACKED) {
+                                s.x = sa[k][mm];
+                                s.y = sb[k][mm];
+                            } else {
+                                s = (int)sa[k][mm];
+                            }
+                            V h = vmax(vmax(diag + s, E[m]), f);
+                            diag = H[m];
+                            H[m] = h;
+                            const V gg = h + vcst;
+                            E[m] = vmax(E[m], gg);
+                            f = vmax(f, gg);
                         }
-                        V h = vmax(vmax(diag + s, E[m]), f);
-                        diag = H[m];
-                        H[m] = h;
-                        const V gg = h + vcst;
-                        E[m] = vmax(E[m], gg);
-                        f = vmax(f, gg);
+                        // slice k is consumed: fetch it for the next column. The
+                        // scheduling barrier keeps the fetch from being hoisted
+                        // above the rows that still read the old slice, so only
+                        // one extra slice is live at a time.
+                        __builtin_amdgcn_sched_barrier(0);
+                        sa[k] = na[k];
+                        if (PACKED) sb[k] = nb[k];
                     }
                     diag_top = htop;
                     if (!last) {
@@ -365,12 +394,12 @@ __global__ void __launch_bounds__(256) sg_plain(PlainParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     stage_profile(p.prof, p.lut, p.prof_words, smem);
     const int16_t *sprof = (const int16_t *)smem;
-    const uint8_t *slut = smem + (size_t)p.prof_words * 4;
+    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
 
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     int2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
-    const int go = p.go, ge = p.ge, Q = p.Q, stride = p.stride;
+    const int go = p.go, ge = p.ge, Q = p.Q;
     const bool qs = p.flags & 1u, ts = p.flags & 4u;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
     const int64_t base_stride = nthreads - (threadIdx.x & 63);
@@ -403,7 +432,7 @@ __global__ void __launch_bounds__(256) sg_plain(PlainParams p) {
             int diag_top = j0 == 0 ? 0 : (qs ? 0 : -go - ge * (j0 - 1));
             for (int i = 0; i < Rw; ++i) {
                 const uint32_t code = i < R ? r[i] : 0u;
-                const int16_t *pa = sprof + (int)slut[code] * stride + j0;
+                const int16_t *pa = sprof + 2 * (int)slut[code] + j0;
                 int htop, f;
                 if (first) {
                     htop = ts ? 0 : -go - ge * i;
