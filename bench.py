@@ -31,10 +31,106 @@ METRIC = "GCUPS on 1 MI355X (bit-exact scores), batched vs plain; p50/p99 query 
 PEAK_INT16_TOPS = 78.6  # 256 CU x 2.4 GHz x 128 packed int16 lane-ops/clk
 PEAK_INT32_TOPS = 39.3  # 256 CU x 2.4 GHz x 64 int32 lane-ops/clk
 OPS_PER_CELL_SG = 8     # semi-global Gotoh: 1 add + 2 max (H), 2 sub + 1 max (E), 1 sub + 1 max (F)
+OPS_PER_CELL_SW = 9     # local: the same plus the max with 0
+
+# name -> (generator kind, default targets, matrix, algo, ends-free, description)
+WORKLOADS = {
+    "protein": (2, 2_000_000, "blosum62", "striped-semi-global", "FFTT",
+                "protein semi-global, blosum62, go 3 ge 1, ends-free FFTT, 1 x 350 aa query vs "
+                "2M log-normal targets (median 350, sigma 0.6, [30,5000])"),
+    "nucleotide": (1, 200_000, "actgn", "striped-semi-global", "FFTT",
+                   "nucleotide semi-global, actgn, go 3 ge 1, ends-free FFTT, 1 x 150 nt query vs "
+                   "200k line records U[100,300] with 0.5% N and 1% planted copies"),
+    "multiquery": (3, 4_000_000, "blosum62", "striped-semi-global", "FFTT",
+                   "batched multi-query semi-global, blosum62, go 3 ge 1, ends-free FFTT, 64 queries "
+                   "U[50,1000] at background frequencies vs 4M line records U[100,300] (ACGT+N codes "
+                   "scored with blosum62), one persistent launch per cell width"),
+    "longread": (4, 500_000, "actgn0", "striped-local", "FFFF",
+                 "local Smith-Waterman, actgn0, go 3 ge 1, 1 x 1000 nt query vs 500k reads "
+                 "(log-normal median 8 kb, sigma 0.5, [1 kb, 50 kb], 8% errors, 2% N)"),
+}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def run_multiquery(args, ish_amd, workloads):
+    """64 queries, lengths U[50,1000], against 4M records in one persistent
+    launch per cell width (config 3)."""
+    kind, n_default, mkind, algo, flags, desc = WORKLOADS["multiquery"]
+    n_targets = args.targets or n_default
+    t0 = time.perf_counter()
+    w = workloads.generate(kind, n_targets, seed=args.seed)
+    rng = np.random.default_rng(args.seed)
+    qlens = rng.integers(50, 1001, 64)
+    queries = [workloads.query(3, args.seed * 1000 + k, int(L)) for k, L in enumerate(qlens)]
+    nres = int(w.targets.offsets[-1])
+    cells = int(qlens.sum()) * nres
+    log(f"workload: {w.targets.n} targets, {nres} residues, 64 queries ({int(qlens.sum())} residues), "
+        f"{cells:.3e} cells, {time.perf_counter() - t0:.1f}s")
+    eng = ish_amd.Engine(0)
+    eng.set_tile(args.tile or 32)
+    t0 = time.perf_counter()
+    db = eng.load_db(w.targets)
+    load_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        eng.align_db_multi(db, queries, mkind, 3, 1, flags)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s, er, eq, st = eng.align_db_multi(db, queries, mkind, 3, 1, flags)
+        kernel_ms.append(eng.last_timing()[0])
+    wall = time.perf_counter() - t0
+    step_s = wall / args.steps
+    kern_s = float(np.mean(kernel_ms)) / 1e3
+    log(f"multi: {step_s * 1e3:.1f} ms/step (kernels {kern_s * 1e3:.1f} ms), "
+        f"{cells / kern_s / 1e9:.1f} GCUPS kernel, widened {int((st & ish_amd.STATUS_WIDENED != 0).sum())}")
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        m = O.Matrix(mkind)
+        sel_q = [0, 21, 42, 63]
+        n_s = int(min(w.targets.n, max(1, args.cpu_cells / (sum(int(qlens[k]) for k in sel_q) * nres / w.targets.n))))
+        idx = np.linspace(0, w.targets.n - 1, n_s).astype(np.int64)
+        lens = (w.targets.offsets[idx + 1] - w.targets.offsets[idx]).astype(np.int64)
+        off = np.zeros(n_s + 1, np.uint64)
+        np.cumsum(lens, out=off[1:])
+        res = np.concatenate([w.targets.residues[int(w.targets.offsets[i]):int(w.targets.offsets[i + 1])] for i in idx])
+        parity = True
+        cpu_s = 0.0
+        used = 0
+        for k in sel_q:
+            t0 = time.perf_counter()
+            os_, oer, oeq, used = O.batch(0, queries[k], res, off, m, 3, 1, O.flags_from_str(flags),
+                                         nthreads=min(16, os.cpu_count() or 1))
+            cpu_s += time.perf_counter() - t0
+            parity = parity and bool(np.array_equal(os_, s[k][idx]) and np.array_equal(oer, er[k][idx])
+                                     and np.array_equal(oeq, eq[k][idx]))
+        sample_cells = sum(int(qlens[k]) for k in sel_q) * int(off[-1])
+        cpu = {"value": round(sample_cells / cpu_s / 1e9, 3), "unit": "GCUPS", "cores": int(used), "kind": "port",
+               "sample": f"queries {sel_q} x {n_s} of the {w.targets.n} targets, {sample_cells / 1e9:.2f} Gcells, "
+                         f"scalar int32 oracle (oracle/oracle.c)"}
+        log(f"cpu oracle: {cpu['value']} GCUPS on {used} threads; parity on sample: {parity}")
+    achieved = cells * OPS_PER_CELL_SG / kern_s / 1e12
+    out = {
+        "metric": METRIC, "value": round(cells / step_s / 1e9, 2), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16",
+        "data": "synthetic",
+        "config": {"workload": desc, "name": "multiquery", "targets": w.targets.n, "residues": nres,
+                   "queries": 64, "query_residues": int(qlens.sum()), "cells": cells, "seed": args.seed,
+                   "content_hash": w.content_hash(), "tile_rows": args.tile or 32},
+        "gcups_kernel": round(cells / kern_s / 1e9, 2), "db_load_ms": round(load_s * 1e3, 1),
+        "parity_sample": parity,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
+                     "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": None,
+                     "ops_per_cell": OPS_PER_CELL_SG, "kernel_ms": round(kern_s * 1e3, 3)},
+        "cpu_baseline": cpu,
+    }
+    db.close()
+    eng.close()
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -42,10 +138,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--targets", type=int, default=2_000_000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="protein")
+    ap.add_argument("--targets", type=int, default=0, help="0 = the workload's default size")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=100_000,
-                    help="targets in the CPU-baseline / parity sample")
+    ap.add_argument("--cpu-cells", type=float, default=1.5e10,
+                    help="cells in the CPU-baseline / parity sample (every k-th target)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0,
@@ -60,17 +157,23 @@ def main():
     import ish_amd
     from ish_amd import workloads
 
+    if args.workload == "multiquery":
+        return run_multiquery(args, ish_amd, workloads)
+
+    kind, n_default, mkind, algo, flags, desc = WORKLOADS[args.workload]
+    n_targets = args.targets or n_default
+    local = algo == "striped-local"
+    ops_per_cell = OPS_PER_CELL_SW if local else OPS_PER_CELL_SG
     t0 = time.perf_counter()
-    w = workloads.generate(2, args.targets, seed=args.seed)
+    w = workloads.generate(kind, n_targets, seed=args.seed)
     log(f"workload: {w.targets.n} targets, {int(w.targets.offsets[-1])} residues, "
         f"hash {w.content_hash()}, {time.perf_counter() - t0:.1f}s")
     Q = len(w.query)
     cells = w.cells
-    flags = "FFTT"
 
     eng = ish_amd.Engine(0)
     eng.set_engine("batched")
-    eng.set_query(w.query, "blosum62", 3, 1, flags)
+    eng.set_query(w.query, mkind, 3, 1, flags, algo)
     t0 = time.perf_counter()
     db = eng.load_db(w.targets)
     load_s = time.perf_counter() - t0
@@ -119,9 +222,9 @@ def main():
     db2.close()
 
     plain = None
-    if not args.no_plain:
+    if not args.no_plain and not local:
         eng.set_engine("plain")
-        eng.set_query(w.query, "blosum62", 3, 1, flags)
+        eng.set_query(w.query, mkind, 3, 1, flags, algo)
         t0 = time.perf_counter()
         ps, per, peq, _ = eng.align(w.targets)
         p_e2e = time.perf_counter() - t0
@@ -138,17 +241,18 @@ def main():
     parity = None
     if not args.no_cpu:
         from oracle import oracle as O
-        n_s = min(args.cpu_sample, w.targets.n)
+        mean_len = float(w.targets.offsets[-1]) / max(w.targets.n, 1)
+        n_s = int(min(w.targets.n, max(1, args.cpu_cells / (Q * mean_len))))
         idx = np.linspace(0, w.targets.n - 1, n_s).astype(np.int64)
         lens = (w.targets.offsets[idx + 1] - w.targets.offsets[idx]).astype(np.int64)
         off = np.zeros(n_s + 1, np.uint64)
         np.cumsum(lens, out=off[1:])
         res = np.concatenate([w.targets.residues[int(w.targets.offsets[i]):int(w.targets.offsets[i + 1])]
                               for i in idx])
-        m = O.Matrix("blosum62")
+        m = O.Matrix(mkind)
         t0 = time.perf_counter()
-        os_, oer, oeq, used = O.batch(0, w.query, res, off, m, 3, 1, O.flags_from_str(flags),
-                                     nthreads=min(16, os.cpu_count() or 1))
+        os_, oer, oeq, used = O.batch(1 if local else 0, w.query, res, off, m, 3, 1,
+                                     O.flags_from_str(flags), nthreads=min(16, os.cpu_count() or 1))
         cpu_s = time.perf_counter() - t0
         sample_cells = Q * int(off[-1])
         cpu = {"value": round(sample_cells / cpu_s / 1e9, 3), "unit": "GCUPS", "cores": int(used),
@@ -157,13 +261,14 @@ def main():
                          f"{sample_cells / 1e9:.2f} Gcells, scalar int32 oracle (oracle/oracle.c), "
                          f"{os.cpu_count()} host threads"}
         parity = bool(np.array_equal(os_, s[idx]) and np.array_equal(oer, er[idx]) and np.array_equal(oeq, eq[idx]))
-        ms_ = int(m.values.reshape(24, 24)[w.query, w.query].sum())
-        emit_gpu = ish_amd.emit("striped-semi-global", s[idx], er[idx], eq[idx], ms_, 0.8)
-        emit_ref = np.array([O.sg_emit(v, ms_, 0.8) for v in os_])
+        ms_ = int(m.values.reshape(m.A, m.A)[w.query, w.query].sum())
+        emit_gpu = ish_amd.emit(algo, s[idx], er[idx], eq[idx], ms_, 0.8)
+        emit_ref = np.array([O.sw_emit(v, a, b, ms_, 0.8) if local else O.sg_emit(v, ms_, 0.8)
+                             for v, a, b in zip(os_, oer, oeq)])
         parity = parity and bool(np.array_equal(emit_gpu, emit_ref))
         log(f"cpu oracle: {cpu['value']} GCUPS on {used} threads; parity on sample: {parity}")
 
-    achieved = cells * OPS_PER_CELL_SG / kern_s / 1e12
+    achieved = cells * ops_per_cell / kern_s / 1e12
     out = {
         "metric": METRIC,
         "value": round(gcups, 2),
@@ -177,8 +282,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int16",
         "data": "synthetic",
-        "config": {"workload": "protein semi-global, blosum62, go 3 ge 1, ends-free FFTT, "
-                               "1 x 350 aa query vs 2M log-normal targets (median 350, sigma 0.6, [30,5000])",
+        "config": {"workload": desc, "name": args.workload,
                    "targets": w.targets.n, "residues": int(w.targets.offsets[-1]), "query_len": Q,
                    "cells": cells, "seed": args.seed, "content_hash": w.content_hash(),
                    "engine": "batched persistent, packed int16 (int32 for wide groups)",
@@ -192,7 +296,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
                      "traffic": None,
-                     "ops_per_cell": OPS_PER_CELL_SG, "kernel_ms": round(kern_s * 1e3, 3)},
+                     "ops_per_cell": ops_per_cell, "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }
     db.close()

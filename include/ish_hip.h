@@ -157,6 +157,40 @@ int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
                          int32_t *end_ref, int32_t *end_query,
                          uint8_t *status);
 
+/* Many queries against one database (semi-global): every target of db is
+ * scored against each of nq queries, query k being
+ * queries[qoffsets[k] .. qoffsets[k+1]), with one shared matrix, gap
+ * penalties and free ends. One persistent launch per cell width drains a
+ * work queue of (query, group) tiles. Outputs are [nq][n_targets]
+ * row-major. This is the batched multi-query mode of the engine; the
+ * reference scores one pattern per run (do_search.mojo:24-274). */
+int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
+                               const uint8_t *queries, const uint64_t *qoffsets,
+                               int32_t nq, const int8_t *matrix, int32_t alphabet,
+                               int32_t matrix_kind, int32_t gap_open,
+                               int32_t gap_extend, uint32_t ends_free,
+                               int32_t *score, int32_t *end_ref,
+                               int32_t *end_query, uint8_t *status);
+
+/* Start coordinates of selected targets of db, for the context's query:
+ * the reverse pass of semi_global_aln_start_end (semi_global_aln/
+ * striped.mojo:160-191; also StripedSemiGlobalMatcher.find_start,
+ * striped_semi_global_matcher.mojo:109-126, called by the reference's GPU
+ * path from cpu/process_batch.mojo:56-61) and of ssw_align (local_aln/
+ * striped.mojo:232-262). For k < n_sel, target sel[k] with forward end
+ * coordinates (end_ref[k], end_query[k]) gets:
+ *   semi-global: target_start = end_ref - rev.end_ref,
+ *                query_start  = qlen - rev.end_query - 1;
+ *   local:       target_start (ref_begin1) = end_ref - rev.end_ref,
+ *                query_start (read_begin1) = end_query - rev.end_query,
+ *                or -1/-1 when end_ref <= 0 or end_query <= 0.
+ * The reverse pass runs on the GPU over rev(target[0..end_ref]) with the
+ * reversed query and swapped free ends. */
+int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db,
+                            const int64_t *sel, const int32_t *end_ref,
+                            const int32_t *end_query, int64_t n_sel,
+                            int32_t *target_start, int32_t *query_start);
+
 /* Timing of the last align call on ctx, measured with HIP events on the
  * context's stream around each kernel launch: total milliseconds and the
  * number of kernel launches it covered. */

@@ -31,7 +31,7 @@ ABI_SYMBOLS = [
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
     "ish_hip_set_tile", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
-    "ish_hip_last_timing", "ish_hip_emit",
+    "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
 ]
 
 
@@ -81,6 +81,9 @@ def lib():
         "ish_hip_db_fetch": ([vp, vp, i32p, i32p, i32p, u8p], i32),
         "ish_hip_last_timing": ([vp, P(ctypes.c_float), i32p], i32),
         "ish_hip_emit": ([i32, i32p, i32p, i32p, i64, i64, ctypes.c_float, u8p], i32),
+        "ish_hip_find_starts": ([vp, vp, P(i64), i32p, i32p, i64, i32p, i32p], i32),
+        "ish_hip_align_db_multi": ([vp, vp, u8p, u64p, i32, i8p, i32, i32, i32, i32, u32,
+                                    i32p, i32p, i32p, u8p], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -248,6 +251,40 @@ class Engine:
                                              _p(out[1], ctypes.c_int32), _p(out[2], ctypes.c_int32),
                                              _p(st, ctypes.c_uint8)))
         return out[0][:n], out[1][:n], out[2][:n], st[:n]
+
+    def find_starts(self, db: "Database", sel, end_ref, end_query):
+        """Start coordinates (target_start, query_start) of the selected
+        targets from their forward end coordinates (reverse pass on the GPU)."""
+        sel = np.ascontiguousarray(sel, np.int64)
+        er = np.ascontiguousarray(end_ref, np.int32)
+        eq = np.ascontiguousarray(end_query, np.int32)
+        n = len(sel)
+        ts = np.zeros(max(n, 1), np.int32)
+        qs = np.zeros(max(n, 1), np.int32)
+        self._check(self._L.ish_hip_find_starts(
+            self._h, db._h, _p(sel if n else np.zeros(1, np.int64), ctypes.c_int64),
+            _p(er if n else np.zeros(1, np.int32), ctypes.c_int32),
+            _p(eq if n else np.zeros(1, np.int32), ctypes.c_int32), n,
+            _p(ts, ctypes.c_int32), _p(qs, ctypes.c_int32)))
+        return ts[:n], qs[:n]
+
+    def align_db_multi(self, db: "Database", queries, matrix_kind: str, gap_open: int = 3,
+                       gap_extend: int = 1, ends_free="FFTT"):
+        """Scores every target of db against every query (semi-global) in one
+        persistent launch per cell width; returns [nq, n] arrays."""
+        qs = [np.ascontiguousarray(q, np.uint8) for q in queries]
+        qoff = np.zeros(len(qs) + 1, np.uint64)
+        np.cumsum([len(q) for q in qs], out=qoff[1:])
+        qbuf = np.concatenate(qs) if qoff[-1] else np.zeros(1, np.uint8)
+        fl = flags_from_str(ends_free) if isinstance(ends_free, str) else int(ends_free)
+        nq, n = len(qs), db.n
+        out = [np.zeros(max(nq * n, 1), np.int32) for _ in range(3)]
+        st = np.zeros(max(nq * n, 1), np.uint8)
+        self._check(self._L.ish_hip_align_db_multi(
+            self._h, db._h, _p(qbuf, ctypes.c_uint8), _p(qoff, ctypes.c_uint64), nq, None, 0,
+            MATRIX[matrix_kind], gap_open, gap_extend, fl, _p(out[0], ctypes.c_int32),
+            _p(out[1], ctypes.c_int32), _p(out[2], ctypes.c_int32), _p(st, ctypes.c_uint8)))
+        return tuple(a[: nq * n].reshape(nq, n) for a in (*out, st))
 
     def last_timing(self):
         ms = ctypes.c_float()

@@ -26,7 +26,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["ish_hip.cpp", "sg_kernels.hip"]
+HIP_SOURCES = ["ish_hip.cpp", "sg_kernels.hip", "sw_kernels.hip"]
 
 
 def _run(cmd):

@@ -23,6 +23,17 @@ struct GroupDesc {
     int32_t first;    // index of the group's first target in sorted arrays
 };
 
+// One query of a multi-query launch.
+struct QueryDesc {
+    const int16_t *prof;  // offset-form profile (staged into LDS)
+    const uint16_t *lut;  // residue code -> profile row offset (dwords)
+    int32_t prof_words;
+    int32_t Q, nchunks, neg_limit;
+    int32_t g_begin, g_end; // groups of this launch for this query
+    int64_t tile_base;      // first work tile of this query (prefix sum)
+    int64_t out_off;        // q * n_targets
+};
+
 // Everything a semi-global / local scoring launch needs.
 struct ScoreParams {
     // query profile (staged into LDS): nrows rows of `stride` int16 scores,
@@ -56,6 +67,11 @@ struct ScoreParams {
     // outputs, original target order
     int32_t *score, *end_ref, *end_q;
     uint8_t *status;
+
+    // multi-query launches: nqueries entries plus a sentinel whose
+    // tile_base is the total number of tiles
+    const QueryDesc *queries;
+    int32_t nqueries;
 };
 
 // Plain (one launch per batch) parameters: targets in input order,
@@ -81,12 +97,24 @@ hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid,
                                  int block, size_t lds, hipStream_t st);
 hipError_t launch_sg_batched_i32(const ScoreParams &p, int chunk, int grid,
                                  int block, size_t lds, hipStream_t st);
+hipError_t launch_sw_batched_i16(const ScoreParams &p, int chunk, int grid,
+                                 int block, size_t lds, hipStream_t st);
+hipError_t launch_sw_batched_i32(const ScoreParams &p, int chunk, int grid,
+                                 int block, size_t lds, hipStream_t st);
+int occupancy_sw_batched_i16(int chunk, int block, size_t lds);
+int occupancy_sw_batched_i32(int chunk, int block, size_t lds);
 hipError_t launch_sg_plain(const PlainParams &p, int grid, int block,
                            size_t lds, hipStream_t st);
 hipError_t launch_pack(const uint8_t *raw, const uint64_t *off,
                        const int32_t *sorted_idx, const GroupDesc *groups,
                        int32_t ngroups, int64_t n, uint2 *res,
                        hipStream_t st);
+hipError_t launch_gather_reversed(const uint8_t *raw, const uint64_t *src_off,
+                                  const int64_t *sel, const uint64_t *dst_off,
+                                  int64_t n_sel, uint8_t *out, hipStream_t st);
+hipError_t launch_sg_multi(const ScoreParams &p, bool packed, int chunk, int grid,
+                           int block, size_t lds, hipStream_t st);
+int occupancy_sg_multi(bool packed, int chunk, int block, size_t lds, uint32_t flags);
 // Occupancy helpers: blocks per CU of each kernel at the given LDS size.
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags);
 int occupancy_sg_batched_i32(int chunk, int block, size_t lds, uint32_t flags);

@@ -58,6 +58,10 @@ struct ish_hip_db {
     int32_t *d_lens = nullptr, *d_orig = nullptr;
     int32_t *d_score = nullptr, *d_end_ref = nullptr, *d_end_q = nullptr;
     uint8_t *d_status = nullptr;
+    // raw residues (offsets rebased to 0) kept for start-coordinate passes
+    uint8_t *d_raw = nullptr;
+    uint64_t *d_off = nullptr;
+    std::vector<uint64_t> h_off;
 };
 
 struct ish_hip_ctx {
@@ -73,7 +77,7 @@ struct ish_hip_ctx {
     int64_t r16max = -1; // longest target routed to the int16 kernel
     int32_t prof_words = 0;
     size_t lds = 0;
-    DevBuf prof_t, prof_p, lut;
+    DevBuf prof_t, prof_rt, prof_p, lut; // forward, reversed-query, plain profiles
     int engine = ISH_ENGINE_BATCHED;
     int chunk = kChunkDefault; // query rows per lane in the batched kernels
     // work buffers
@@ -228,7 +232,7 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (auto e : ctx->ev) hipEventDestroy(e);
-    for (DevBuf *b : {&ctx->prof_t, &ctx->prof_p, &ctx->lut, &ctx->scratch, &ctx->counter,
+    for (DevBuf *b : {&ctx->prof_t, &ctx->prof_rt, &ctx->prof_p, &ctx->lut, &ctx->scratch, &ctx->counter,
                       &ctx->in_res, &ctx->in_off, &ctx->out_score, &ctx->out_er, &ctx->out_eq,
                       &ctx->out_st})
         b->release();
@@ -282,42 +286,34 @@ static int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, i
     return best < 0 ? -1 : best;
 }
 
-extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t Q,
-                                     const int8_t *matrix, int32_t A, int32_t kind, int32_t go,
-                                     int32_t ge, uint32_t ends_free, int32_t algo) {
-    if (!ctx) return ISH_E_INVALID;
-    if (Q < 0 || (Q > 0 && !q)) return fail(ctx, ISH_E_INVALID, "bad query");
-    if (algo != ISH_ALGO_SEMI_GLOBAL && algo != ISH_ALGO_LOCAL)
-        return fail(ctx, ISH_E_INVALID, "unknown algo %d", algo);
-    if (algo == ISH_ALGO_LOCAL) return fail(ctx, ISH_E_INVALID, "local alignment not built yet");
-    if (go < 0 || ge < 0) return fail(ctx, ISH_E_INVALID, "gap penalties must be >= 0");
-    std::vector<int8_t> mat;
-    if (!matrix) {
-        A = alphabet_of(kind);
-        if (A < 0) return fail(ctx, ISH_E_INVALID, "unknown matrix kind %d", kind);
-        mat.resize((size_t)A * A);
-        ish_hip_matrix(kind, mat.data(), A * A);
-    } else {
-        if (A <= 0 || A > 256) return fail(ctx, ISH_E_INVALID, "alphabet must be 1..256");
-        mat.assign(matrix, matrix + (size_t)A * A);
-    }
-    for (int32_t j = 0; j < Q; ++j)
-        if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query code %d at %d >= alphabet %d", q[j], j, A);
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
+// Local alignment in packed int16: with the state shifted down every 256
+// columns, every value lies in [-(go+ge+|min S|), Q*M + (256+Q+8)*ge + M],
+// whatever the target length.
+static bool local_fits_int16(int32_t qpad, int32_t go, int32_t ge, int32_t max_s, int32_t min_s) {
+    const int64_t M = std::max(max_s, 0), m = std::max(-min_s, 0);
+    const int64_t lim = 32767 - 64;
+    const int64_t up = (int64_t)qpad * M + (int64_t)(256 + qpad + 8) * ge + M + go;
+    const int64_t lo = 3LL * go + 3LL * ge + m;
+    return up <= lim && lo <= lim && qpad <= 65534;
+}
 
-    int32_t mn = 0, mx = 0; // both seeded at 0 (striped.mojo:62-71)
-    for (int8_t v : mat) {
-        mn = std::min<int32_t>(mn, v);
-        mx = std::max<int32_t>(mx, v);
-    }
-    const int32_t nl = (-go < mn) ? -32768 + go + 1 : -32768 - mn + 1;
+// Query profile in the LDS layout the kernels read: one row of `stride`
+// int16 scores per group of residue codes whose score column over the query
+// is identical (for the ascii matrix this collapses 256 codes to the
+// distinct query bytes plus one "absent" row); lut maps a code to the dword
+// offset of its row.
+struct Profile {
+    int32_t stride = 0, nrows = 0, qpad = 0;
+    std::vector<int16_t> fwd, rev, plain; // offset form, reversed query, S
+    uint16_t lut[256];
+    size_t lds() const { return (size_t)nrows * stride * 2 + 512; }
+    int32_t words() const { return nrows * stride / 2; }
+};
 
-    // Profile rows: target codes whose score column over the query is
-    // identical share a row (for the ascii matrix this collapses 256 codes
-    // to the distinct query bytes plus one "absent" row).
+static void build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
+                          int32_t ge, bool want_rev_plain, Profile &pf) {
     std::map<std::vector<int8_t>, int> row_of;
     std::vector<std::vector<int8_t>> rows;
-    uint16_t lut[256];
     for (int c = 0; c < 256; ++c) {
         const int cc = c < A ? c : A - 1; // out-of-alphabet codes score as the last code
         std::vector<int8_t> col((size_t)Q);
@@ -327,31 +323,89 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
             it = row_of.emplace(col, (int)rows.size()).first;
             rows.push_back(col);
         }
-        lut[c] = (uint16_t)it->second; // row index; scaled to an offset below
+        pf.lut[c] = (uint16_t)it->second;
     }
-    const int32_t qpad = (Q + kChunkAlign - 1) / kChunkAlign * kChunkAlign;
+    pf.qpad = (Q + kChunkAlign - 1) / kChunkAlign * kChunkAlign;
     // rows start 16 bytes apart modulo 128: 16-byte slices of different rows
     // spread over the LDS banks and every slice read is 16-byte aligned
-    const int32_t stride = qpad + 8;
-    const int32_t nrows = (int32_t)rows.size();
-    const size_t prof_elems = (size_t)nrows * stride;
-    const size_t lds = prof_elems * 2 + 512;
-    for (int c = 0; c < 256; ++c) lut[c] = (uint16_t)(lut[c] * (stride / 2));
+    pf.stride = pf.qpad + 8;
+    pf.nrows = (int32_t)rows.size();
+    for (int c = 0; c < 256; ++c) pf.lut[c] = (uint16_t)(pf.lut[c] * (pf.stride / 2));
+    const size_t elems = (size_t)pf.nrows * pf.stride;
+    pf.fwd.assign(elems, 0);
+    if (want_rev_plain) {
+        pf.rev.assign(elems, 0);
+        pf.plain.assign(elems, 0);
+    }
+    for (int32_t r = 0; r < pf.nrows; ++r)
+        for (int32_t j = 0; j < Q; ++j) {
+            const int16_t v = (int16_t)(rows[r][j] + 2 * ge);
+            pf.fwd[(size_t)r * pf.stride + j] = v;
+            if (want_rev_plain) {
+                pf.plain[(size_t)r * pf.stride + j] = rows[r][j];
+                // reversed query: the profile of the reverse passes
+                // (Profile of rev_pattern, striped_semi_global_matcher.mojo:68-75)
+                pf.rev[(size_t)r * pf.stride + (Q - 1 - j)] = v;
+            }
+        }
+}
+
+static int32_t resolve_matrix(ish_hip_ctx *ctx, const int8_t *matrix, int32_t &A, int32_t kind,
+                              std::vector<int8_t> &mat) {
+    if (!matrix) {
+        A = alphabet_of(kind);
+        if (A < 0) return fail(ctx, ISH_E_INVALID, "unknown matrix kind %d", kind);
+        mat.resize((size_t)A * A);
+        ish_hip_matrix(kind, mat.data(), A * A);
+    } else {
+        if (A <= 0 || A > 256) return fail(ctx, ISH_E_INVALID, "alphabet must be 1..256");
+        mat.assign(matrix, matrix + (size_t)A * A);
+    }
+    return ISH_OK;
+}
+
+// int16 floor seeding the last-row maximum (striped.mojo:329-334), from the
+// matrix minimum seeded at 0 (:62-71).
+static int32_t neg_limit_of(int32_t go, int32_t mn) {
+    const int32_t nl = (-go < mn) ? -32768 + go + 1 : -32768 - mn + 1;
+    return (int32_t)(int16_t)nl;
+}
+
+extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t Q,
+                                     const int8_t *matrix, int32_t A, int32_t kind, int32_t go,
+                                     int32_t ge, uint32_t ends_free, int32_t algo) {
+    if (!ctx) return ISH_E_INVALID;
+    if (Q < 0 || (Q > 0 && !q)) return fail(ctx, ISH_E_INVALID, "bad query");
+    if (algo != ISH_ALGO_SEMI_GLOBAL && algo != ISH_ALGO_LOCAL)
+        return fail(ctx, ISH_E_INVALID, "unknown algo %d", algo);
+    if (go < 0 || ge < 0) return fail(ctx, ISH_E_INVALID, "gap penalties must be >= 0");
+    std::vector<int8_t> mat;
+    int32_t rc = resolve_matrix(ctx, matrix, A, kind, mat);
+    if (rc) return rc;
+    for (int32_t j = 0; j < Q; ++j)
+        if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query code %d at %d >= alphabet %d", q[j], j, A);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+
+    int32_t mn = 0, mx = 0; // both seeded at 0 (striped.mojo:62-71)
+    for (int8_t v : mat) {
+        mn = std::min<int32_t>(mn, v);
+        mx = std::max<int32_t>(mx, v);
+    }
+    Profile pf;
+    build_profile(q, Q, mat, A, ge, true, pf);
+    const size_t lds = pf.lds();
     if (lds > kLdsLimit)
         return fail(ctx, ISH_E_RANGE, "query profile needs %zu bytes of LDS (limit %zu)", lds,
                     kLdsLimit);
-    std::vector<int16_t> pt(prof_elems, 0), pp(prof_elems, 0);
-    for (int32_t r = 0; r < nrows; ++r)
-        for (int32_t j = 0; j < Q; ++j) {
-            pp[(size_t)r * stride + j] = rows[r][j];
-            pt[(size_t)r * stride + j] = (int16_t)(rows[r][j] + 2 * ge);
-        }
-    HIP_TRY(ctx, ctx->prof_t.ensure(prof_elems * 2));
-    HIP_TRY(ctx, ctx->prof_p.ensure(prof_elems * 2));
+    const size_t bytes = pf.fwd.size() * 2;
+    HIP_TRY(ctx, ctx->prof_t.ensure(bytes));
+    HIP_TRY(ctx, ctx->prof_p.ensure(bytes));
+    HIP_TRY(ctx, ctx->prof_rt.ensure(bytes));
     HIP_TRY(ctx, ctx->lut.ensure(512));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_t.p, pt.data(), prof_elems * 2, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_p.p, pp.data(), prof_elems * 2, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->lut.p, lut, 512, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_t.p, pf.fwd.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_rt.p, pf.rev.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_p.p, pf.plain.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->lut.p, pf.lut, 512, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
     ctx->Q = Q;
@@ -361,14 +415,15 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     ctx->kind = kind;
     ctx->flags = ends_free & 15u;
     ctx->nchunks = 0; // set per launch from the tile height
-    ctx->stride = stride;
-    ctx->rows = nrows;
-    ctx->neg_limit = (int32_t)(int16_t)nl;
+    ctx->stride = pf.stride;
+    ctx->rows = pf.nrows;
+    ctx->neg_limit = neg_limit_of(go, mn);
     ctx->min_score = mn;
     ctx->max_score = mx;
-    ctx->prof_words = (int32_t)(prof_elems / 2);
+    ctx->prof_words = pf.words();
     ctx->lds = lds;
-    ctx->r16max = int16_max_len(Q, go, ge, mx, mn);
+    ctx->r16max = algo == ISH_ALGO_LOCAL ? (local_fits_int16(pf.qpad, go, ge, mx, mn) ? INT64_MAX : -1)
+                                         : int16_max_len(Q, go, ge, mx, mn);
     ctx->have_query = true;
     return ISH_OK;
 }
@@ -437,24 +492,24 @@ static int32_t size_grid(ish_hip_ctx *ctx, int occ_blocks, int64_t cols, int *gr
 // ---------------------------------------------------------------------------
 // device-resident database
 // ---------------------------------------------------------------------------
-extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
-                                   const uint64_t *offsets, int64_t n, ish_hip_db **out) {
-    if (!ctx || !out || n < 0 || (n > 0 && (!offsets || !residues)))
-        return fail(ctx, ISH_E_INVALID, "bad db arguments");
-    if (n > (int64_t)INT32_MAX - kGroup) return fail(ctx, ISH_E_RANGE, "too many targets");
-    *out = nullptr;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const uint64_t base = n > 0 ? offsets[0] : 0;
-    const uint64_t nres = n > 0 ? offsets[n] - base : 0;
+// Builds the packed, length-sorted layout from residues already on the
+// device (d_raw, offsets d_off rebased to 0). Takes ownership of both
+// buffers, which the database keeps for start-coordinate passes.
+static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
+                        std::vector<uint64_t> h_off, ish_hip_db **out) {
+    const int64_t n = (int64_t)h_off.size() - 1;
+    ish_hip_db *db = new ish_hip_db();
+    db->device = ctx->device;
+    db->n = n;
+    db->nres = n > 0 ? (int64_t)h_off[(size_t)n] : 0;
+    db->d_raw = d_raw;
+    db->d_off = d_off;
     int64_t maxlen = 0;
-    for (int64_t t = 0; t < n; ++t) {
-        if (offsets[t + 1] < offsets[t]) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
-        maxlen = std::max<int64_t>(maxlen, (int64_t)(offsets[t + 1] - offsets[t]));
-    }
-    if (maxlen > (int64_t)INT32_MAX / 2) return fail(ctx, ISH_E_RANGE, "target too long");
+    for (int64_t t = 0; t < n; ++t)
+        maxlen = std::max<int64_t>(maxlen, (int64_t)(h_off[(size_t)t + 1] - h_off[(size_t)t]));
     // counting sort by length, descending, stable in input order
     std::vector<int64_t> hist((size_t)maxlen + 2, 0);
-    for (int64_t t = 0; t < n; ++t) hist[(size_t)(maxlen - (int64_t)(offsets[t + 1] - offsets[t]))]++;
+    for (int64_t t = 0; t < n; ++t) hist[(size_t)(maxlen - (int64_t)(h_off[(size_t)t + 1] - h_off[(size_t)t]))]++;
     int64_t acc = 0;
     for (auto &h : hist) {
         const int64_t v = h;
@@ -465,15 +520,11 @@ extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
     const int64_t nslots = (int64_t)ng * kGroup;
     std::vector<int32_t> sorted((size_t)nslots, -1), lens((size_t)nslots, 0);
     for (int64_t t = 0; t < n; ++t) {
-        const int64_t L = (int64_t)(offsets[t + 1] - offsets[t]);
+        const int64_t L = (int64_t)(h_off[(size_t)t + 1] - h_off[(size_t)t]);
         const int64_t pos = hist[(size_t)(maxlen - L)]++;
         sorted[(size_t)pos] = (int32_t)t;
         lens[(size_t)pos] = (int32_t)L;
     }
-    ish_hip_db *db = new ish_hip_db();
-    db->device = ctx->device;
-    db->n = n;
-    db->nres = (int64_t)nres;
     db->ngroups = ng;
     std::vector<GroupDesc> groups((size_t)ng);
     uint64_t off = 0;
@@ -486,6 +537,7 @@ extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
         db->maxcols = std::max(db->maxcols, groups[g].ncols);
         off += (uint64_t)(groups[g].ncols / 4) * 64;
     }
+    db->h_off = std::move(h_off);
     const uint64_t res_words = off + 64; // one spare block: empty groups read block 0
     auto cleanup = [&](int32_t code) {
         ish_hip_db_free(db);
@@ -509,27 +561,7 @@ extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
         DB_TRY(hipMemcpyAsync(db->d_groups, groups.data(), sizeof(GroupDesc) * ng, hipMemcpyHostToDevice, ctx->stream));
         DB_TRY(hipMemcpyAsync(db->d_lens, lens.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
         DB_TRY(hipMemcpyAsync(db->d_orig, sorted.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
-        // raw residues + offsets, rebased to 0, packed on the device
-        uint8_t *d_raw = nullptr;
-        uint64_t *d_off = nullptr;
-        std::vector<uint64_t> rebased((size_t)n + 1);
-        for (int64_t t = 0; t <= n; ++t) rebased[(size_t)t] = offsets[t] - base;
-        DB_TRY(hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1)));
-        if ((e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (n + 1))) != hipSuccess) {
-            hipFree(d_raw);
-            return cleanup(fail(ctx, ISH_E_NOMEM, "hipMalloc offsets: %s", hipGetErrorString(e)));
-        }
-        hipError_t e1 = hipMemcpyAsync(d_raw, residues + base, nres, hipMemcpyHostToDevice, ctx->stream);
-        hipError_t e2 = hipMemcpyAsync(d_off, rebased.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
-        hipError_t e3 = (e1 == hipSuccess && e2 == hipSuccess)
-                            ? launch_pack(d_raw, d_off, db->d_orig, db->d_groups, ng, n, db->d_res, ctx->stream)
-                            : (e1 != hipSuccess ? e1 : e2);
-        hipError_t e4 = hipStreamSynchronize(ctx->stream);
-        hipFree(d_raw);
-        hipFree(d_off);
-        if (e3 != hipSuccess || e4 != hipSuccess)
-            return cleanup(fail(ctx, ISH_E_HIP, "db packing failed: %s",
-                                hipGetErrorString(e3 != hipSuccess ? e3 : e4)));
+        DB_TRY(launch_pack(db->d_raw, db->d_off, db->d_orig, db->d_groups, ng, n, db->d_res, ctx->stream));
     }
     DB_TRY(hipStreamSynchronize(ctx->stream));
 #undef DB_TRY
@@ -537,13 +569,45 @@ extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
     return ISH_OK;
 }
 
+extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
+                                   const uint64_t *offsets, int64_t n, ish_hip_db **out) {
+    if (!ctx || !out || n < 0 || (n > 0 && (!offsets || !residues)))
+        return fail(ctx, ISH_E_INVALID, "bad db arguments");
+    if (n > (int64_t)INT32_MAX - kGroup) return fail(ctx, ISH_E_RANGE, "too many targets");
+    *out = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const uint64_t base = n > 0 ? offsets[0] : 0;
+    std::vector<uint64_t> h_off((size_t)n + 1, 0);
+    for (int64_t t = 0; t < n; ++t) {
+        if (offsets[t + 1] < offsets[t]) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
+        if (offsets[t + 1] - offsets[t] > (uint64_t)INT32_MAX / 2) return fail(ctx, ISH_E_RANGE, "target too long");
+        h_off[(size_t)t + 1] = offsets[t + 1] - base;
+    }
+    const uint64_t nres = h_off[(size_t)n];
+    uint8_t *d_raw = nullptr;
+    uint64_t *d_off = nullptr;
+    hipError_t e = hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (size_t)(n + 1));
+    if (e == hipSuccess && nres > 0)
+        e = hipMemcpyAsync(d_raw, residues + base, nres, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (d_raw) (void)hipFree(d_raw);
+        if (d_off) (void)hipFree(d_off);
+        return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "db upload: %s", hipGetErrorString(e));
+    }
+    return db_build(ctx, d_raw, d_off, std::move(h_off), out);
+}
+
 extern "C" void ish_hip_db_free(ish_hip_db *db) {
     if (!db) return;
-    hipSetDevice(db->device);
+    (void)hipSetDevice(db->device);
     for (void *p : {(void *)db->d_groups, (void *)db->d_res, (void *)db->d_lens, (void *)db->d_orig,
                     (void *)db->d_score, (void *)db->d_end_ref, (void *)db->d_end_q,
-                    (void *)db->d_status})
-        if (p) hipFree(p);
+                    (void *)db->d_status, (void *)db->d_raw, (void *)db->d_off})
+        if (p) (void)hipFree(p);
     delete db;
 }
 
@@ -554,15 +618,12 @@ extern "C" int32_t ish_hip_db_size(const ish_hip_db *db, int64_t *n, int64_t *nr
     return ISH_OK;
 }
 
-extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
-    if (!ctx || !db) return ISH_E_INVALID;
-    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
-    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
+// Enqueues the scoring kernels over every group of db with the given
+// profile (forward or reversed query) and free-end flags.
+static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, uint32_t flags) {
     ctx->launches = 0;
     if (db->ngroups == 0) return ISH_OK;
     HIP_TRY(ctx, ctx->counter.ensure(256));
-
     // groups whose longest target exceeds the int16 bound come first
     // (descending lengths); they run in the int32 kernel.
     int32_t g16 = 0;
@@ -576,7 +637,7 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
         g16 = lo;
     }
     ScoreParams p{};
-    p.prof = (const int16_t *)ctx->prof_t.p;
+    p.prof = prof;
     p.lut = (const uint16_t *)ctx->lut.p;
     p.prof_words = ctx->prof_words;
     p.stride = ctx->stride;
@@ -584,7 +645,7 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     p.nchunks = (ctx->Q + ctx->chunk - 1) / ctx->chunk;
     p.go = ctx->go;
     p.ge = ctx->ge;
-    p.flags = ctx->flags;
+    p.flags = flags;
     p.neg_limit = ctx->neg_limit;
     p.groups = db->d_groups;
     p.res = db->d_res;
@@ -596,10 +657,12 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     p.end_q = db->d_end_q;
     p.status = db->d_status;
 
+    const bool local = ctx->algo == ISH_ALGO_LOCAL;
     if (g16 > 0) {
         int grid;
         int64_t ss;
-        const int occ = occupancy_sg_batched_i32(ctx->chunk, kBlock, ctx->lds, ctx->flags);
+        const int occ = local ? occupancy_sw_batched_i32(ctx->chunk, kBlock, ctx->lds)
+                              : occupancy_sg_batched_i32(ctx->chunk, kBlock, ctx->lds, flags);
         int32_t rc = size_grid(ctx, occ, db->group_maxlen[0] + 3, &grid, &ss);
         if (rc) return rc;
         p.bnd = (uint2 *)ctx->scratch.p;
@@ -608,13 +671,15 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
         p.g_end = g16;
         HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
         HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, launch_sg_batched_i32(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, local ? launch_sw_batched_i32(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream)
+                           : launch_sg_batched_i32(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
         HIP_TRY(ctx, timed_end(ctx));
     }
     if (g16 < db->ngroups) {
         int grid;
         int64_t ss;
-        const int occ = occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, ctx->flags);
+        const int occ = local ? occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds)
+                              : occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, flags);
         int32_t rc = size_grid(ctx, occ, db->group_maxlen[g16] + 3, &grid, &ss);
         if (rc) return rc;
         p.bnd = (uint2 *)ctx->scratch.p;
@@ -623,10 +688,19 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
         p.g_end = db->ngroups;
         HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
         HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, launch_sg_batched_i16(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, local ? launch_sw_batched_i16(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream)
+                           : launch_sg_batched_i16(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
         HIP_TRY(ctx, timed_end(ctx));
     }
     return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
+    if (!ctx || !db) return ISH_E_INVALID;
+    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
+    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return launch_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags);
 }
 
 extern "C" int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
@@ -727,14 +801,242 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
     if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
     if (n == 0) return ISH_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (ctx->engine == ISH_ENGINE_PLAIN)
+    if (ctx->engine == ISH_ENGINE_PLAIN) {
+        if (ctx->algo != ISH_ALGO_SEMI_GLOBAL)
+            return fail(ctx, ISH_E_INVALID, "the plain engine scores semi-global alignments only");
         return align_plain(ctx, residues, offsets, n, score, end_ref, end_q, status);
+    }
     ish_hip_db *db = nullptr;
     int32_t rc = ish_hip_db_load(ctx, residues, offsets, n, &db);
     if (rc) return rc;
     rc = ish_hip_align_db(ctx, db, score, end_ref, end_q, status);
     ish_hip_db_free(db);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// many queries against one database, one persistent launch per cell width
+// ---------------------------------------------------------------------------
+extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
+                                          const uint8_t *queries, const uint64_t *qoffsets,
+                                          int32_t nq, const int8_t *matrix, int32_t A,
+                                          int32_t kind, int32_t go, int32_t ge,
+                                          uint32_t ends_free, int32_t *score, int32_t *end_ref,
+                                          int32_t *end_q, uint8_t *status) {
+    if (!ctx || !db || nq < 0 || (nq > 0 && (!queries || !qoffsets)))
+        return fail(ctx, ISH_E_INVALID, "bad multi-query arguments");
+    if (go < 0 || ge < 0) return fail(ctx, ISH_E_INVALID, "gap penalties must be >= 0");
+    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
+    const int64_t n = db->n;
+    if (nq == 0 || n == 0) return ISH_OK;
+    if (!score || !end_ref || !end_q) return fail(ctx, ISH_E_INVALID, "null outputs");
+    std::vector<int8_t> mat;
+    int32_t rc = resolve_matrix(ctx, matrix, A, kind, mat);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int32_t mn = 0, mx = 0;
+    for (int8_t v : mat) {
+        mn = std::min<int32_t>(mn, v);
+        mx = std::max<int32_t>(mx, v);
+    }
+    const uint32_t flags = ends_free & 15u;
+    // profiles of every query, back to back
+    std::vector<Profile> pfs((size_t)nq);
+    std::vector<size_t> pofs((size_t)nq + 1, 0);
+    size_t lds = 0;
+    for (int32_t k = 0; k < nq; ++k) {
+        if (qoffsets[k + 1] < qoffsets[k]) return fail(ctx, ISH_E_INVALID, "query offsets must be non-decreasing");
+        const int32_t Q = (int32_t)(qoffsets[k + 1] - qoffsets[k]);
+        const uint8_t *q = queries + qoffsets[k];
+        for (int32_t j = 0; j < Q; ++j)
+            if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query %d code %d >= alphabet %d", k, q[j], A);
+        build_profile(q, Q, mat, A, ge, false, pfs[(size_t)k]);
+        lds = std::max(lds, pfs[(size_t)k].lds());
+        pofs[(size_t)k + 1] = pofs[(size_t)k] + ((pfs[(size_t)k].fwd.size() * 2 + 511) & ~(size_t)255);
+    }
+    if (lds > kLdsLimit) return fail(ctx, ISH_E_RANGE, "a query profile needs %zu bytes of LDS", lds);
+    const size_t luts_off = pofs[(size_t)nq];
+    std::vector<uint8_t> blob(luts_off + (size_t)nq * 512, 0);
+    for (int32_t k = 0; k < nq; ++k) {
+        memcpy(blob.data() + pofs[(size_t)k], pfs[(size_t)k].fwd.data(), pfs[(size_t)k].fwd.size() * 2);
+        memcpy(blob.data() + luts_off + (size_t)k * 512, pfs[(size_t)k].lut, 512);
+    }
+    DevBuf d_blob, d_desc, d_out;
+    auto release = [&]() {
+        d_blob.release();
+        d_desc.release();
+        d_out.release();
+    };
+    const size_t tot = (size_t)nq * (size_t)n;
+    hipError_t e = d_blob.ensure(blob.size());
+    if (e == hipSuccess) e = d_desc.ensure(sizeof(QueryDesc) * ((size_t)nq + 1) * 2);
+    if (e == hipSuccess) e = d_out.ensure(tot * 13);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = ctx->counter.ensure(256);
+    if (e != hipSuccess) {
+        release();
+        return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "multi-query setup: %s", hipGetErrorString(e));
+    }
+    int32_t *o_score = (int32_t *)d_out.p;
+    int32_t *o_er = o_score + tot;
+    int32_t *o_eq = o_er + tot;
+    uint8_t *o_st = (uint8_t *)(o_eq + tot);
+    const int waves = kBlock / 64;
+    ctx->launches = 0;
+    // two launches: int32 items (groups beyond each query's int16 bound,
+    // which come first in the length order) and packed int16 items
+    for (int pass = 0; pass < 2 && !rc; ++pass) {
+        const bool packed = pass == 1;
+        std::vector<QueryDesc> qd((size_t)nq + 1);
+        int64_t tiles = 0;
+        int32_t maxcols = 0;
+        for (int32_t k = 0; k < nq; ++k) {
+            const Profile &pf = pfs[(size_t)k];
+            const int32_t Q = (int32_t)(qoffsets[k + 1] - qoffsets[k]);
+            const int64_t r16 = int16_max_len(Q, go, ge, mx, mn);
+            int32_t lo = 0, hi = db->ngroups;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) / 2;
+                if ((int64_t)db->group_maxlen[mid] > r16) lo = mid + 1;
+                else hi = mid;
+            }
+            QueryDesc &d = qd[(size_t)k];
+            d.prof = (const int16_t *)((uint8_t *)d_blob.p + pofs[(size_t)k]);
+            d.lut = (const uint16_t *)((uint8_t *)d_blob.p + luts_off + (size_t)k * 512);
+            d.prof_words = pf.words();
+            d.Q = Q;
+            d.nchunks = (Q + ctx->chunk - 1) / ctx->chunk;
+            d.neg_limit = neg_limit_of(go, mn);
+            d.g_begin = packed ? lo : 0;
+            d.g_end = packed ? db->ngroups : lo;
+            d.tile_base = tiles;
+            d.out_off = (int64_t)k * n;
+            const int64_t items = (int64_t)(d.g_end - d.g_begin) * (packed ? 1 : 2);
+            tiles += (items + waves - 1) / waves;
+            if (d.g_end > d.g_begin) maxcols = std::max(maxcols, db->group_maxlen[d.g_begin] + 3);
+        }
+        qd[(size_t)nq].tile_base = tiles;
+        if (tiles == 0) continue;
+        QueryDesc *dqd = (QueryDesc *)d_desc.p + (size_t)pass * ((size_t)nq + 1);
+        HIP_TRY(ctx, hipMemcpyAsync(dqd, qd.data(), sizeof(QueryDesc) * ((size_t)nq + 1), hipMemcpyHostToDevice, ctx->stream));
+        ScoreParams p{};
+        p.go = go;
+        p.ge = ge;
+        p.flags = flags;
+        p.groups = db->d_groups;
+        p.res = db->d_res;
+        p.lens = db->d_lens;
+        p.orig = db->d_orig;
+        p.counter = (uint32_t *)ctx->counter.p;
+        p.score = o_score;
+        p.end_ref = o_er;
+        p.end_q = o_eq;
+        p.status = o_st;
+        p.queries = dqd;
+        p.nqueries = nq;
+        int grid;
+        int64_t ss;
+        const int occ = occupancy_sg_multi(packed, ctx->chunk, kBlock, lds, flags);
+        rc = size_grid(ctx, occ, maxcols, &grid, &ss);
+        if (rc) break;
+        p.bnd = (uint2 *)ctx->scratch.p;
+        p.slot_stride = ss;
+        HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
+        HIP_TRY(ctx, timed_begin(ctx));
+        HIP_TRY(ctx, launch_sg_multi(p, packed, ctx->chunk, grid, kBlock, lds, ctx->stream));
+        HIP_TRY(ctx, timed_end(ctx));
+    }
+    if (!rc) {
+        e = hipMemcpyAsync(score, o_score, tot * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(end_ref, o_er, tot * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(end_q, o_eq, tot * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess && status) e = hipMemcpyAsync(status, o_st, tot, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, ISH_E_HIP, "multi-query results: %s", hipGetErrorString(e));
+        collect_timing(ctx);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    release();
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// start coordinates
+// ---------------------------------------------------------------------------
+extern "C" int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db, const int64_t *sel,
+                                       const int32_t *end_ref, const int32_t *end_query,
+                                       int64_t n_sel, int32_t *target_start,
+                                       int32_t *query_start) {
+    if (!ctx || !db || n_sel < 0 ||
+        (n_sel > 0 && (!sel || !end_ref || !end_query || !target_start || !query_start)))
+        return fail(ctx, ISH_E_INVALID, "bad find_starts arguments");
+    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
+    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
+    if (n_sel == 0) return ISH_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const bool local = ctx->algo == ISH_ALGO_LOCAL;
+    // reversed prefixes rev(r[0..end_ref]) of the selected targets
+    std::vector<uint64_t> off((size_t)n_sel + 1, 0);
+    std::vector<int64_t> src((size_t)n_sel);
+    for (int64_t k = 0; k < n_sel; ++k) {
+        const int64_t t = sel[k];
+        if (t < 0 || t >= db->n) return fail(ctx, ISH_E_INVALID, "selected target %lld out of range", (long long)t);
+        const int64_t R = (int64_t)(db->h_off[(size_t)t + 1] - db->h_off[(size_t)t]);
+        int64_t L = (int64_t)end_ref[k] + 1;
+        if (L < 0 || L > R) return fail(ctx, ISH_E_INVALID, "end_ref %d out of range for target %lld", end_ref[k], (long long)t);
+        if (local && (end_ref[k] <= 0 || end_query[k] <= 0)) L = 0; // no begin pass (ssw_align)
+        src[(size_t)k] = t;
+        off[(size_t)k + 1] = off[(size_t)k] + (uint64_t)L;
+    }
+    const uint64_t nres = off[(size_t)n_sel];
+    uint8_t *d_raw = nullptr;
+    uint64_t *d_off = nullptr;
+    int64_t *d_sel = nullptr;
+    hipError_t e = hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (size_t)(n_sel + 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_sel, sizeof(int64_t) * (size_t)n_sel);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_off, off.data(), sizeof(uint64_t) * (size_t)(n_sel + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_sel, src.data(), sizeof(int64_t) * (size_t)n_sel, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && nres > 0)
+        e = launch_gather_reversed(db->d_raw, db->d_off, d_sel, d_off, n_sel, d_raw, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (d_sel) (void)hipFree(d_sel);
+    if (e != hipSuccess) {
+        if (d_raw) (void)hipFree(d_raw);
+        if (d_off) (void)hipFree(d_off);
+        return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "reverse gather: %s", hipGetErrorString(e));
+    }
+    ish_hip_db *rdb = nullptr;
+    int32_t rc = db_build(ctx, d_raw, d_off, std::move(off), &rdb);
+    if (rc) return rc;
+    // free ends swap for the reverse pass (striped.mojo:165-183)
+    const uint32_t f = ctx->flags;
+    const uint32_t rflags = ((f & 1u) << 1) | ((f >> 1) & 1u) | ((f & 4u) << 1) | ((f >> 1) & 4u);
+    rc = launch_db(ctx, rdb, (const int16_t *)ctx->prof_rt.p, local ? 0u : rflags);
+    std::vector<int32_t> rs((size_t)n_sel), rer((size_t)n_sel), req((size_t)n_sel);
+    if (!rc) rc = ish_hip_db_fetch(ctx, rdb, rs.data(), rer.data(), req.data(), nullptr);
+    ish_hip_db_free(rdb);
+    if (rc) return rc;
+    for (int64_t k = 0; k < n_sel; ++k) {
+        if (local) {
+            if (end_ref[k] <= 0 || end_query[k] <= 0) {
+                target_start[k] = -1;
+                query_start[k] = -1;
+            } else {
+                // begin = position where the reverse running maximum last
+                // rose, in original coordinates (local_aln/striped.mojo:232-262)
+                target_start[k] = rer[(size_t)k] < 0 ? -1 : end_ref[k] - rer[(size_t)k];
+                query_start[k] = end_query[k] - req[(size_t)k];
+            }
+        } else {
+            // semi_global_aln_start_end (striped.mojo:185-191)
+            target_start[k] = end_ref[k] - rer[(size_t)k];
+            query_start[k] = ctx->Q - req[(size_t)k] - 1;
+        }
+    }
+    return ISH_OK;
 }
 
 // ---------------------------------------------------------------------------

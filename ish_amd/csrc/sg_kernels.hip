@@ -37,52 +37,9 @@
 #include <stdint.h>
 
 #include "engine.h"
+#include "kernel_common.h"
 
 namespace ish {
-
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) {
-    return __builtin_elementwise_max(a, b);
-}
-__device__ __forceinline__ int vmax(int a, int b) { return a > b ? a : b; }
-
-template <typename V> __device__ __forceinline__ V splat(int v);
-template <> __device__ __forceinline__ s16x2 splat<s16x2>(int v) {
-    s16x2 r;
-    r.x = (short)v;
-    r.y = (short)v;
-    return r;
-}
-template <> __device__ __forceinline__ int splat<int>(int v) { return v; }
-
-__device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
-__device__ __forceinline__ uint32_t as_u32(int v) { return (uint32_t)v; }
-template <typename V> __device__ __forceinline__ V from_u32(uint32_t v);
-template <> __device__ __forceinline__ s16x2 from_u32<s16x2>(uint32_t v) {
-    return __builtin_bit_cast(s16x2, v);
-}
-template <> __device__ __forceinline__ int from_u32<int>(uint32_t v) { return (int)v; }
-
-__device__ __forceinline__ uint32_t wave_broadcast0(uint32_t v) {
-    return __builtin_amdgcn_readfirstlane(v);
-}
-
-// Stage the query profile and the residue -> profile-row table into LDS.
-// The table holds each row's offset in dwords (256 x uint16), so a residue
-// code turns into an LDS address with one shift-add.
-__device__ __forceinline__ void stage_profile(const int16_t *prof, const uint16_t *lut,
-                                              int prof_words, uint8_t *smem) {
-    const int *src = (const int *)prof;
-    int *dst = (int *)smem;
-    for (int x = threadIdx.x; x < prof_words; x += blockDim.x) dst[x] = src[x];
-    const int *lsrc = (const int *)lut;
-    int *ldst = (int *)(smem + (size_t)prof_words * 4);
-    for (int x = threadIdx.x; x < 128; x += blockDim.x) ldst[x] = lsrc[x];
-    __syncthreads();
-}
-
-typedef short short8 __attribute__((ext_vector_type(8)));
 
 // Running (value, first index) maximum as one ordered key per target:
 // value in the high 16 bits (signed), 0xFFFF - index in the low 16 bits, so
@@ -147,6 +104,239 @@ __device__ __forceinline__ void finalize(int R, int Q, bool te, bool qe, bool ha
 //   V = s16x2 : lane owns targets (first + lane) and (first + 64 + lane)
 //   V = int   : work item = (group, half); lane owns one target
 // ---------------------------------------------------------------------------
+// Per-query values a group pass needs (one query, or one of many).
+struct QueryView {
+    int Q, nchunks, neg_limit;
+    int64_t out_off; // offset of this query's results in the output arrays
+};
+
+// Scores one group (PACKED: 128 targets, two per lane) or one half group
+// (int32: 64 targets) against the query whose profile is staged in smem.
+template <typename V, int K, bool TE, bool QE>
+__device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &qv,
+                                         const uint8_t *smem, const uint16_t *slut, uint2 *bnd,
+                                         int g, int half, int lane) {
+    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
+    const int go = p.go, ge = p.ge, cst = ge - go, Q = qv.Q;
+    const bool qs = p.flags & 1u, ts = p.flags & 4u;
+    const V vcst = splat<V>(cst);
+    const GroupDesc gd = p.groups[g];
+    const int tA = gd.first + half * 64 + lane;
+    const int tB = gd.first + 64 + lane;
+    const int RA = p.lens[tA];
+    const int RB = PACKED ? p.lens[tB] : 0;
+    const uint2 *res = p.res + gd.res_off;
+    const int nblk = gd.ncols >> 2;
+
+    TrackI16 t16;
+    TrackI32 t32;
+    t16.te_a = t16.te_b = t16.qe_a = t16.qe_b = INT_MIN;
+    t16.corner_a = t16.corner_b = 0;
+    t32.te_v = INT_MIN;
+    t32.te_i = -1;
+    t32.qe_v = INT_MIN;
+    t32.qe_j = 0;
+    t32.corner = 0;
+
+    for (int c = 0; c < qv.nchunks; ++c) {
+        const int j0 = c * K;
+        const bool first = c == 0;
+        const bool last = c == qv.nchunks - 1;
+        const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
+        // LDS address of this chunk's rows in profile row `code`
+        const uint8_t *chunk_base = smem + 2 * j0;
+        auto row_ptr = [&](uint32_t code) -> const short8 * {
+            return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
+        };
+        V H[K], E[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const int j = j0 + m;
+            const int hl = qs ? (j - 1) * ge : -go - ge; // H^(j,-1)
+            H[m] = splat<V>(hl);
+            E[m] = splat<V>(hl + cst); // E^(j,0)
+        }
+        V diag_top = splat<V>(j0 == 0 ? -2 * ge : (qs ? (j0 - 2) * ge : -go - ge));
+
+        uint2 rr_next = res[lane];
+        uint2 bn_next[4];
+        if (!first) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
+        }
+        // profile slices of column 0
+        short8 sa[NS], sb[NS];
+        {
+            const short8 *na = row_ptr((half ? rr_next.y : rr_next.x) & 0xffu);
+            const short8 *nb = row_ptr(rr_next.y & 0xffu);
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                sa[k] = na[k];
+                if (PACKED) sb[k] = nb[k];
+            }
+        }
+        for (int b = 0; b < nblk; ++b) {
+            const uint2 rr = rr_next;
+            uint2 bn[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
+            if (b + 1 < nblk) {
+                rr_next = res[(size_t)(b + 1) * 64 + lane];
+                if (!first) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int i = b * 4 + t;
+                // residue codes of the next column (past the end: any code)
+                const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
+                const uint32_t wb = t < 3 ? rr.y : rr_next.y;
+                const int sh = 8 * ((t + 1) & 3);
+                const short8 *na = row_ptr((wa >> sh) & 0xffu);
+                const short8 *nb = row_ptr((wb >> sh) & 0xffu);
+                V htop, f;
+                if (first) {
+                    const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
+                    htop = splat<V>(ht);
+                    f = splat<V>(ht + cst); // F^(0,i)
+                } else {
+                    htop = from_u32<V>(bn[t].x);
+                    f = from_u32<V>(bn[t].y);
+                }
+                V diag = diag_top;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+#pragma unroll
+                    for (int mm = 0; mm < 8; ++mm) {
+                        const int m = k * 8 + mm;
+                        V s;
+                        if constexpr (PACKED) {
+                            s.x = sa[k][mm];
+                            s.y = sb[k][mm];
+                        } else {
+                            s = (int)sa[k][mm];
+                        }
+                        V h = vmax(vmax(diag + s, E[m]), f);
+                        diag = H[m];
+                        H[m] = h;
+                        const V gg = h + vcst;
+                        E[m] = vmax(E[m], gg);
+                        f = vmax(f, gg);
+                    }
+                    // slice k is consumed: fetch it for the next column. The
+                    // scheduling barrier keeps the fetch from being hoisted
+                    // above the rows that still read the old slice, so only
+                    // one extra slice is live at a time.
+                    __builtin_amdgcn_sched_barrier(0);
+                    sa[k] = na[k];
+                    if (PACKED) sb[k] = nb[k];
+                }
+                diag_top = htop;
+                if (!last) {
+                    uint2 o;
+                    o.x = as_u32(H[K - 1]);
+                    o.y = as_u32(f);
+                    bnd[(size_t)i * 64 + lane] = o;
+                } else {
+                    // row Q-1 lives in register mstar of the last chunk
+                    V hl = H[0];
+#pragma unroll
+                    for (int m = 1; m < K; ++m)
+                        if (m == mstar) hl = H[m];
+                    // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
+                    const V u = hl - splat<V>(i * ge);
+                    if constexpr (PACKED) {
+                        const uint32_t uu = as_u32(u);
+                        if (TE || QE) {
+                            if (i < RA) t16.te_a = max(t16.te_a, key_lo(uu, i));
+                            if (i < RB) t16.te_b = max(t16.te_b, key_hi(uu, i));
+                        } else {
+                            if (i == RA - 1) t16.corner_a = (int)(short)(uu & 0xffffu);
+                            if (i == RB - 1) t16.corner_b = (int)(short)(uu >> 16);
+                        }
+                    } else {
+                        const int uv = (int)as_u32(u);
+                        if (TE || QE) {
+                            if (i < RA && uv > t32.te_v) {
+                                t32.te_v = uv;
+                                t32.te_i = i;
+                            }
+                        } else if (i == RA - 1) {
+                            t32.corner = uv;
+                        }
+                    }
+                }
+                if (QE) {
+                    const bool cap_a = i == RA - 1;
+                    const bool cap_b = PACKED && i == RB - 1;
+                    // wave-uniform test keeps this once-per-target path a
+                    // real branch instead of predicated code on every column
+                    if (__builtin_amdgcn_ballot_w64(cap_a || cap_b) != 0) {
+#pragma unroll
+                        for (int m = 0; m < K; ++m) {
+                            if (last && m > mstar) continue;
+                            const int j = j0 + m;
+                            // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
+                            const V wv = H[m] - splat<V>(j * ge);
+                            if constexpr (PACKED) {
+                                const uint32_t ww = as_u32(wv);
+                                if (cap_a) t16.qe_a = max(t16.qe_a, key_lo(ww, j));
+                                if (cap_b) t16.qe_b = max(t16.qe_b, key_hi(ww, j));
+                            } else {
+                                const int wi = (int)as_u32(wv);
+                                if (cap_a && wi > t32.qe_v) {
+                                    t32.qe_v = wi;
+                                    t32.qe_j = j;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // results
+    const int qoff = (Q - 1) * ge;
+    if constexpr (PACKED) {
+        for (int which = 0; which < 2; ++which) {
+            const int t = which ? tB : tA;
+            const int R = which ? RB : RA;
+            const int o = p.orig[t];
+            if (o < 0) continue;
+            const int kte = which ? t16.te_b : t16.te_a;
+            const int kqe = which ? t16.qe_b : t16.qe_a;
+            const int cor = which ? t16.corner_b : t16.corner_a;
+            const int te_val = (kte >> 16) - qoff;
+            const int te_idx = 0xFFFF - (kte & 0xFFFF);
+            const int qe_val = (kqe >> 16) - (R - 1) * ge;
+            const int qe_j = 0xFFFF - (kqe & 0xFFFF);
+            int sc, er, eq;
+            finalize(R, Q, TE, QE, kte != INT_MIN, te_val, te_idx, kqe != INT_MIN, qe_val,
+                     qe_j, cor - qoff, qv.neg_limit, sc, er, eq);
+            p.score[qv.out_off + o] = sc;
+            p.end_ref[qv.out_off + o] = er;
+            p.end_q[qv.out_off + o] = eq;
+            p.status[qv.out_off + o] = 16;
+        }
+    } else {
+        const int o = p.orig[tA];
+        if (o >= 0) {
+            int sc, er, eq;
+            finalize(RA, Q, TE, QE, t32.te_i >= 0, t32.te_v - qoff, t32.te_i,
+                     t32.qe_v != INT_MIN, t32.qe_v - (RA - 1) * ge, t32.qe_j,
+                     t32.corner - qoff, qv.neg_limit, sc, er, eq);
+            p.score[qv.out_off + o] = sc;
+            p.end_ref[qv.out_off + o] = er;
+            p.end_q[qv.out_off + o] = eq;
+            p.status[qv.out_off + o] = 32 | 0x80;
+        }
+    }
+}
+
 template <typename V, int K, bool TE, bool QE>
 __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
@@ -154,14 +344,15 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
     const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
 
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
-    constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     uint2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
-    const int go = p.go, ge = p.ge, cst = ge - go, Q = p.Q;
-    const bool qs = p.flags & 1u, ts = p.flags & 4u;
-    const V vcst = splat<V>(cst);
     const int items = PACKED ? (p.g_end - p.g_begin) : 2 * (p.g_end - p.g_begin);
+    QueryView qv;
+    qv.Q = p.Q;
+    qv.nchunks = p.nchunks;
+    qv.neg_limit = p.neg_limit;
+    qv.out_off = 0;
 
     for (;;) {
         uint32_t w = 0;
@@ -170,218 +361,71 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
         if ((int)w >= items) break;
         const int g = p.g_begin + (PACKED ? (int)w : (int)(w >> 1));
         const int half = PACKED ? 0 : (int)(w & 1);
-        const GroupDesc gd = p.groups[g];
-        const int tA = gd.first + half * 64 + lane;
-        const int tB = gd.first + 64 + lane;
-        const int RA = p.lens[tA];
-        const int RB = PACKED ? p.lens[tB] : 0;
-        const uint2 *res = p.res + gd.res_off;
-        const int nblk = gd.ncols >> 2;
+        sg_group<V, K, TE, QE>(p, qv, smem, slut, bnd, g, half, lane);
+    }
+}
 
-        TrackI16 t16;
-        TrackI32 t32;
-        t16.te_a = t16.te_b = t16.qe_a = t16.qe_b = INT_MIN;
-        t16.corner_a = t16.corner_b = 0;
-        t32.te_v = INT_MIN;
-        t32.te_i = -1;
-        t32.qe_v = INT_MIN;
-        t32.qe_j = 0;
-        t32.corner = 0;
+// ---------------------------------------------------------------------------
+// Multi-query persistent kernel: one launch drains tiles of (query, 4 wave
+// items) in query-major order. A workgroup restages the profile only when
+// its next tile belongs to another query. Wave item = group (PACKED) or
+// half group (int32).
+// ---------------------------------------------------------------------------
+template <typename V, int K, bool TE, bool QE>
+__global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ int s_tile[3]; // query, first wave item, items of that query
+    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int waves = blockDim.x >> 6;
+    const int slot = blockIdx.x * waves + wave;
+    uint2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
+    int cur_q = -1;
+    const uint16_t *slut = nullptr;
 
-        for (int c = 0; c < p.nchunks; ++c) {
-            const int j0 = c * K;
-            const bool first = c == 0;
-            const bool last = c == p.nchunks - 1;
-            const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
-            // LDS address of this chunk's rows in profile row `code`
-            const uint8_t *chunk_base = smem + 2 * j0;
-            auto row_ptr = [&](uint32_t code) -> const short8 * {
-                return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
-            };
-            V H[K], E[K];
-#pragma unroll
-            for (int m = 0; m < K; ++m) {
-                const int j = j0 + m;
-                const int hl = qs ? (j - 1) * ge : -go - ge; // H^(j,-1)
-                H[m] = splat<V>(hl);
-                E[m] = splat<V>(hl + cst); // E^(j,0)
-            }
-            V diag_top = splat<V>(j0 == 0 ? -2 * ge : (qs ? (j0 - 2) * ge : -go - ge));
-
-            uint2 rr_next = res[lane];
-            uint2 bn_next[4];
-            if (!first) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
-            }
-            // profile slices of column 0
-            short8 sa[NS], sb[NS];
-            {
-                const short8 *na = row_ptr((half ? rr_next.y : rr_next.x) & 0xffu);
-                const short8 *nb = row_ptr(rr_next.y & 0xffu);
-#pragma unroll
-                for (int k = 0; k < NS; ++k) {
-                    sa[k] = na[k];
-                    if (PACKED) sb[k] = nb[k];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int64_t tile = (int64_t)atomicAdd(p.counter, 1u);
+            int q = -1;
+            if (tile < p.queries[p.nqueries].tile_base) {
+                int lo = 0, hi = p.nqueries - 1; // last query whose tile_base <= tile
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (p.queries[mid].tile_base <= tile) lo = mid;
+                    else hi = mid - 1;
                 }
+                q = lo;
             }
-            for (int b = 0; b < nblk; ++b) {
-                const uint2 rr = rr_next;
-                uint2 bn[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
-                if (b + 1 < nblk) {
-                    rr_next = res[(size_t)(b + 1) * 64 + lane];
-                    if (!first) {
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int i = b * 4 + t;
-                    // residue codes of the next column (past the end: any code)
-                    const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
-                    const uint32_t wb = t < 3 ? rr.y : rr_next.y;
-                    const int sh = 8 * ((t + 1) & 3);
-                    const short8 *na = row_ptr((wa >> sh) & 0xffu);
-                    const short8 *nb = row_ptr((wb >> sh) & 0xffu);
-                    V htop, f;
-                    if (first) {
-                        const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
-                        htop = splat<V>(ht);
-                        f = splat<V>(ht + cst); // F^(0,i)
-                    } else {
-                        htop = from_u32<V>(bn[t].x);
-                        f = from_u32<V>(bn[t].y);
-                    }
-                    V diag = diag_top;
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) {
-#pragma unroll
-                        for (int mm = 0; mm < 8; ++mm) {
-                            const int m = k * 8 + mm;
-                            V s;
-                            if constexpr (PACKED) {
-                                s.x = sa[k][mm];
-                                s.y = sb[k][mm];
-                            } else {
-                                s = (int)sa[k][mm];
-                            }
-                            V h = vmax(vmax(diag + s, E[m]), f);
-                            diag = H[m];
-                            H[m] = h;
-                            const V gg = h + vcst;
-                            E[m] = vmax(E[m], gg);
-                            f = vmax(f, gg);
-                        }
-                        // slice k is consumed: fetch it for the next column. The
-                        // scheduling barrier keeps the fetch from being hoisted
-                        // above the rows that still read the old slice, so only
-                        // one extra slice is live at a time.
-                        __builtin_amdgcn_sched_barrier(0);
-                        sa[k] = na[k];
-                        if (PACKED) sb[k] = nb[k];
-                    }
-                    diag_top = htop;
-                    if (!last) {
-                        uint2 o;
-                        o.x = as_u32(H[K - 1]);
-                        o.y = as_u32(f);
-                        bnd[(size_t)i * 64 + lane] = o;
-                    } else {
-                        // row Q-1 lives in register mstar of the last chunk
-                        V hl = H[0];
-#pragma unroll
-                        for (int m = 1; m < K; ++m)
-                            if (m == mstar) hl = H[m];
-                        // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
-                        const V u = hl - splat<V>(i * ge);
-                        if constexpr (PACKED) {
-                            const uint32_t uu = as_u32(u);
-                            if (TE || QE) {
-                                if (i < RA) t16.te_a = max(t16.te_a, key_lo(uu, i));
-                                if (i < RB) t16.te_b = max(t16.te_b, key_hi(uu, i));
-                            } else {
-                                if (i == RA - 1) t16.corner_a = (int)(short)(uu & 0xffffu);
-                                if (i == RB - 1) t16.corner_b = (int)(short)(uu >> 16);
-                            }
-                        } else {
-                            const int uv = (int)as_u32(u);
-                            if (TE || QE) {
-                                if (i < RA && uv > t32.te_v) {
-                                    t32.te_v = uv;
-                                    t32.te_i = i;
-                                }
-                            } else if (i == RA - 1) {
-                                t32.corner = uv;
-                            }
-                        }
-                    }
-                    if (QE) {
-                        const bool cap_a = i == RA - 1;
-                        const bool cap_b = PACKED && i == RB - 1;
-                        if (cap_a || cap_b) {
-#pragma unroll
-                            for (int m = 0; m < K; ++m) {
-                                if (last && m > mstar) continue;
-                                const int j = j0 + m;
-                                // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
-                                const V wv = H[m] - splat<V>(j * ge);
-                                if constexpr (PACKED) {
-                                    const uint32_t ww = as_u32(wv);
-                                    if (cap_a) t16.qe_a = max(t16.qe_a, key_lo(ww, j));
-                                    if (cap_b) t16.qe_b = max(t16.qe_b, key_hi(ww, j));
-                                } else {
-                                    const int wi = (int)as_u32(wv);
-                                    if (wi > t32.qe_v) {
-                                        t32.qe_v = wi;
-                                        t32.qe_j = j;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
+            s_tile[0] = q;
+            if (q >= 0) {
+                const QueryDesc &qd = p.queries[q];
+                const int nitems = PACKED ? (qd.g_end - qd.g_begin) : 2 * (qd.g_end - qd.g_begin);
+                s_tile[1] = (int)((tile - qd.tile_base) * waves);
+                s_tile[2] = nitems;
             }
         }
-
-        // results
-        const int qoff = (Q - 1) * ge;
-        if constexpr (PACKED) {
-            for (int which = 0; which < 2; ++which) {
-                const int t = which ? tB : tA;
-                const int R = which ? RB : RA;
-                const int o = p.orig[t];
-                if (o < 0) continue;
-                const int kte = which ? t16.te_b : t16.te_a;
-                const int kqe = which ? t16.qe_b : t16.qe_a;
-                const int cor = which ? t16.corner_b : t16.corner_a;
-                const int te_val = (kte >> 16) - qoff;
-                const int te_idx = 0xFFFF - (kte & 0xFFFF);
-                const int qe_val = (kqe >> 16) - (R - 1) * ge;
-                const int qe_j = 0xFFFF - (kqe & 0xFFFF);
-                int sc, er, eq;
-                finalize(R, Q, TE, QE, kte != INT_MIN, te_val, te_idx, kqe != INT_MIN, qe_val,
-                         qe_j, cor - qoff, p.neg_limit, sc, er, eq);
-                p.score[o] = sc;
-                p.end_ref[o] = er;
-                p.end_q[o] = eq;
-                p.status[o] = 16;
-            }
-        } else {
-            const int o = p.orig[tA];
-            if (o >= 0) {
-                int sc, er, eq;
-                finalize(RA, Q, TE, QE, t32.te_i >= 0, t32.te_v - qoff, t32.te_i,
-                         t32.qe_v != INT_MIN, t32.qe_v - (RA - 1) * ge, t32.qe_j,
-                         t32.corner - qoff, p.neg_limit, sc, er, eq);
-                p.score[o] = sc;
-                p.end_ref[o] = er;
-                p.end_q[o] = eq;
-                p.status[o] = 32 | 0x80;
-            }
+        __syncthreads();
+        const int q = s_tile[0];
+        if (q < 0) break;
+        const QueryDesc qd = p.queries[q];
+        if (q != cur_q) {
+            stage_profile(qd.prof, qd.lut, qd.prof_words, smem); // ends in a barrier
+            slut = (const uint16_t *)(smem + (size_t)qd.prof_words * 4);
+            cur_q = q;
         }
+        const int item = s_tile[1] + wave;
+        if (item < s_tile[2]) {
+            QueryView qv;
+            qv.Q = qd.Q;
+            qv.nchunks = qd.nchunks;
+            qv.neg_limit = qd.neg_limit;
+            qv.out_off = qd.out_off;
+            const int g = qd.g_begin + (PACKED ? item : (item >> 1));
+            const int half = PACKED ? 0 : (item & 1);
+            sg_group<V, K, TE, QE>(p, qv, smem, slut, bnd, g, half, lane);
+        }
+        __syncthreads(); // everyone is done with s_tile and the profile
     }
 }
 
@@ -525,6 +569,28 @@ __global__ void __launch_bounds__(256) pack_groups(const uint8_t *raw, const uin
 
 __global__ void reset_counter(uint32_t *c) { *c = 0u; }
 
+// rev(raw[src_off[sel[k]] .. + L_k)) -> out[dst_off[k] .. dst_off[k+1]),
+// L_k = dst_off[k+1] - dst_off[k] (callers check L_k <= target length).
+__global__ void __launch_bounds__(256) gather_reversed(const uint8_t *raw, const uint64_t *src_off,
+                                                       const int64_t *sel, const uint64_t *dst_off,
+                                                       int64_t n_sel, uint8_t *out) {
+    for (int64_t k = blockIdx.x; k < n_sel; k += gridDim.x) {
+        const uint64_t s0 = src_off[sel[k]];
+        const uint64_t d0 = dst_off[k];
+        const uint64_t L = dst_off[k + 1] - d0;
+        for (uint64_t x = threadIdx.x; x < L; x += blockDim.x) out[d0 + x] = raw[s0 + L - 1 - x];
+    }
+}
+
+hipError_t launch_gather_reversed(const uint8_t *raw, const uint64_t *src_off, const int64_t *sel,
+                                  const uint64_t *dst_off, int64_t n_sel, uint8_t *out,
+                                  hipStream_t st) {
+    const int64_t grid = n_sel < 65536 ? n_sel : 65536;
+    hipLaunchKernelGGL(gather_reversed, dim3((unsigned)grid), dim3(256), 0, st, raw, src_off, sel,
+                       dst_off, n_sel, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -541,6 +607,28 @@ static hipError_t launch_batched(const ScoreParams &p, int grid, int block, size
     else
         hipLaunchKernelGGL((sg_batched<V, K, false, false>), dim3(grid), dim3(block), lds, st, p);
     return hipGetLastError();
+}
+
+template <typename V, int K>
+static hipError_t launch_multi(const ScoreParams &p, int grid, int block, size_t lds, hipStream_t st) {
+    const bool te = p.flags & 8u, qe = p.flags & 2u;
+    if (te && qe)
+        hipLaunchKernelGGL((sg_multi<V, K, true, true>), dim3(grid), dim3(block), lds, st, p);
+    else if (te)
+        hipLaunchKernelGGL((sg_multi<V, K, true, false>), dim3(grid), dim3(block), lds, st, p);
+    else if (qe)
+        hipLaunchKernelGGL((sg_multi<V, K, false, true>), dim3(grid), dim3(block), lds, st, p);
+    else
+        hipLaunchKernelGGL((sg_multi<V, K, false, false>), dim3(grid), dim3(block), lds, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sg_multi(const ScoreParams &p, bool packed, int chunk, int grid, int block,
+                           size_t lds, hipStream_t st) {
+    if (packed) return chunk == 16 ? launch_multi<s16x2, 16>(p, grid, block, lds, st)
+                                   : launch_multi<s16x2, 32>(p, grid, block, lds, st);
+    return chunk == 16 ? launch_multi<int, 16>(p, grid, block, lds, st)
+                       : launch_multi<int, 32>(p, grid, block, lds, st);
 }
 
 hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid, int block,
@@ -601,6 +689,19 @@ template <typename V, int K> static int occ_batched(int block, size_t lds, uint3
     if (qe) return occ(sg_batched<V, K, false, true>, block, lds);
     return occ(sg_batched<V, K, false, false>, block, lds);
 }
+template <typename V, int K> static int occ_multi(int block, size_t lds, uint32_t flags) {
+    const bool te = flags & 8u, qe = flags & 2u;
+    if (te && qe) return occ(sg_multi<V, K, true, true>, block, lds);
+    if (te) return occ(sg_multi<V, K, true, false>, block, lds);
+    if (qe) return occ(sg_multi<V, K, false, true>, block, lds);
+    return occ(sg_multi<V, K, false, false>, block, lds);
+}
+int occupancy_sg_multi(bool packed, int chunk, int block, size_t lds, uint32_t flags) {
+    if (packed) return chunk == 16 ? occ_multi<s16x2, 16>(block, lds, flags)
+                                   : occ_multi<s16x2, 32>(block, lds, flags);
+    return chunk == 16 ? occ_multi<int, 16>(block, lds, flags) : occ_multi<int, 32>(block, lds, flags);
+}
+
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags) {
     return chunk == 16 ? occ_batched<s16x2, 16>(block, lds, flags)
                        : occ_batched<s16x2, 32>(block, lds, flags);

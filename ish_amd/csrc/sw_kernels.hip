@@ -1,0 +1,301 @@
+// Local (Smith-Waterman) scoring kernel for gfx950.
+//
+// Semantics: sw / ssw_align of the reference's CPU striped path
+// (ishlib/matcher/alignment/local_aln/striped.mojo:142-434):
+//   H = max(0, Hdiag + S[r_i][q_j], E, F), E and F as in Gotoh, every
+//   boundary 0 (E and F never need their floor: H takes the max with 0);
+//   score = max H; end_ref = first column whose column maximum reaches the
+//   final maximum (-1 when it is 0); end_query = smallest row of that column
+//   holding it (0 when the maximum is 0) (:395-428).
+//
+// Layout: the same as the semi-global batched kernel (lane = two targets in
+// packed int16 halves, K query rows in registers, per-wave boundary rows).
+// Offset form H^ = H + (i - i0 + j)*ge, with i0 the start of the current
+// 256-column window: the floor 0 becomes (i - i0 + j)*ge, one wave-uniform
+// value per row, and the state is shifted down by 256*ge at every window
+// start, so int16 stays exact on reads of any length.
+//
+// End coordinates are tracked per query chunk c: its maximum M_c, the first
+// column i_c where the chunk reaches it and the smallest row r_c of that
+// column holding it (the row scan runs only on columns where some lane's
+// chunk maximum strictly rises). Chunks combine by value, then first
+// column, then smallest row, which is exactly the whole-column rule.
+// The padding rows below the query (profile score -2*ge, i.e. <= 0) take
+// part in the column maximum: their values derive from real rows of the
+// same chunk through a non-positive diagonal score or a gap, so they never
+// strictly exceed the real rows' maximum so far, and the row scan meets the
+// real row first.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <stdint.h>
+
+#include "engine.h"
+#include "kernel_common.h"
+
+namespace ish {
+
+constexpr int kRenorm = 256; // columns per offset window (multiple of 4)
+
+__device__ __forceinline__ void sw_combine(int mc, int ic, int rc, int &best, int &er, int &eq) {
+    if (mc > best) {
+        best = mc;
+        er = ic;
+        eq = rc;
+    } else if (mc == best && mc > 0) {
+        if (ic < er) {
+            er = ic;
+            eq = rc;
+        } else if (ic == er && rc < eq) {
+            eq = rc;
+        }
+    }
+}
+
+template <typename V, int K>
+__global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    stage_profile(p.prof, p.lut, p.prof_words, smem);
+    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
+
+    constexpr bool PACKED = !__is_same(V, int);
+    constexpr int NS = K / 8;
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    uint2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
+    const int go = p.go, ge = p.ge, cst = ge - go, Q = p.Q;
+    const V vcst = splat<V>(cst);
+    const V vshift = splat<V>(kRenorm * ge);
+    const int items = PACKED ? (p.g_end - p.g_begin) : 2 * (p.g_end - p.g_begin);
+
+    for (;;) {
+        uint32_t w = 0;
+        if (lane == 0) w = atomicAdd(p.counter, 1u);
+        w = wave_broadcast0(__shfl(w, 0));
+        if ((int)w >= items) break;
+        const int g = p.g_begin + (PACKED ? (int)w : (int)(w >> 1));
+        const int half = PACKED ? 0 : (int)(w & 1);
+        const GroupDesc gd = p.groups[g];
+        const int tA = gd.first + half * 64 + lane;
+        const int tB = gd.first + 64 + lane;
+        const int RA = p.lens[tA];
+        const int RB = PACKED ? p.lens[tB] : 0;
+        const uint2 *res = p.res + gd.res_off;
+        const int nblk = gd.ncols >> 2;
+
+        int bestA = 0, erA = -1, eqA = 0, bestB = 0, erB = -1, eqB = 0;
+
+        for (int c = 0; c < p.nchunks; ++c) {
+            const int j0 = c * K;
+            const bool first = c == 0;
+            const bool last = c == p.nchunks - 1;
+            const uint8_t *chunk_base = smem + 2 * j0;
+            auto row_ptr = [&](uint32_t code) -> const short8 * {
+                return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
+            };
+            V H[K], E[K];
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                H[m] = splat<V>((j0 + m - 1) * ge); // H^(j,-1) = (j-1)*ge
+                E[m] = H[m] + vcst;                 // E^(j,0)
+            }
+            V diag_top = splat<V>((j0 - 2) * ge); // H^(j0-1,-1)
+            V run = splat<V>(0);                  // chunk maximum so far (plain values)
+            int icA = -1, rcA = 0, icB = -1, rcB = 0;
+            int ibase = 0;
+
+            uint2 rr_next = res[lane];
+            uint2 bn_next[4];
+            if (!first) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
+            }
+            short8 sa[NS], sb[NS];
+            {
+                const short8 *na = row_ptr((half ? rr_next.y : rr_next.x) & 0xffu);
+                const short8 *nb = row_ptr(rr_next.y & 0xffu);
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    sa[k] = na[k];
+                    if (PACKED) sb[k] = nb[k];
+                }
+            }
+            for (int b = 0; b < nblk; ++b) {
+                if (b > 0 && ((b * 4) % kRenorm) == 0) {
+                    // new offset window: shift the carried state down
+#pragma unroll
+                    for (int m = 0; m < K; ++m) {
+                        H[m] = H[m] - vshift;
+                        E[m] = E[m] - vshift;
+                    }
+                    diag_top = diag_top - vshift;
+                    ibase = b * 4;
+                }
+                const uint2 rr = rr_next;
+                uint2 bn[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
+                if (b + 1 < nblk) {
+                    rr_next = res[(size_t)(b + 1) * 64 + lane];
+                    if (!first) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int i = b * 4 + t;
+                    const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
+                    const uint32_t wb = t < 3 ? rr.y : rr_next.y;
+                    const int sh = 8 * ((t + 1) & 3);
+                    const short8 *na = row_ptr((wa >> sh) & 0xffu);
+                    const short8 *nb = row_ptr((wb >> sh) & 0xffu);
+                    V htop, f;
+                    if (first) {
+                        const int ht = (i - ibase - 1) * ge; // H^(-1,i)
+                        htop = splat<V>(ht);
+                        f = splat<V>(ht + cst);
+                    } else {
+                        htop = from_u32<V>(bn[t].x);
+                        f = from_u32<V>(bn[t].y);
+                    }
+                    const int zc = (i - ibase + j0) * ge; // floor of row j0 at column i
+                    V diag = diag_top;
+                    V cms[NS]; // column maximum, one partial per 8-row slice
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) cms[k] = splat<V>(0);
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+#pragma unroll
+                        for (int mm = 0; mm < 8; ++mm) {
+                            const int m = k * 8 + mm;
+                            V s;
+                            if constexpr (PACKED) {
+                                s.x = sa[k][mm];
+                                s.y = sb[k][mm];
+                            } else {
+                                s = (int)sa[k][mm];
+                            }
+                            const V z = splat<V>(zc + m * ge);
+                            V h = vmax(vmax(vmax(diag + s, E[m]), f), z);
+                            diag = H[m];
+                            H[m] = h;
+                            const V gg = h + vcst;
+                            E[m] = vmax(E[m], gg);
+                            f = vmax(f, gg);
+                            cms[k] = vmax(cms[k], h - z);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+                        sa[k] = na[k];
+                        if (PACKED) sb[k] = nb[k];
+                    }
+                    diag_top = htop;
+                    if (!last) {
+                        uint2 o;
+                        o.x = as_u32(H[K - 1]);
+                        o.y = as_u32(f);
+                        bnd[(size_t)i * 64 + lane] = o;
+                    }
+                    V cm = cms[0];
+#pragma unroll
+                    for (int k = 1; k < NS; ++k) cm = vmax(cm, cms[k]);
+                    // columns past a target's end do not count
+                    if constexpr (PACKED) {
+                        const uint32_t mask = (i < RA ? 0x0000FFFFu : 0u) | (i < RB ? 0xFFFF0000u : 0u);
+                        cm = from_u32<V>(as_u32(cm) & mask);
+                    } else {
+                        cm = i < RA ? cm : 0;
+                    }
+                    const V nr = vmax(run, cm);
+                    // A rise matters only when it reaches the target's best
+                    // over the chunks above: a chunk that ends below it cannot
+                    // win, and one that ends at or above it records its last
+                    // rise here, which is its (M_c, i_c, r_c).
+                    bool up_a, up_b;
+                    if constexpr (PACKED) {
+                        up_a = nr.x > run.x && (int)nr.x >= bestA;
+                        up_b = nr.y > run.y && (int)nr.y >= bestB;
+                    } else {
+                        up_a = nr > run && nr >= bestA;
+                        up_b = false;
+                    }
+                    // wave-uniform test keeps this rare path a real branch
+                    if (__builtin_amdgcn_ballot_w64(up_a || up_b) != 0) {
+                        // smallest row of this column holding the new maximum
+                        int ra = K, rb = K;
+#pragma unroll
+                        for (int m = 0; m < K; ++m) {
+                            const V d = H[m] - splat<V>(zc + m * ge);
+                            if constexpr (PACKED) {
+                                if (ra == K && d.x == nr.x) ra = m;
+                                if (rb == K && d.y == nr.y) rb = m;
+                            } else {
+                                if (ra == K && d == nr) ra = m;
+                            }
+                        }
+                        if (up_a) {
+                            icA = i;
+                            rcA = j0 + ra;
+                        }
+                        if (up_b) {
+                            icB = i;
+                            rcB = j0 + rb;
+                        }
+                    }
+                    run = nr;
+                }
+            }
+            if constexpr (PACKED) {
+                sw_combine((int)run.x, icA, rcA, bestA, erA, eqA);
+                sw_combine((int)run.y, icB, rcB, bestB, erB, eqB);
+            } else {
+                sw_combine((int)run, icA, rcA, bestA, erA, eqA);
+            }
+        }
+
+        for (int which = 0; which < (PACKED ? 2 : 1); ++which) {
+            const int t = which ? tB : tA;
+            const int o = p.orig[t];
+            if (o < 0) continue;
+            const int best = which ? bestB : bestA;
+            p.score[o] = best;
+            p.end_ref[o] = best > 0 ? (which ? erB : erA) : -1;
+            p.end_q[o] = best > 0 ? (which ? eqB : eqA) : 0;
+            p.status[o] = PACKED ? 16 : (32 | 0x80);
+        }
+    }
+}
+
+hipError_t launch_sw_batched_i16(const ScoreParams &p, int chunk, int grid, int block,
+                                 size_t lds, hipStream_t st) {
+    if (chunk == 16)
+        hipLaunchKernelGGL((sw_batched<s16x2, 16>), dim3(grid), dim3(block), lds, st, p);
+    else
+        hipLaunchKernelGGL((sw_batched<s16x2, 32>), dim3(grid), dim3(block), lds, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_batched_i32(const ScoreParams &p, int chunk, int grid, int block,
+                                 size_t lds, hipStream_t st) {
+    if (chunk == 16)
+        hipLaunchKernelGGL((sw_batched<int, 16>), dim3(grid), dim3(block), lds, st, p);
+    else
+        hipLaunchKernelGGL((sw_batched<int, 32>), dim3(grid), dim3(block), lds, st, p);
+    return hipGetLastError();
+}
+
+template <typename KernelT> static int occ_of(KernelT k, int block, size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+int occupancy_sw_batched_i16(int chunk, int block, size_t lds) {
+    return chunk == 16 ? occ_of(sw_batched<s16x2, 16>, block, lds)
+                       : occ_of(sw_batched<s16x2, 32>, block, lds);
+}
+int occupancy_sw_batched_i32(int chunk, int block, size_t lds) {
+    return chunk == 16 ? occ_of(sw_batched<int, 16>, block, lds)
+                       : occ_of(sw_batched<int, 32>, block, lds);
+}
+
+} // namespace ish

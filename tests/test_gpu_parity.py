@@ -172,3 +172,238 @@ def test_database_reuse_across_queries(eng):
         assert np.array_equal(s, os_) and np.array_equal(er, oer) and np.array_equal(eq, oeq), (qlen, flags)
         assert (st == ish_amd.STATUS_I16).all()
     db.close()
+
+
+# ---- local (Smith-Waterman) ------------------------------------------------
+def check_local(eng, q, targets, kind, go, ge, tile=32, label=""):
+    eng.set_engine("batched")
+    eng.set_tile(tile)
+    eng.set_query(q, kind, go, ge, "FFFF", "striped-local")
+    s, er, eq, st = eng.align(targets)
+    m = O.Matrix(kind)
+    os_, oer, oeq, _ = O.batch(1, q, targets.residues, targets.offsets, m, go, ge, 0)
+    bad = np.nonzero((s != os_) | (er != oer) | (eq != oeq))[0]
+    if len(bad):
+        t = bad[0]
+        raise AssertionError(
+            f"{label} local tile={tile} {len(bad)}/{targets.n} differ; first t={t} "
+            f"R={targets.offsets[t + 1] - targets.offsets[t]} gpu=({s[t]},{er[t]},{eq[t]}) "
+            f"oracle=({os_[t]},{oer[t]},{oeq[t]}) status={st[t]}")
+    return s, er, eq, st
+
+
+@pytest.mark.parametrize("tile", [32, 16])
+def test_local_known_answers(eng, tile):
+    eng.set_tile(tile)
+    for c in VEC["local"]:
+        m = O.Matrix(c["matrix"])
+        q, t = m.encode(c["read"]), m.encode(c["reference"])
+        eng.set_query(q, c["matrix"], c["go"], c["ge"], "FFFF", "striped-local")
+        s, er, eq, _ = eng.align(Targets.from_list([t]))
+        assert (s[0], er[0], eq[0]) == (c["score"], c["ref_end"], c["read_end"]), c
+
+
+@pytest.mark.parametrize("tile", [32, 16])
+@pytest.mark.parametrize("kind", ["actgn", "actgn0", "blosum62", "ascii"])
+def test_local_random(eng, tile, kind):
+    rng = np.random.default_rng(zlib.crc32(f"local-{kind}-{tile}".encode()))
+    alpha = {"actgn": 5, "actgn0": 5, "blosum62": 24, "ascii": 256}[kind]
+    for trial in range(6):
+        Q = int(rng.integers(1, 200))
+        base = 40 if kind == "ascii" else 0
+        span = 60 if kind == "ascii" else alpha
+        q = (rng.integers(0, span, Q) + base).astype(np.uint8)
+        seqs = []
+        for t in range(250):
+            R = int(rng.integers(0, 700 if t % 7 else 2600))
+            r = (rng.integers(0, span, R) + base).astype(np.uint8)
+            if R and t % 4 == 0:
+                k = min(Q, R)
+                off = int(rng.integers(0, R - k + 1))
+                r[off:off + k] = q[:k]
+            seqs.append(r)
+        ge = int(rng.integers(0, 3))
+        go = ge + int(rng.integers(1, 6))
+        check_local(eng, q, Targets.from_list(seqs), kind, go, ge, tile, f"trial={trial}")
+
+
+def test_local_long_reads_and_windows(eng):
+    """Reads far longer than the 256-column offset window, hits near the end."""
+    rng = np.random.default_rng(21)
+    q = rng.integers(0, 4, 500).astype(np.uint8)
+    seqs = []
+    for L in (20000, 9000, 2049, 1023, 257, 256, 255, 3):
+        r = rng.integers(0, 4, L).astype(np.uint8)
+        if L > 600:
+            r[L - 520:L - 20] = q
+        seqs.append(r)
+    for tile in (32, 16):
+        s, er, eq, st = check_local(eng, q, Targets.from_list(seqs), "actgn0", 3, 1, tile)
+        assert s[0] == 2 * 500 and er[0] == 20000 - 21
+        assert (st == ish_amd.STATUS_I16).all()
+
+
+def test_local_int32_path(eng):
+    rng = np.random.default_rng(22)
+    q = rng.integers(0, 24, 600).astype(np.uint8)
+    seqs = [rng.integers(0, 24, int(rng.integers(1, 900))).astype(np.uint8) for _ in range(300)]
+    s, er, eq, st = check_local(eng, q, Targets.from_list(seqs), "blosum62", 60, 50)
+    assert (st & ish_amd.STATUS_WIDENED).all()
+
+
+def test_local_long_read_workload(eng):
+    w = workloads.generate(4, 160, seed=4)
+    check_local(eng, w.query, w.targets, "actgn0", 3, 1, 32, "wl4")
+
+
+def _records(text, rtype):
+    if rtype == "line":
+        lines = text.split("\n")
+        if text.endswith("\n"):
+            lines = lines[:-1]
+        return [(l, l) for l in lines]
+    recs, name, seq = [], None, []
+    for l in text.split("\n"):
+        if l.startswith(">"):
+            if name is not None:
+                recs.append((name, "".join(seq)))
+            name, seq = l, []
+        elif name is not None and l:
+            seq.append(l)
+    if name is not None:
+        recs.append((name, "".join(seq)))
+    return recs
+
+
+def test_searcher_fixtures_on_gpu(eng):
+    """tests/test_searcher.mojo:17-200 through the GPU engine: 5 inputs x
+    {striped-local, striped-semi-global} x {TTTT, FFTT}, output compared
+    exactly (line) or case-insensitively (fastx)."""
+    cases = json.load(open(os.path.join(HERE, "golden", "searcher_cases.json")))
+    eng.set_engine("batched")
+    eng.set_tile(32)
+    for c in cases["cases"]:
+        kind = c["matrix"]
+        recs = _records(cases["inputs"][c["input"]], c["record_type"])
+        q, ms = ish_amd.encode_and_score(kind, c["pattern"])
+        dec = ish_amd.decoding(kind)
+        targets = Targets.from_list([ish_amd.encode_and_score(kind, seq.encode())[0] for _, seq in recs])
+        for algo in ("striped-local", "striped-semi-global"):
+            for ends in ("TTTT", "FFTT"):
+                eng.set_query(q, kind, 3, 1, ends, algo)
+                s, er, eq, _ = eng.align(targets)
+                emit = ish_amd.emit(algo, s, er, eq, ms, 0.8)
+                out = []
+                for k, (name, seq) in enumerate(recs):
+                    if emit[k]:
+                        if c["record_type"] == "line":
+                            out.append(name + "\n")
+                        else:
+                            enc = ish_amd.encode_and_score(kind, seq.encode())[0]
+                            out.append(name + "\n" + bytes(dec[enc]).decode() + "\n")
+                got = "".join(out)
+                exp = c["expected"]
+                assert (got == exp) if c["record_type"] == "line" else (got.upper() == exp.upper()), (kind, algo, ends)
+
+
+# ---- start coordinates (reverse pass on the GPU) ---------------------------
+def test_starts_known_answers(eng):
+    eng.set_engine("batched")
+    eng.set_tile(32)
+    for c in VEC["semi_global"]:
+        if "target_start" not in c:
+            continue
+        m = O.Matrix(c["matrix"])
+        q, t = m.encode(c["query"]), m.encode(c["target"])
+        eng.set_query(q, c["matrix"], c["go"], c["ge"], c["flags"])
+        db = eng.load_db(Targets.from_list([t]))
+        s, er, eq, _ = eng.align_db(db)
+        ts, qs = eng.find_starts(db, [0], er, eq)
+        assert (ts[0], er[0]) == (c["target_start"], c["target_end"]), c
+        db.close()
+    for c in VEC["local"]:
+        m = O.Matrix(c["matrix"])
+        q, t = m.encode(c["read"]), m.encode(c["reference"])
+        eng.set_query(q, c["matrix"], c["go"], c["ge"], "FFFF", "striped-local")
+        db = eng.load_db(Targets.from_list([t]))
+        s, er, eq, _ = eng.align_db(db)
+        ts, qs = eng.find_starts(db, [0], er, eq)
+        assert (ts[0], qs[0]) == (c["ref_begin"], c["read_begin"]), c
+        db.close()
+
+
+@pytest.mark.parametrize("flags", range(16))
+def test_semi_global_starts_random(eng, flags):
+    rng = np.random.default_rng(300 + flags)
+    m = O.Matrix("blosum62")
+    q = rng.integers(0, 20, 90).astype(np.uint8)
+    seqs = []
+    for t in range(200):
+        R = int(rng.integers(1, 400))
+        r = rng.integers(0, 20, R).astype(np.uint8)
+        if t % 3 == 0:
+            k = min(len(q), R)
+            off = int(rng.integers(0, R - k + 1))
+            r[off:off + k] = q[:k]
+        seqs.append(r)
+    T = Targets.from_list(seqs)
+    eng.set_engine("batched")
+    eng.set_tile(32)
+    eng.set_query(q, "blosum62", 3, 1, flags)
+    db = eng.load_db(T)
+    s, er, eq, _ = eng.align_db(db)
+    sel = np.arange(T.n)
+    ts, qs = eng.find_starts(db, sel, er, eq)
+    for k in sel:
+        se = O.sg_start_end(q, seqs[k], m, 3, 1, flags)
+        assert (ts[k], qs[k]) == (se["target_start"], se["query_start"]), (k, flags)
+    db.close()
+
+
+def test_local_starts_random(eng):
+    rng = np.random.default_rng(77)
+    m = O.Matrix("actgn")
+    q = rng.integers(0, 5, 120).astype(np.uint8)
+    seqs = []
+    for t in range(300):
+        R = int(rng.integers(1, 900))
+        r = rng.integers(0, 5, R).astype(np.uint8)
+        if t % 2 == 0:
+            k = min(len(q) - 10, R)
+            off = int(rng.integers(0, R - k + 1))
+            r[off:off + k] = q[5:5 + k]
+        seqs.append(r)
+    T = Targets.from_list(seqs)
+    eng.set_engine("batched")
+    eng.set_query(q, "actgn", 3, 1, "FFFF", "striped-local")
+    db = eng.load_db(T)
+    s, er, eq, _ = eng.align_db(db)
+    sel = np.arange(T.n)
+    ts, qs = eng.find_starts(db, sel, er, eq)
+    for k in sel:
+        a = O.sw_align(q, seqs[k], m, 3, 1, cutoff=-1.0)
+        assert a is not None
+        assert (ts[k], qs[k]) == (a["ref_begin1"], a["read_begin1"]), k
+    db.close()
+
+
+# ---- many queries in one launch --------------------------------------------
+@pytest.mark.parametrize("flags", ["FFTT", "TTTT", "FFFF", "TFTF"])
+def test_multi_query(eng, flags):
+    rng = np.random.default_rng(zlib.crc32(flags.encode()))
+    m = O.Matrix("blosum62")
+    seqs = [rng.integers(0, 24, int(rng.integers(0, 700))).astype(np.uint8) for _ in range(1500)]
+    seqs += [rng.integers(0, 24, L).astype(np.uint8) for L in (15000, 12000)]  # int32 for the longest query
+    T = Targets.from_list(seqs)
+    queries = [rng.integers(0, 20, L).astype(np.uint8) for L in (1, 7, 33, 64, 100, 257, 350, 600, 1000)]
+    queries[5][:50] = seqs[3][:50] if len(seqs[3]) >= 50 else queries[5][:50]
+    eng.set_engine("batched")
+    eng.set_tile(32)
+    db = eng.load_db(T)
+    s, er, eq, st = eng.align_db_multi(db, queries, "blosum62", 3, 2, flags)
+    fl = O.flags_from_str(flags)
+    for k, q in enumerate(queries):
+        os_, oer, oeq, _ = O.batch(0, q, T.residues, T.offsets, m, 3, 2, fl)
+        assert np.array_equal(s[k], os_) and np.array_equal(er[k], oer) and np.array_equal(eq[k], oeq), (k, len(q))
+    assert (st[-1, -2:] & ish_amd.STATUS_WIDENED).all()
+    db.close()
