@@ -49,6 +49,7 @@ extern "C" {
 #define ISH_E_NO_QUERY (-4)  /* ish_hip_set_query not called */
 #define ISH_E_RANGE (-5)     /* size beyond what the engine supports */
 #define ISH_E_NO_DEVICE (-6) /* no gfx950 device */
+#define ISH_E_TIMEOUT (-7)   /* ish_hip_server_wait: not done in time */
 
 /* matrix kinds, MatrixKind (ishlib/matcher/alignment/scoring_matrix.mojo:
  * 176-254) */
@@ -210,6 +211,93 @@ int32_t ish_hip_emit(int32_t algo, const int32_t *score,
                      const int32_t *end_ref, const int32_t *end_query,
                      int64_t n, int64_t max_score, float threshold,
                      uint8_t *emit);
+
+/* ------------------------------------------------------------------------
+ * Serving loop: a device-resident database answers semi-global queries that
+ * arrive over time, with continuous batching. No counterpart in the
+ * reference, which scores one pattern per run (do_search.mojo:24-274);
+ * per query it computes what StripedSemiGlobalMatcher.first_match computes
+ * for every target (striped_semi_global_matcher.mojo:77-107) and returns
+ * the emitted targets.
+ *
+ * A worker thread owns `slots` launch slots. Whenever a slot is free and
+ * requests are waiting, it takes them FIFO (at most max_batch queries and,
+ * beyond the first, at most batch_cells cells) and enqueues one multi-query
+ * pass plus the --score emission on the GPU; a batch never waits for the
+ * batches already in flight. A request's results do not depend on the
+ * other requests of its batch.
+ *
+ * The database must stay loaded until ish_hip_server_destroy returns.
+ * Submit / wait / hits / release may be called from any thread.
+ * ------------------------------------------------------------------------ */
+typedef struct ish_hip_server ish_hip_server;
+
+typedef struct ish_hip_server_config {
+    int32_t matrix_kind;      /* ISH_MATRIX_* */
+    const int8_t *matrix;     /* alphabet*alphabet, NULL = built-in */
+    int32_t alphabet;
+    int32_t gap_open;         /* absolute values, as ish_hip_set_query */
+    int32_t gap_extend;
+    float threshold;          /* --score (float32 rule of ish_hip_emit) */
+    int32_t max_batch;        /* queries per batch, 1..256 (0 = 16) */
+    int32_t slots;            /* batches in flight, 1..4 (0 = 2) */
+    int64_t batch_cells;      /* cells per batch beyond its first query
+                                 (0 = 4e11) */
+    int64_t hits_per_query;   /* device hit-list capacity per query; a query
+                                 with more hits is emitted on the host from
+                                 its full results (0 = 65536) */
+    int32_t keep_scores;      /* 1: keep every request's full per-target
+                                 score / end_ref / end_query */
+} ish_hip_server_config;
+
+typedef struct ish_hip_request_info {
+    int64_t n_hits;           /* emitted targets */
+    int64_t cells;            /* qlen * residues in the database */
+    double t_submit;          /* ish_hip_server_clock() times (seconds) */
+    double t_launch;
+    double t_done;
+    int32_t batch_id;
+    int32_t batch_size;       /* queries in the request's batch */
+    int32_t status;           /* 0 or ISH_E_* */
+} ish_hip_request_info;
+
+typedef struct ish_hip_server_stats {
+    int64_t batches;
+    int64_t queries;          /* completed */
+    int64_t cells;            /* of completed queries */
+    double kernel_ms;         /* scoring kernels, HIP events, summed */
+    int64_t pending;          /* waiting for a slot */
+    int64_t busy_slots;
+} ish_hip_server_stats;
+
+int32_t ish_hip_server_create(ish_hip_db *db, const ish_hip_server_config *cfg,
+                              ish_hip_server **out);
+/* Finishes every submitted request, then frees the server. */
+void ish_hip_server_destroy(ish_hip_server *srv);
+const char *ish_hip_server_last_error(ish_hip_server *srv);
+/* Monotonic clock of the request timestamps, in seconds. */
+double ish_hip_server_clock(void);
+/* Queues one query (encoded codes, free ends as ISH_FREE_* bits). Rejects a
+ * query whose profile does not fit the LDS with ISH_E_RANGE. */
+int32_t ish_hip_server_submit(ish_hip_server *srv, const uint8_t *enc_query,
+                              int32_t qlen, uint32_t ends_free,
+                              int64_t *ticket);
+/* Waits up to timeout_ms (< 0: no limit) for a request; returns 0, the
+ * request's error code, or ISH_E_TIMEOUT. info may be NULL. */
+int32_t ish_hip_server_wait(ish_hip_server *srv, int64_t ticket,
+                            int32_t timeout_ms, ish_hip_request_info *info);
+/* Copies up to cap emitted targets, ascending by target index (original DB
+ * order), with score and 0-based end coordinates. NULL arrays are skipped. */
+int32_t ish_hip_server_hits(ish_hip_server *srv, int64_t ticket,
+                            int64_t *target, int32_t *score, int32_t *end_ref,
+                            int32_t *end_query, int64_t cap);
+/* Full per-target results (n_targets each); keep_scores servers only. */
+int32_t ish_hip_server_scores(ish_hip_server *srv, int64_t ticket,
+                              int32_t *score, int32_t *end_ref,
+                              int32_t *end_query);
+/* Forgets a finished request. */
+int32_t ish_hip_server_release(ish_hip_server *srv, int64_t ticket);
+int32_t ish_hip_server_get_stats(ish_hip_server *srv, ish_hip_server_stats *out);
 
 #ifdef __cplusplus
 }

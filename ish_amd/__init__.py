@@ -17,7 +17,8 @@ LIB_DIR = os.path.join(HERE, "lib")
 
 ISH_OK = 0
 ERRORS = {-1: "ISH_E_INVALID", -2: "ISH_E_HIP", -3: "ISH_E_NOMEM", -4: "ISH_E_NO_QUERY",
-          -5: "ISH_E_RANGE", -6: "ISH_E_NO_DEVICE"}
+          -5: "ISH_E_RANGE", -6: "ISH_E_NO_DEVICE", -7: "ISH_E_TIMEOUT"}
+ISH_E_TIMEOUT = -7
 MATRIX = {"ascii": 0, "actgn": 1, "actgn0": 2, "blosum62": 3}
 ALGO = {"striped-semi-global": 0, "semi-global": 0, "striped-local": 1, "local": 1}
 ENGINE = {"batched": 0, "plain": 1}
@@ -32,7 +33,32 @@ ABI_SYMBOLS = [
     "ish_hip_set_tile", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
+    "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
+    "ish_hip_server_clock", "ish_hip_server_submit", "ish_hip_server_wait", "ish_hip_server_hits",
+    "ish_hip_server_scores", "ish_hip_server_release", "ish_hip_server_get_stats",
 ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("matrix_kind", ctypes.c_int32), ("matrix", ctypes.POINTER(ctypes.c_int8)),
+                ("alphabet", ctypes.c_int32), ("gap_open", ctypes.c_int32),
+                ("gap_extend", ctypes.c_int32), ("threshold", ctypes.c_float),
+                ("max_batch", ctypes.c_int32), ("slots", ctypes.c_int32),
+                ("batch_cells", ctypes.c_int64), ("hits_per_query", ctypes.c_int64),
+                ("keep_scores", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("n_hits", ctypes.c_int64), ("cells", ctypes.c_int64), ("t_submit", ctypes.c_double),
+                ("t_launch", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("batch_id", ctypes.c_int32), ("batch_size", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_int64), ("queries", ctypes.c_int64), ("cells", ctypes.c_int64),
+                ("kernel_ms", ctypes.c_double), ("pending", ctypes.c_int64),
+                ("busy_slots", ctypes.c_int64)]
 
 
 class IshError(RuntimeError):
@@ -84,6 +110,16 @@ def lib():
         "ish_hip_find_starts": ([vp, vp, P(i64), i32p, i32p, i64, i32p, i32p], i32),
         "ish_hip_align_db_multi": ([vp, vp, u8p, u64p, i32, i8p, i32, i32, i32, i32, u32,
                                     i32p, i32p, i32p, u8p], i32),
+        "ish_hip_server_create": ([vp, P(ServerConfig), P(vp)], i32),
+        "ish_hip_server_destroy": ([vp], None),
+        "ish_hip_server_last_error": ([vp], ctypes.c_char_p),
+        "ish_hip_server_clock": ([], ctypes.c_double),
+        "ish_hip_server_submit": ([vp, u8p, i32, u32, P(i64)], i32),
+        "ish_hip_server_wait": ([vp, i64, i32, P(RequestInfo)], i32),
+        "ish_hip_server_hits": ([vp, i64, P(i64), i32p, i32p, i32p, i64], i32),
+        "ish_hip_server_scores": ([vp, i64, i32p, i32p, i32p], i32),
+        "ish_hip_server_release": ([vp, i64], i32),
+        "ish_hip_server_get_stats": ([vp, P(ServerStats)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -302,6 +338,94 @@ class Database:
     def close(self):
         if getattr(self, "_h", None):
             self._engine._L.ish_hip_db_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Serving loop over a device-resident database (semi-global): queries
+    are submitted as they arrive and batched onto the GPU continuously.
+    submit() returns a ticket; result() waits and returns the emitted
+    targets (ascending target index) with scores and end coordinates."""
+
+    def __init__(self, db: Database, matrix_kind: str, gap_open: int = 3, gap_extend: int = 1,
+                 threshold: float = 0.8, max_batch: int = 16, slots: int = 2,
+                 batch_cells: int = 0, hits_per_query: int = 0, keep_scores: bool = False):
+        self._L = lib()
+        self._db = db  # keeps the database alive while the server runs
+        cfg = ServerConfig(MATRIX[matrix_kind], None, 0, gap_open, gap_extend, threshold,
+                           max_batch, slots, batch_cells, hits_per_query, int(keep_scores))
+        h = ctypes.c_void_p()
+        rc = self._L.ish_hip_server_create(db._h, ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise IshError(f"ish_hip_server_create failed: {ERRORS.get(rc, rc)}")
+        self._h = h
+        self.n = db.n
+
+    @staticmethod
+    def clock() -> float:
+        return float(lib().ish_hip_server_clock())
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._L.ish_hip_server_last_error(self._h)
+            raise IshError(f"{ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def submit(self, query, ends_free="FFTT") -> int:
+        q = np.ascontiguousarray(query, np.uint8)
+        qb = q if len(q) else np.zeros(1, np.uint8)
+        fl = flags_from_str(ends_free) if isinstance(ends_free, str) else int(ends_free)
+        t = ctypes.c_int64()
+        self._check(self._L.ish_hip_server_submit(self._h, _p(qb, ctypes.c_uint8), len(q), fl,
+                                                  ctypes.byref(t)))
+        return int(t.value)
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """Request info as a dict, or None on timeout."""
+        info = RequestInfo()
+        rc = self._L.ish_hip_server_wait(self._h, ticket, timeout_ms, ctypes.byref(info))
+        if rc == ISH_E_TIMEOUT:
+            return None
+        self._check(rc)
+        return {k: getattr(info, k) for k, _ in RequestInfo._fields_}
+
+    def hits(self, ticket: int, n_hits: int):
+        m = max(n_hits, 1)
+        t = np.zeros(m, np.int64)
+        a = [np.zeros(m, np.int32) for _ in range(3)]
+        self._check(self._L.ish_hip_server_hits(self._h, ticket, _p(t, ctypes.c_int64),
+                                                _p(a[0], ctypes.c_int32), _p(a[1], ctypes.c_int32),
+                                                _p(a[2], ctypes.c_int32), n_hits))
+        return t[:n_hits], a[0][:n_hits], a[1][:n_hits], a[2][:n_hits]
+
+    def scores(self, ticket: int):
+        n = max(self.n, 1)
+        a = [np.zeros(n, np.int32) for _ in range(3)]
+        self._check(self._L.ish_hip_server_scores(self._h, ticket, _p(a[0], ctypes.c_int32),
+                                                  _p(a[1], ctypes.c_int32), _p(a[2], ctypes.c_int32)))
+        return tuple(x[: self.n] for x in a)
+
+    def release(self, ticket: int):
+        self._check(self._L.ish_hip_server_release(self._h, ticket))
+
+    def result(self, ticket: int, release: bool = True):
+        """Waits for a request; returns (info, (target, score, end_ref, end_query))."""
+        info = self.wait(ticket)
+        hits = self.hits(ticket, info["n_hits"])
+        if release:
+            self.release(ticket)
+        return info, hits
+
+    def stats(self):
+        st = ServerStats()
+        self._check(self._L.ish_hip_server_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ish_hip_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -30,8 +30,10 @@ struct QueryDesc {
     int32_t prof_words;
     int32_t Q, nchunks, neg_limit;
     int32_t g_begin, g_end; // groups of this launch for this query
+    uint32_t flags;         // free ends of this query: qs|qe<<1|ts<<2|te<<3
+    int32_t reserved;
     int64_t tile_base;      // first work tile of this query (prefix sum)
-    int64_t out_off;        // q * n_targets
+    int64_t out_off;        // offset of this query's results in the outputs
 };
 
 // Everything a semi-global / local scoring launch needs.
@@ -112,9 +114,30 @@ hipError_t launch_pack(const uint8_t *raw, const uint64_t *off,
 hipError_t launch_gather_reversed(const uint8_t *raw, const uint64_t *src_off,
                                   const int64_t *sel, const uint64_t *dst_off,
                                   int64_t n_sel, uint8_t *out, hipStream_t st);
+// Multi-query launch. p.flags selects the kernel variant: its te / qe bits
+// must be the union over the launch's queries, and either every query has
+// te or qe free or none has (the per-query bits come from QueryDesc.flags).
 hipError_t launch_sg_multi(const ScoreParams &p, bool packed, int chunk, int grid,
                            int block, size_t lds, hipStream_t st);
 int occupancy_sg_multi(bool packed, int chunk, int block, size_t lds, uint32_t flags);
+// Emission at the --score threshold for the results of nq queries (query
+// k's results at k * n): hit (target, score, end_ref, end_query) records go
+// to hits[k * cap ...] and count[k] counts every emitted target (it may
+// exceed cap; the caller then falls back to the full arrays).
+struct EmitParams {
+    const int32_t *score, *end_ref, *end_q;
+    int64_t n;
+    int32_t nq;
+    int32_t algo;
+    const float *max_score; // per query, float32(max_score)
+    float threshold;
+    int4 *hits;
+    int64_t cap;
+    uint32_t *count;
+};
+hipError_t launch_emit_hits(const EmitParams &p, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
+
 // Occupancy helpers: blocks per CU of each kernel at the given LDS size.
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags);
 int occupancy_sg_batched_i32(int chunk, int block, size_t lds, uint32_t flags);

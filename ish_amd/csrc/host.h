@@ -1,0 +1,127 @@
+// Host-side internals shared by the C ABI (ish_hip.cpp) and the serving
+// loop (server.cpp). Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace ish {
+
+constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
+constexpr int kBlock = 256;              // 4 waves per workgroup
+
+// Grow-only device buffer (power-of-two capacity).
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t c = 1;
+        while (c < bytes) c <<= 1;
+        if (c < 256) c = 256;
+        hipError_t e = hipMalloc(&p, c);
+        if (e == hipSuccess) cap = c;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Query profile in the LDS layout the kernels read: one row of `stride`
+// int16 scores per group of residue codes whose score column over the query
+// is identical (for the ascii matrix this collapses 256 codes to the
+// distinct query bytes plus one "absent" row); lut maps a code to the dword
+// offset of its row.
+struct Profile {
+    int32_t stride = 0, nrows = 0, qpad = 0;
+    std::vector<int16_t> fwd, rev, plain; // offset form, reversed query, S
+    uint16_t lut[256];
+    size_t lds() const { return (size_t)nrows * stride * 2 + 512; }
+    int32_t words() const { return nrows * stride / 2; }
+};
+
+void build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
+                   int32_t ge, bool want_rev_plain, Profile &pf);
+// Alphabet size of a built-in matrix kind (-1 when unknown).
+int32_t alphabet_of(int32_t kind);
+// The A*A matrix to use: the caller's, or the built-in one of `kind` when
+// matrix is null. Returns 0 or an ISH_E_* code with a message in err.
+int32_t matrix_of(const int8_t *matrix, int32_t &A, int32_t kind, std::vector<int8_t> &mat,
+                  std::string &err);
+// Longest target the packed int16 semi-global kernels may score for a
+// query (-1: none).
+int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s);
+// int16 floor seeding the last-row maximum.
+int32_t neg_limit_of(int32_t go, int32_t mn);
+// Grid of a persistent launch (occ_blocks per CU on `cus` CUs) and the
+// per-wave boundary scratch for groups of up to `cols` columns; grows
+// `scratch` and may shrink the grid to stay within the memory budget.
+hipError_t persistent_grid(DevBuf &scratch, int occ_blocks, int cus, int64_t cols, int *grid,
+                           int64_t *slot_stride);
+
+} // namespace ish
+
+// Device-resident target database (ish_hip_db of the C ABI).
+struct ish_hip_db {
+    int device = 0;
+    int64_t n = 0, nres = 0;
+    int32_t ngroups = 0, maxcols = 0;
+    std::vector<int32_t> group_maxlen; // per group, descending
+    ish::GroupDesc *d_groups = nullptr;
+    uint2 *d_res = nullptr;
+    int32_t *d_lens = nullptr, *d_orig = nullptr;
+    int32_t *d_score = nullptr, *d_end_ref = nullptr, *d_end_q = nullptr;
+    uint8_t *d_status = nullptr;
+    // raw residues (offsets rebased to 0) kept for start-coordinate passes
+    uint8_t *d_raw = nullptr;
+    uint64_t *d_off = nullptr;
+    std::vector<uint64_t> h_off;
+};
+
+namespace ish {
+
+// One query of a multi-query pass over a database.
+struct MultiItem {
+    const Profile *pf;
+    int32_t Q;
+    uint32_t flags;  // free ends
+    int64_t out_off; // offset of its results in the output arrays
+};
+
+// Buffers a multi-query pass uses; the host vectors must stay alive until
+// the stream has consumed the copies.
+struct MultiBuffers {
+    DevBuf *blob = nullptr;    // profiles + luts
+    DevBuf *desc = nullptr;    // QueryDesc tables
+    DevBuf *scratch = nullptr; // boundary rows
+    uint32_t *counter = nullptr;
+    std::vector<uint8_t> *host_blob = nullptr;
+    std::vector<QueryDesc> *host_desc = nullptr;
+};
+
+// Enqueues on `st` every launch that scores the items against db
+// (semi-global): int32 groups first, then packed int16, one persistent
+// launch per (cell width, end-rule variant) that has work. `on_launch` is
+// called with true before and false after each kernel launch (timing).
+// Returns 0 or an ISH_E_* code with a message in err.
+int32_t enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &items, int32_t go,
+                      int32_t ge, int32_t mn, int32_t mx, int chunk, int cus, int32_t *score,
+                      int32_t *end_ref, int32_t *end_q, uint8_t *status, MultiBuffers &mb,
+                      hipStream_t st, std::string &err,
+                      const std::function<hipError_t(bool)> &on_launch);
+
+} // namespace ish

@@ -13,56 +13,10 @@
 #include <vector>
 
 #include "engine.h"
+#include "host.h"
 #include "tables.h"
 
 using namespace ish;
-
-namespace {
-
-constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
-constexpr int kBlock = 256;              // 4 waves per workgroup
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap && p) return hipSuccess;
-        if (p) {
-            hipFree(p);
-            p = nullptr;
-            cap = 0;
-        }
-        size_t c = 1;
-        while (c < bytes) c <<= 1;
-        if (c < 256) c = 256;
-        hipError_t e = hipMalloc(&p, c);
-        if (e == hipSuccess) cap = c;
-        return e;
-    }
-    void release() {
-        if (p) hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-} // namespace
-
-struct ish_hip_db {
-    int device = 0;
-    int64_t n = 0, nres = 0;
-    int32_t ngroups = 0, maxcols = 0;
-    std::vector<int32_t> group_maxlen; // per group, descending
-    GroupDesc *d_groups = nullptr;
-    uint2 *d_res = nullptr;
-    int32_t *d_lens = nullptr, *d_orig = nullptr;
-    int32_t *d_score = nullptr, *d_end_ref = nullptr, *d_end_q = nullptr;
-    uint8_t *d_status = nullptr;
-    // raw residues (offsets rebased to 0) kept for start-coordinate passes
-    uint8_t *d_raw = nullptr;
-    uint64_t *d_off = nullptr;
-    std::vector<uint64_t> h_off;
-};
 
 struct ish_hip_ctx {
     int device = 0;
@@ -125,7 +79,7 @@ extern "C" int32_t ish_hip_device_count(void) {
     return ok;
 }
 
-static int32_t alphabet_of(int32_t kind) {
+int32_t ish::alphabet_of(int32_t kind) {
     switch (kind) {
     case ISH_MATRIX_ASCII: return 256;
     case ISH_MATRIX_ACTGN:
@@ -229,14 +183,14 @@ extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
 
 extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    for (auto e : ctx->ev) hipEventDestroy(e);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto e : ctx->ev) (void)hipEventDestroy(e);
     for (DevBuf *b : {&ctx->prof_t, &ctx->prof_rt, &ctx->prof_p, &ctx->lut, &ctx->scratch, &ctx->counter,
                       &ctx->in_res, &ctx->in_off, &ctx->out_score, &ctx->out_er, &ctx->out_eq,
                       &ctx->out_st})
         b->release();
-    hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -268,7 +222,7 @@ extern "C" int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine) {
 //   H(j,i) >= -2go - ge*(i+j)               (gap from the boundary)
 //   H^ = H + (i+j)*ge in [-2go, Q*M + (R+Q)*ge]
 // plus one profile term (<= M + 2ge) and one gap term (>= -go - m).
-static int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s) {
+int64_t ish::int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s) {
     const int64_t M = std::max(max_s, 0), m = std::max(-min_s, 0);
     const int64_t lim = 32767 - 16;
     if (3LL * go + 2LL * ge + m > lim) return -1;
@@ -297,21 +251,8 @@ static bool local_fits_int16(int32_t qpad, int32_t go, int32_t ge, int32_t max_s
     return up <= lim && lo <= lim && qpad <= 65534;
 }
 
-// Query profile in the LDS layout the kernels read: one row of `stride`
-// int16 scores per group of residue codes whose score column over the query
-// is identical (for the ascii matrix this collapses 256 codes to the
-// distinct query bytes plus one "absent" row); lut maps a code to the dword
-// offset of its row.
-struct Profile {
-    int32_t stride = 0, nrows = 0, qpad = 0;
-    std::vector<int16_t> fwd, rev, plain; // offset form, reversed query, S
-    uint16_t lut[256];
-    size_t lds() const { return (size_t)nrows * stride * 2 + 512; }
-    int32_t words() const { return nrows * stride / 2; }
-};
-
-static void build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
-                          int32_t ge, bool want_rev_plain, Profile &pf) {
+void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
+                        int32_t ge, bool want_rev_plain, Profile &pf) {
     std::map<std::vector<int8_t>, int> row_of;
     std::vector<std::vector<int8_t>> rows;
     for (int c = 0; c < 256; ++c) {
@@ -350,23 +291,37 @@ static void build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t>
         }
 }
 
-static int32_t resolve_matrix(ish_hip_ctx *ctx, const int8_t *matrix, int32_t &A, int32_t kind,
-                              std::vector<int8_t> &mat) {
+int32_t ish::matrix_of(const int8_t *matrix, int32_t &A, int32_t kind, std::vector<int8_t> &mat,
+                       std::string &err) {
     if (!matrix) {
         A = alphabet_of(kind);
-        if (A < 0) return fail(ctx, ISH_E_INVALID, "unknown matrix kind %d", kind);
+        if (A < 0) {
+            err = "unknown matrix kind " + std::to_string(kind);
+            return ISH_E_INVALID;
+        }
         mat.resize((size_t)A * A);
         ish_hip_matrix(kind, mat.data(), A * A);
     } else {
-        if (A <= 0 || A > 256) return fail(ctx, ISH_E_INVALID, "alphabet must be 1..256");
+        if (A <= 0 || A > 256) {
+            err = "alphabet must be 1..256";
+            return ISH_E_INVALID;
+        }
         mat.assign(matrix, matrix + (size_t)A * A);
     }
     return ISH_OK;
 }
 
+static int32_t resolve_matrix(ish_hip_ctx *ctx, const int8_t *matrix, int32_t &A, int32_t kind,
+                              std::vector<int8_t> &mat) {
+    std::string err;
+    const int32_t rc = matrix_of(matrix, A, kind, mat, err);
+    if (rc) return fail(ctx, rc, "%s", err.c_str());
+    return ISH_OK;
+}
+
 // int16 floor seeding the last-row maximum (striped.mojo:329-334), from the
 // matrix minimum seeded at 0 (:62-71).
-static int32_t neg_limit_of(int32_t go, int32_t mn) {
+int32_t ish::neg_limit_of(int32_t go, int32_t mn) {
     const int32_t nl = (-go < mn) ? -32768 + go + 1 : -32768 - mn + 1;
     return (int32_t)(int16_t)nl;
 }
@@ -474,18 +429,25 @@ extern "C" int32_t ish_hip_sync(ish_hip_ctx *ctx) {
 // Scratch for the boundary rows: one slot of `cols` columns x 64 lanes x 8
 // bytes per resident wave. The grid is capped so that scratch stays within
 // half of the free device memory.
-static int32_t size_grid(ish_hip_ctx *ctx, int occ_blocks, int64_t cols, int *grid_out,
-                         int64_t *slot_stride) {
-    int64_t grid = (int64_t)occ_blocks * ctx->cus;
+hipError_t ish::persistent_grid(DevBuf &scratch, int occ_blocks, int cus, int64_t cols,
+                                int *grid_out, int64_t *slot_stride) {
+    int64_t grid = (int64_t)occ_blocks * cus;
     const int64_t per_block = (int64_t)(kBlock / 64) * std::max<int64_t>(cols, 4) * 64 * 8;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
-    const int64_t budget = std::min<int64_t>((int64_t)(free_b / 2) + (int64_t)ctx->scratch.cap,
+    const int64_t budget = std::min<int64_t>((int64_t)(free_b / 2) + (int64_t)scratch.cap,
                                              (int64_t)64 << 30);
     if (grid * per_block > budget) grid = std::max<int64_t>(1, budget / per_block);
-    HIP_TRY(ctx, ctx->scratch.ensure((size_t)(grid * per_block)));
+    hipError_t e = scratch.ensure((size_t)(grid * per_block));
+    if (e != hipSuccess) return e;
     *grid_out = (int)grid;
     *slot_stride = std::max<int64_t>(cols, 4) * 64;
+    return hipSuccess;
+}
+
+static int32_t size_grid(ish_hip_ctx *ctx, int occ_blocks, int64_t cols, int *grid_out,
+                         int64_t *slot_stride) {
+    HIP_TRY(ctx, persistent_grid(ctx->scratch, occ_blocks, ctx->cus, cols, grid_out, slot_stride));
     return ISH_OK;
 }
 
@@ -658,38 +620,41 @@ static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, 
     p.status = db->d_status;
 
     const bool local = ctx->algo == ISH_ALGO_LOCAL;
+    // size the boundary scratch for both launches before either is enqueued
+    // (growing it between them would free memory the first one uses)
+    int grid32 = 0, grid16 = 0;
+    int64_t ss32 = 0, ss16 = 0;
     if (g16 > 0) {
-        int grid;
-        int64_t ss;
         const int occ = local ? occupancy_sw_batched_i32(ctx->chunk, kBlock, ctx->lds)
                               : occupancy_sg_batched_i32(ctx->chunk, kBlock, ctx->lds, flags);
-        int32_t rc = size_grid(ctx, occ, db->group_maxlen[0] + 3, &grid, &ss);
+        int32_t rc = size_grid(ctx, occ, db->group_maxlen[0] + 3, &grid32, &ss32);
         if (rc) return rc;
-        p.bnd = (uint2 *)ctx->scratch.p;
-        p.slot_stride = ss;
+    }
+    if (g16 < db->ngroups) {
+        const int occ = local ? occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds)
+                              : occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, flags);
+        int32_t rc = size_grid(ctx, occ, db->group_maxlen[g16] + 3, &grid16, &ss16);
+        if (rc) return rc;
+    }
+    p.bnd = (uint2 *)ctx->scratch.p;
+    if (g16 > 0) {
+        p.slot_stride = ss32;
         p.g_begin = 0;
         p.g_end = g16;
         HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
         HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, local ? launch_sw_batched_i32(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream)
-                           : launch_sg_batched_i32(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, local ? launch_sw_batched_i32(p, ctx->chunk, grid32, kBlock, ctx->lds, ctx->stream)
+                           : launch_sg_batched_i32(p, ctx->chunk, grid32, kBlock, ctx->lds, ctx->stream));
         HIP_TRY(ctx, timed_end(ctx));
     }
     if (g16 < db->ngroups) {
-        int grid;
-        int64_t ss;
-        const int occ = local ? occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds)
-                              : occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, flags);
-        int32_t rc = size_grid(ctx, occ, db->group_maxlen[g16] + 3, &grid, &ss);
-        if (rc) return rc;
-        p.bnd = (uint2 *)ctx->scratch.p;
-        p.slot_stride = ss;
+        p.slot_stride = ss16;
         p.g_begin = g16;
         p.g_end = db->ngroups;
         HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
         HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, local ? launch_sw_batched_i16(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream)
-                           : launch_sg_batched_i16(p, ctx->chunk, grid, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, local ? launch_sw_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream)
+                           : launch_sg_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream));
         HIP_TRY(ctx, timed_end(ctx));
     }
     return ISH_OK;
@@ -817,6 +782,143 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
 // ---------------------------------------------------------------------------
 // many queries against one database, one persistent launch per cell width
 // ---------------------------------------------------------------------------
+int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &items, int32_t go,
+                           int32_t ge, int32_t mn, int32_t mx, int chunk, int cus, int32_t *score,
+                           int32_t *end_ref, int32_t *end_q, uint8_t *status, MultiBuffers &mb,
+                           hipStream_t st, std::string &err,
+                           const std::function<hipError_t(bool)> &on_launch) {
+    const size_t nq = items.size();
+    if (nq == 0 || db->n == 0) return ISH_OK;
+    auto hip_fail = [&](hipError_t e, const char *what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP;
+    };
+    // profiles and luts of every query, back to back
+    std::vector<size_t> pofs(nq + 1, 0);
+    size_t lds = 0;
+    for (size_t k = 0; k < nq; ++k) {
+        const Profile &pf = *items[k].pf;
+        lds = std::max(lds, pf.lds());
+        pofs[k + 1] = pofs[k] + ((pf.fwd.size() * 2 + 511) & ~(size_t)255);
+    }
+    if (lds > kLdsLimit) {
+        err = "a query profile needs " + std::to_string(lds) + " bytes of LDS";
+        return ISH_E_RANGE;
+    }
+    const size_t luts_off = pofs[nq];
+    std::vector<uint8_t> &blob = *mb.host_blob;
+    blob.assign(luts_off + nq * 512, 0);
+    for (size_t k = 0; k < nq; ++k) {
+        const Profile &pf = *items[k].pf;
+        memcpy(blob.data() + pofs[k], pf.fwd.data(), pf.fwd.size() * 2);
+        memcpy(blob.data() + luts_off + k * 512, pf.lut, 512);
+    }
+    hipError_t e = mb.blob->ensure(blob.size());
+    if (e == hipSuccess) e = mb.desc->ensure(sizeof(QueryDesc) * (nq + 1) * 4);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(mb.blob->p, blob.data(), blob.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "multi-query setup");
+    // queries whose end rules need the last-row / last-column tracking, and
+    // queries that only need the corner cell, run in different variants
+    std::vector<QueryDesc> &hd = *mb.host_desc;
+    hd.assign((nq + 1) * 4, QueryDesc{});
+    const int waves = kBlock / 64;
+    struct Launch {
+        ScoreParams p;
+        bool packed;
+        int32_t maxcols, ntab;
+    };
+    std::vector<Launch> launches;
+    for (int cls = 0; cls < 2; ++cls) {
+        std::vector<size_t> members;
+        uint32_t vflags = 0;
+        for (size_t k = 0; k < nq; ++k) {
+            const bool track = items[k].flags & 10u;
+            if (track == (cls == 1)) {
+                members.push_back(k);
+                vflags |= items[k].flags & 10u;
+            }
+        }
+        if (members.empty()) continue;
+        for (int pass = 0; pass < 2; ++pass) {
+            const bool packed = pass == 1;
+            const int table = (int)launches.size();
+            QueryDesc *qd = hd.data() + (size_t)table * (nq + 1);
+            int64_t tiles = 0;
+            int32_t maxcols = 0;
+            for (size_t m = 0; m < members.size(); ++m) {
+                const MultiItem &it = items[members[m]];
+                const int64_t r16 = int16_max_len(it.Q, go, ge, mx, mn);
+                int32_t lo = 0, hi = db->ngroups;
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi) / 2;
+                    if ((int64_t)db->group_maxlen[mid] > r16) lo = mid + 1;
+                    else hi = mid;
+                }
+                QueryDesc &d = qd[m];
+                d.prof = (const int16_t *)((uint8_t *)mb.blob->p + pofs[members[m]]);
+                d.lut = (const uint16_t *)((uint8_t *)mb.blob->p + luts_off + members[m] * 512);
+                d.prof_words = it.pf->words();
+                d.Q = it.Q;
+                d.nchunks = (it.Q + chunk - 1) / chunk;
+                d.neg_limit = neg_limit_of(go, mn);
+                d.g_begin = packed ? lo : 0;
+                d.g_end = packed ? db->ngroups : lo;
+                d.flags = it.flags;
+                d.tile_base = tiles;
+                d.out_off = it.out_off;
+                const int64_t witems = (int64_t)(d.g_end - d.g_begin) * (packed ? 1 : 2);
+                tiles += (witems + waves - 1) / waves;
+                if (d.g_end > d.g_begin) maxcols = std::max(maxcols, db->group_maxlen[d.g_begin] + 3);
+            }
+            qd[members.size()].tile_base = tiles;
+            if (tiles == 0) continue;
+            Launch L{};
+            ScoreParams &p = L.p;
+            p.go = go;
+            p.ge = ge;
+            p.flags = vflags;
+            p.groups = db->d_groups;
+            p.res = db->d_res;
+            p.lens = db->d_lens;
+            p.orig = db->d_orig;
+            p.counter = mb.counter;
+            p.score = score;
+            p.end_ref = end_ref;
+            p.end_q = end_q;
+            p.status = status;
+            p.queries = (QueryDesc *)mb.desc->p + (size_t)table * (nq + 1);
+            p.nqueries = (int32_t)members.size();
+            L.packed = packed;
+            L.maxcols = maxcols;
+            L.ntab = (int32_t)members.size() + 1;
+            launches.push_back(L);
+        }
+    }
+    // size the boundary scratch for every launch before the first one is
+    // enqueued (growing it later would free memory a queued launch uses)
+    std::vector<int> grids(launches.size());
+    for (size_t l = 0; l < launches.size(); ++l) {
+        int64_t ss;
+        const int occ = occupancy_sg_multi(launches[l].packed, chunk, kBlock, lds, launches[l].p.flags);
+        e = persistent_grid(*mb.scratch, occ, cus, launches[l].maxcols, &grids[l], &ss);
+        if (e != hipSuccess) return hip_fail(e, "boundary scratch");
+        launches[l].p.slot_stride = ss;
+    }
+    for (size_t l = 0; l < launches.size(); ++l) {
+        ScoreParams &p = launches[l].p;
+        p.bnd = (uint2 *)mb.scratch->p;
+        e = hipMemcpyAsync((void *)p.queries, hd.data() + l * (nq + 1),
+                           sizeof(QueryDesc) * (size_t)launches[l].ntab, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_reset(p.counter, st);
+        if (e == hipSuccess) e = on_launch(true);
+        if (e == hipSuccess) e = launch_sg_multi(p, launches[l].packed, chunk, grids[l], kBlock, lds, st);
+        if (e == hipSuccess) e = on_launch(false);
+        if (e != hipSuccess) return hip_fail(e, "multi-query launch");
+    }
+    return ISH_OK;
+}
+
 extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
                                           const uint8_t *queries, const uint64_t *qoffsets,
                                           int32_t nq, const int8_t *matrix, int32_t A,
@@ -839,11 +941,8 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
         mn = std::min<int32_t>(mn, v);
         mx = std::max<int32_t>(mx, v);
     }
-    const uint32_t flags = ends_free & 15u;
-    // profiles of every query, back to back
     std::vector<Profile> pfs((size_t)nq);
-    std::vector<size_t> pofs((size_t)nq + 1, 0);
-    size_t lds = 0;
+    std::vector<MultiItem> items((size_t)nq);
     for (int32_t k = 0; k < nq; ++k) {
         if (qoffsets[k + 1] < qoffsets[k]) return fail(ctx, ISH_E_INVALID, "query offsets must be non-decreasing");
         const int32_t Q = (int32_t)(qoffsets[k + 1] - qoffsets[k]);
@@ -851,102 +950,38 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
         for (int32_t j = 0; j < Q; ++j)
             if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query %d code %d >= alphabet %d", k, q[j], A);
         build_profile(q, Q, mat, A, ge, false, pfs[(size_t)k]);
-        lds = std::max(lds, pfs[(size_t)k].lds());
-        pofs[(size_t)k + 1] = pofs[(size_t)k] + ((pfs[(size_t)k].fwd.size() * 2 + 511) & ~(size_t)255);
+        items[(size_t)k] = MultiItem{&pfs[(size_t)k], Q, ends_free & 15u, (int64_t)k * n};
     }
-    if (lds > kLdsLimit) return fail(ctx, ISH_E_RANGE, "a query profile needs %zu bytes of LDS", lds);
-    const size_t luts_off = pofs[(size_t)nq];
-    std::vector<uint8_t> blob(luts_off + (size_t)nq * 512, 0);
-    for (int32_t k = 0; k < nq; ++k) {
-        memcpy(blob.data() + pofs[(size_t)k], pfs[(size_t)k].fwd.data(), pfs[(size_t)k].fwd.size() * 2);
-        memcpy(blob.data() + luts_off + (size_t)k * 512, pfs[(size_t)k].lut, 512);
-    }
-    DevBuf d_blob, d_desc, d_out;
-    auto release = [&]() {
-        d_blob.release();
-        d_desc.release();
-        d_out.release();
-    };
     const size_t tot = (size_t)nq * (size_t)n;
-    hipError_t e = d_blob.ensure(blob.size());
-    if (e == hipSuccess) e = d_desc.ensure(sizeof(QueryDesc) * ((size_t)nq + 1) * 2);
-    if (e == hipSuccess) e = d_out.ensure(tot * 13);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_blob.p, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
+    DevBuf d_blob, d_desc, d_out;
+    std::vector<uint8_t> h_blob;
+    std::vector<QueryDesc> h_desc;
+    hipError_t e = d_out.ensure(tot * 13);
     if (e == hipSuccess) e = ctx->counter.ensure(256);
     if (e != hipSuccess) {
-        release();
-        return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "multi-query setup: %s", hipGetErrorString(e));
+        d_out.release();
+        return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "multi-query outputs: %s",
+                    hipGetErrorString(e));
     }
     int32_t *o_score = (int32_t *)d_out.p;
     int32_t *o_er = o_score + tot;
     int32_t *o_eq = o_er + tot;
     uint8_t *o_st = (uint8_t *)(o_eq + tot);
-    const int waves = kBlock / 64;
+    MultiBuffers mb;
+    mb.blob = &d_blob;
+    mb.desc = &d_desc;
+    mb.scratch = &ctx->scratch;
+    mb.counter = (uint32_t *)ctx->counter.p;
+    mb.host_blob = &h_blob;
+    mb.host_desc = &h_desc;
     ctx->launches = 0;
-    // two launches: int32 items (groups beyond each query's int16 bound,
-    // which come first in the length order) and packed int16 items
-    for (int pass = 0; pass < 2 && !rc; ++pass) {
-        const bool packed = pass == 1;
-        std::vector<QueryDesc> qd((size_t)nq + 1);
-        int64_t tiles = 0;
-        int32_t maxcols = 0;
-        for (int32_t k = 0; k < nq; ++k) {
-            const Profile &pf = pfs[(size_t)k];
-            const int32_t Q = (int32_t)(qoffsets[k + 1] - qoffsets[k]);
-            const int64_t r16 = int16_max_len(Q, go, ge, mx, mn);
-            int32_t lo = 0, hi = db->ngroups;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi) / 2;
-                if ((int64_t)db->group_maxlen[mid] > r16) lo = mid + 1;
-                else hi = mid;
-            }
-            QueryDesc &d = qd[(size_t)k];
-            d.prof = (const int16_t *)((uint8_t *)d_blob.p + pofs[(size_t)k]);
-            d.lut = (const uint16_t *)((uint8_t *)d_blob.p + luts_off + (size_t)k * 512);
-            d.prof_words = pf.words();
-            d.Q = Q;
-            d.nchunks = (Q + ctx->chunk - 1) / ctx->chunk;
-            d.neg_limit = neg_limit_of(go, mn);
-            d.g_begin = packed ? lo : 0;
-            d.g_end = packed ? db->ngroups : lo;
-            d.tile_base = tiles;
-            d.out_off = (int64_t)k * n;
-            const int64_t items = (int64_t)(d.g_end - d.g_begin) * (packed ? 1 : 2);
-            tiles += (items + waves - 1) / waves;
-            if (d.g_end > d.g_begin) maxcols = std::max(maxcols, db->group_maxlen[d.g_begin] + 3);
-        }
-        qd[(size_t)nq].tile_base = tiles;
-        if (tiles == 0) continue;
-        QueryDesc *dqd = (QueryDesc *)d_desc.p + (size_t)pass * ((size_t)nq + 1);
-        HIP_TRY(ctx, hipMemcpyAsync(dqd, qd.data(), sizeof(QueryDesc) * ((size_t)nq + 1), hipMemcpyHostToDevice, ctx->stream));
-        ScoreParams p{};
-        p.go = go;
-        p.ge = ge;
-        p.flags = flags;
-        p.groups = db->d_groups;
-        p.res = db->d_res;
-        p.lens = db->d_lens;
-        p.orig = db->d_orig;
-        p.counter = (uint32_t *)ctx->counter.p;
-        p.score = o_score;
-        p.end_ref = o_er;
-        p.end_q = o_eq;
-        p.status = o_st;
-        p.queries = dqd;
-        p.nqueries = nq;
-        int grid;
-        int64_t ss;
-        const int occ = occupancy_sg_multi(packed, ctx->chunk, kBlock, lds, flags);
-        rc = size_grid(ctx, occ, maxcols, &grid, &ss);
-        if (rc) break;
-        p.bnd = (uint2 *)ctx->scratch.p;
-        p.slot_stride = ss;
-        HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
-        HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, launch_sg_multi(p, packed, ctx->chunk, grid, kBlock, lds, ctx->stream));
-        HIP_TRY(ctx, timed_end(ctx));
-    }
-    if (!rc) {
+    std::string err;
+    rc = enqueue_multi(db, items, go, ge, mn, mx, ctx->chunk, ctx->cus, o_score, o_er, o_eq, o_st,
+                       mb, ctx->stream, err,
+                       [ctx](bool begin) { return begin ? timed_begin(ctx) : timed_end(ctx); });
+    if (rc) {
+        fail(ctx, rc, "%s", err.c_str());
+    } else {
         e = hipMemcpyAsync(score, o_score, tot * 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(end_ref, o_er, tot * 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(end_q, o_eq, tot * 4, hipMemcpyDeviceToHost, ctx->stream);
@@ -956,7 +991,9 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
         collect_timing(ctx);
     }
     (void)hipStreamSynchronize(ctx->stream);
-    release();
+    d_blob.release();
+    d_desc.release();
+    d_out.release();
     return rc;
 }
 

@@ -1,0 +1,69 @@
+// Device-side emission for the serving loop: applies the matchers' --score
+// rule to the results of a multi-query pass and compacts the emitted
+// targets into per-query hit lists, so that only hits cross PCIe.
+//
+// The rule is the float32 one of the reference matchers:
+//   semi-global (striped_semi_global_matcher.mojo:77-107):
+//     reject if f32(score) < f32(max_score) * thr, then emit iff
+//     f32(score) / f32(max_score) >= thr;
+//   local (striped_local_matcher.mojo:61-90, local_aln/striped.mojo:213-230):
+//     reject unless f32(score) > thr * f32(max_score), end_ref > 0 and
+//     end_query > 0, then emit iff f32(score) / f32(max_score) >= thr.
+// __fmul_rn / __fdiv_rn give IEEE round-to-nearest products and quotients,
+// the arithmetic of the host rule.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine.h"
+
+namespace ish {
+
+__global__ void __launch_bounds__(256) emit_hits(EmitParams p) {
+    const int k = blockIdx.y;
+    const float ms = p.max_score[k];
+    const float thr = p.threshold;
+    const int64_t base = (int64_t)k * p.n;
+    uint32_t *cnt = p.count + k;
+    int4 *hits = p.hits + (int64_t)k * p.cap;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p.n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t sc = p.score[base + t];
+        const float s = (float)sc;
+        bool ok;
+        if (p.algo == 0) {
+            ok = !(s < __fmul_rn(ms, thr));
+        } else {
+            ok = s > __fmul_rn(thr, ms) && p.end_ref[base + t] > 0 && p.end_q[base + t] > 0;
+        }
+        if (ok) ok = __fdiv_rn(s, ms) >= thr;
+        if (ok) {
+            const uint32_t slot = atomicAdd(cnt, 1u);
+            if ((int64_t)slot < p.cap)
+                hits[slot] = make_int4((int)t, sc, p.end_ref[base + t], p.end_q[base + t]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_emit_hits(const EmitParams &p, hipStream_t st) {
+    if (p.n <= 0 || p.nq <= 0) return hipSuccess;
+    int64_t bx = (p.n + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(emit_hits, dim3((unsigned)bx, (unsigned)p.nq), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int64_t b = (n + 255) / 256;
+    if (b > 1024) b = 1024;
+    hipLaunchKernelGGL(fill_u32, dim3((unsigned)b), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+} // namespace ish

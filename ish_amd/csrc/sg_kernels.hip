@@ -107,11 +107,15 @@ __device__ __forceinline__ void finalize(int R, int Q, bool te, bool qe, bool ha
 // Per-query values a group pass needs (one query, or one of many).
 struct QueryView {
     int Q, nchunks, neg_limit;
+    uint32_t flags;  // free ends of this query (qs|qe<<1|ts<<2|te<<3)
     int64_t out_off; // offset of this query's results in the output arrays
 };
 
 // Scores one group (PACKED: 128 targets, two per lane) or one half group
 // (int32: 64 targets) against the query whose profile is staged in smem.
+// TE / QE say which end rules the variant can apply; the query's own flags
+// select among them at run time (a variant with TE or QE must only see
+// queries that have te or qe free, since it tracks no corner cell).
 template <typename V, int K, bool TE, bool QE>
 __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &qv,
                                          const uint8_t *smem, const uint16_t *slut, uint2 *bnd,
@@ -119,7 +123,8 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
     constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
     const int go = p.go, ge = p.ge, cst = ge - go, Q = qv.Q;
-    const bool qs = p.flags & 1u, ts = p.flags & 4u;
+    const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
+    const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
     const V vcst = splat<V>(cst);
     const GroupDesc gd = p.groups[g];
     const int tA = gd.first + half * 64 + lane;
@@ -269,7 +274,7 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
                         }
                     }
                 }
-                if (QE) {
+                if (qe_rt) {
                     const bool cap_a = i == RA - 1;
                     const bool cap_b = PACKED && i == RB - 1;
                     // wave-uniform test keeps this once-per-target path a
@@ -315,7 +320,7 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
             const int qe_val = (kqe >> 16) - (R - 1) * ge;
             const int qe_j = 0xFFFF - (kqe & 0xFFFF);
             int sc, er, eq;
-            finalize(R, Q, TE, QE, kte != INT_MIN, te_val, te_idx, kqe != INT_MIN, qe_val,
+            finalize(R, Q, te_rt, qe_rt, kte != INT_MIN, te_val, te_idx, kqe != INT_MIN, qe_val,
                      qe_j, cor - qoff, qv.neg_limit, sc, er, eq);
             p.score[qv.out_off + o] = sc;
             p.end_ref[qv.out_off + o] = er;
@@ -326,7 +331,7 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
         const int o = p.orig[tA];
         if (o >= 0) {
             int sc, er, eq;
-            finalize(RA, Q, TE, QE, t32.te_i >= 0, t32.te_v - qoff, t32.te_i,
+            finalize(RA, Q, te_rt, qe_rt, t32.te_i >= 0, t32.te_v - qoff, t32.te_i,
                      t32.qe_v != INT_MIN, t32.qe_v - (RA - 1) * ge, t32.qe_j,
                      t32.corner - qoff, qv.neg_limit, sc, er, eq);
             p.score[qv.out_off + o] = sc;
@@ -352,6 +357,7 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
     qv.Q = p.Q;
     qv.nchunks = p.nchunks;
     qv.neg_limit = p.neg_limit;
+    qv.flags = p.flags;
     qv.out_off = 0;
 
     for (;;) {
@@ -420,6 +426,7 @@ __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
             qv.Q = qd.Q;
             qv.nchunks = qd.nchunks;
             qv.neg_limit = qd.neg_limit;
+            qv.flags = qd.flags;
             qv.out_off = qd.out_off;
             const int g = qd.g_begin + (PACKED ? item : (item >> 1));
             const int half = PACKED ? 0 : (item & 1);

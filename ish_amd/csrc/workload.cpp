@@ -21,6 +21,9 @@
 //    [1000, 50000]; per base 3 % substitution, 3 % insertion, 2 % deletion,
 //    2 % N; 2 % of reads carry the query passed through the same error
 //    model.
+//  5 serving database: protein targets as in 2 but with the length
+//    distribution of 1, R ~ U[100, 300]; 0.1 % carry a homolog of the
+//    350 aa base query (its first R residues).
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
@@ -189,7 +192,7 @@ void gen_target(const Job &jb, int64_t t) {
             plant(r, copy, dst, R);
             jb.planted[t] = 1;
         }
-    } else if (jb.kind == 2) {
+    } else if (jb.kind == 2 || jb.kind == 5) {
         const AATables &T = aa();
         for (int64_t k = 0; k < R; ++k) dst[k] = draw_cdf(r, T.cdf, 20);
         if (r.uniform() < 0.001) {
@@ -237,7 +240,7 @@ extern "C" {
 // Query of a workload (codes). Returns its length.
 int32_t ish_wl_query(int32_t kind, uint64_t seed, int32_t qlen, uint8_t *out) {
     Rng r(seed, 0);
-    if (kind == 2 || kind == 3) {
+    if (kind == 2 || kind == 3 || kind == 5) {
         for (int32_t k = 0; k < qlen; ++k) out[k] = draw_cdf(r, aa().cdf, 20);
     } else {
         for (int32_t k = 0; k < qlen; ++k) out[k] = (uint8_t)r.below(4);
@@ -251,7 +254,7 @@ int64_t ish_wl_lengths(int32_t kind, uint64_t seed, int64_t n, int64_t *lens) {
     for (int64_t t = 0; t < n; ++t) {
         Rng r(seed, (uint64_t)t + 1);
         int64_t L;
-        if (kind == 1 || kind == 3) L = 100 + (int64_t)r.below(201);
+        if (kind == 1 || kind == 3 || kind == 5) L = 100 + (int64_t)r.below(201);
         else if (kind == 2) L = lognormal_len(r, 350.0, 0.6, 30, 5000);
         else L = lognormal_len(r, 8000.0, 0.5, 1000, 50000);
         lens[t] = L;

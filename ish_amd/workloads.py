@@ -16,8 +16,8 @@ from . import LIB_DIR, IshError, Targets
 
 _wl = None
 
-KIND_QUERY_LEN = {1: 150, 2: 350, 4: 1000}
-KIND_MATRIX = {1: "actgn", 2: "blosum62", 3: "blosum62", 4: "actgn0"}
+KIND_QUERY_LEN = {1: 150, 2: 350, 4: 1000, 5: 350}
+KIND_MATRIX = {1: "actgn", 2: "blosum62", 3: "blosum62", 4: "actgn0", 5: "blosum62"}
 
 
 def _lib():
@@ -71,7 +71,8 @@ def query(kind: int, seed: int, qlen: int | None = None) -> np.ndarray:
 
 
 def generate(kind: int, n: int, seed: int = 1, qlen: int | None = None, threads: int = 0) -> Workload:
-    """kind 1: nucleotide lines, 2: protein FASTA, 4: long reads (see workload.cpp)."""
+    """kind 1: nucleotide lines, 2: protein FASTA, 3: nucleotide targets with
+    protein queries, 4: long reads, 5: serving database (see workload.cpp)."""
     L = _lib()
     q = query(kind, seed, qlen)
     lens = np.zeros(max(n, 1), np.int64)
@@ -87,3 +88,30 @@ def generate(kind: int, n: int, seed: int = 1, qlen: int | None = None, threads:
                      res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                      planted.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), threads)
     return Workload(kind, seed, q, Targets(res, off), planted[:n].astype(bool), KIND_MATRIX[kind])
+
+
+@dataclass
+class Request:
+    arrival: float      # seconds after the start of the stream
+    query: np.ndarray   # BLOSUM62 codes at background frequencies
+    ends_free: str      # "FFTT" or "TTTT"
+
+
+def serving_requests(count: int, seed: int = 1, rate: float = 200.0, median: float = 300.0,
+                     sigma: float = 0.5, lo: int = 20, hi: int = 2000,
+                     ttt_fraction: float = 0.5) -> list[Request]:
+    """Open-loop query stream of the serving workload: Poisson arrivals at
+    `rate` per second, ln Q ~ N(ln median, sigma^2) rounded and clipped to
+    [lo, hi], ends-free TTTT with probability ttt_fraction, else FFTT.
+    Query residues come from the workload generator (background
+    frequencies), one stream per request."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    lens = np.clip(np.rint(np.exp(np.log(median) + sigma * rng.standard_normal(count))), lo, hi).astype(int)
+    ttt = rng.random(count) < ttt_fraction
+    out = []
+    for k in range(count):
+        q = query(5, (seed << 20) + k + 1, int(lens[k]))
+        out.append(Request(float(arrivals[k]), q, "TTTT" if ttt[k] else "FFTT"))
+    return out
