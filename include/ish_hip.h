@@ -127,6 +127,17 @@ int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine);
  * 305-306). */
 int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane);
 
+/* How the batched local kernel spreads a group of 128 targets: AUTO picks
+ * WORKGROUP when the longest groups are far longer than the average work
+ * per wave (long reads), else WAVE. WAVE: one wave scores a group through
+ * every query chunk. WORKGROUP: the four waves of a workgroup split the
+ * query chunks of one group and pipeline them through LDS. A tuning knob
+ * only; results never depend on it. No counterpart in the reference. */
+#define ISH_SCHEDULE_AUTO 0
+#define ISH_SCHEDULE_WAVE 1
+#define ISH_SCHEDULE_WORKGROUP 2
+int32_t ish_hip_set_schedule(ish_hip_ctx *ctx, int32_t schedule);
+
 /* One synchronous batch (the batch_match_coarse + copy_outputs_to_host
  * step). Targets are residues[offsets[t] .. offsets[t+1]) for t < n.
  * Outputs (n entries each): score, 0-based end_ref / end_query, status.

@@ -22,6 +22,7 @@ ISH_E_TIMEOUT = -7
 MATRIX = {"ascii": 0, "actgn": 1, "actgn0": 2, "blosum62": 3}
 ALGO = {"striped-semi-global": 0, "semi-global": 0, "striped-local": 1, "local": 1}
 ENGINE = {"batched": 0, "plain": 1}
+SCHEDULE = {"auto": 0, "wave": 1, "workgroup": 2}
 FREE_QUERY_START, FREE_QUERY_END, FREE_TARGET_START, FREE_TARGET_END = 1, 2, 4, 8
 STATUS_I16, STATUS_I32, STATUS_WIDENED = 16, 32, 0x80
 
@@ -30,7 +31,7 @@ ABI_SYMBOLS = [
     "ish_hip_version", "ish_hip_device_count", "ish_hip_create", "ish_hip_destroy",
     "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
-    "ish_hip_set_tile", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
+    "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
@@ -97,6 +98,7 @@ def lib():
         "ish_hip_set_query": ([vp, u8p, i32, i8p, i32, i32, i32, i32, u32, i32], i32),
         "ish_hip_set_engine": ([vp, i32], i32),
         "ish_hip_set_tile": ([vp, i32], i32),
+        "ish_hip_set_schedule": ([vp, i32], i32),
         "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
         "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
         "ish_hip_db_free": ([vp], None),
@@ -241,6 +243,9 @@ class Engine:
 
     def set_tile(self, rows: int):
         self._check(self._L.ish_hip_set_tile(self._h, rows))
+
+    def set_schedule(self, schedule: str):
+        self._check(self._L.ish_hip_set_schedule(self._h, SCHEDULE[schedule]))
 
     def set_query(self, query: np.ndarray, matrix_kind: str, gap_open: int = 3,
                   gap_extend: int = 1, ends_free="FFTT", algo="striped-semi-global"):

@@ -104,6 +104,12 @@ hipError_t launch_sw_batched_i16(const ScoreParams &p, int chunk, int grid,
 hipError_t launch_sw_batched_i32(const ScoreParams &p, int chunk, int grid,
                                  int block, size_t lds, hipStream_t st);
 int occupancy_sw_batched_i16(int chunk, int block, size_t lds);
+// Workgroup-pipelined local kernel (packed int16, 4 waves share a group):
+// extra LDS beyond the profile, launcher (block of 256) and occupancy.
+size_t sw_pipe_extra_lds();
+hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+                              hipStream_t st);
+int occupancy_sw_pipe_i16(int chunk, size_t lds);
 int occupancy_sw_batched_i32(int chunk, int block, size_t lds);
 hipError_t launch_sg_plain(const PlainParams &p, int grid, int block,
                            size_t lds, hipStream_t st);

@@ -34,6 +34,7 @@ struct ish_hip_ctx {
     DevBuf prof_t, prof_rt, prof_p, lut; // forward, reversed-query, plain profiles
     int engine = ISH_ENGINE_BATCHED;
     int chunk = kChunkDefault; // query rows per lane in the batched kernels
+    int schedule = ISH_SCHEDULE_AUTO; // one wave or one workgroup per group
     // work buffers
     DevBuf scratch, counter, in_res, in_off, out_score, out_er, out_eq, out_st;
     // timing
@@ -203,6 +204,14 @@ extern "C" int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lan
     if (query_rows_per_lane != 16 && query_rows_per_lane != 32)
         return fail(ctx, ISH_E_INVALID, "tile height must be 16 or 32 (got %d)", query_rows_per_lane);
     ctx->chunk = query_rows_per_lane;
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_set_schedule(ish_hip_ctx *ctx, int32_t schedule) {
+    if (!ctx) return ISH_E_INVALID;
+    if (schedule != ISH_SCHEDULE_AUTO && schedule != ISH_SCHEDULE_WAVE && schedule != ISH_SCHEDULE_WORKGROUP)
+        return fail(ctx, ISH_E_INVALID, "unknown schedule %d", schedule);
+    ctx->schedule = schedule;
     return ISH_OK;
 }
 
@@ -630,9 +639,27 @@ static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, 
         int32_t rc = size_grid(ctx, occ, db->group_maxlen[0] + 3, &grid32, &ss32);
         if (rc) return rc;
     }
+    // Local alignment over groups much longer than the average work per
+    // wave (long reads): the four waves of a workgroup share each group.
+    bool pipe = false;
+    const size_t pipe_lds = ctx->lds + sw_pipe_extra_lds();
+    if (local && g16 < db->ngroups && pipe_lds <= kLdsLimit && ctx->schedule != ISH_SCHEDULE_WAVE) {
+        const int nch = (ctx->Q + ctx->chunk - 1) / ctx->chunk;
+        const int rounds = (nch + 3) / 4;
+        if (ctx->schedule == ISH_SCHEDULE_WORKGROUP) {
+            pipe = true;
+        } else if (nch >= 4 && 4 * rounds - nch <= rounds) {
+            double cols = 0;
+            for (int32_t g = g16; g < db->ngroups; ++g) cols += db->group_maxlen[g] + 3;
+            const int occ = occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds);
+            const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
+            pipe = db->group_maxlen[g16] > 1.5 * per_wave;
+        }
+    }
     if (g16 < db->ngroups) {
-        const int occ = local ? occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds)
-                              : occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, flags);
+        const int occ = pipe ? occupancy_sw_pipe_i16(ctx->chunk, pipe_lds)
+                        : local ? occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds)
+                                : occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, flags);
         int32_t rc = size_grid(ctx, occ, db->group_maxlen[g16] + 3, &grid16, &ss16);
         if (rc) return rc;
     }
@@ -653,8 +680,9 @@ static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, 
         p.g_end = db->ngroups;
         HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
         HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, local ? launch_sw_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream)
-                           : launch_sg_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream));
+        HIP_TRY(ctx, pipe ? launch_sw_pipe_i16(p, ctx->chunk, grid16, pipe_lds, ctx->stream)
+                     : local ? launch_sw_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream)
+                             : launch_sg_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream));
         HIP_TRY(ctx, timed_end(ctx));
     }
     return ISH_OK;

@@ -265,6 +265,276 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Workgroup-pipelined local kernel (packed int16): the four waves of a
+// workgroup share one group of 128 targets and split its query chunks,
+// wave w taking chunks w, w+4, w+8, ...  Time advances in steps of one
+// 4-column block, closed by a workgroup barrier; at step s wave w works on
+// block s - w - r*P of its round-r chunk (P = max(blocks, 5) steps per
+// round), one block behind the wave that owns the chunk above. The
+// chunk-bottom rows (H^, F^) flow to the next wave through a double-
+// buffered LDS ring, and from wave 3 to wave 0 of the next round through
+// the workgroup's boundary slot in global memory (written P-4 or more
+// steps before it is read, and overwritten 3 steps after). A group then
+// takes a quarter of the time one wave needs, so reads of 50 kb no longer
+// set the length of the launch. Results are the same as sw_batched's: each
+// chunk runs the same per-column code, and the per-chunk (M_c, i_c, r_c)
+// combine is order-free (value, then first column, then smallest row).
+// ---------------------------------------------------------------------------
+constexpr int kPipeWaves = 4;
+constexpr size_t kPipeRingBytes = (size_t)(kPipeWaves - 1) * 2 * 4 * 64 * sizeof(uint2);
+constexpr size_t kPipeCombBytes = (size_t)kPipeWaves * 64 * 6 * sizeof(int);
+
+size_t sw_pipe_extra_lds() { return kPipeRingBytes + kPipeCombBytes + 16; }
+
+template <int K>
+__global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
+    using V = s16x2;
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ int s_item;
+    stage_profile(p.prof, p.lut, p.prof_words, smem);
+    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
+    uint8_t *tail = smem + (((size_t)p.prof_words * 4 + 512 + 15) & ~(size_t)15);
+    uint2 *ring = (uint2 *)tail;                         // [wave 0..2][parity][t][lane]
+    int *comb = (int *)(tail + kPipeRingBytes);          // [wave][lane][6]
+
+    constexpr int NS = K / 8;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    uint2 *gbnd = p.bnd + (size_t)blockIdx.x * p.slot_stride;
+    const int go = p.go, ge = p.ge, cst = ge - go;
+    const V vcst = splat<V>(cst);
+    const V vshift = splat<V>(kRenorm * ge);
+    const int items = p.g_end - p.g_begin;
+    const int nch = p.nchunks;
+    const int rounds = (nch + kPipeWaves - 1) / kPipeWaves;
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads(); // s_item is rewritten at the top of the next group
+        if (item >= items) break;
+        const GroupDesc gd = p.groups[p.g_begin + item];
+        const int tA = gd.first + lane;
+        const int tB = gd.first + 64 + lane;
+        const int RA = p.lens[tA];
+        const int RB = p.lens[tB];
+        const uint2 *res = p.res + gd.res_off;
+        const int nblk = gd.ncols >> 2;
+        const int P = nblk > 5 ? nblk : 5;
+        const int steps = (rounds - 1) * P + nblk + (kPipeWaves - 1);
+
+        int bestA = 0, erA = -1, eqA = 0, bestB = 0, erB = -1, eqB = 0;
+        V H[K], E[K];
+        V diag_top = splat<V>(0), run = splat<V>(0);
+        int icA = -1, rcA = 0, icB = -1, rcB = 0, ibase = 0;
+        uint2 rr_next = make_uint2(0u, 0u);
+        uint2 bn_next[4];
+        short8 sa[NS], sb[NS];
+#pragma unroll
+        for (int m = 0; m < K; ++m) H[m] = E[m] = splat<V>(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) bn_next[t] = make_uint2(0u, 0u);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) sa[k] = sb[k] = (short8)0;
+
+        int b = -w, r = 0; // block and round of this wave at the current step
+        for (int s = 0; s < steps; ++s) {
+            const int c = kPipeWaves * r + w;
+            if (b >= 0 && b < nblk && r < rounds && c < nch) {
+                const int j0 = c * K;
+                const bool first = c == 0;
+                const bool last = c == nch - 1;
+                const int par = (r * P + b) & 1; // ring parity of this (round, block)
+                const uint8_t *chunk_base = smem + 2 * j0;
+                auto row_ptr = [&](uint32_t code) -> const short8 * {
+                    return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
+                };
+                if (b == 0) {
+#pragma unroll
+                    for (int m = 0; m < K; ++m) {
+                        H[m] = splat<V>((j0 + m - 1) * ge);
+                        E[m] = H[m] + vcst;
+                    }
+                    diag_top = splat<V>((j0 - 2) * ge);
+                    run = splat<V>(0);
+                    icA = -1;
+                    rcA = 0;
+                    icB = -1;
+                    rcB = 0;
+                    ibase = 0;
+                    rr_next = res[lane];
+                    if (!first && w == 0) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
+                    }
+                    const short8 *na = row_ptr(rr_next.x & 0xffu);
+                    const short8 *nb = row_ptr(rr_next.y & 0xffu);
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+                        sa[k] = na[k];
+                        sb[k] = nb[k];
+                    }
+                }
+                if (b > 0 && ((b * 4) % kRenorm) == 0) {
+#pragma unroll
+                    for (int m = 0; m < K; ++m) {
+                        H[m] = H[m] - vshift;
+                        E[m] = E[m] - vshift;
+                    }
+                    diag_top = diag_top - vshift;
+                    ibase = b * 4;
+                }
+                const uint2 rr = rr_next;
+                uint2 bn[4];
+                if (!first) {
+                    if (w == 0) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
+                        if (b + 1 < nblk) {
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                bn_next[t] = gbnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
+                        }
+                    } else {
+                        const uint2 *src = ring + (size_t)((w - 1) * 2 + par) * 256;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) bn[t] = src[t * 64 + lane];
+                    }
+                }
+                if (b + 1 < nblk) rr_next = res[(size_t)(b + 1) * 64 + lane];
+                uint2 *dst_ring = ring + (size_t)(w * 2 + par) * 256;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int i = b * 4 + t;
+                    const uint32_t wa = t < 3 ? rr.x : rr_next.x;
+                    const uint32_t wb = t < 3 ? rr.y : rr_next.y;
+                    const int sh = 8 * ((t + 1) & 3);
+                    const short8 *na = row_ptr((wa >> sh) & 0xffu);
+                    const short8 *nb = row_ptr((wb >> sh) & 0xffu);
+                    V htop, f;
+                    if (first) {
+                        const int ht = (i - ibase - 1) * ge;
+                        htop = splat<V>(ht);
+                        f = splat<V>(ht + cst);
+                    } else {
+                        htop = from_u32<V>(bn[t].x);
+                        f = from_u32<V>(bn[t].y);
+                    }
+                    const int zc = (i - ibase + j0) * ge;
+                    V diag = diag_top;
+                    V cms[NS];
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) cms[k] = splat<V>(0);
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+#pragma unroll
+                        for (int mm = 0; mm < 8; ++mm) {
+                            const int m = k * 8 + mm;
+                            V sv;
+                            sv.x = sa[k][mm];
+                            sv.y = sb[k][mm];
+                            const V z = splat<V>(zc + m * ge);
+                            V h = vmax(vmax(vmax(diag + sv, E[m]), f), z);
+                            diag = H[m];
+                            H[m] = h;
+                            const V gg = h + vcst;
+                            E[m] = vmax(E[m], gg);
+                            f = vmax(f, gg);
+                            cms[k] = vmax(cms[k], h - z);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+                        sa[k] = na[k];
+                        sb[k] = nb[k];
+                    }
+                    diag_top = htop;
+                    if (!last) {
+                        uint2 o;
+                        o.x = as_u32(H[K - 1]);
+                        o.y = as_u32(f);
+                        if (w < kPipeWaves - 1) dst_ring[t * 64 + lane] = o;
+                        else gbnd[(size_t)i * 64 + lane] = o;
+                    }
+                    V cm = cms[0];
+#pragma unroll
+                    for (int k = 1; k < NS; ++k) cm = vmax(cm, cms[k]);
+                    const uint32_t mask = (i < RA ? 0x0000FFFFu : 0u) | (i < RB ? 0xFFFF0000u : 0u);
+                    cm = from_u32<V>(as_u32(cm) & mask);
+                    const V nr = vmax(run, cm);
+                    // bestA / bestB cover this wave's earlier chunks only: a
+                    // lower bound of the other chunks' maximum, which is all
+                    // the filter needs
+                    const bool up_a = nr.x > run.x && (int)nr.x >= bestA;
+                    const bool up_b = nr.y > run.y && (int)nr.y >= bestB;
+                    if (__builtin_amdgcn_ballot_w64(up_a || up_b) != 0) {
+                        int ra = K, rb = K;
+#pragma unroll
+                        for (int m = 0; m < K; ++m) {
+                            const V d = H[m] - splat<V>(zc + m * ge);
+                            if (ra == K && d.x == nr.x) ra = m;
+                            if (rb == K && d.y == nr.y) rb = m;
+                        }
+                        if (up_a) {
+                            icA = i;
+                            rcA = j0 + ra;
+                        }
+                        if (up_b) {
+                            icB = i;
+                            rcB = j0 + rb;
+                        }
+                    }
+                    run = nr;
+                }
+                if (b == nblk - 1) {
+                    sw_combine((int)run.x, icA, rcA, bestA, erA, eqA);
+                    sw_combine((int)run.y, icB, rcB, bestB, erB, eqB);
+                }
+            }
+            __syncthreads();
+            if (++b == P) {
+                b = 0;
+                ++r;
+            }
+        }
+        int *cw = comb + (size_t)(w * 64 + lane) * 6;
+        cw[0] = bestA;
+        cw[1] = erA;
+        cw[2] = eqA;
+        cw[3] = bestB;
+        cw[4] = erB;
+        cw[5] = eqB;
+        __syncthreads();
+        if (w == 0) {
+            for (int v = 1; v < kPipeWaves; ++v) {
+                const int *cv = comb + (size_t)(v * 64 + lane) * 6;
+                sw_combine(cv[0], cv[1], cv[2], bestA, erA, eqA);
+                sw_combine(cv[3], cv[4], cv[5], bestB, erB, eqB);
+            }
+            for (int which = 0; which < 2; ++which) {
+                const int t = which ? tB : tA;
+                const int o = p.orig[t];
+                if (o < 0) continue;
+                const int best = which ? bestB : bestA;
+                p.score[o] = best;
+                p.end_ref[o] = best > 0 ? (which ? erB : erA) : -1;
+                p.end_q[o] = best > 0 ? (which ? eqB : eqA) : 0;
+                p.status[o] = 16;
+            }
+        }
+        __syncthreads(); // comb is rewritten by the next group
+    }
+}
+
+hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+                              hipStream_t st) {
+    if (chunk == 16)
+        hipLaunchKernelGGL((sw_pipe<16>), dim3(grid), dim3(64 * kPipeWaves), lds, st, p);
+    else
+        hipLaunchKernelGGL((sw_pipe<32>), dim3(grid), dim3(64 * kPipeWaves), lds, st, p);
+    return hipGetLastError();
+}
+
 hipError_t launch_sw_batched_i16(const ScoreParams &p, int chunk, int grid, int block,
                                  size_t lds, hipStream_t st) {
     if (chunk == 16)
@@ -287,6 +557,11 @@ template <typename KernelT> static int occ_of(KernelT k, int block, size_t lds) 
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, lds) != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
+}
+
+int occupancy_sw_pipe_i16(int chunk, size_t lds) {
+    return chunk == 16 ? occ_of(sw_pipe<16>, 64 * kPipeWaves, lds)
+                       : occ_of(sw_pipe<32>, 64 * kPipeWaves, lds);
 }
 
 int occupancy_sw_batched_i16(int chunk, int block, size_t lds) {

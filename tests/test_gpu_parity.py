@@ -175,18 +175,22 @@ def test_database_reuse_across_queries(eng):
 
 
 # ---- local (Smith-Waterman) ------------------------------------------------
-def check_local(eng, q, targets, kind, go, ge, tile=32, label=""):
+def check_local(eng, q, targets, kind, go, ge, tile=32, label="", schedule="auto"):
     eng.set_engine("batched")
     eng.set_tile(tile)
+    eng.set_schedule(schedule)
     eng.set_query(q, kind, go, ge, "FFFF", "striped-local")
-    s, er, eq, st = eng.align(targets)
+    try:
+        s, er, eq, st = eng.align(targets)
+    finally:
+        eng.set_schedule("auto")
     m = O.Matrix(kind)
     os_, oer, oeq, _ = O.batch(1, q, targets.residues, targets.offsets, m, go, ge, 0)
     bad = np.nonzero((s != os_) | (er != oer) | (eq != oeq))[0]
     if len(bad):
         t = bad[0]
         raise AssertionError(
-            f"{label} local tile={tile} {len(bad)}/{targets.n} differ; first t={t} "
+            f"{label} local tile={tile} schedule={schedule} {len(bad)}/{targets.n} differ; first t={t} "
             f"R={targets.offsets[t + 1] - targets.offsets[t]} gpu=({s[t]},{er[t]},{eq[t]}) "
             f"oracle=({os_[t]},{oer[t]},{oeq[t]}) status={st[t]}")
     return s, er, eq, st
@@ -203,9 +207,10 @@ def test_local_known_answers(eng, tile):
         assert (s[0], er[0], eq[0]) == (c["score"], c["ref_end"], c["read_end"]), c
 
 
+@pytest.mark.parametrize("schedule", ["wave", "workgroup"])
 @pytest.mark.parametrize("tile", [32, 16])
 @pytest.mark.parametrize("kind", ["actgn", "actgn0", "blosum62", "ascii"])
-def test_local_random(eng, tile, kind):
+def test_local_random(eng, tile, kind, schedule):
     rng = np.random.default_rng(zlib.crc32(f"local-{kind}-{tile}".encode()))
     alpha = {"actgn": 5, "actgn0": 5, "blosum62": 24, "ascii": 256}[kind]
     for trial in range(6):
@@ -224,10 +229,11 @@ def test_local_random(eng, tile, kind):
             seqs.append(r)
         ge = int(rng.integers(0, 3))
         go = ge + int(rng.integers(1, 6))
-        check_local(eng, q, Targets.from_list(seqs), kind, go, ge, tile, f"trial={trial}")
+        check_local(eng, q, Targets.from_list(seqs), kind, go, ge, tile, f"trial={trial}", schedule)
 
 
-def test_local_long_reads_and_windows(eng):
+@pytest.mark.parametrize("schedule", ["wave", "workgroup"])
+def test_local_long_reads_and_windows(eng, schedule):
     """Reads far longer than the 256-column offset window, hits near the end."""
     rng = np.random.default_rng(21)
     q = rng.integers(0, 4, 500).astype(np.uint8)
@@ -238,7 +244,8 @@ def test_local_long_reads_and_windows(eng):
             r[L - 520:L - 20] = q
         seqs.append(r)
     for tile in (32, 16):
-        s, er, eq, st = check_local(eng, q, Targets.from_list(seqs), "actgn0", 3, 1, tile)
+        s, er, eq, st = check_local(eng, q, Targets.from_list(seqs), "actgn0", 3, 1, tile,
+                                    schedule=schedule)
         assert s[0] == 2 * 500 and er[0] == 20000 - 21
         assert (st == ish_amd.STATUS_I16).all()
 
@@ -251,9 +258,24 @@ def test_local_int32_path(eng):
     assert (st & ish_amd.STATUS_WIDENED).all()
 
 
-def test_local_long_read_workload(eng):
+@pytest.mark.parametrize("schedule", ["auto", "wave", "workgroup"])
+def test_local_long_read_workload(eng, schedule):
     w = workloads.generate(4, 160, seed=4)
-    check_local(eng, w.query, w.targets, "actgn0", 3, 1, 32, "wl4")
+    check_local(eng, w.query, w.targets, "actgn0", 3, 1, 32, "wl4", schedule)
+
+
+@pytest.mark.parametrize("Q", [130, 31])
+def test_local_workgroup_many_groups(eng, Q):
+    """More groups than workgroups, so every workgroup scores several groups
+    (LDS ring and combine area reused); Q=130 leaves waves 1-3 idle in the
+    last round, Q=31 has a single chunk (waves 1-3 idle throughout)."""
+    rng = np.random.default_rng(Q)
+    q = rng.integers(0, 4, Q).astype(np.uint8)
+    seqs = [rng.integers(0, 5, int(rng.integers(0, 150))).astype(np.uint8) for _ in range(70000)]
+    for k in range(0, 70000, 97):
+        if len(seqs[k]) >= Q // 2:
+            seqs[k][: Q // 2] = q[: Q // 2]
+    check_local(eng, q, Targets.from_list(seqs), "actgn", 3, 1, 32, f"many Q={Q}", "workgroup")
 
 
 def _records(text, rtype):
