@@ -54,8 +54,10 @@ struct Profile {
     int32_t words() const { return nrows * stride / 2; }
 };
 
+// Profile entries are S + bias (the kernels' offset form); entries past the
+// query are 0.
 void build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
-                   int32_t ge, bool want_rev_plain, Profile &pf);
+                   int32_t bias, bool want_rev_plain, Profile &pf);
 // Alphabet size of a built-in matrix kind (-1 when unknown).
 int32_t alphabet_of(int32_t kind);
 // The A*A matrix to use: the caller's, or the built-in one of `kind` when

@@ -261,7 +261,7 @@ static bool local_fits_int16(int32_t qpad, int32_t go, int32_t ge, int32_t max_s
 }
 
 void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
-                        int32_t ge, bool want_rev_plain, Profile &pf) {
+                        int32_t bias, bool want_rev_plain, Profile &pf) {
     std::map<std::vector<int8_t>, int> row_of;
     std::vector<std::vector<int8_t>> rows;
     for (int c = 0; c < 256; ++c) {
@@ -289,7 +289,7 @@ void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &
     }
     for (int32_t r = 0; r < pf.nrows; ++r)
         for (int32_t j = 0; j < Q; ++j) {
-            const int16_t v = (int16_t)(rows[r][j] + 2 * ge);
+            const int16_t v = (int16_t)(rows[r][j] + bias);
             pf.fwd[(size_t)r * pf.stride + j] = v;
             if (want_rev_plain) {
                 pf.plain[(size_t)r * pf.stride + j] = rows[r][j];
@@ -356,7 +356,9 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
         mx = std::max<int32_t>(mx, v);
     }
     Profile pf;
-    build_profile(q, Q, mat, A, ge, true, pf);
+    // offset-form scores: S + 2ge for the semi-global kernels, S + ge for
+    // the local ones (their offsets are explained in the kernel sources)
+    build_profile(q, Q, mat, A, algo == ISH_ALGO_LOCAL ? ge : 2 * ge, true, pf);
     const size_t lds = pf.lds();
     if (lds > kLdsLimit)
         return fail(ctx, ISH_E_RANGE, "query profile needs %zu bytes of LDS (limit %zu)", lds,
@@ -977,7 +979,7 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
         const uint8_t *q = queries + qoffsets[k];
         for (int32_t j = 0; j < Q; ++j)
             if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query %d code %d >= alphabet %d", k, q[j], A);
-        build_profile(q, Q, mat, A, ge, false, pfs[(size_t)k]);
+        build_profile(q, Q, mat, A, 2 * ge, false, pfs[(size_t)k]);
         items[(size_t)k] = MultiItem{&pfs[(size_t)k], Q, ends_free & 15u, (int64_t)k * n};
     }
     const size_t tot = (size_t)nq * (size_t)n;

@@ -1,4 +1,4 @@
-// Local (Smith-Waterman) scoring kernel for gfx950.
+// Local (Smith-Waterman) scoring kernels for gfx950.
 //
 // Semantics: sw / ssw_align of the reference's CPU striped path
 // (ishlib/matcher/alignment/local_aln/striped.mojo:142-434):
@@ -9,22 +9,30 @@
 //   holding it (0 when the maximum is 0) (:395-428).
 //
 // Layout: the same as the semi-global batched kernel (lane = two targets in
-// packed int16 halves, K query rows in registers, per-wave boundary rows).
-// Offset form H^ = H + (i - i0 + j)*ge, with i0 the start of the current
-// 256-column window: the floor 0 becomes (i - i0 + j)*ge, one wave-uniform
-// value per row, and the state is shifted down by 256*ge at every window
-// start, so int16 stays exact on reads of any length.
+// packed int16 halves, K query rows in registers, chunk-bottom rows handed
+// to the next chunk).
+//
+// Column-offset form: with i' = i - i0 the column inside the current
+// 256-column window, H~ = H + i'*ge (likewise E~, F~). Then
+//   G   = H~ + (ge - go)
+//   E~  = max(E~, G)            (next column)
+//   F~  = max(F~, G) - ge       (next row)
+//   H~  = max(H~diag + (S + ge), E~, F~, i'*ge)
+// The floor i'*ge is one value for the whole column, and so is the offset
+// of every row, so the column maximum is a plain max over the rows minus
+// one scalar. The state is shifted down by 256*ge at every window start,
+// so int16 stays exact on reads of any length.
 //
 // End coordinates are tracked per query chunk c: its maximum M_c, the first
 // column i_c where the chunk reaches it and the smallest row r_c of that
 // column holding it (the row scan runs only on columns where some lane's
 // chunk maximum strictly rises). Chunks combine by value, then first
 // column, then smallest row, which is exactly the whole-column rule.
-// The padding rows below the query (profile score -2*ge, i.e. <= 0) take
-// part in the column maximum: their values derive from real rows of the
-// same chunk through a non-positive diagonal score or a gap, so they never
-// strictly exceed the real rows' maximum so far, and the row scan meets the
-// real row first.
+// The padding rows below the query (profile entry 0, i.e. S = -ge <= 0)
+// take part in the column maximum: their values derive from real rows of
+// the same chunk through a non-positive diagonal score or a gap, so they
+// never strictly exceed the real rows' maximum so far, and the row scan
+// meets the real row first.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <stdint.h>
@@ -51,6 +59,168 @@ __device__ __forceinline__ void sw_combine(int mc, int ic, int rc, int &best, in
     }
 }
 
+// State of one lane while it scores one query chunk of its target(s).
+template <typename V, int K> struct SwChunk {
+    V H[K], E[K];
+    V diag_top; // H~ of the row above the chunk, previous column
+    V run;      // chunk maximum so far (plain values)
+    int icA, rcA, icB, rcB;
+    int ibase; // first column of the current offset window
+};
+
+template <typename V, int K>
+__device__ __forceinline__ void sw_chunk_init(SwChunk<V, K> &st, int go, int ge) {
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        st.H[m] = splat<V>(-ge); // H~(j, -1): H = 0 at column -1, offset -ge
+        st.E[m] = splat<V>(-go); // E~(j, 0) = H(j, -1) - go
+    }
+    st.diag_top = splat<V>(-ge);
+    st.run = splat<V>(0);
+    st.icA = st.rcA = st.icB = st.rcB = 0;
+    st.icA = st.icB = -1;
+    st.ibase = 0;
+}
+
+// Scores the 4 columns of block b (columns 4b .. 4b+3) of one chunk.
+//   rr / rr_next: residue words of this block and the next (the profile
+//   slices of the next column are fetched while a column is scored);
+//   bn: chunk-top rows (H~, F~ of the row above) when the chunk is not the
+//   first; out: where the chunk-bottom rows go (4 x 64 uint2, t*64 + lane),
+//   null for the last chunk. bestA / bestB: a lower bound of the maximum
+//   over the target's other chunks (filters the row scans).
+template <typename V, int K>
+__device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool first,
+                                         uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
+                                         short8 (&sa)[K / 8], short8 (&sb)[K / 8],
+                                         const uint8_t *chunk_base, const uint16_t *slut, int half,
+                                         int RA, int RB, int bestA, int bestB, int go, int ge,
+                                         uint2 *out, int lane) {
+    constexpr bool PACKED = !__is_same(V, int);
+    constexpr int NS = K / 8;
+    const V vcst = splat<V>(ge - go);
+    const V vge = splat<V>(ge);
+    auto row_ptr = [&](uint32_t code) -> const short8 * {
+        return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
+    };
+    if (b > 0 && ((b * 4) % kRenorm) == 0) {
+        // new offset window: shift the carried state down
+        const V vshift = splat<V>(kRenorm * ge);
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            st.H[m] = st.H[m] - vshift;
+            st.E[m] = st.E[m] - vshift;
+        }
+        st.diag_top = st.diag_top - vshift;
+        st.ibase = b * 4;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int i = b * 4 + t;
+        const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
+        const uint32_t wb = t < 3 ? rr.y : rr_next.y;
+        const int sh = 8 * ((t + 1) & 3);
+        const short8 *na = row_ptr((wa >> sh) & 0xffu);
+        const short8 *nb = row_ptr((wb >> sh) & 0xffu);
+        const int zi = (i - st.ibase) * ge; // floor of column i (all rows)
+        const V vz = splat<V>(zi);
+        V htop, f;
+        if (first) {
+            htop = vz;                // H~(-1, i): H = 0 above the query
+            f = splat<V>(zi - go);    // F~(0, i) = H(-1, i) - go
+        } else {
+            htop = from_u32<V>(bn[t].x);
+            f = from_u32<V>(bn[t].y);
+        }
+        V diag = st.diag_top;
+        V cms[NS]; // column maximum, one partial per 8-row slice
+#pragma unroll
+        for (int k = 0; k < NS; ++k) cms[k] = vz;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+#pragma unroll
+            for (int mm = 0; mm < 8; ++mm) {
+                const int m = k * 8 + mm;
+                V sv;
+                if constexpr (PACKED) {
+                    sv.x = sa[k][mm];
+                    sv.y = sb[k][mm];
+                } else {
+                    sv = (int)sa[k][mm];
+                }
+                const V h = vmax(vmax(vmax(diag + sv, st.E[m]), f), vz);
+                diag = st.H[m];
+                st.H[m] = h;
+                const V g = h + vcst;
+                st.E[m] = vmax(st.E[m], g);
+                f = vmax(f, g) - vge;
+                cms[k] = vmax(cms[k], h);
+            }
+            // slice k is consumed: fetch it for the next column (the
+            // scheduling barrier keeps only one extra slice live)
+            __builtin_amdgcn_sched_barrier(0);
+            sa[k] = na[k];
+            if (PACKED) sb[k] = nb[k];
+        }
+        st.diag_top = htop;
+        if (out) {
+            uint2 o;
+            o.x = as_u32(st.H[K - 1]);
+            o.y = as_u32(f);
+            out[t * 64 + lane] = o;
+        }
+        V cm = cms[0];
+#pragma unroll
+        for (int k = 1; k < NS; ++k) cm = vmax(cm, cms[k]);
+        cm = cm - vz; // plain column maximum (>= 0)
+        // columns past a target's end do not count
+        if constexpr (PACKED) {
+            const uint32_t mask = (i < RA ? 0x0000FFFFu : 0u) | (i < RB ? 0xFFFF0000u : 0u);
+            cm = from_u32<V>(as_u32(cm) & mask);
+        } else {
+            cm = i < RA ? cm : 0;
+        }
+        const V nr = vmax(st.run, cm);
+        // A rise matters only when it reaches the target's best over the
+        // other chunks: a chunk that ends below it cannot win, and one that
+        // ends at or above it records its last rise here, which is its
+        // (M_c, i_c, r_c).
+        bool up_a, up_b;
+        if constexpr (PACKED) {
+            up_a = nr.x > st.run.x && (int)nr.x >= bestA;
+            up_b = nr.y > st.run.y && (int)nr.y >= bestB;
+        } else {
+            up_a = nr > st.run && nr >= bestA;
+            up_b = false;
+        }
+        // wave-uniform test keeps this rare path a real branch
+        if (__builtin_amdgcn_ballot_w64(up_a || up_b) != 0) {
+            // smallest row of this column holding the new maximum
+            const V key = nr + vz;
+            int ra = K, rb = K;
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                if constexpr (PACKED) {
+                    if (ra == K && st.H[m].x == key.x) ra = m;
+                    if (rb == K && st.H[m].y == key.y) rb = m;
+                } else {
+                    if (ra == K && st.H[m] == key) ra = m;
+                }
+            }
+            if (up_a) {
+                st.icA = i;
+                st.rcA = j0 + ra;
+            }
+            if (up_b) {
+                st.icB = i;
+                st.rcB = j0 + rb;
+            }
+        }
+        st.run = nr;
+    }
+}
+
+// One wave per group: the wave runs every query chunk of its group in turn.
 template <typename V, int K>
 __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
@@ -62,9 +232,7 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     uint2 *bnd = p.bnd + (size_t)slot * p.slot_stride;
-    const int go = p.go, ge = p.ge, cst = ge - go, Q = p.Q;
-    const V vcst = splat<V>(cst);
-    const V vshift = splat<V>(kRenorm * ge);
+    const int go = p.go, ge = p.ge;
     const int items = PACKED ? (p.g_end - p.g_begin) : 2 * (p.g_end - p.g_begin);
 
     for (;;) {
@@ -83,53 +251,28 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
         const int nblk = gd.ncols >> 2;
 
         int bestA = 0, erA = -1, eqA = 0, bestB = 0, erB = -1, eqB = 0;
-
         for (int c = 0; c < p.nchunks; ++c) {
             const int j0 = c * K;
             const bool first = c == 0;
             const bool last = c == p.nchunks - 1;
             const uint8_t *chunk_base = smem + 2 * j0;
-            auto row_ptr = [&](uint32_t code) -> const short8 * {
-                return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
-            };
-            V H[K], E[K];
-#pragma unroll
-            for (int m = 0; m < K; ++m) {
-                H[m] = splat<V>((j0 + m - 1) * ge); // H^(j,-1) = (j-1)*ge
-                E[m] = H[m] + vcst;                 // E^(j,0)
-            }
-            V diag_top = splat<V>((j0 - 2) * ge); // H^(j0-1,-1)
-            V run = splat<V>(0);                  // chunk maximum so far (plain values)
-            int icA = -1, rcA = 0, icB = -1, rcB = 0;
-            int ibase = 0;
-
-            uint2 rr_next = res[lane];
+            SwChunk<V, K> st;
+            sw_chunk_init(st, go, ge);
+            uint2 rr_next = nblk > 0 ? res[lane] : make_uint2(0u, 0u);
             uint2 bn_next[4];
-            if (!first) {
 #pragma unroll
-                for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
-            }
+            for (int t = 0; t < 4; ++t) bn_next[t] = (!first && nblk > 0) ? bnd[t * 64 + lane] : make_uint2(0u, 0u);
             short8 sa[NS], sb[NS];
             {
-                const short8 *na = row_ptr((half ? rr_next.y : rr_next.x) & 0xffu);
-                const short8 *nb = row_ptr(rr_next.y & 0xffu);
+                const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[(half ? rr_next.y : rr_next.x) & 0xffu]);
+                const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
 #pragma unroll
                 for (int k = 0; k < NS; ++k) {
                     sa[k] = na[k];
-                    if (PACKED) sb[k] = nb[k];
+                    sb[k] = PACKED ? nb[k] : na[k];
                 }
             }
             for (int b = 0; b < nblk; ++b) {
-                if (b > 0 && ((b * 4) % kRenorm) == 0) {
-                    // new offset window: shift the carried state down
-#pragma unroll
-                    for (int m = 0; m < K; ++m) {
-                        H[m] = H[m] - vshift;
-                        E[m] = E[m] - vshift;
-                    }
-                    diag_top = diag_top - vshift;
-                    ibase = b * 4;
-                }
                 const uint2 rr = rr_next;
                 uint2 bn[4];
 #pragma unroll
@@ -141,114 +284,14 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
                         for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
                     }
                 }
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int i = b * 4 + t;
-                    const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
-                    const uint32_t wb = t < 3 ? rr.y : rr_next.y;
-                    const int sh = 8 * ((t + 1) & 3);
-                    const short8 *na = row_ptr((wa >> sh) & 0xffu);
-                    const short8 *nb = row_ptr((wb >> sh) & 0xffu);
-                    V htop, f;
-                    if (first) {
-                        const int ht = (i - ibase - 1) * ge; // H^(-1,i)
-                        htop = splat<V>(ht);
-                        f = splat<V>(ht + cst);
-                    } else {
-                        htop = from_u32<V>(bn[t].x);
-                        f = from_u32<V>(bn[t].y);
-                    }
-                    const int zc = (i - ibase + j0) * ge; // floor of row j0 at column i
-                    V diag = diag_top;
-                    V cms[NS]; // column maximum, one partial per 8-row slice
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) cms[k] = splat<V>(0);
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) {
-#pragma unroll
-                        for (int mm = 0; mm < 8; ++mm) {
-                            const int m = k * 8 + mm;
-                            V s;
-                            if constexpr (PACKED) {
-                                s.x = sa[k][mm];
-                                s.y = sb[k][mm];
-                            } else {
-                                s = (int)sa[k][mm];
-                            }
-                            const V z = splat<V>(zc + m * ge);
-                            V h = vmax(vmax(vmax(diag + s, E[m]), f), z);
-                            diag = H[m];
-                            H[m] = h;
-                            const V gg = h + vcst;
-                            E[m] = vmax(E[m], gg);
-                            f = vmax(f, gg);
-                            cms[k] = vmax(cms[k], h - z);
-                        }
-                        __builtin_amdgcn_sched_barrier(0);
-                        sa[k] = na[k];
-                        if (PACKED) sb[k] = nb[k];
-                    }
-                    diag_top = htop;
-                    if (!last) {
-                        uint2 o;
-                        o.x = as_u32(H[K - 1]);
-                        o.y = as_u32(f);
-                        bnd[(size_t)i * 64 + lane] = o;
-                    }
-                    V cm = cms[0];
-#pragma unroll
-                    for (int k = 1; k < NS; ++k) cm = vmax(cm, cms[k]);
-                    // columns past a target's end do not count
-                    if constexpr (PACKED) {
-                        const uint32_t mask = (i < RA ? 0x0000FFFFu : 0u) | (i < RB ? 0xFFFF0000u : 0u);
-                        cm = from_u32<V>(as_u32(cm) & mask);
-                    } else {
-                        cm = i < RA ? cm : 0;
-                    }
-                    const V nr = vmax(run, cm);
-                    // A rise matters only when it reaches the target's best
-                    // over the chunks above: a chunk that ends below it cannot
-                    // win, and one that ends at or above it records its last
-                    // rise here, which is its (M_c, i_c, r_c).
-                    bool up_a, up_b;
-                    if constexpr (PACKED) {
-                        up_a = nr.x > run.x && (int)nr.x >= bestA;
-                        up_b = nr.y > run.y && (int)nr.y >= bestB;
-                    } else {
-                        up_a = nr > run && nr >= bestA;
-                        up_b = false;
-                    }
-                    // wave-uniform test keeps this rare path a real branch
-                    if (__builtin_amdgcn_ballot_w64(up_a || up_b) != 0) {
-                        // smallest row of this column holding the new maximum
-                        int ra = K, rb = K;
-#pragma unroll
-                        for (int m = 0; m < K; ++m) {
-                            const V d = H[m] - splat<V>(zc + m * ge);
-                            if constexpr (PACKED) {
-                                if (ra == K && d.x == nr.x) ra = m;
-                                if (rb == K && d.y == nr.y) rb = m;
-                            } else {
-                                if (ra == K && d == nr) ra = m;
-                            }
-                        }
-                        if (up_a) {
-                            icA = i;
-                            rcA = j0 + ra;
-                        }
-                        if (up_b) {
-                            icB = i;
-                            rcB = j0 + rb;
-                        }
-                    }
-                    run = nr;
-                }
+                sw_block<V, K>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, half, RA,
+                               RB, bestA, bestB, go, ge, last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
             }
             if constexpr (PACKED) {
-                sw_combine((int)run.x, icA, rcA, bestA, erA, eqA);
-                sw_combine((int)run.y, icB, rcB, bestB, erB, eqB);
+                sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
+                sw_combine((int)st.run.y, st.icB, st.rcB, bestB, erB, eqB);
             } else {
-                sw_combine((int)run, icA, rcA, bestA, erA, eqA);
+                sw_combine((int)st.run, st.icA, st.rcA, bestA, erA, eqA);
             }
         }
 
@@ -272,7 +315,7 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
 // 4-column block, closed by a workgroup barrier; at step s wave w works on
 // block s - w - r*P of its round-r chunk (P = max(blocks, 5) steps per
 // round), one block behind the wave that owns the chunk above. The
-// chunk-bottom rows (H^, F^) flow to the next wave through a double-
+// chunk-bottom rows (H~, F~) flow to the next wave through a double-
 // buffered LDS ring, and from wave 3 to wave 0 of the next round through
 // the workgroup's boundary slot in global memory (written P-4 or more
 // steps before it is read, and overwritten 3 steps after). A group then
@@ -295,16 +338,14 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     stage_profile(p.prof, p.lut, p.prof_words, smem);
     const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
     uint8_t *tail = smem + (((size_t)p.prof_words * 4 + 512 + 15) & ~(size_t)15);
-    uint2 *ring = (uint2 *)tail;                         // [wave 0..2][parity][t][lane]
-    int *comb = (int *)(tail + kPipeRingBytes);          // [wave][lane][6]
+    uint2 *ring = (uint2 *)tail;                // [wave 0..2][parity][t][lane]
+    int *comb = (int *)(tail + kPipeRingBytes); // [wave][lane][6]
 
     constexpr int NS = K / 8;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     uint2 *gbnd = p.bnd + (size_t)blockIdx.x * p.slot_stride;
-    const int go = p.go, ge = p.ge, cst = ge - go;
-    const V vcst = splat<V>(cst);
-    const V vshift = splat<V>(kRenorm * ge);
+    const int go = p.go, ge = p.ge;
     const int items = p.g_end - p.g_begin;
     const int nch = p.nchunks;
     const int rounds = (nch + kPipeWaves - 1) / kPipeWaves;
@@ -326,14 +367,11 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
         const int steps = (rounds - 1) * P + nblk + (kPipeWaves - 1);
 
         int bestA = 0, erA = -1, eqA = 0, bestB = 0, erB = -1, eqB = 0;
-        V H[K], E[K];
-        V diag_top = splat<V>(0), run = splat<V>(0);
-        int icA = -1, rcA = 0, icB = -1, rcB = 0, ibase = 0;
+        SwChunk<V, K> st;
+        sw_chunk_init(st, go, ge);
         uint2 rr_next = make_uint2(0u, 0u);
         uint2 bn_next[4];
         short8 sa[NS], sb[NS];
-#pragma unroll
-        for (int m = 0; m < K; ++m) H[m] = E[m] = splat<V>(0);
 #pragma unroll
         for (int t = 0; t < 4; ++t) bn_next[t] = make_uint2(0u, 0u);
 #pragma unroll
@@ -348,46 +386,25 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                 const bool last = c == nch - 1;
                 const int par = (r * P + b) & 1; // ring parity of this (round, block)
                 const uint8_t *chunk_base = smem + 2 * j0;
-                auto row_ptr = [&](uint32_t code) -> const short8 * {
-                    return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
-                };
                 if (b == 0) {
-#pragma unroll
-                    for (int m = 0; m < K; ++m) {
-                        H[m] = splat<V>((j0 + m - 1) * ge);
-                        E[m] = H[m] + vcst;
-                    }
-                    diag_top = splat<V>((j0 - 2) * ge);
-                    run = splat<V>(0);
-                    icA = -1;
-                    rcA = 0;
-                    icB = -1;
-                    rcB = 0;
-                    ibase = 0;
+                    sw_chunk_init(st, go, ge);
                     rr_next = res[lane];
                     if (!first && w == 0) {
 #pragma unroll
                         for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
                     }
-                    const short8 *na = row_ptr(rr_next.x & 0xffu);
-                    const short8 *nb = row_ptr(rr_next.y & 0xffu);
+                    const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.x & 0xffu]);
+                    const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
 #pragma unroll
                     for (int k = 0; k < NS; ++k) {
                         sa[k] = na[k];
                         sb[k] = nb[k];
                     }
                 }
-                if (b > 0 && ((b * 4) % kRenorm) == 0) {
-#pragma unroll
-                    for (int m = 0; m < K; ++m) {
-                        H[m] = H[m] - vshift;
-                        E[m] = E[m] - vshift;
-                    }
-                    diag_top = diag_top - vshift;
-                    ibase = b * 4;
-                }
                 const uint2 rr = rr_next;
                 uint2 bn[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bn[t] = make_uint2(0u, 0u);
                 if (!first) {
                     if (w == 0) {
 #pragma unroll
@@ -404,91 +421,18 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                     }
                 }
                 if (b + 1 < nblk) rr_next = res[(size_t)(b + 1) * 64 + lane];
-                uint2 *dst_ring = ring + (size_t)(w * 2 + par) * 256;
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int i = b * 4 + t;
-                    const uint32_t wa = t < 3 ? rr.x : rr_next.x;
-                    const uint32_t wb = t < 3 ? rr.y : rr_next.y;
-                    const int sh = 8 * ((t + 1) & 3);
-                    const short8 *na = row_ptr((wa >> sh) & 0xffu);
-                    const short8 *nb = row_ptr((wb >> sh) & 0xffu);
-                    V htop, f;
-                    if (first) {
-                        const int ht = (i - ibase - 1) * ge;
-                        htop = splat<V>(ht);
-                        f = splat<V>(ht + cst);
-                    } else {
-                        htop = from_u32<V>(bn[t].x);
-                        f = from_u32<V>(bn[t].y);
-                    }
-                    const int zc = (i - ibase + j0) * ge;
-                    V diag = diag_top;
-                    V cms[NS];
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) cms[k] = splat<V>(0);
-#pragma unroll
-                    for (int k = 0; k < NS; ++k) {
-#pragma unroll
-                        for (int mm = 0; mm < 8; ++mm) {
-                            const int m = k * 8 + mm;
-                            V sv;
-                            sv.x = sa[k][mm];
-                            sv.y = sb[k][mm];
-                            const V z = splat<V>(zc + m * ge);
-                            V h = vmax(vmax(vmax(diag + sv, E[m]), f), z);
-                            diag = H[m];
-                            H[m] = h;
-                            const V gg = h + vcst;
-                            E[m] = vmax(E[m], gg);
-                            f = vmax(f, gg);
-                            cms[k] = vmax(cms[k], h - z);
-                        }
-                        __builtin_amdgcn_sched_barrier(0);
-                        sa[k] = na[k];
-                        sb[k] = nb[k];
-                    }
-                    diag_top = htop;
-                    if (!last) {
-                        uint2 o;
-                        o.x = as_u32(H[K - 1]);
-                        o.y = as_u32(f);
-                        if (w < kPipeWaves - 1) dst_ring[t * 64 + lane] = o;
-                        else gbnd[(size_t)i * 64 + lane] = o;
-                    }
-                    V cm = cms[0];
-#pragma unroll
-                    for (int k = 1; k < NS; ++k) cm = vmax(cm, cms[k]);
-                    const uint32_t mask = (i < RA ? 0x0000FFFFu : 0u) | (i < RB ? 0xFFFF0000u : 0u);
-                    cm = from_u32<V>(as_u32(cm) & mask);
-                    const V nr = vmax(run, cm);
-                    // bestA / bestB cover this wave's earlier chunks only: a
-                    // lower bound of the other chunks' maximum, which is all
-                    // the filter needs
-                    const bool up_a = nr.x > run.x && (int)nr.x >= bestA;
-                    const bool up_b = nr.y > run.y && (int)nr.y >= bestB;
-                    if (__builtin_amdgcn_ballot_w64(up_a || up_b) != 0) {
-                        int ra = K, rb = K;
-#pragma unroll
-                        for (int m = 0; m < K; ++m) {
-                            const V d = H[m] - splat<V>(zc + m * ge);
-                            if (ra == K && d.x == nr.x) ra = m;
-                            if (rb == K && d.y == nr.y) rb = m;
-                        }
-                        if (up_a) {
-                            icA = i;
-                            rcA = j0 + ra;
-                        }
-                        if (up_b) {
-                            icB = i;
-                            rcB = j0 + rb;
-                        }
-                    }
-                    run = nr;
-                }
+                uint2 *out = nullptr;
+                if (!last)
+                    out = w < kPipeWaves - 1 ? ring + (size_t)(w * 2 + par) * 256
+                                             : gbnd + (size_t)b * 4 * 64;
+                // bestA / bestB cover this wave's earlier chunks only: a
+                // lower bound of the other chunks' maximum, which is all the
+                // row-scan filter needs
+                sw_block<V, K>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, 0, RA,
+                               RB, bestA, bestB, go, ge, out, lane);
                 if (b == nblk - 1) {
-                    sw_combine((int)run.x, icA, rcA, bestA, erA, eqA);
-                    sw_combine((int)run.y, icB, rcB, bestB, erB, eqB);
+                    sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
+                    sw_combine((int)st.run.y, st.icB, st.rcB, bestB, erB, eqB);
                 }
             }
             __syncthreads();
