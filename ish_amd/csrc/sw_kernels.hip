@@ -193,18 +193,33 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             up_a = nr > st.run && nr >= bestA;
             up_b = false;
         }
-        // wave-uniform test keeps this rare path a real branch
+        // wave-uniform tests keep these rare paths real branches
         if (__builtin_amdgcn_ballot_w64(up_a || up_b) != 0) {
-            // smallest row of this column holding the new maximum
+            // smallest row of this column holding the new maximum: the first
+            // 8-row slice whose maximum is the key holds it
             const V key = nr + vz;
             int ra = K, rb = K;
 #pragma unroll
-            for (int m = 0; m < K; ++m) {
+            for (int k = 0; k < NS; ++k) {
+                bool ha, hb;
                 if constexpr (PACKED) {
-                    if (ra == K && st.H[m].x == key.x) ra = m;
-                    if (rb == K && st.H[m].y == key.y) rb = m;
+                    ha = up_a && ra == K && cms[k].x == key.x;
+                    hb = up_b && rb == K && cms[k].y == key.y;
                 } else {
-                    if (ra == K && st.H[m] == key) ra = m;
+                    ha = up_a && ra == K && cms[k] == key;
+                    hb = false;
+                }
+                if (__builtin_amdgcn_ballot_w64(ha || hb) != 0) {
+#pragma unroll
+                    for (int mm = 0; mm < 8; ++mm) {
+                        const int m = k * 8 + mm;
+                        if constexpr (PACKED) {
+                            if (ha && ra == K && st.H[m].x == key.x) ra = m;
+                            if (hb && rb == K && st.H[m].y == key.y) rb = m;
+                        } else {
+                            if (ha && ra == K && st.H[m] == key) ra = m;
+                        }
+                    }
                 }
             }
             if (up_a) {
@@ -327,8 +342,9 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
 constexpr int kPipeWaves = 4;
 constexpr size_t kPipeRingBytes = (size_t)(kPipeWaves - 1) * 2 * 4 * 64 * sizeof(uint2);
 constexpr size_t kPipeCombBytes = (size_t)kPipeWaves * 64 * 6 * sizeof(int);
+constexpr size_t kPipePubBytes = (size_t)kPipeWaves * 64 * sizeof(uint32_t);
 
-size_t sw_pipe_extra_lds() { return kPipeRingBytes + kPipeCombBytes + 16; }
+size_t sw_pipe_extra_lds() { return kPipeRingBytes + kPipeCombBytes + kPipePubBytes + 16; }
 
 template <int K>
 __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
@@ -340,6 +356,9 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     uint8_t *tail = smem + (((size_t)p.prof_words * 4 + 512 + 15) & ~(size_t)15);
     uint2 *ring = (uint2 *)tail;                // [wave 0..2][parity][t][lane]
     int *comb = (int *)(tail + kPipeRingBytes); // [wave][lane][6]
+    // each wave's best so far per lane (packed A|B): the other waves read it
+    // as a lower bound of the maximum over the chunks they do not own
+    uint32_t *pub = (uint32_t *)(tail + kPipeRingBytes + kPipeCombBytes); // [wave][lane]
 
     constexpr int NS = K / 8;
     const int lane = threadIdx.x & 63;
@@ -354,7 +373,9 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
         __syncthreads();
         const int item = s_item;
-        __syncthreads(); // s_item is rewritten at the top of the next group
+        pub[w * 64 + lane] = 0u;
+        __syncthreads(); // s_item is rewritten at the top of the next group;
+                         // every pub entry is reset before anyone reads it
         if (item >= items) break;
         const GroupDesc gd = p.groups[p.g_begin + item];
         const int tA = gd.first + lane;
@@ -425,11 +446,27 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                 if (!last)
                     out = w < kPipeWaves - 1 ? ring + (size_t)(w * 2 + par) * 256
                                              : gbnd + (size_t)b * 4 * 64;
-                // bestA / bestB cover this wave's earlier chunks only: a
-                // lower bound of the other chunks' maximum, which is all the
-                // row-scan filter needs
+                // Row-scan filter: a lower bound of the maximum over the
+                // target's other chunks. This wave's finished chunks give
+                // bestA / bestB; the other waves publish their best so far
+                // (finished chunks and running chunk maxima, which only
+                // grow, so a stale entry is still a lower bound).
+                int lbA = bestA, lbB = bestB;
+#pragma unroll
+                for (int v = 0; v < kPipeWaves; ++v) {
+                    if (v == w) continue;
+                    const V o = from_u32<V>(pub[v * 64 + lane]);
+                    lbA = max(lbA, (int)o.x);
+                    lbB = max(lbB, (int)o.y);
+                }
                 sw_block<V, K>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, 0, RA,
-                               RB, bestA, bestB, go, ge, out, lane);
+                               RB, lbA, lbB, go, ge, out, lane);
+                {
+                    V mine;
+                    mine.x = (short)max(bestA, (int)st.run.x);
+                    mine.y = (short)max(bestB, (int)st.run.y);
+                    pub[w * 64 + lane] = as_u32(mine);
+                }
                 if (b == nblk - 1) {
                     sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
                     sw_combine((int)st.run.y, st.icB, st.rcB, bestB, erB, eqB);
