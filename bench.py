@@ -60,6 +60,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(content_hash, tile):
+    """HBM bytes per dispatch of the dominant kernel, as rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters measured them for this exact workload
+    (profiles/traffic.json, scaled by the copy-kernel calibration); None
+    when that workload and tile were not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        entry = json.load(open(path)).get(content_hash)
+    except (OSError, ValueError):
+        return None
+    if not entry or entry.get("tile_rows") != tile:
+        return None
+    return entry["bytes"]
+
+
 def serve_stream(ish_amd, srv, reqs):
     """Open-loop client: submits each request at its arrival time (one
     thread), collects results; latency = completion - scheduled arrival on
@@ -443,7 +458,7 @@ def main():
         "parity_sample": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(w.content_hash(), best_tile),
                      "ops_per_cell": ops_per_cell, "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }
