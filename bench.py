@@ -333,19 +333,20 @@ def main():
     load_s = time.perf_counter() - t0
     log(f"db load (H2D + sort + pack): {load_s * 1e3:.1f} ms")
 
-    # tile choice: time one pass per candidate (results never depend on it)
-    tiles = [args.tile] if args.tile else [32, 16]
-    tile_ms = {}
-    for tile in tiles:
-        eng.set_tile(tile)
-        eng.align_db_async(db)
-        eng.sync()
-        eng.align_db_async(db)
-        eng.sync()
-        tile_ms[tile] = eng.last_timing()[0]
-        log(f"tile {tile}: kernel {tile_ms[tile]:.2f} ms")
-    best_tile = min(tile_ms, key=tile_ms.get)
-    eng.set_tile(best_tile)
+    # launch configuration: the library's autotuner times tile heights (and
+    # for local alignment the schedule) per target-length bucket; results
+    # never depend on it. --tile fixes the tile instead.
+    t0 = time.perf_counter()
+    if args.tile:
+        eng.set_tile(args.tile)
+        tuning = None
+    else:
+        tuning = eng.autotune(db)
+        log(f"autotune ({time.perf_counter() - t0:.1f}s): " +
+            ", ".join(f">={lo}: tile {t} {sc}" for lo, t, sc in tuning))
+    best_tile = args.tile or 0
+    # repeated passes replay one captured HIP graph
+    eng.set_graphs(True)
     for _ in range(args.warmup):
         eng.align_db_async(db)
         eng.sync()
@@ -367,6 +368,7 @@ def main():
     log(f"batched: {step_s * 1e3:.2f} ms/step, {gcups:.1f} GCUPS (kernel {gcups_kernel:.1f}), "
         f"widened targets {widened}")
 
+    eng.set_graphs(False)
     # end-to-end: upload + pack + score + fetch, for one query
     t0 = time.perf_counter()
     db2 = eng.load_db(w.targets)
@@ -445,8 +447,9 @@ def main():
         "config": {"workload": desc, "name": args.workload,
                    "targets": w.targets.n, "residues": int(w.targets.offsets[-1]), "query_len": Q,
                    "cells": cells, "seed": args.seed, "content_hash": w.content_hash(),
-                   "engine": "batched persistent, packed int16 (int32 for wide groups)",
-                   "tile_rows": best_tile, "tile_kernel_ms": {str(k): round(v, 3) for k, v in tile_ms.items()}},
+                   "engine": "batched persistent, packed int16 (int32 for wide groups), HIP graph replay",
+                   "tile_rows": best_tile or None,
+                   "tuning": [{"min_len": lo, "tile": t, "schedule": sc} for lo, t, sc in tuning] if tuning else None},
         "gcups_kernel": round(gcups_kernel, 2),
         "gcups_e2e": round(cells / e2e_s / 1e9, 2),
         "db_load_ms": round(load_s * 1e3, 1),
@@ -458,7 +461,8 @@ def main():
         "parity_sample": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": measured_traffic(w.content_hash(), best_tile),
+                     "traffic": measured_traffic(w.content_hash(), best_tile or (
+                         tuning[0][1] if tuning and len({t for _, t, _ in tuning}) == 1 else None)),
                      "ops_per_cell": ops_per_cell, "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }

@@ -138,6 +138,29 @@ int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane);
 #define ISH_SCHEDULE_WORKGROUP 2
 int32_t ish_hip_set_schedule(ish_hip_ctx *ctx, int32_t schedule);
 
+/* Autotuner: times every candidate launch configuration (tile height 16
+ * or 32; for local alignment also the WAVE / WORKGROUP schedule) on a
+ * sample of each target-length bucket of db (longest target >= 4096,
+ * >= 1024, >= 256, shorter), `reps` timed runs each (0 = 2), for the
+ * context's current query, and keeps the fastest per bucket under the key
+ * (matrix kind, algorithm, ceil(log2(query length))). Later passes over any
+ * database with a query of the same key launch each bucket with its tuned
+ * configuration. ish_hip_set_tile switches the table off until the next
+ * ish_hip_autotune. Results never depend on it. No counterpart in the
+ * reference. */
+int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t reps);
+/* The tuned (tile, schedule) per target-length bucket for the current
+ * query's key, with each bucket's smallest longest-target length; returns
+ * the number of buckets (4) or 0 when that key has not been tuned. n >= 4. */
+int32_t ish_hip_tuning(const ish_hip_ctx *ctx, int32_t *tile, int32_t *schedule,
+                       int32_t *bucket_min_len, int32_t n);
+/* HIP graphs for database passes: when on, ish_hip_align_db[_async]
+ * captures its launches into a graph the first time and replays it while
+ * the launches (kernels, grids, parameters and buffers) stay the same; a
+ * change of query, database, tile or tuning re-captures. Counters are reset
+ * by kernels, not memsets, inside the graph. */
+int32_t ish_hip_set_graphs(ish_hip_ctx *ctx, int32_t on);
+
 /* One synchronous batch (the batch_match_coarse + copy_outputs_to_host
  * step). Targets are residues[offsets[t] .. offsets[t+1]) for t < n.
  * Outputs (n entries each): score, 0-based end_ref / end_query, status.

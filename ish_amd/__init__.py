@@ -31,7 +31,8 @@ ABI_SYMBOLS = [
     "ish_hip_version", "ish_hip_device_count", "ish_hip_create", "ish_hip_destroy",
     "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
-    "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
+    "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_autotune", "ish_hip_tuning",
+    "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
@@ -99,6 +100,9 @@ def lib():
         "ish_hip_set_engine": ([vp, i32], i32),
         "ish_hip_set_tile": ([vp, i32], i32),
         "ish_hip_set_schedule": ([vp, i32], i32),
+        "ish_hip_autotune": ([vp, vp, i32], i32),
+        "ish_hip_tuning": ([vp, i32p, i32p, i32p, i32], i32),
+        "ish_hip_set_graphs": ([vp, i32], i32),
         "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
         "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
         "ish_hip_db_free": ([vp], None),
@@ -246,6 +250,26 @@ class Engine:
 
     def set_schedule(self, schedule: str):
         self._check(self._L.ish_hip_set_schedule(self._h, SCHEDULE[schedule]))
+
+    def set_graphs(self, on: bool):
+        self._check(self._L.ish_hip_set_graphs(self._h, int(bool(on))))
+
+    def autotune(self, db: "Database", reps: int = 2):
+        """Times the launch configurations per target-length bucket for the
+        current query and keeps the fastest; returns tuning()."""
+        self._check(self._L.ish_hip_autotune(self._h, db._h, reps))
+        return self.tuning()
+
+    def tuning(self):
+        """[(bucket min length, tile, schedule)] for the current query's key,
+        or [] when it has not been tuned."""
+        t, sc, lo = (np.zeros(4, np.int32) for _ in range(3))
+        n = self._L.ish_hip_tuning(self._h, _p(t, ctypes.c_int32), _p(sc, ctypes.c_int32),
+                                   _p(lo, ctypes.c_int32), 4)
+        if n < 0:
+            self._check(n)
+        inv = {v: k for k, v in SCHEDULE.items()}
+        return [(int(lo[b]), int(t[b]), inv.get(int(sc[b]), "auto")) for b in range(n)]
 
     def set_query(self, query: np.ndarray, matrix_kind: str, gap_open: int = 3,
                   gap_extend: int = 1, ends_free="FFTT", algo="striped-semi-global"):

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,20 @@
 #include "tables.h"
 
 using namespace ish;
+
+// Target-length buckets of the autotuner: a group belongs to the first
+// bucket whose lower bound its longest target reaches.
+constexpr int kLenBuckets = 4;
+constexpr int32_t kLenBucketLo[kLenBuckets] = {4096, 1024, 256, 0};
+struct TuneChoice {
+    int8_t chunk, schedule;
+};
+struct TuneKey {
+    int32_t kind, algo, qbucket;
+    bool operator<(const TuneKey &o) const {
+        return kind != o.kind ? kind < o.kind : algo != o.algo ? algo < o.algo : qbucket < o.qbucket;
+    }
+};
 
 struct ish_hip_ctx {
     int device = 0;
@@ -35,6 +50,15 @@ struct ish_hip_ctx {
     int engine = ISH_ENGINE_BATCHED;
     int chunk = kChunkDefault; // query rows per lane in the batched kernels
     int schedule = ISH_SCHEDULE_AUTO; // one wave or one workgroup per group
+    // autotuning: (matrix, algorithm, query-length bucket) -> choice per
+    // target-length bucket
+    std::map<TuneKey, std::array<TuneChoice, kLenBuckets>> tune;
+    bool tuned = false;
+    // HIP graph of the last db pass, replayed while its launches match
+    bool graphs = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint8_t> graph_sig;
     // work buffers
     DevBuf scratch, counter, in_res, in_off, out_score, out_er, out_eq, out_st;
     // timing
@@ -187,6 +211,8 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (DevBuf *b : {&ctx->prof_t, &ctx->prof_rt, &ctx->prof_p, &ctx->lut, &ctx->scratch, &ctx->counter,
                       &ctx->in_res, &ctx->in_off, &ctx->out_score, &ctx->out_er, &ctx->out_eq,
                       &ctx->out_st})
@@ -204,6 +230,7 @@ extern "C" int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lan
     if (query_rows_per_lane != 16 && query_rows_per_lane != 32)
         return fail(ctx, ISH_E_INVALID, "tile height must be 16 or 32 (got %d)", query_rows_per_lane);
     ctx->chunk = query_rows_per_lane;
+    ctx->tuned = false; // an explicit tile overrides the tuning table
     return ISH_OK;
 }
 
@@ -593,10 +620,57 @@ extern "C" int32_t ish_hip_db_size(const ish_hip_db *db, int64_t *n, int64_t *nr
 
 // Enqueues the scoring kernels over every group of db with the given
 // profile (forward or reversed query) and free-end flags.
-static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, uint32_t flags) {
-    ctx->launches = 0;
-    if (db->ngroups == 0) return ISH_OK;
+// One persistent launch of a db pass.
+struct DbLaunch {
+    int32_t kind; // kSg16, kSg32, kSw16, kSw32, kSwPipe
+    int32_t chunk;
+    int32_t g_begin, g_end;
+    int32_t grid;
+    int32_t pad;
+    int64_t slot_stride;
+    uint64_t lds;
+    ScoreParams p;
+};
+enum { kSg16 = 0, kSg32 = 1, kSw16 = 2, kSw32 = 3, kSwPipe = 4 };
+
+static int32_t len_bucket(int32_t maxlen) {
+    for (int32_t b = 0; b < kLenBuckets - 1; ++b)
+        if (maxlen >= kLenBucketLo[b]) return b;
+    return kLenBuckets - 1;
+}
+
+static TuneKey tune_key(const ish_hip_ctx *ctx) {
+    int32_t qb = 0;
+    while ((1 << qb) < ctx->Q) ++qb;
+    return TuneKey{ctx->kind, ctx->algo, qb};
+}
+
+// Heuristic schedule of the local int16 groups [g0, g1) at one tile height:
+// the workgroup pipeline when the longest group is far longer than the
+// average work per wave (long reads), else one wave per group.
+static bool pipe_heuristic(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t g0, int32_t g1,
+                           int32_t chunk) {
+    const int nch = (ctx->Q + chunk - 1) / chunk;
+    const int rounds = (nch + 3) / 4;
+    if (nch < 4 || 4 * rounds - nch > rounds) return false;
+    double cols = 0;
+    for (int32_t g = g0; g < g1; ++g) cols += db->group_maxlen[g] + 3;
+    const int occ = occupancy_sw_batched_i16(chunk, kBlock, ctx->lds);
+    const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
+    return db->group_maxlen[g0] > 1.5 * per_wave;
+}
+
+// Plans the launches that score groups [g_lo, g_hi) of db (int32 groups
+// first), sizes the boundary scratch for all of them, then fills in every
+// launch's parameters. `forced` >= 0 fixes (chunk, schedule) for the int16
+// groups (autotuning); otherwise the tuning table or the defaults decide.
+static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, uint32_t flags,
+                       int32_t g_lo, int32_t g_hi, int forced_chunk, int forced_sched,
+                       std::vector<DbLaunch> &out) {
+    out.clear();
+    if (g_hi <= g_lo) return ISH_OK;
     HIP_TRY(ctx, ctx->counter.ensure(256));
+    const bool local = ctx->algo == ISH_ALGO_LOCAL;
     // groups whose longest target exceeds the int16 bound come first
     // (descending lengths); they run in the int32 kernel.
     int32_t g16 = 0;
@@ -609,85 +683,261 @@ static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, 
         }
         g16 = lo;
     }
-    ScoreParams p{};
-    p.prof = prof;
-    p.lut = (const uint16_t *)ctx->lut.p;
-    p.prof_words = ctx->prof_words;
-    p.stride = ctx->stride;
-    p.Q = ctx->Q;
-    p.nchunks = (ctx->Q + ctx->chunk - 1) / ctx->chunk;
-    p.go = ctx->go;
-    p.ge = ctx->ge;
-    p.flags = flags;
-    p.neg_limit = ctx->neg_limit;
-    p.groups = db->d_groups;
-    p.res = db->d_res;
-    p.lens = db->d_lens;
-    p.orig = db->d_orig;
-    p.counter = (uint32_t *)ctx->counter.p;
-    p.score = db->d_score;
-    p.end_ref = db->d_end_ref;
-    p.end_q = db->d_end_q;
-    p.status = db->d_status;
-
-    const bool local = ctx->algo == ISH_ALGO_LOCAL;
-    // size the boundary scratch for both launches before either is enqueued
-    // (growing it between them would free memory the first one uses)
-    int grid32 = 0, grid16 = 0;
-    int64_t ss32 = 0, ss16 = 0;
-    if (g16 > 0) {
-        const int occ = local ? occupancy_sw_batched_i32(ctx->chunk, kBlock, ctx->lds)
-                              : occupancy_sg_batched_i32(ctx->chunk, kBlock, ctx->lds, flags);
-        int32_t rc = size_grid(ctx, occ, db->group_maxlen[0] + 3, &grid32, &ss32);
-        if (rc) return rc;
+    const int32_t a = g_lo, m16 = std::max(g_lo, std::min(g16, g_hi));
+    if (m16 > a) {
+        DbLaunch L{};
+        L.kind = local ? kSw32 : kSg32;
+        L.chunk = ctx->chunk;
+        L.g_begin = a;
+        L.g_end = m16;
+        out.push_back(L);
     }
-    // Local alignment over groups much longer than the average work per
-    // wave (long reads): the four waves of a workgroup share each group.
-    bool pipe = false;
     const size_t pipe_lds = ctx->lds + sw_pipe_extra_lds();
-    if (local && g16 < db->ngroups && pipe_lds <= kLdsLimit && ctx->schedule != ISH_SCHEDULE_WAVE) {
-        const int nch = (ctx->Q + ctx->chunk - 1) / ctx->chunk;
-        const int rounds = (nch + 3) / 4;
-        if (ctx->schedule == ISH_SCHEDULE_WORKGROUP) {
-            pipe = true;
-        } else if (nch >= 4 && 4 * rounds - nch <= rounds) {
-            double cols = 0;
-            for (int32_t g = g16; g < db->ngroups; ++g) cols += db->group_maxlen[g] + 3;
-            const int occ = occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds);
-            const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
-            pipe = db->group_maxlen[g16] > 1.5 * per_wave;
+    auto add16 = [&](int32_t g0, int32_t g1, int chunk, int sched) {
+        if (g1 <= g0) return;
+        DbLaunch L{};
+        bool pipe = false;
+        if (local && pipe_lds <= kLdsLimit) {
+            if (sched == ISH_SCHEDULE_WORKGROUP) pipe = true;
+            else if (sched == ISH_SCHEDULE_AUTO) pipe = pipe_heuristic(ctx, db, g0, g1, chunk);
+        }
+        L.kind = pipe ? kSwPipe : local ? kSw16 : kSg16;
+        L.chunk = chunk;
+        L.g_begin = g0;
+        L.g_end = g1;
+        out.push_back(L);
+    };
+    if (forced_chunk > 0) {
+        add16(m16, g_hi, forced_chunk, forced_sched);
+    } else {
+        auto it = ctx->tuned ? ctx->tune.find(tune_key(ctx)) : ctx->tune.end();
+        if (it == ctx->tune.end()) {
+            add16(m16, g_hi, ctx->chunk, ctx->schedule);
+        } else {
+            // one launch per run of target-length buckets with the same
+            // tuned choice
+            int32_t g = m16;
+            while (g < g_hi) {
+                const TuneChoice &c = it->second[(size_t)len_bucket(db->group_maxlen[g])];
+                int32_t e = g;
+                while (e < g_hi) {
+                    const TuneChoice &d = it->second[(size_t)len_bucket(db->group_maxlen[e])];
+                    if (d.chunk != c.chunk || d.schedule != c.schedule) break;
+                    ++e;
+                }
+                add16(g, e, c.chunk, ctx->schedule != ISH_SCHEDULE_AUTO ? ctx->schedule : c.schedule);
+                g = e;
+            }
         }
     }
-    if (g16 < db->ngroups) {
-        const int occ = pipe ? occupancy_sw_pipe_i16(ctx->chunk, pipe_lds)
-                        : local ? occupancy_sw_batched_i16(ctx->chunk, kBlock, ctx->lds)
-                                : occupancy_sg_batched_i16(ctx->chunk, kBlock, ctx->lds, flags);
-        int32_t rc = size_grid(ctx, occ, db->group_maxlen[g16] + 3, &grid16, &ss16);
+    // grids and scratch for every launch before any is enqueued (growing the
+    // scratch between launches would free memory a queued launch uses)
+    for (DbLaunch &L : out) {
+        L.lds = L.kind == kSwPipe ? pipe_lds : ctx->lds;
+        int occ = 1;
+        switch (L.kind) {
+        case kSg32: occ = occupancy_sg_batched_i32(L.chunk, kBlock, L.lds, flags); break;
+        case kSg16: occ = occupancy_sg_batched_i16(L.chunk, kBlock, L.lds, flags); break;
+        case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
+        case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
+        default: occ = occupancy_sw_pipe_i16(L.chunk, L.lds); break;
+        }
+        int grid;
+        int64_t ss;
+        int32_t rc = size_grid(ctx, occ, db->group_maxlen[L.g_begin] + 3, &grid, &ss);
         if (rc) return rc;
+        L.grid = grid;
+        L.slot_stride = ss;
     }
-    p.bnd = (uint2 *)ctx->scratch.p;
-    if (g16 > 0) {
-        p.slot_stride = ss32;
-        p.g_begin = 0;
-        p.g_end = g16;
-        HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
-        HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, local ? launch_sw_batched_i32(p, ctx->chunk, grid32, kBlock, ctx->lds, ctx->stream)
-                           : launch_sg_batched_i32(p, ctx->chunk, grid32, kBlock, ctx->lds, ctx->stream));
-        HIP_TRY(ctx, timed_end(ctx));
-    }
-    if (g16 < db->ngroups) {
-        p.slot_stride = ss16;
-        p.g_begin = g16;
-        p.g_end = db->ngroups;
-        HIP_TRY(ctx, launch_reset(p.counter, ctx->stream));
-        HIP_TRY(ctx, timed_begin(ctx));
-        HIP_TRY(ctx, pipe ? launch_sw_pipe_i16(p, ctx->chunk, grid16, pipe_lds, ctx->stream)
-                     : local ? launch_sw_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream)
-                             : launch_sg_batched_i16(p, ctx->chunk, grid16, kBlock, ctx->lds, ctx->stream));
-        HIP_TRY(ctx, timed_end(ctx));
+    for (DbLaunch &L : out) {
+        ScoreParams &p = L.p;
+        p.prof = prof;
+        p.lut = (const uint16_t *)ctx->lut.p;
+        p.prof_words = ctx->prof_words;
+        p.stride = ctx->stride;
+        p.Q = ctx->Q;
+        p.nchunks = (ctx->Q + L.chunk - 1) / L.chunk;
+        p.go = ctx->go;
+        p.ge = ctx->ge;
+        p.flags = flags;
+        p.neg_limit = ctx->neg_limit;
+        p.groups = db->d_groups;
+        p.res = db->d_res;
+        p.lens = db->d_lens;
+        p.orig = db->d_orig;
+        p.counter = (uint32_t *)ctx->counter.p;
+        p.score = db->d_score;
+        p.end_ref = db->d_end_ref;
+        p.end_q = db->d_end_q;
+        p.status = db->d_status;
+        p.bnd = (uint2 *)ctx->scratch.p;
+        p.slot_stride = L.slot_stride;
+        p.g_begin = L.g_begin;
+        p.g_end = L.g_end;
     }
     return ISH_OK;
+}
+
+static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed) {
+    hipError_t e = launch_reset(L.p.counter, ctx->stream);
+    if (e == hipSuccess && timed) e = timed_begin(ctx);
+    if (e != hipSuccess) return e;
+    switch (L.kind) {
+    case kSg32: e = launch_sg_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
+    case kSg16: e = launch_sg_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
+    case kSw32: e = launch_sw_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
+    case kSw16: e = launch_sw_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
+    default: e = launch_sw_pipe_i16(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
+    }
+    if (e == hipSuccess && timed) e = timed_end(ctx);
+    return e;
+}
+
+static void drop_graph(ish_hip_ctx *ctx) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_sig.clear();
+}
+
+// Scores db against the context's query: eager launches, or a replay of the
+// HIP graph captured for exactly these launches (same kernels, grids and
+// parameters, pointers included) when graphs are on.
+static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, uint32_t flags,
+                         bool allow_graph = true) {
+    ctx->launches = 0;
+    if (db->ngroups == 0) return ISH_OK;
+    std::vector<DbLaunch> plan;
+    int32_t rc = plan_db(ctx, db, prof, flags, 0, db->ngroups, 0, 0, plan);
+    if (rc) return rc;
+    if (!(ctx->graphs && allow_graph)) {
+        for (const DbLaunch &L : plan) HIP_TRY(ctx, enqueue_launch(ctx, L, true));
+        return ISH_OK;
+    }
+    std::vector<uint8_t> sig(plan.size() * sizeof(DbLaunch));
+    if (!plan.empty()) memcpy(sig.data(), plan.data(), sig.size());
+    if (!ctx->graph_exec || sig != ctx->graph_sig) {
+        drop_graph(ctx);
+        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (const DbLaunch &L : plan)
+            if (e == hipSuccess) e = enqueue_launch(ctx, L, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            ctx->graph_exec = nullptr;
+            return fail(ctx, ISH_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        }
+        ctx->graph = g;
+        ctx->graph_sig = sig;
+    }
+    HIP_TRY(ctx, timed_begin(ctx));
+    HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    HIP_TRY(ctx, timed_end(ctx));
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_set_graphs(ish_hip_ctx *ctx, int32_t on) {
+    if (!ctx) return ISH_E_INVALID;
+    ctx->graphs = on != 0;
+    if (!ctx->graphs) {
+        (void)hipStreamSynchronize(ctx->stream);
+        drop_graph(ctx);
+    }
+    return ISH_OK;
+}
+
+// Times each candidate (tile height, and for local alignment the
+// schedule) on a sample of every target-length bucket of db for the
+// context's query, and keeps the fastest per bucket for the query's
+// (matrix, algorithm, query-length bucket).
+extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t reps) {
+    if (!ctx || !db) return ISH_E_INVALID;
+    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
+    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
+    if (ctx->engine != ISH_ENGINE_BATCHED) return fail(ctx, ISH_E_INVALID, "autotuning needs the batched engine");
+    if (reps <= 0) reps = 2;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const bool local = ctx->algo == ISH_ALGO_LOCAL;
+    const int32_t chunks[2] = {16, 32};
+    std::array<TuneChoice, kLenBuckets> best;
+    for (auto &c : best) c = TuneChoice{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO};
+    int32_t g = 0;
+    // int16 groups only: the int32 ones keep the default tile
+    {
+        int32_t lo = 0, hi = db->ngroups;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) / 2;
+            if ((int64_t)db->group_maxlen[mid] > ctx->r16max) lo = mid + 1;
+            else hi = mid;
+        }
+        g = lo;
+    }
+    // sample: up to 4 waves' worth of groups per resident wave slot
+    const int32_t sample = std::max<int32_t>(64, ctx->cus * 8 * 2);
+    const int saved_launches = ctx->launches;
+    while (g < db->ngroups) {
+        const int32_t bkt = len_bucket(db->group_maxlen[g]);
+        int32_t e = g;
+        while (e < db->ngroups && len_bucket(db->group_maxlen[e]) == bkt) ++e;
+        int32_t s0 = g, s1 = e;
+        if (s1 - s0 > sample) {
+            s0 = g + (e - g - sample) / 2;
+            s1 = s0 + sample;
+        }
+        float best_ms = 1e30f;
+        for (int32_t ch : chunks) {
+            const int nsched = local ? 2 : 1;
+            for (int si = 0; si < nsched; ++si) {
+                const int sched = si == 0 ? ISH_SCHEDULE_WAVE : ISH_SCHEDULE_WORKGROUP;
+                std::vector<DbLaunch> plan;
+                int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, s0, s1, ch,
+                                     sched, plan);
+                if (rc) return rc;
+                if (sched == ISH_SCHEDULE_WORKGROUP && (plan.empty() || plan.back().kind != kSwPipe))
+                    continue; // the pipeline does not fit this query
+                float total = 0.f;
+                for (int rep = 0; rep <= reps; ++rep) {
+                    ctx->launches = 0;
+                    for (const DbLaunch &L : plan) HIP_TRY(ctx, enqueue_launch(ctx, L, true));
+                    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                    collect_timing(ctx);
+                    if (rep > 0) total += ctx->last_ms; // the first run warms up
+                }
+                if (total < best_ms) {
+                    best_ms = total;
+                    best[(size_t)bkt] = TuneChoice{(int8_t)ch, (int8_t)sched};
+                }
+            }
+        }
+        g = e;
+    }
+    ctx->launches = saved_launches;
+    ctx->tune[tune_key(ctx)] = best;
+    ctx->tuned = true;
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_tuning(const ish_hip_ctx *ctx, int32_t *chunk, int32_t *schedule,
+                                  int32_t *bucket_min_len, int32_t n) {
+    if (!ctx || n < kLenBuckets) return ISH_E_INVALID;
+    if (!ctx->have_query) return ISH_E_NO_QUERY;
+    auto it = ctx->tune.find(tune_key(ctx));
+    for (int32_t b = 0; b < kLenBuckets; ++b) {
+        if (bucket_min_len) bucket_min_len[b] = kLenBucketLo[b];
+        if (it == ctx->tune.end()) {
+            if (chunk) chunk[b] = 0;
+            if (schedule) schedule[b] = 0;
+        } else {
+            if (chunk) chunk[b] = it->second[(size_t)b].chunk;
+            if (schedule) schedule[b] = it->second[(size_t)b].schedule;
+        }
+    }
+    return it == ctx->tune.end() ? 0 : kLenBuckets;
 }
 
 extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
@@ -1081,7 +1331,7 @@ extern "C" int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db, const i
     // free ends swap for the reverse pass (striped.mojo:165-183)
     const uint32_t f = ctx->flags;
     const uint32_t rflags = ((f & 1u) << 1) | ((f >> 1) & 1u) | ((f & 4u) << 1) | ((f >> 1) & 4u);
-    rc = launch_db(ctx, rdb, (const int16_t *)ctx->prof_rt.p, local ? 0u : rflags);
+    rc = launch_db(ctx, rdb, (const int16_t *)ctx->prof_rt.p, local ? 0u : rflags, false);
     std::vector<int32_t> rs((size_t)n_sel), rer((size_t)n_sel), req((size_t)n_sel);
     if (!rc) rc = ish_hip_db_fetch(ctx, rdb, rs.data(), rer.data(), req.data(), nullptr);
     ish_hip_db_free(rdb);

@@ -429,3 +429,71 @@ def test_multi_query(eng, flags):
         assert np.array_equal(s[k], os_) and np.array_equal(er[k], oer) and np.array_equal(eq[k], oeq), (k, len(q))
     assert (st[-1, -2:] & ish_amd.STATUS_WIDENED).all()
     db.close()
+
+
+# ---- autotuner and HIP graphs ------------------------------------------------
+def test_autotune_keeps_results(eng):
+    """The tuned launches (per length bucket) give the same results as the
+    defaults, for semi-global and local queries."""
+    w = workloads.generate(2, 30000, seed=8)
+    eng.set_engine("batched")
+    eng.set_tile(32)
+    db = eng.load_db(w.targets)
+    try:
+        eng.set_query(w.query, "blosum62", 3, 1, "FFTT")
+        ref = eng.align_db(db)
+        tuned = eng.autotune(db)
+        assert len(tuned) == 4 and all(t in (16, 32) for _, t, _ in tuned)
+        got = eng.align_db(db)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+        lr = workloads.generate(4, 400, seed=9)
+        db2 = eng.load_db(lr.targets)
+        eng.set_tile(32)
+        eng.set_query(lr.query, "actgn0", 3, 1, "FFFF", "striped-local")
+        ref = eng.align_db(db2)
+        tuned = eng.autotune(db2, 1)
+        assert len(tuned) == 4
+        got = eng.align_db(db2)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+        m = O.Matrix("actgn0")
+        os_, oer, oeq, _ = O.batch(1, lr.query, lr.targets.residues, lr.targets.offsets, m, 3, 1, 0)
+        assert np.array_equal(got[0], os_) and np.array_equal(got[1], oer) and np.array_equal(got[2], oeq)
+        db2.close()
+    finally:
+        eng.set_tile(32)  # switches the tuning table off for later tests
+        db.close()
+
+
+def test_graph_replays_match_eager(eng):
+    """Captured passes replayed several times, with eager work and
+    synchronisations in between; every replay equals the eager result."""
+    w = workloads.generate(2, 20000, seed=12)
+    rng = np.random.default_rng(3)
+    qa, qb = w.query, rng.integers(0, 20, 120).astype(np.uint8)
+    eng.set_engine("batched")
+    eng.set_tile(32)
+    db = eng.load_db(w.targets)
+    other = Targets.from_list([rng.integers(0, 20, 300).astype(np.uint8) for _ in range(500)])
+    try:
+        eager = {}
+        for name, q, fl in (("a", qa, "FFTT"), ("b", qb, "TTTT")):
+            eng.set_query(q, "blosum62", 3, 1, fl)
+            eager[name] = eng.align_db(db)
+        eng.set_graphs(True)
+        for rnd in range(3):
+            for name, q, fl in (("a", qa, "FFTT"), ("b", qb, "TTTT")):
+                eng.set_query(q, "blosum62", 3, 1, fl)
+                for rep in range(3):
+                    got = eng.align_db(db)
+                    for x, y in zip(got, eager[name]):
+                        assert np.array_equal(x, y), (rnd, name, rep)
+                    # eager work between replays
+                    eng.set_engine("plain")
+                    eng.align(other)
+                    eng.set_engine("batched")
+                    eng.sync()
+    finally:
+        eng.set_graphs(False)
+        db.close()
