@@ -247,6 +247,76 @@ int32_t ish_hip_emit(int32_t algo, const int32_t *score,
                      uint8_t *emit);
 
 /* ------------------------------------------------------------------------
+ * File search runner: the search loop of ish's parallel runners over one
+ * file (ishlib/parallel_line_search_runner.mojo:75-161,
+ * ishlib/parallel_fastx_search_runner.mojo:79-182) with GPU scoring. Reads
+ * line or FASTA/FASTQ records, plain or gzip (zlib, as the reference's
+ * GZFile), with the reference's kseq record rules; encodes them with the
+ * matrix's encoding; scores batches on the GPU while the next batch is
+ * parsed; writes each emitted record to out_fd in input order, in the
+ * runners' non-tty format:
+ *   line:  decode(encode(line)) "\n"
+ *   fastx: ">" name "\n" decode(encode(seq)) ["\n+\n" qual] "\n"
+ * Binary files (a NUL byte in the first 8192 bytes) are skipped in line
+ * mode. `pattern` is the ASCII pattern (encoded and scored as
+ * convert_ascii_to_encoding_and_score does). Sets the context's query.
+ * ------------------------------------------------------------------------ */
+#define ISH_RECORD_LINE 0
+#define ISH_RECORD_FASTX 1
+
+typedef struct ish_hip_search_config {
+    int32_t matrix_kind;  /* ISH_MATRIX_* */
+    int32_t algo;         /* ISH_ALGO_* */
+    int32_t gap_open;     /* absolute values */
+    int32_t gap_extend;
+    uint32_t ends_free;   /* ISH_FREE_* bits (semi-global) */
+    float threshold;      /* --score */
+    int32_t record_type;  /* ISH_RECORD_* */
+    int64_t batch_bytes;  /* record bytes per GPU batch (0 = 256 MiB);
+                             output never depends on it */
+} ish_hip_search_config;
+
+typedef struct ish_hip_search_stats {
+    int64_t records;
+    int64_t residues;
+    int64_t matches;
+    int64_t batches;
+    double cells;          /* residues * pattern length */
+    double seconds_total;
+    double seconds_parse;  /* parser thread, overlapped with the GPU */
+    double seconds_gpu;    /* upload + scoring + results, per batch */
+    double seconds_write;
+    int32_t reader_status;  /* 0, or the reader code that ended the file
+                               early (-2 truncated quality, -3 stream error;
+                               the reference's runners stop there too) */
+} ish_hip_search_stats;
+
+/* The record reader of the runner, on its own (host only, no device):
+ * reads every record of a file (raw bytes, not encoded) with the same rules.
+ * flags: bit 0 = binary (a NUL in the first 8192 bytes), bit 1 = FASTQ
+ * (decided from the first 8192 bytes). status: 0, or the reader's code that
+ * ended the file (-2 truncated quality, -3 stream error); a FASTX record with
+ * an empty sequence also ends the file (status 0). */
+typedef struct ish_hip_records ish_hip_records;
+int32_t ish_hip_records_read(const char *path, int32_t record_type,
+                             ish_hip_records **out);
+int64_t ish_hip_records_count(const ish_hip_records *r);
+int32_t ish_hip_records_flags(const ish_hip_records *r);
+int32_t ish_hip_records_status(const ish_hip_records *r);
+/* Pointers stay valid until ish_hip_records_free. name / qual are empty for
+ * line records. */
+int32_t ish_hip_records_get(const ish_hip_records *r, int64_t i,
+                            const uint8_t **seq, int64_t *seq_len,
+                            const uint8_t **name, int64_t *name_len,
+                            const uint8_t **qual, int64_t *qual_len);
+void ish_hip_records_free(ish_hip_records *r);
+
+int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path,
+                            const uint8_t *pattern, int32_t pattern_len,
+                            const ish_hip_search_config *cfg, int32_t out_fd,
+                            ish_hip_search_stats *stats);
+
+/* ------------------------------------------------------------------------
  * Serving loop: a device-resident database answers semi-global queries that
  * arrive over time, with continuous batching. No counterpart in the
  * reference, which scores one pattern per run (do_search.mojo:24-274);

@@ -38,7 +38,23 @@ ABI_SYMBOLS = [
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
     "ish_hip_server_clock", "ish_hip_server_submit", "ish_hip_server_wait", "ish_hip_server_hits",
     "ish_hip_server_scores", "ish_hip_server_release", "ish_hip_server_get_stats",
+    "ish_hip_search_file", "ish_hip_records_read", "ish_hip_records_count", "ish_hip_records_flags",
+    "ish_hip_records_status", "ish_hip_records_get", "ish_hip_records_free",
 ]
+RECORD = {"line": 0, "fastx": 1}
+
+
+class SearchConfig(ctypes.Structure):
+    _fields_ = [("matrix_kind", ctypes.c_int32), ("algo", ctypes.c_int32), ("gap_open", ctypes.c_int32),
+                ("gap_extend", ctypes.c_int32), ("ends_free", ctypes.c_uint32), ("threshold", ctypes.c_float),
+                ("record_type", ctypes.c_int32), ("batch_bytes", ctypes.c_int64)]
+
+
+class SearchStats(ctypes.Structure):
+    _fields_ = [("records", ctypes.c_int64), ("residues", ctypes.c_int64), ("matches", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("cells", ctypes.c_double), ("seconds_total", ctypes.c_double),
+                ("seconds_parse", ctypes.c_double), ("seconds_gpu", ctypes.c_double),
+                ("seconds_write", ctypes.c_double), ("reader_status", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -126,6 +142,13 @@ def lib():
         "ish_hip_server_scores": ([vp, i64, i32p, i32p, i32p], i32),
         "ish_hip_server_release": ([vp, i64], i32),
         "ish_hip_server_get_stats": ([vp, P(ServerStats)], i32),
+        "ish_hip_search_file": ([vp, ctypes.c_char_p, u8p, i32, P(SearchConfig), i32, P(SearchStats)], i32),
+        "ish_hip_records_read": ([ctypes.c_char_p, i32, P(vp)], i32),
+        "ish_hip_records_count": ([vp], i64),
+        "ish_hip_records_flags": ([vp], i32),
+        "ish_hip_records_status": ([vp], i32),
+        "ish_hip_records_get": ([vp, i64, P(u8p), P(i64), P(u8p), P(i64), P(u8p), P(i64)], i32),
+        "ish_hip_records_free": ([vp], None),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -193,6 +216,30 @@ def flags_from_str(s: str) -> int:
     if len(s) != 4 or set(s) - {"T", "F"}:
         raise ValueError("ends-free must be 4 letters of T/F, like FFTT")
     return sum(1 << k for k, c in enumerate(s) if c == "T")
+
+
+def read_records(path, record_type: str = "line"):
+    """Every record of a file as the runner reads it (host only): returns
+    (records, flags, status); records are (seq, name, qual) bytes, flags bit 0
+    = binary, bit 1 = FASTQ."""
+    L = lib()
+    h = ctypes.c_void_p()
+    rc = L.ish_hip_records_read(os.fsencode(path), RECORD[record_type], ctypes.byref(h))
+    if rc != 0:
+        raise IshError(f"ish_hip_records_read failed: {ERRORS.get(rc, rc)}")
+    try:
+        n = L.ish_hip_records_count(h)
+        out = []
+        P = ctypes.POINTER(ctypes.c_uint8)
+        ptrs = [P() for _ in range(3)]
+        lens = [ctypes.c_int64() for _ in range(3)]
+        for i in range(n):
+            L.ish_hip_records_get(h, i, ctypes.byref(ptrs[0]), ctypes.byref(lens[0]), ctypes.byref(ptrs[1]),
+                                  ctypes.byref(lens[1]), ctypes.byref(ptrs[2]), ctypes.byref(lens[2]))
+            out.append(tuple(ctypes.string_at(p, n_.value) if n_.value else b"" for p, n_ in zip(ptrs, lens)))
+        return out, int(L.ish_hip_records_flags(h)), int(L.ish_hip_records_status(h))
+    finally:
+        L.ish_hip_records_free(h)
 
 
 class Targets:
@@ -350,6 +397,21 @@ class Engine:
             MATRIX[matrix_kind], gap_open, gap_extend, fl, _p(out[0], ctypes.c_int32),
             _p(out[1], ctypes.c_int32), _p(out[2], ctypes.c_int32), _p(st, ctypes.c_uint8)))
         return tuple(a[: nq * n].reshape(nq, n) for a in (*out, st))
+
+    def search_file(self, path, pattern: bytes, matrix_kind: str, algo="striped-semi-global",
+                    gap_open: int = 3, gap_extend: int = 1, ends_free="FFTT", threshold: float = 0.8,
+                    record_type: str = "line", out_fd: int = 1, batch_bytes: int = 0):
+        """Searches one file (plain or gzip) and writes the emitted records to
+        out_fd in the reference runners' format; returns the run's stats."""
+        pat = np.frombuffer(bytes(pattern), np.uint8).copy()
+        pb = pat if len(pat) else np.zeros(1, np.uint8)
+        fl = flags_from_str(ends_free) if isinstance(ends_free, str) else int(ends_free)
+        cfg = SearchConfig(MATRIX[matrix_kind], ALGO[algo] if isinstance(algo, str) else int(algo), gap_open,
+                           gap_extend, fl, threshold, RECORD[record_type], batch_bytes)
+        st = SearchStats()
+        self._check(self._L.ish_hip_search_file(self._h, os.fsencode(path), _p(pb, ctypes.c_uint8), len(pat),
+                                                ctypes.byref(cfg), out_fd, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in SearchStats._fields_}
 
     def last_timing(self):
         ms = ctypes.c_float()

@@ -26,7 +26,8 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["ish_hip.cpp", "server.cpp", "sg_kernels.hip", "sw_kernels.hip", "serve_kernels.hip"]
+HIP_SOURCES = ["ish_hip.cpp", "server.cpp", "search.cpp", "sg_kernels.hip", "sw_kernels.hip",
+               "serve_kernels.hip"]
 
 
 def _run(cmd):
@@ -50,7 +51,7 @@ def build(verbose: bool = True) -> None:
         objs.append(obj)
     out = os.path.join(LIB, "libish_hip.so")
     tmp = out + ".tmp"
-    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp])
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-lz", "-o", tmp])
     os.replace(tmp, out)
     wl = os.path.join(LIB, "libish_workload.so")
     _run(["g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-pthread",
