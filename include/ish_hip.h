@@ -176,6 +176,12 @@ int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
  * (searcher_device.mojo:419-450). */
 int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
                         const uint64_t *offsets, int64_t n, ish_hip_db **out);
+/* The same from raw record bytes (ASCII): they are translated to residue
+ * codes with matrix_kind's encoding (ish_hip_encoding) on the device after
+ * the upload, so the host only splits records. */
+int32_t ish_hip_db_load_ascii(ish_hip_ctx *ctx, const uint8_t *ascii,
+                              const uint64_t *offsets, int64_t n,
+                              int32_t matrix_kind, ish_hip_db **out);
 void ish_hip_db_free(ish_hip_db *db);
 int32_t ish_hip_db_size(const ish_hip_db *db, int64_t *n_targets,
                         int64_t *n_residues);

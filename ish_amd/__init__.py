@@ -32,7 +32,7 @@ ABI_SYMBOLS = [
     "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
     "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_autotune", "ish_hip_tuning",
-    "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_free", "ish_hip_db_size",
+    "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_load_ascii", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
@@ -121,6 +121,7 @@ def lib():
         "ish_hip_set_graphs": ([vp, i32], i32),
         "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
         "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
+        "ish_hip_db_load_ascii": ([vp, u8p, u64p, i64, i32, P(vp)], i32),
         "ish_hip_db_free": ([vp], None),
         "ish_hip_db_size": ([vp, P(i64), P(i64)], i32),
         "ish_hip_align_db": ([vp, vp, i32p, i32p, i32p, u8p], i32),
@@ -343,6 +344,15 @@ class Engine:
         self._check(self._L.ish_hip_db_load(self._h, _p(targets.residues, ctypes.c_uint8),
                                             _p(targets.offsets, ctypes.c_uint64), targets.n,
                                             ctypes.byref(h)))
+        return Database(self, h, targets.n)
+
+    def load_db_ascii(self, targets: Targets, matrix_kind: str) -> "Database":
+        """Device-resident database from raw record bytes, encoded on the
+        device with the matrix's encoding."""
+        h = ctypes.c_void_p()
+        self._check(self._L.ish_hip_db_load_ascii(self._h, _p(targets.residues, ctypes.c_uint8),
+                                                  _p(targets.offsets, ctypes.c_uint64), targets.n,
+                                                  MATRIX[matrix_kind], ctypes.byref(h)))
         return Database(self, h, targets.n)
 
     def align_db(self, db: "Database"):

@@ -143,6 +143,12 @@ struct EmitParams {
 };
 hipError_t launch_emit_hits(const EmitParams &p, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
+// Byte -> residue code table, passed by value to the encoding kernel.
+struct EncodeTable {
+    uint8_t code[256];
+};
+// p[i] = t.code[p[i]] for i < n (p 16-byte aligned).
+hipError_t launch_encode_bytes(uint8_t *p, uint64_t n, const EncodeTable &t, hipStream_t st);
 
 // Occupancy helpers: blocks per CU of each kernel at the given LDS size.
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags);

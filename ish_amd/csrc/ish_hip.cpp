@@ -569,8 +569,11 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     return ISH_OK;
 }
 
-extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
-                                   const uint64_t *offsets, int64_t n, ish_hip_db **out) {
+// Uploads residues[offsets[0] .. offsets[n]) and builds the database;
+// with `encode` the bytes are translated to residue codes on the device
+// first.
+static int32_t db_load_impl(ish_hip_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
+                            int64_t n, const EncodeTable *encode, ish_hip_db **out) {
     if (!ctx || !out || n < 0 || (n > 0 && (!offsets || !residues)))
         return fail(ctx, ISH_E_INVALID, "bad db arguments");
     if (n > (int64_t)INT32_MAX - kGroup) return fail(ctx, ISH_E_RANGE, "too many targets");
@@ -592,6 +595,7 @@ extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
         e = hipMemcpyAsync(d_raw, residues + base, nres, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, ctx->stream);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(ctx->stream);
         if (d_raw) (void)hipFree(d_raw);
@@ -599,6 +603,20 @@ extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "db upload: %s", hipGetErrorString(e));
     }
     return db_build(ctx, d_raw, d_off, std::move(h_off), out);
+}
+
+extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
+                                   const uint64_t *offsets, int64_t n, ish_hip_db **out) {
+    return db_load_impl(ctx, residues, offsets, n, nullptr, out);
+}
+
+extern "C" int32_t ish_hip_db_load_ascii(ish_hip_ctx *ctx, const uint8_t *ascii,
+                                         const uint64_t *offsets, int64_t n, int32_t matrix_kind,
+                                         ish_hip_db **out) {
+    EncodeTable t;
+    if (ish_hip_encoding(matrix_kind, t.code) != ISH_OK)
+        return fail(ctx, ISH_E_INVALID, "unknown matrix kind %d", matrix_kind);
+    return db_load_impl(ctx, ascii, offsets, n, &t, out);
 }
 
 extern "C" void ish_hip_db_free(ish_hip_db *db) {

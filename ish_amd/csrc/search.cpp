@@ -16,9 +16,10 @@
 // whether the records are FASTQ is decided by the first 8192 bytes of the
 // file, as peek_file does for the first file.
 //
-// Pipeline: a parser thread reads, splits and encodes records into batches
-// (two in flight) while the calling thread scores the previous batch on
-// the GPU and writes its output.
+// Pipeline: a parser thread reads and splits records into batches of raw
+// bytes (two in flight) while the calling thread uploads the previous batch
+// (the device translates the bytes to residue codes), scores it on the GPU
+// and writes its output.
 #include <zlib.h>
 
 #include <algorithm>
@@ -309,11 +310,8 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
                 more = false;
                 b->error = src.status;
             } else {
-                const size_t o = b->res.size();
-                b->res.resize(o + seq->size());
-                uint8_t *dst = b->res.data() + o;
-                const uint8_t *sp = seq->data();
-                for (size_t k = 0; k < seq->size(); ++k) dst[k] = enc[sp[k]];
+                // raw bytes: the device encodes them after the upload
+                b->res.insert(b->res.end(), seq->begin(), seq->end());
                 b->off.push_back(b->res.size());
                 if (name) {
                     b->names.insert(b->names.end(), name->begin(), name->end());
@@ -343,7 +341,7 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
             std::vector<uint8_t> emit((size_t)n);
             ish_hip_db *db = nullptr;
             if (b->res.empty()) b->res.push_back(0);
-            rc = ish_hip_db_load(ctx, b->res.data(), b->off.data(), n, &db);
+            rc = ish_hip_db_load_ascii(ctx, b->res.data(), b->off.data(), n, cfg->matrix_kind, &db);
             if (rc == ISH_OK) rc = ish_hip_align_db(ctx, db, score.data(), er.data(), eq.data(), nullptr);
             if (db) ish_hip_db_free(db);
             if (rc == ISH_OK)
@@ -358,7 +356,7 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
                     st.matches++;
                     const uint64_t a = b->off[(size_t)k], e = b->off[(size_t)k + 1];
                     tmp.resize(e - a);
-                    for (uint64_t x = a; x < e; ++x) tmp[x - a] = dec[b->res[x]];
+                    for (uint64_t x = a; x < e; ++x) tmp[x - a] = dec[enc[b->res[x]]];
                     if (cfg->record_type == ISH_RECORD_LINE) {
                         out.put(tmp.data(), tmp.size());
                         out.put("\n");

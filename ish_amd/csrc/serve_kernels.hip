@@ -50,6 +50,39 @@ __global__ void __launch_bounds__(256) fill_u32(uint32_t *p, int64_t n, uint32_t
         p[i] = v;
 }
 
+// In-place byte translation (ASCII -> residue codes) of a device buffer:
+// 16 bytes per thread, grid-stride.
+__global__ void __launch_bounds__(256) encode_bytes(uint8_t *p, uint64_t n, EncodeTable tp) {
+    __shared__ EncodeTable t;
+    if (threadIdx.x < 64) ((uint32_t *)t.code)[threadIdx.x] = ((const uint32_t *)tp.code)[threadIdx.x];
+    __syncthreads();
+    const uint64_t nv = n / 16;
+    uint4 *v = (uint4 *)p;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint4 x = v[i];
+        uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t a = w[k];
+            w[k] = (uint32_t)t.code[a & 0xffu] | ((uint32_t)t.code[(a >> 8) & 0xffu] << 8) |
+                   ((uint32_t)t.code[(a >> 16) & 0xffu] << 16) | ((uint32_t)t.code[a >> 24] << 24);
+        }
+        v[i] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    const uint64_t tail = nv * 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tail < n && tail < nv * 16 + 16) p[tail] = t.code[p[tail]];
+}
+
+hipError_t launch_encode_bytes(uint8_t *p, uint64_t n, const EncodeTable &t, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t b = (n / 16 + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(encode_bytes, dim3((unsigned)b), dim3(256), 0, st, p, n, t);
+    return hipGetLastError();
+}
+
 hipError_t launch_emit_hits(const EmitParams &p, hipStream_t st) {
     if (p.n <= 0 || p.nq <= 0) return hipSuccess;
     int64_t bx = (p.n + 255) / 256;

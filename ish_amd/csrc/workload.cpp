@@ -25,6 +25,7 @@
 //    distribution of 1, R ~ U[100, 300]; 0.1 % carry a homolog of the
 //    350 aa base query (its first R residues).
 #include <math.h>
+#include <stdio.h>
 #include <pthread.h>
 #include <stdint.h>
 #include <string.h>
@@ -287,6 +288,49 @@ int32_t ish_wl_targets(int32_t kind, uint64_t seed, const uint8_t *query, int32_
     for (auto &t : th) pthread_join(t, nullptr);
     pthread_mutex_destroy(&jb.mu);
     return nthreads;
+}
+
+// Writes the targets as a FASTA (fastq = 0) or FASTQ (fastq = 1) file:
+// record t is named "t<t> synthetic", its residues decoded through
+// decode[code], sequence lines of at most `width` residues (0 = one line),
+// FASTQ qualities all 'I'. Returns 0, or -1 when the file cannot be written.
+int32_t ish_wl_write_fastx(const char *path, const uint8_t *res, const uint64_t *off, int64_t n,
+                           const uint8_t *decode, int32_t fastq, int32_t width) {
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    std::vector<char> buf;
+    buf.reserve(1 << 22);
+    auto flush = [&]() {
+        if (!buf.empty() && fwrite(buf.data(), 1, buf.size(), f) != buf.size()) return false;
+        buf.clear();
+        return true;
+    };
+    bool ok = true;
+    char hdr[64];
+    for (int64_t t = 0; t < n && ok; ++t) {
+        const int k = snprintf(hdr, sizeof hdr, "%ct%lld synthetic\n", fastq ? '@' : '>', (long long)t);
+        buf.insert(buf.end(), hdr, hdr + k);
+        const uint64_t a = off[t], e = off[t + 1];
+        uint64_t col = 0;
+        for (uint64_t x = a; x < e; ++x) {
+            buf.push_back((char)decode[res[x]]);
+            if (width > 0 && !fastq && ++col == (uint64_t)width && x + 1 < e) {
+                buf.push_back('\n');
+                col = 0;
+            }
+        }
+        buf.push_back('\n');
+        if (fastq) {
+            buf.push_back('+');
+            buf.push_back('\n');
+            buf.insert(buf.end(), (size_t)(e - a), 'I');
+            buf.push_back('\n');
+        }
+        if (buf.size() >= (1 << 22)) ok = flush();
+    }
+    ok = ok && flush();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : -1;
 }
 
 // FNV-1a 64 of a buffer (content hash recorded with each workload).

@@ -63,6 +63,25 @@ class Workload:
         return f"{h2:016x}-{h1:016x}"
 
 
+def write_fastx(path: str, targets: Targets, matrix: str, fastq: bool = False, width: int = 60) -> None:
+    """Writes targets as FASTA (or FASTQ) text, residues decoded with the
+    matrix's decoding table (records named "t<index> synthetic")."""
+    from . import decoding
+    L = _lib()
+    dec = np.zeros(256, np.uint8)
+    d = decoding(matrix)
+    dec[: len(d)] = d
+    P = ctypes.POINTER
+    L.ish_wl_write_fastx.argtypes = [ctypes.c_char_p, P(ctypes.c_uint8), P(ctypes.c_uint64), ctypes.c_int64,
+                                     P(ctypes.c_uint8), ctypes.c_int32, ctypes.c_int32]
+    L.ish_wl_write_fastx.restype = ctypes.c_int32
+    rc = L.ish_wl_write_fastx(os.fsencode(path), targets.residues.ctypes.data_as(P(ctypes.c_uint8)),
+                              targets.offsets.ctypes.data_as(P(ctypes.c_uint64)), targets.n,
+                              dec.ctypes.data_as(P(ctypes.c_uint8)), int(fastq), width)
+    if rc != 0:
+        raise IshError(f"cannot write {path}")
+
+
 def query(kind: int, seed: int, qlen: int | None = None) -> np.ndarray:
     qlen = KIND_QUERY_LEN.get(kind, 350) if qlen is None else qlen
     out = np.zeros(max(qlen, 1), np.uint8)

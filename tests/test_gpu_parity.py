@@ -497,3 +497,27 @@ def test_graph_replays_match_eager(eng):
     finally:
         eng.set_graphs(False)
         db.close()
+
+
+@pytest.mark.parametrize("kind", ["actgn", "blosum62", "ascii"])
+def test_db_load_ascii_encodes_on_device(eng, kind):
+    """Raw bytes (every byte value, lengths that leave 16-byte tails)
+    encoded on the device score exactly like host-encoded residues."""
+    rng = np.random.default_rng(zlib.crc32(kind.encode()))
+    raw = [rng.integers(0, 256, int(rng.integers(0, 300))).astype(np.uint8) for _ in range(777)]
+    raw[5] = np.arange(256, dtype=np.uint8)
+    enc = ish_amd.encoding(kind)
+    A = len(ish_amd.decoding(kind))
+    q = rng.integers(0, A if kind != "ascii" else 256, 90).astype(np.uint8)
+    q = enc[q] if kind == "ascii" else q
+    eng.set_engine("batched")
+    eng.set_tile(32)
+    eng.set_query(q, kind, 3, 1, "TTTT")
+    db_a = eng.load_db_ascii(Targets.from_list(raw), kind)
+    db_e = eng.load_db(Targets.from_list([enc[r] for r in raw]))
+    try:
+        for x, y in zip(eng.align_db(db_a), eng.align_db(db_e)):
+            assert np.array_equal(x, y)
+    finally:
+        db_a.close()
+        db_e.close()
