@@ -362,12 +362,4 @@ int32_t oracle_batch(int32_t algo, const uint8_t *q, int32_t Q,
                      score, end_ref, end_query, nthreads);
 }
 
-int32_t oracle_batch_striped(int32_t algo, const uint8_t *q, int32_t Q,
-                             const uint8_t *residues, const uint64_t *offsets,
-                             int64_t n, const int8_t *S, int32_t A, int32_t go,
-                             int32_t ge, uint32_t flags, int32_t *score,
-                             int32_t *end_ref, int32_t *end_query,
-                             int32_t nthreads) {
-    return run_batch(1, algo, q, Q, residues, offsets, n, S, A, go, ge, flags,
-                     score, end_ref, end_query, nthreads);
-}
+/* oracle_batch_striped lives in striped.cpp (query state built once). */

@@ -104,7 +104,7 @@ int32_t oracle_batch(int32_t algo, const uint8_t *q, int32_t Q,
                      int32_t *end_ref, int32_t *end_query, int32_t nthreads);
 
 /* ---- Farrar-striped restatement (oracle/striped.c) --------------------- */
-/* Striped int16 semi-global, width W lanes (8, 16 or 32). Follows
+/* Striped int16 semi-global, width W lanes (8, 16 or 32; 0 = native). Follows
  * semi_global_aln (striped.mojo:252-498) including its lazy-F loop and its
  * strided visiting order in the free-query-end pass. */
 oracle_end_t oracle_sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
@@ -116,8 +116,10 @@ oracle_end_t oracle_sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
 oracle_end_t oracle_sw_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
                                int32_t R, const int8_t *S, int32_t A,
                                int32_t go, int32_t ge, int32_t W);
-/* Batch driver for the striped restatement: the CPU baseline that is timed.
- * Returns the number of threads used. */
+/* Batch driver for the striped restatement at the host's native int16
+ * width (32 lanes with AVX-512BW, else 16): the CPU baseline that is timed.
+ * The query profile is built once per call, as the reference's matcher
+ * does. Returns the number of threads used. */
 int32_t oracle_batch_striped(int32_t algo, const uint8_t *q, int32_t Q,
                              const uint8_t *residues, const uint64_t *offsets,
                              int64_t n, const int8_t *S, int32_t A, int32_t go,

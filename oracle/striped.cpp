@@ -20,7 +20,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
+
+#if defined(__AVX2__)
+#include <immintrin.h>
+#endif
 
 #include "oracle.h"
 
@@ -44,6 +50,22 @@ template <typename T> static inline T sat(int32_t v) {
 template <typename T, int W> struct Vec {
     T v[W];
 };
+
+/* Lane access (the algorithms touch single lanes only outside the
+ * per-segment loops). */
+template <typename T, int W> static inline T lget(const Vec<T, W> &a, int k) { return a.v[k]; }
+template <typename T, int W> static inline void lset(Vec<T, W> &a, int k, T x) { a.v[k] = x; }
+
+template <typename T, int W> static inline bool vall_eq(const Vec<T, W> &a, const Vec<T, W> &b) {
+    bool eq = true;
+    for (int k = 0; k < W; ++k) eq &= a.v[k] == b.v[k];
+    return eq;
+}
+template <typename T, int W> static inline int32_t vhmax(const Vec<T, W> &a) {
+    int32_t m = 0;
+    for (int k = 0; k < W; ++k) m = std::max<int32_t>(m, a.v[k]);
+    return m;
+}
 
 template <typename T, int W> static inline Vec<T, W> vset(int32_t x) {
     Vec<T, W> r;
@@ -82,6 +104,114 @@ static inline bool vany_gt(const Vec<T, W> &a, const Vec<T, W> &b) {
     return any;
 }
 
+/* Native-width lanes. The generic helpers above are the definition; these
+ * overloads compute the same thing with AVX2 / AVX-512BW instructions
+ * (saturating add/sub, max, whole-vector shift by one lane), so that the
+ * timed CPU baseline runs at the speed of the reference's SIMD code. */
+#if defined(__AVX2__)
+#define ORACLE_SIMD_DEFS(TYPE, LANES, REG, PFX, ADDS, SUBS, MAX, GT_ANY, SHR1)       \
+    template <> struct Vec<TYPE, LANES> {                                             \
+        REG m;                                                                        \
+    };                                                                                \
+    template <> inline TYPE lget<TYPE, LANES>(const Vec<TYPE, LANES> &a, int k) {     \
+        alignas(64) TYPE t[LANES];                                                    \
+        memcpy(t, &a.m, sizeof t);                                                    \
+        return t[k];                                                                  \
+    }                                                                                 \
+    template <> inline void lset<TYPE, LANES>(Vec<TYPE, LANES> & a, int k, TYPE x) {  \
+        alignas(64) TYPE t[LANES];                                                    \
+        memcpy(t, &a.m, sizeof t);                                                    \
+        t[k] = x;                                                                     \
+        memcpy(&a.m, t, sizeof t);                                                    \
+    }                                                                                 \
+    template <> inline bool vall_eq<TYPE, LANES>(const Vec<TYPE, LANES> &a, const Vec<TYPE, LANES> &b) { \
+        return memcmp(&a.m, &b.m, sizeof(REG)) == 0;                                  \
+    }                                                                                 \
+    template <> inline int32_t vhmax<TYPE, LANES>(const Vec<TYPE, LANES> &a) {        \
+        alignas(64) TYPE t[LANES];                                                    \
+        memcpy(t, &a.m, sizeof t);                                                    \
+        int32_t m = 0;                                                                \
+        for (int k = 0; k < LANES; ++k) m = std::max<int32_t>(m, t[k]);               \
+        return m;                                                                     \
+    }                                                                                 \
+    template <> inline Vec<TYPE, LANES> vset<TYPE, LANES>(int32_t x) {                \
+        alignas(64) TYPE t[LANES];                                                    \
+        for (int k = 0; k < LANES; ++k) t[k] = (TYPE)x;                               \
+        Vec<TYPE, LANES> r;                                                           \
+        memcpy(&r.m, t, sizeof t);                                                    \
+        return r;                                                                     \
+    }                                                                                 \
+    static inline Vec<TYPE, LANES> vadds(const Vec<TYPE, LANES> &a, const Vec<TYPE, LANES> &b) { \
+        Vec<TYPE, LANES> r;                                                           \
+        r.m = ADDS(a.m, b.m);                                                         \
+        return r;                                                                     \
+    }                                                                                 \
+    static inline Vec<TYPE, LANES> vsubs(const Vec<TYPE, LANES> &a, const Vec<TYPE, LANES> &b) { \
+        Vec<TYPE, LANES> r;                                                           \
+        r.m = SUBS(a.m, b.m);                                                         \
+        return r;                                                                     \
+    }                                                                                 \
+    static inline Vec<TYPE, LANES> vmax(const Vec<TYPE, LANES> &a, const Vec<TYPE, LANES> &b) { \
+        Vec<TYPE, LANES> r;                                                           \
+        r.m = MAX(a.m, b.m);                                                          \
+        return r;                                                                     \
+    }                                                                                 \
+    static inline bool vany_gt(const Vec<TYPE, LANES> &a, const Vec<TYPE, LANES> &b) { \
+        return GT_ANY(a.m, b.m);                                                      \
+    }                                                                                 \
+    static inline Vec<TYPE, LANES> vshr1(const Vec<TYPE, LANES> &a) {                 \
+        Vec<TYPE, LANES> r;                                                           \
+        r.m = SHR1(a.m);                                                              \
+        return r;                                                                     \
+    }
+
+/* 256 bits: lane shift toward higher indices by `bytes`, zero fill. */
+template <int BYTES> static inline __m256i shr_lanes256(__m256i a) {
+    const __m256i t = _mm256_permute2x128_si256(a, a, 0x08); /* [0, a.lo] */
+    return _mm256_alignr_epi8(a, t, 16 - BYTES);
+}
+static inline bool gt_any_i16_256(__m256i a, __m256i b) {
+    return _mm256_movemask_epi8(_mm256_cmpgt_epi16(a, b)) != 0;
+}
+static inline bool gt_any_u8_256(__m256i a, __m256i b) {
+    return _mm256_movemask_epi8(_mm256_cmpeq_epi8(_mm256_max_epu8(a, b), b)) != -1;
+}
+static inline bool gt_any_u16_256(__m256i a, __m256i b) {
+    return _mm256_movemask_epi8(_mm256_cmpeq_epi16(_mm256_max_epu16(a, b), b)) != -1;
+}
+ORACLE_SIMD_DEFS(int16_t, 16, __m256i, 256, _mm256_adds_epi16, _mm256_subs_epi16, _mm256_max_epi16,
+                 gt_any_i16_256, shr_lanes256<2>)
+ORACLE_SIMD_DEFS(uint8_t, 32, __m256i, 256, _mm256_adds_epu8, _mm256_subs_epu8, _mm256_max_epu8,
+                 gt_any_u8_256, shr_lanes256<1>)
+ORACLE_SIMD_DEFS(uint16_t, 16, __m256i, 256, _mm256_adds_epu16, _mm256_subs_epu16, _mm256_max_epu16,
+                 gt_any_u16_256, shr_lanes256<2>)
+#endif
+
+#if defined(__AVX512BW__)
+/* 512 bits: [0, a.l0, a.l1, a.l2] by 128-bit lanes, then a byte align. */
+template <int BYTES> static inline __m512i shr_lanes512(__m512i a) {
+    const __m512i t = _mm512_alignr_epi64(a, _mm512_setzero_si512(), 6);
+    return _mm512_alignr_epi8(a, t, 16 - BYTES);
+}
+static inline bool gt_any_i16_512(__m512i a, __m512i b) { return _mm512_cmpgt_epi16_mask(a, b) != 0; }
+static inline bool gt_any_u8_512(__m512i a, __m512i b) { return _mm512_cmpgt_epu8_mask(a, b) != 0; }
+static inline bool gt_any_u16_512(__m512i a, __m512i b) { return _mm512_cmpgt_epu16_mask(a, b) != 0; }
+ORACLE_SIMD_DEFS(int16_t, 32, __m512i, 512, _mm512_adds_epi16, _mm512_subs_epi16, _mm512_max_epi16,
+                 gt_any_i16_512, shr_lanes512<2>)
+ORACLE_SIMD_DEFS(uint8_t, 64, __m512i, 512, _mm512_adds_epu8, _mm512_subs_epu8, _mm512_max_epu8,
+                 gt_any_u8_512, shr_lanes512<1>)
+ORACLE_SIMD_DEFS(uint16_t, 32, __m512i, 512, _mm512_adds_epu16, _mm512_subs_epu16, _mm512_max_epu16,
+                 gt_any_u16_512, shr_lanes512<2>)
+#endif
+
+/* The widest int16 lane count of this build (the reference's
+ * simd_width_of for int16 on this host). */
+#if defined(__AVX512BW__)
+constexpr int kNativeW = 32;
+#else
+constexpr int kNativeW = 16;
+#endif
+
 /* Profile.generate_query_profile: P[c*seg + s].lane[k] = S[c][q[s + k*seg]],
  * padding rows score `pad`. */
 template <typename T, int W>
@@ -95,45 +225,71 @@ static void make_profile(const uint8_t *q, int32_t Q, const int8_t *S,
             for (int k = 0; k < W; ++k) {
                 int32_t j = s + k * seg;
                 int32_t val = j >= Q ? bias : (int32_t)S[(int64_t)c * A + q[j]] + bias;
-                P[(size_t)c * seg + s].v[k] = (T)val;
+                lset(P[(size_t)c * seg + s], k, (T)val);
             }
 }
 
+/* Query-side state of semi_global_aln (the matcher builds it once per
+ * query, striped_semi_global_matcher.mojo:41-75). */
+template <int W> struct SgQuery {
+    typedef Vec<int16_t, W> V;
+    std::vector<V> P;
+    int32_t seg = 0, Q = 0, go = 0, ge = 0, min_score = 0, max_score = 0;
+    uint32_t flags = 0;
+    void prepare(const uint8_t *q, int32_t Qn, const int8_t *S, int32_t A, int32_t gop,
+                 int32_t gex, uint32_t fl) {
+        Q = Qn;
+        go = gop;
+        ge = gex;
+        flags = fl;
+        min_score = max_score = 0;
+        for (int64_t k = 0; k < (int64_t)A * A; ++k) {
+            min_score = std::min<int32_t>(min_score, S[k]);
+            max_score = std::max<int32_t>(max_score, S[k]);
+        }
+        if (Q > 0) make_profile<int16_t, W>(q, Q, S, A, 0, P, seg);
+    }
+};
+
+/* Per-thread buffers of semi_global_aln (the reference allocates them per
+ * call, striped.mojo:288-293; reusing them changes no result). */
+template <int W> struct SgWork {
+    std::vector<Vec<int16_t, W>> h_store, h_load, pv_e;
+    std::vector<int16_t> boundary;
+};
+
 template <int W>
-oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
-                        int32_t R, const int8_t *S, int32_t A, int32_t go,
-                        int32_t ge, uint32_t flags) {
+oracle_end_t sg_striped_run(const SgQuery<W> &qy, const uint8_t *r, int32_t R, SgWork<W> &wk) {
     typedef int16_t T;
     typedef Vec<T, W> V;
     const int32_t MIN = Lim<T>::MIN, MAX = Lim<T>::MAX;
     oracle_end_t res = {0, 0, 0};
+    const int32_t Q = qy.Q, go = qy.go, ge = qy.ge, seg = qy.seg;
     if (Q == 0 || R == 0) return res;
+    const uint32_t flags = qy.flags;
     const bool fqs = flags & ORACLE_QS, fqe = flags & ORACLE_QE;
     const bool fts = flags & ORACLE_TS, fte = flags & ORACLE_TE;
-
-    int32_t min_score = 0, max_score = 0;
-    for (int64_t k = 0; k < (int64_t)A * A; ++k) {
-        min_score = std::min<int32_t>(min_score, S[k]);
-        max_score = std::max<int32_t>(max_score, S[k]);
-    }
-    std::vector<V> P;
-    int32_t seg;
-    make_profile<T, W>(q, Q, S, A, 0, P, seg);
+    const int32_t min_score = qy.min_score;
+    const std::vector<V> &P = qy.P;
 
     int32_t end_query = Q - 1, end_reference = -1;
     const int32_t offset = (Q - 1) % seg;
     const int32_t position = (W - 1) - (Q - 1) / seg;
     const int pos_lane = W - 1 - position; /* lane holding row Q-1 */
 
-    std::vector<V> h_store(seg), h_load(seg), pv_e(seg);
-    std::vector<T> boundary((size_t)R + 1);
+    std::vector<V> &h_store = wk.h_store, &h_load = wk.h_load, &pv_e = wk.pv_e;
+    std::vector<T> &boundary = wk.boundary;
+    h_store.resize(seg);
+    h_load.resize(seg);
+    pv_e.resize(seg);
+    boundary.resize((size_t)R + 1);
     for (int32_t s = 0; s < seg; ++s) {
-        V h, e;
+        V h = vset<T, W>(0), e = vset<T, W>(0);
         for (int k = 0; k < W; ++k) {
             int32_t tmp = fqs ? 0 : (int32_t)(int16_t)(-go - (int16_t)(ge * (k * seg + s)));
-            h.v[k] = (T)(tmp < MIN ? MIN : tmp);
+            lset(h, k, (T)(tmp < MIN ? MIN : tmp));
             tmp = (int32_t)(int16_t)(tmp - go);
-            e.v[k] = (T)(tmp < MIN ? MIN : tmp);
+            lset(e, k, (T)(tmp < MIN ? MIN : tmp));
         }
         h_store[s] = h;
         pv_e[s] = e;
@@ -155,7 +311,7 @@ oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
         V v_h = vshr1(h_store[seg - 1]);
         const V *prof = &P[(size_t)r[i] * seg];
         std::swap(h_load, h_store);
-        v_h.v[0] = boundary[i];
+        lset(v_h, 0, boundary[i]);
         for (int32_t j = 0; j < seg; ++j) {
             v_h = vadds(v_h, prof[j]);
             v_e = pv_e[j];
@@ -176,7 +332,7 @@ oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
             int32_t tmp = fts ? -go : (int32_t)boundary[i + 1] - go;
             T tmp2 = (T)(tmp < MIN ? MIN : tmp);
             v_f = vshr1(v_f);
-            v_f.v[0] = tmp2;
+            lset(v_f, 0, tmp2);
             for (int32_t j = 0; j < seg; ++j) {
                 v_h = h_store[j];
                 v_h = vmax(v_h, v_f);
@@ -190,12 +346,12 @@ oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
             }
         }
         v_h = h_store[offset];
-        bool cmp = v_h.v[pos_lane] > v_max_h.v[pos_lane];
+        bool cmp = lget(v_h, pos_lane) > lget(v_max_h, pos_lane);
         v_max_h = vmax(v_h, v_max_h);
         if (cmp) end_reference = i;
     }
     if (fte) {
-        score = v_max_h.v[pos_lane];
+        score = lget(v_max_h, pos_lane);
         end_query = Q - 1;
     }
     if (fqe) {
@@ -203,7 +359,7 @@ oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
             for (int k = 0; k < W; ++k) {
                 int32_t temp = s + k * seg;
                 if (temp >= Q) continue;
-                T hv = h_store[s].v[k];
+                T hv = lget(h_store[s], k);
                 if (hv > score) {
                     score = hv;
                     end_query = temp;
@@ -214,7 +370,7 @@ oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
             }
     }
     if (!fte && !fqe) {
-        score = h_store[offset].v[pos_lane];
+        score = lget(h_store[offset], pos_lane);
         end_reference = R - 1;
         end_query = Q - 1;
     }
@@ -225,20 +381,34 @@ oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
     return res;
 }
 
+
+template <int W>
+oracle_end_t sg_striped(const uint8_t *q, int32_t Q, const uint8_t *r, int32_t R, const int8_t *S,
+                        int32_t A, int32_t go, int32_t ge, uint32_t flags) {
+    SgQuery<W> qy;
+    qy.prepare(q, Q, S, A, go, ge, flags);
+    SgWork<W> wk;
+    return sg_striped_run<W>(qy, r, R, wk);
+}
+
 /* sw[dt, width] (local_aln/striped.mojo:272-434), forward direction,
  * terminate disabled. Returns the best with overflow flagged by
  * best+bias >= MAX (the caller reruns wider). */
+template <typename T, int W> struct SwWork {
+    std::vector<Vec<T, W>> h_store, h_load, pv_e, h_max;
+};
+
 template <typename T, int W>
-oracle_end_t sw_striped_t(const uint8_t *q, int32_t Q, const uint8_t *r,
-                          int32_t R, const int8_t *S, int32_t A, int32_t go,
-                          int32_t ge, int32_t bias, bool *overflow) {
+oracle_end_t sw_striped_t(const std::vector<Vec<T, W>> &P, int32_t seg, int32_t Q,
+                          const uint8_t *r, int32_t R, int32_t go, int32_t ge, int32_t bias,
+                          bool *overflow, SwWork<T, W> &wk) {
     typedef Vec<T, W> V;
     const int32_t MAXV = Lim<T>::MAX;
-    std::vector<V> P;
-    int32_t seg;
-    make_profile<T, W>(q, Q, S, A, bias, P, seg);
-    std::vector<V> h_store(seg, vset<T, W>(0)), h_load(seg, vset<T, W>(0)),
-        pv_e(seg, vset<T, W>(0)), h_max(seg, vset<T, W>(0));
+    std::vector<V> &h_store = wk.h_store, &h_load = wk.h_load, &pv_e = wk.pv_e, &h_max = wk.h_max;
+    h_store.assign(seg, vset<T, W>(0));
+    h_load.assign(seg, vset<T, W>(0));
+    pv_e.assign(seg, vset<T, W>(0));
+    h_max.assign(seg, vset<T, W>(0));
     const V zero = vset<T, W>(0), v_go = vset<T, W>(go), v_ge = vset<T, W>(ge),
             v_bias = vset<T, W>(bias);
     V v_max_score = zero, v_max_mark = zero;
@@ -282,11 +452,8 @@ oracle_end_t sw_striped_t(const uint8_t *q, int32_t Q, const uint8_t *r,
             }
         }
         v_max_score = vmax(v_max_score, v_max_col);
-        bool all_eq = true;
-        for (int k = 0; k < W; ++k) all_eq &= v_max_mark.v[k] == v_max_score.v[k];
-        if (!all_eq) {
-            int32_t temp = 0;
-            for (int k = 0; k < W; ++k) temp = std::max<int32_t>(temp, v_max_score.v[k]);
+        if (!vall_eq(v_max_mark, v_max_score)) {
+            const int32_t temp = vhmax(v_max_score);
             v_max_mark = v_max_score;
             if (temp > max_score) {
                 max_score = temp;
@@ -301,7 +468,7 @@ oracle_end_t sw_striped_t(const uint8_t *q, int32_t Q, const uint8_t *r,
     }
     for (int32_t s = 0; s < seg; ++s)
         for (int k = 0; k < W; ++k)
-            if ((int32_t)h_max[s].v[k] == max_score) {
+            if ((int32_t)lget(h_max[s], k) == max_score) {
                 int32_t temp = s + k * seg;
                 if (temp < end_query) end_query = temp;
             }
@@ -309,21 +476,96 @@ oracle_end_t sw_striped_t(const uint8_t *q, int32_t Q, const uint8_t *r,
     return res;
 }
 
+/* Query-side state of ssw_align: the biased u8 and u16 profiles
+ * (local_aln/striped.mojo:110-139), built once per query. */
+template <int W> struct SwQuery {
+    std::vector<Vec<uint8_t, 2 * W>> P8;
+    std::vector<Vec<uint16_t, W>> P16;
+    int32_t seg8 = 0, seg16 = 0, Q = 0, go = 0, ge = 0, bias = 0;
+    void prepare(const uint8_t *q, int32_t Qn, const int8_t *S, int32_t A, int32_t gop, int32_t gex) {
+        Q = Qn;
+        go = gop;
+        ge = gex;
+        bias = 0;
+        for (int64_t k = 0; k < (int64_t)A * A; ++k) bias = std::min<int32_t>(bias, S[k]);
+        bias = -bias;
+        make_profile<uint8_t, 2 * W>(q, Q, S, A, bias, P8, seg8);
+        make_profile<uint16_t, W>(q, Q, S, A, bias, P16, seg16);
+    }
+};
+template <int W> struct SwWorkAll {
+    SwWork<uint8_t, 2 * W> w8;
+    SwWork<uint16_t, W> w16;
+};
+
 template <int W>
-oracle_end_t sw_striped(const uint8_t *q, int32_t Q, const uint8_t *r,
-                        int32_t R, const int8_t *S, int32_t A, int32_t go,
-                        int32_t ge) {
-    int32_t bias = 0;
-    for (int64_t k = 0; k < (int64_t)A * A; ++k) bias = std::min<int32_t>(bias, S[k]);
-    bias = -bias;
+oracle_end_t sw_striped_run(const SwQuery<W> &qy, const uint8_t *r, int32_t R, SwWorkAll<W> &wk) {
     bool ov = false;
-    oracle_end_t e = sw_striped_t<uint8_t, 2 * W>(q, Q, r, R, S, A, go, ge, bias, &ov);
-    if (e.score + bias >= 255)
-        e = sw_striped_t<uint16_t, W>(q, Q, r, R, S, A, go, ge, bias, &ov);
+    oracle_end_t e = sw_striped_t<uint8_t, 2 * W>(qy.P8, qy.seg8, qy.Q, r, R, qy.go, qy.ge, qy.bias,
+                                                  &ov, wk.w8);
+    if (e.score + qy.bias >= 255)
+        e = sw_striped_t<uint16_t, W>(qy.P16, qy.seg16, qy.Q, r, R, qy.go, qy.ge, qy.bias, &ov,
+                                      wk.w16);
     return e;
 }
 
+template <int W>
+oracle_end_t sw_striped(const uint8_t *q, int32_t Q, const uint8_t *r, int32_t R, const int8_t *S,
+                        int32_t A, int32_t go, int32_t ge) {
+    SwQuery<W> qy;
+    qy.prepare(q, Q, S, A, go, ge);
+    SwWorkAll<W> wk;
+    return sw_striped_run<W>(qy, r, R, wk);
+}
+
+/* Threaded batch over targets with the query state built once, at lane
+ * count W. */
+template <int W>
+int32_t striped_batch(int32_t algo, const uint8_t *q, int32_t Q, const uint8_t *res,
+                      const uint64_t *off, int64_t n, const int8_t *S, int32_t A, int32_t go,
+                      int32_t ge, uint32_t flags, int32_t *score, int32_t *end_ref,
+                      int32_t *end_query, int32_t nthreads) {
+    SgQuery<W> sgq;
+    SwQuery<W> swq;
+    if (algo == 0) sgq.prepare(q, Q, S, A, go, ge, flags);
+    else swq.prepare(q, Q, S, A, go, ge);
+    std::atomic<int64_t> next(0);
+    auto worker = [&]() {
+        SgWork<W> sgw;
+        SwWorkAll<W> sww;
+        for (;;) {
+            const int64_t lo = next.fetch_add(64);
+            if (lo >= n) break;
+            const int64_t hi = std::min<int64_t>(n, lo + 64);
+            for (int64_t t = lo; t < hi; ++t) {
+                const uint8_t *r = res + off[t];
+                const int32_t R = (int32_t)(off[t + 1] - off[t]);
+                const oracle_end_t e = algo == 0 ? sg_striped_run<W>(sgq, r, R, sgw)
+                                                 : sw_striped_run<W>(swq, r, R, sww);
+                score[t] = e.score;
+                end_ref[t] = e.end_ref;
+                end_query[t] = e.end_query;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int32_t k = 0; k < nthreads; ++k) th.emplace_back(worker);
+    for (auto &x : th) x.join();
+    return nthreads;
+}
+
 } // namespace
+
+extern "C" int32_t oracle_batch_striped(int32_t algo, const uint8_t *q, int32_t Q,
+                                        const uint8_t *residues, const uint64_t *offsets,
+                                        int64_t n, const int8_t *S, int32_t A, int32_t go,
+                                        int32_t ge, uint32_t flags, int32_t *score,
+                                        int32_t *end_ref, int32_t *end_query, int32_t nthreads) {
+    if (nthreads <= 0) nthreads = (int32_t)std::thread::hardware_concurrency();
+    nthreads = std::max(1, std::min(nthreads, 256));
+    return striped_batch<kNativeW>(algo, q, Q, residues, offsets, n, S, A, go, ge, flags, score,
+                                   end_ref, end_query, nthreads);
+}
 
 extern "C" oracle_end_t oracle_sg_striped(const uint8_t *q, int32_t Q,
                                           const uint8_t *r, int32_t R,
@@ -333,7 +575,8 @@ extern "C" oracle_end_t oracle_sg_striped(const uint8_t *q, int32_t Q,
     switch (W) {
     case 8: return sg_striped<8>(q, Q, r, R, S, A, go, ge, flags);
     case 32: return sg_striped<32>(q, Q, r, R, S, A, go, ge, flags);
-    default: return sg_striped<16>(q, Q, r, R, S, A, go, ge, flags);
+    case 16: return sg_striped<16>(q, Q, r, R, S, A, go, ge, flags);
+    default: return sg_striped<kNativeW>(q, Q, r, R, S, A, go, ge, flags);
     }
 }
 
@@ -344,6 +587,7 @@ extern "C" oracle_end_t oracle_sw_striped(const uint8_t *q, int32_t Q,
     switch (W) {
     case 8: return sw_striped<8>(q, Q, r, R, S, A, go, ge);
     case 32: return sw_striped<32>(q, Q, r, R, S, A, go, ge);
-    default: return sw_striped<16>(q, Q, r, R, S, A, go, ge);
+    case 16: return sw_striped<16>(q, Q, r, R, S, A, go, ge);
+    default: return sw_striped<kNativeW>(q, Q, r, R, S, A, go, ge);
     }
 }
