@@ -68,7 +68,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(content_hash, tile):
+def measured_traffic(content_hash, tile, schedule=None):
     """HBM bytes per dispatch of the dominant kernel, as rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters measured them for this exact workload
     (profiles/traffic.json, scaled by the copy-kernel calibration); None
@@ -78,7 +78,7 @@ def measured_traffic(content_hash, tile):
         entry = json.load(open(path)).get(content_hash)
     except (OSError, ValueError):
         return None
-    if not entry or entry.get("tile_rows") != tile:
+    if not entry or entry.get("tile_rows") != tile or entry.get("schedule", "wave") != (schedule or "wave"):
         return None
     return entry["bytes"]
 
@@ -407,9 +407,8 @@ def main():
         tuning = None
     else:
         tuning = eng.autotune(db)
-        log(f"autotune ({time.perf_counter() - t0:.1f}s): " +
-            ", ".join(f">={lo}: tile {t} {sc}" for lo, t, sc in tuning))
-    best_tile = args.tile or 0
+        log(f"autotune ({time.perf_counter() - t0:.1f}s): tile {tuning[0]}, schedule {tuning[1]}")
+    best_tile = args.tile or (tuning[0] if tuning else 0)
     # repeated passes replay one captured HIP graph
     eng.set_graphs(True)
     for _ in range(args.warmup):
@@ -520,7 +519,7 @@ def main():
                    "cells": cells, "seed": args.seed, "content_hash": w.content_hash(),
                    "engine": "batched persistent, packed int16 (int32 for wide groups), HIP graph replay",
                    "tile_rows": best_tile or None,
-                   "tuning": [{"min_len": lo, "tile": t, "schedule": sc} for lo, t, sc in tuning] if tuning else None},
+                   "tuning": {"tile": tuning[0], "schedule": tuning[1]} if tuning else None},
         "gcups_kernel": round(gcups_kernel, 2),
         "gcups_e2e": round(cells / e2e_s / 1e9, 2),
         "db_load_ms": round(load_s * 1e3, 1),
@@ -532,8 +531,8 @@ def main():
         "parity_sample": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": measured_traffic(w.content_hash(), best_tile or (
-                         tuning[0][1] if tuning and len({t for _, t, _ in tuning}) == 1 else None)),
+                     "traffic": measured_traffic(w.content_hash(), best_tile,
+                                                 tuning[1] if tuning else None),
                      "ops_per_cell": ops_per_cell, "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }

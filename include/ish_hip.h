@@ -127,33 +127,33 @@ int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine);
  * 305-306). */
 int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane);
 
-/* How the batched local kernel spreads a group of 128 targets: AUTO picks
- * WORKGROUP when the longest groups are far longer than the average work
- * per wave (long reads), else WAVE. WAVE: one wave scores a group through
- * every query chunk. WORKGROUP: the four waves of a workgroup split the
- * query chunks of one group and pipeline them through LDS. A tuning knob
+/* How the batched kernels spread a group of 128 targets. WAVE: one wave
+ * scores a group through every query chunk. WORKGROUP: the four waves of a
+ * workgroup split the query chunks of one group and pipeline them through
+ * LDS (every group). AUTO: local alignment pipelines every group when the
+ * longest groups are far longer than the average work per wave (long
+ * reads); semi-global pipelines just the groups longer than half the
+ * average work per wave, in the same launch as the others. A tuning knob
  * only; results never depend on it. No counterpart in the reference. */
 #define ISH_SCHEDULE_AUTO 0
 #define ISH_SCHEDULE_WAVE 1
 #define ISH_SCHEDULE_WORKGROUP 2
 int32_t ish_hip_set_schedule(ish_hip_ctx *ctx, int32_t schedule);
 
-/* Autotuner: times every candidate launch configuration (tile height 16
- * or 32; for local alignment also the WAVE / WORKGROUP schedule) on a
- * sample of each target-length bucket of db (longest target >= 4096,
- * >= 1024, >= 256, shorter), `reps` timed runs each (0 = 2), for the
- * context's current query, and keeps the fastest per bucket under the key
- * (matrix kind, algorithm, ceil(log2(query length))). Later passes over any
- * database with a query of the same key launch each bucket with its tuned
- * configuration. ish_hip_set_tile switches the table off until the next
- * ish_hip_autotune. Results never depend on it. No counterpart in the
- * reference. */
+/* Autotuner: times whole passes over db for the context's current query
+ * with every launch configuration (tile height 32 or 16; schedule WAVE,
+ * AUTO or WORKGROUP), `reps` timed runs each after a warm-up (0 = 2), and
+ * keeps the fastest under the key (matrix kind, algorithm,
+ * ceil(log2(query length)), length bucket of the database's longest
+ * target: >= 4096, >= 1024, >= 256, shorter). Later passes whose query and
+ * database share that key use the tuned configuration, in one launch.
+ * ish_hip_set_tile switches the table off until the next ish_hip_autotune.
+ * Results never depend on it. No counterpart in the reference. */
 int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t reps);
-/* The tuned (tile, schedule) per target-length bucket for the current
- * query's key, with each bucket's smallest longest-target length; returns
- * the number of buckets (4) or 0 when that key has not been tuned. n >= 4. */
-int32_t ish_hip_tuning(const ish_hip_ctx *ctx, int32_t *tile, int32_t *schedule,
-                       int32_t *bucket_min_len, int32_t n);
+/* The tuned (tile, schedule) for the current query and db's key: returns
+ * 1, or 0 when that key has not been tuned. */
+int32_t ish_hip_tuning(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t *tile,
+                       int32_t *schedule);
 /* HIP graphs for database passes: when on, ish_hip_align_db[_async]
  * captures its launches into a graph the first time and replays it while
  * the launches (kernels, grids, parameters and buffers) stay the same; a

@@ -117,7 +117,7 @@ def lib():
         "ish_hip_set_tile": ([vp, i32], i32),
         "ish_hip_set_schedule": ([vp, i32], i32),
         "ish_hip_autotune": ([vp, vp, i32], i32),
-        "ish_hip_tuning": ([vp, i32p, i32p, i32p, i32], i32),
+        "ish_hip_tuning": ([vp, vp, i32p, i32p], i32),
         "ish_hip_set_graphs": ([vp, i32], i32),
         "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
         "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
@@ -303,21 +303,21 @@ class Engine:
         self._check(self._L.ish_hip_set_graphs(self._h, int(bool(on))))
 
     def autotune(self, db: "Database", reps: int = 2):
-        """Times the launch configurations per target-length bucket for the
-        current query and keeps the fastest; returns tuning()."""
+        """Times whole passes over db with every launch configuration for
+        the current query and keeps the fastest; returns tuning(db)."""
         self._check(self._L.ish_hip_autotune(self._h, db._h, reps))
-        return self.tuning()
+        return self.tuning(db)
 
-    def tuning(self):
-        """[(bucket min length, tile, schedule)] for the current query's key,
-        or [] when it has not been tuned."""
-        t, sc, lo = (np.zeros(4, np.int32) for _ in range(3))
-        n = self._L.ish_hip_tuning(self._h, _p(t, ctypes.c_int32), _p(sc, ctypes.c_int32),
-                                   _p(lo, ctypes.c_int32), 4)
+    def tuning(self, db: "Database"):
+        """(tile, schedule) tuned for the current query and db, or None."""
+        t, sc = ctypes.c_int32(), ctypes.c_int32()
+        n = self._L.ish_hip_tuning(self._h, db._h, ctypes.byref(t), ctypes.byref(sc))
         if n < 0:
             self._check(n)
+        if n == 0:
+            return None
         inv = {v: k for k, v in SCHEDULE.items()}
-        return [(int(lo[b]), int(t[b]), inv.get(int(sc[b]), "auto")) for b in range(n)]
+        return int(t.value), inv.get(int(sc.value), "auto")
 
     def set_query(self, query: np.ndarray, matrix_kind: str, gap_open: int = 3,
                   gap_extend: int = 1, ends_free="FFTT", algo="striped-semi-global"):

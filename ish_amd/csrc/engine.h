@@ -74,6 +74,10 @@ struct ScoreParams {
     // tile_base is the total number of tiles
     const QueryDesc *queries;
     int32_t nqueries;
+
+    // hybrid launches: the first n_long groups of [g_begin, g_end) are
+    // scored by whole workgroups
+    int32_t n_long;
 };
 
 // Plain (one launch per batch) parameters: targets in input order,
@@ -104,6 +108,12 @@ hipError_t launch_sw_batched_i16(const ScoreParams &p, int chunk, int grid,
 hipError_t launch_sw_batched_i32(const ScoreParams &p, int chunk, int grid,
                                  int block, size_t lds, hipStream_t st);
 int occupancy_sw_batched_i16(int chunk, int block, size_t lds);
+// Hybrid semi-global launch (packed int16, block of 256): the first n_long
+// groups are pipelined across a workgroup's four waves.
+size_t sg_pipe_extra_lds();
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+                                hipStream_t st);
+int occupancy_sg_hybrid_i16(int chunk, size_t lds, uint32_t flags);
 // Workgroup-pipelined local kernel (packed int16, 4 waves share a group):
 // extra LDS beyond the profile, launcher (block of 256) and occupancy.
 size_t sw_pipe_extra_lds();

@@ -19,17 +19,20 @@
 
 using namespace ish;
 
-// Target-length buckets of the autotuner: a group belongs to the first
-// bucket whose lower bound its longest target reaches.
+// Autotuning table: (matrix kind, algorithm, ceil(log2 Q), length bucket
+// of the database's longest target) -> the launch configuration of a pass.
 constexpr int kLenBuckets = 4;
 constexpr int32_t kLenBucketLo[kLenBuckets] = {4096, 1024, 256, 0};
 struct TuneChoice {
     int8_t chunk, schedule;
 };
 struct TuneKey {
-    int32_t kind, algo, qbucket;
+    int32_t kind, algo, qbucket, dbucket;
     bool operator<(const TuneKey &o) const {
-        return kind != o.kind ? kind < o.kind : algo != o.algo ? algo < o.algo : qbucket < o.qbucket;
+        if (kind != o.kind) return kind < o.kind;
+        if (algo != o.algo) return algo < o.algo;
+        if (qbucket != o.qbucket) return qbucket < o.qbucket;
+        return dbucket < o.dbucket;
     }
 };
 
@@ -52,7 +55,7 @@ struct ish_hip_ctx {
     int schedule = ISH_SCHEDULE_AUTO; // one wave or one workgroup per group
     // autotuning: (matrix, algorithm, query-length bucket) -> choice per
     // target-length bucket
-    std::map<TuneKey, std::array<TuneChoice, kLenBuckets>> tune;
+    std::map<TuneKey, TuneChoice> tune;
     bool tuned = false;
     // HIP graph of the last db pass, replayed while its launches match
     bool graphs = false;
@@ -640,16 +643,16 @@ extern "C" int32_t ish_hip_db_size(const ish_hip_db *db, int64_t *n, int64_t *nr
 // profile (forward or reversed query) and free-end flags.
 // One persistent launch of a db pass.
 struct DbLaunch {
-    int32_t kind; // kSg16, kSg32, kSw16, kSw32, kSwPipe
+    int32_t kind; // kSg16, kSg32, kSw16, kSw32, kSwPipe, kSgHybrid
     int32_t chunk;
     int32_t g_begin, g_end;
     int32_t grid;
-    int32_t pad;
+    int32_t n_long; // kSgHybrid: leading groups scored by whole workgroups
     int64_t slot_stride;
     uint64_t lds;
     ScoreParams p;
 };
-enum { kSg16 = 0, kSg32 = 1, kSw16 = 2, kSw32 = 3, kSwPipe = 4 };
+enum { kSg16 = 0, kSg32 = 1, kSw16 = 2, kSw32 = 3, kSwPipe = 4, kSgHybrid = 5 };
 
 static int32_t len_bucket(int32_t maxlen) {
     for (int32_t b = 0; b < kLenBuckets - 1; ++b)
@@ -657,10 +660,11 @@ static int32_t len_bucket(int32_t maxlen) {
     return kLenBuckets - 1;
 }
 
-static TuneKey tune_key(const ish_hip_ctx *ctx) {
+static TuneKey tune_key(const ish_hip_ctx *ctx, const ish_hip_db *db) {
     int32_t qb = 0;
     while ((1 << qb) < ctx->Q) ++qb;
-    return TuneKey{ctx->kind, ctx->algo, qb};
+    const int32_t longest = db->ngroups > 0 ? db->group_maxlen[0] : 0;
+    return TuneKey{ctx->kind, ctx->algo, qb, len_bucket(longest)};
 }
 
 // Heuristic schedule of the local int16 groups [g0, g1) at one tile height:
@@ -676,6 +680,21 @@ static bool pipe_heuristic(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t
     const int occ = occupancy_sw_batched_i16(chunk, kBlock, ctx->lds);
     const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
     return db->group_maxlen[g0] > 1.5 * per_wave;
+}
+
+// Semi-global groups [g0, g1) whose longest target exceeds half the
+// average work per resident wave: the leading ones that a hybrid launch
+// scores with whole workgroups (so that no single wave holds the launch
+// open); 0 when none is that long.
+static int32_t long_groups(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t g0, int32_t g1,
+                           int32_t chunk, uint32_t flags) {
+    double cols = 0;
+    for (int32_t g = g0; g < g1; ++g) cols += db->group_maxlen[g] + 3;
+    const int occ = occupancy_sg_hybrid_i16(chunk, ctx->lds + sg_pipe_extra_lds(), flags);
+    const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
+    int32_t n = 0;
+    while (g0 + n < g1 && db->group_maxlen[g0 + n] > 0.5 * per_wave) ++n;
+    return n;
 }
 
 // Plans the launches that score groups [g_lo, g_hi) of db (int32 groups
@@ -711,53 +730,50 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         out.push_back(L);
     }
     const size_t pipe_lds = ctx->lds + sw_pipe_extra_lds();
+    const size_t hybrid_lds = ctx->lds + sg_pipe_extra_lds();
     auto add16 = [&](int32_t g0, int32_t g1, int chunk, int sched) {
         if (g1 <= g0) return;
         DbLaunch L{};
         bool pipe = false;
+        int32_t n_long = 0;
         if (local && pipe_lds <= kLdsLimit) {
             if (sched == ISH_SCHEDULE_WORKGROUP) pipe = true;
             else if (sched == ISH_SCHEDULE_AUTO) pipe = pipe_heuristic(ctx, db, g0, g1, chunk);
+        } else if (!local && hybrid_lds <= kLdsLimit) {
+            if (sched == ISH_SCHEDULE_WORKGROUP) n_long = g1 - g0;
+            else if (sched == ISH_SCHEDULE_AUTO) n_long = long_groups(ctx, db, g0, g1, chunk, flags);
         }
-        L.kind = pipe ? kSwPipe : local ? kSw16 : kSg16;
+        L.kind = pipe ? kSwPipe : local ? kSw16 : n_long > 0 ? kSgHybrid : kSg16;
         L.chunk = chunk;
         L.g_begin = g0;
         L.g_end = g1;
+        L.n_long = n_long;
         out.push_back(L);
     };
     if (forced_chunk > 0) {
         add16(m16, g_hi, forced_chunk, forced_sched);
     } else {
-        auto it = ctx->tuned ? ctx->tune.find(tune_key(ctx)) : ctx->tune.end();
+        auto it = ctx->tuned ? ctx->tune.find(tune_key(ctx, db)) : ctx->tune.end();
         if (it == ctx->tune.end()) {
             add16(m16, g_hi, ctx->chunk, ctx->schedule);
         } else {
-            // one launch per run of target-length buckets with the same
-            // tuned choice
-            int32_t g = m16;
-            while (g < g_hi) {
-                const TuneChoice &c = it->second[(size_t)len_bucket(db->group_maxlen[g])];
-                int32_t e = g;
-                while (e < g_hi) {
-                    const TuneChoice &d = it->second[(size_t)len_bucket(db->group_maxlen[e])];
-                    if (d.chunk != c.chunk || d.schedule != c.schedule) break;
-                    ++e;
-                }
-                add16(g, e, c.chunk, ctx->schedule != ISH_SCHEDULE_AUTO ? ctx->schedule : c.schedule);
-                g = e;
-            }
+            // the tuned configuration for the whole pass (one launch: a
+            // launch per length range would add a tail per launch)
+            const TuneChoice &c = it->second;
+            add16(m16, g_hi, c.chunk, ctx->schedule != ISH_SCHEDULE_AUTO ? ctx->schedule : c.schedule);
         }
     }
     // grids and scratch for every launch before any is enqueued (growing the
     // scratch between launches would free memory a queued launch uses)
     for (DbLaunch &L : out) {
-        L.lds = L.kind == kSwPipe ? pipe_lds : ctx->lds;
+        L.lds = L.kind == kSwPipe ? pipe_lds : L.kind == kSgHybrid ? hybrid_lds : ctx->lds;
         int occ = 1;
         switch (L.kind) {
         case kSg32: occ = occupancy_sg_batched_i32(L.chunk, kBlock, L.lds, flags); break;
         case kSg16: occ = occupancy_sg_batched_i16(L.chunk, kBlock, L.lds, flags); break;
         case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
         case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
+        case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.lds, flags); break;
         default: occ = occupancy_sw_pipe_i16(L.chunk, L.lds); break;
         }
         int grid;
@@ -792,6 +808,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         p.slot_stride = L.slot_stride;
         p.g_begin = L.g_begin;
         p.g_end = L.g_end;
+        p.n_long = L.n_long;
     }
     return ISH_OK;
 }
@@ -805,6 +822,7 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
     case kSg16: e = launch_sg_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw32: e = launch_sw_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw16: e = launch_sw_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
+    case kSgHybrid: e = launch_sg_hybrid_i16(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
     default: e = launch_sw_pipe_i16(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
     }
     if (e == hipSuccess && timed) e = timed_end(ctx);
@@ -880,82 +898,54 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     if (ctx->engine != ISH_ENGINE_BATCHED) return fail(ctx, ISH_E_INVALID, "autotuning needs the batched engine");
     if (reps <= 0) reps = 2;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const bool local = ctx->algo == ISH_ALGO_LOCAL;
-    const int32_t chunks[2] = {16, 32};
-    std::array<TuneChoice, kLenBuckets> best;
-    for (auto &c : best) c = TuneChoice{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO};
-    int32_t g = 0;
-    // int16 groups only: the int32 ones keep the default tile
-    {
-        int32_t lo = 0, hi = db->ngroups;
-        while (lo < hi) {
-            const int32_t mid = (lo + hi) / 2;
-            if ((int64_t)db->group_maxlen[mid] > ctx->r16max) lo = mid + 1;
-            else hi = mid;
-        }
-        g = lo;
-    }
-    // sample: up to 4 waves' worth of groups per resident wave slot
-    const int32_t sample = std::max<int32_t>(64, ctx->cus * 8 * 2);
+    const int32_t chunks[2] = {32, 16};
+    const int scheds[3] = {ISH_SCHEDULE_WAVE, ISH_SCHEDULE_AUTO, ISH_SCHEDULE_WORKGROUP};
+    TuneChoice best{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO};
+    float best_ms = 1e30f;
     const int saved_launches = ctx->launches;
-    while (g < db->ngroups) {
-        const int32_t bkt = len_bucket(db->group_maxlen[g]);
-        int32_t e = g;
-        while (e < db->ngroups && len_bucket(db->group_maxlen[e]) == bkt) ++e;
-        int32_t s0 = g, s1 = e;
-        if (s1 - s0 > sample) {
-            s0 = g + (e - g - sample) / 2;
-            s1 = s0 + sample;
-        }
-        float best_ms = 1e30f;
-        for (int32_t ch : chunks) {
-            const int nsched = local ? 2 : 1;
-            for (int si = 0; si < nsched; ++si) {
-                const int sched = si == 0 ? ISH_SCHEDULE_WAVE : ISH_SCHEDULE_WORKGROUP;
-                std::vector<DbLaunch> plan;
-                int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, s0, s1, ch,
-                                     sched, plan);
-                if (rc) return rc;
-                if (sched == ISH_SCHEDULE_WORKGROUP && (plan.empty() || plan.back().kind != kSwPipe))
-                    continue; // the pipeline does not fit this query
-                float total = 0.f;
-                for (int rep = 0; rep <= reps; ++rep) {
-                    ctx->launches = 0;
-                    for (const DbLaunch &L : plan) HIP_TRY(ctx, enqueue_launch(ctx, L, true));
-                    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                    collect_timing(ctx);
-                    if (rep > 0) total += ctx->last_ms; // the first run warms up
-                }
-                if (total < best_ms) {
-                    best_ms = total;
-                    best[(size_t)bkt] = TuneChoice{(int8_t)ch, (int8_t)sched};
-                }
+    std::vector<std::vector<DbLaunch>> seen;
+    for (int32_t ch : chunks) {
+        for (int sched : scheds) {
+            // whole passes: the launch tail is part of what is being tuned
+            std::vector<DbLaunch> plan;
+            int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, 0, db->ngroups,
+                                 ch, sched, plan);
+            if (rc) return rc;
+            bool dup = false; // a schedule that plans the same launches as one already timed
+            for (const auto &q : seen)
+                dup = dup || (q.size() == plan.size() &&
+                              memcmp(q.data(), plan.data(), plan.size() * sizeof(DbLaunch)) == 0);
+            if (dup) continue;
+            seen.push_back(plan);
+            float total = 0.f;
+            for (int rep = 0; rep <= reps; ++rep) {
+                ctx->launches = 0;
+                for (const DbLaunch &L : plan) HIP_TRY(ctx, enqueue_launch(ctx, L, true));
+                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                collect_timing(ctx);
+                if (rep > 0) total += ctx->last_ms; // the first run warms up
+            }
+            if (total < best_ms) {
+                best_ms = total;
+                best = TuneChoice{(int8_t)ch, (int8_t)sched};
             }
         }
-        g = e;
     }
     ctx->launches = saved_launches;
-    ctx->tune[tune_key(ctx)] = best;
+    ctx->tune[tune_key(ctx, db)] = best;
     ctx->tuned = true;
     return ISH_OK;
 }
 
-extern "C" int32_t ish_hip_tuning(const ish_hip_ctx *ctx, int32_t *chunk, int32_t *schedule,
-                                  int32_t *bucket_min_len, int32_t n) {
-    if (!ctx || n < kLenBuckets) return ISH_E_INVALID;
+extern "C" int32_t ish_hip_tuning(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t *tile,
+                                  int32_t *schedule) {
+    if (!ctx || !db) return ISH_E_INVALID;
     if (!ctx->have_query) return ISH_E_NO_QUERY;
-    auto it = ctx->tune.find(tune_key(ctx));
-    for (int32_t b = 0; b < kLenBuckets; ++b) {
-        if (bucket_min_len) bucket_min_len[b] = kLenBucketLo[b];
-        if (it == ctx->tune.end()) {
-            if (chunk) chunk[b] = 0;
-            if (schedule) schedule[b] = 0;
-        } else {
-            if (chunk) chunk[b] = it->second[(size_t)b].chunk;
-            if (schedule) schedule[b] = it->second[(size_t)b].schedule;
-        }
-    }
-    return it == ctx->tune.end() ? 0 : kLenBuckets;
+    auto it = ctx->tune.find(tune_key(ctx, db));
+    if (it == ctx->tune.end()) return 0;
+    if (tile) *tile = it->second.chunk;
+    if (schedule) *schedule = it->second.schedule;
+    return 1;
 }
 
 extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {

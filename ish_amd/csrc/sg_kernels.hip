@@ -111,31 +111,27 @@ struct QueryView {
     int64_t out_off; // offset of this query's results in the output arrays
 };
 
-// Scores one group (PACKED: 128 targets, two per lane) or one half group
-// (int32: 64 targets) against the query whose profile is staged in smem.
-// TE / QE say which end rules the variant can apply; the query's own flags
-// select among them at run time (a variant with TE or QE must only see
-// queries that have te or qe free, since it tracks no corner cell).
-template <typename V, int K, bool TE, bool QE>
-__device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &qv,
-                                         const uint8_t *smem, const uint16_t *slut, uint2 *bnd,
-                                         int g, int half, int lane) {
-    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
-    constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
-    const int go = p.go, ge = p.ge, cst = ge - go, Q = qv.Q;
-    const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
-    const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
-    const V vcst = splat<V>(cst);
-    const GroupDesc gd = p.groups[g];
-    const int tA = gd.first + half * 64 + lane;
-    const int tB = gd.first + 64 + lane;
-    const int RA = p.lens[tA];
-    const int RB = PACKED ? p.lens[tB] : 0;
-    const uint2 *res = p.res + gd.res_off;
-    const int nblk = gd.ncols >> 2;
+// State of one lane while it scores one query chunk (K rows) of its
+// target(s).
+template <typename V, int K> struct SgChunk {
+    V H[K], E[K];
+    V diag_top; // H^ of the row above the chunk, previous column
+};
 
-    TrackI16 t16;
-    TrackI32 t32;
+template <typename V, int K>
+__device__ __forceinline__ void sg_chunk_init(SgChunk<V, K> &st, int j0, bool qs, int go, int ge) {
+    const int cst = ge - go;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const int j = j0 + m;
+        const int hl = qs ? (j - 1) * ge : -go - ge; // H^(j,-1)
+        st.H[m] = splat<V>(hl);
+        st.E[m] = splat<V>(hl + cst); // E^(j,0)
+    }
+    st.diag_top = splat<V>(j0 == 0 ? -2 * ge : (qs ? (j0 - 2) * ge : -go - ge));
+}
+
+__device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
     t16.te_a = t16.te_b = t16.qe_a = t16.qe_b = INT_MIN;
     t16.corner_a = t16.corner_b = 0;
     t32.te_v = INT_MIN;
@@ -143,168 +139,147 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
     t32.qe_v = INT_MIN;
     t32.qe_j = 0;
     t32.corner = 0;
+}
 
-    for (int c = 0; c < qv.nchunks; ++c) {
-        const int j0 = c * K;
-        const bool first = c == 0;
-        const bool last = c == qv.nchunks - 1;
-        const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
-        // LDS address of this chunk's rows in profile row `code`
-        const uint8_t *chunk_base = smem + 2 * j0;
-        auto row_ptr = [&](uint32_t code) -> const short8 * {
-            return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
-        };
-        V H[K], E[K];
+// Scores columns 4b .. 4b+3 of one chunk (rows j0 .. j0+K-1).
+//   rr / rr_next: residue words of this block and the next (the profile
+//   slices of the next column are fetched while a column is scored);
+//   bn: the chunk-top rows (H^, F^ of the row above) unless `first`;
+//   out: where the chunk-bottom rows go (t*64 + lane), null for the last
+//   chunk, which tracks row Q-1 (register mstar) instead.
+// TE / QE say which end rules the variant can apply; qe_rt is the query's.
+template <typename V, int K, bool TE, bool QE>
+__device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool first, bool last,
+                                         int mstar, uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
+                                         short8 (&sa)[K / 8], short8 (&sb)[K / 8],
+                                         const uint8_t *chunk_base, const uint16_t *slut,
+                                         int half, int RA, int RB, bool ts, bool qe_rt, int go,
+                                         int ge, TrackI16 &t16, TrackI32 &t32, uint2 *out,
+                                         int lane) {
+    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    constexpr int NS = K / 8;
+    const int cst = ge - go;
+    const V vcst = splat<V>(cst);
+    auto row_ptr = [&](uint32_t code) -> const short8 * {
+        return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
+    };
 #pragma unroll
-        for (int m = 0; m < K; ++m) {
-            const int j = j0 + m;
-            const int hl = qs ? (j - 1) * ge : -go - ge; // H^(j,-1)
-            H[m] = splat<V>(hl);
-            E[m] = splat<V>(hl + cst); // E^(j,0)
+    for (int t = 0; t < 4; ++t) {
+        const int i = b * 4 + t;
+        // residue codes of the next column (past the end: any code)
+        const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
+        const uint32_t wb = t < 3 ? rr.y : rr_next.y;
+        const int sh = 8 * ((t + 1) & 3);
+        const short8 *na = row_ptr((wa >> sh) & 0xffu);
+        const short8 *nb = row_ptr((wb >> sh) & 0xffu);
+        V htop, f;
+        if (first) {
+            const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
+            htop = splat<V>(ht);
+            f = splat<V>(ht + cst); // F^(0,i)
+        } else {
+            htop = from_u32<V>(bn[t].x);
+            f = from_u32<V>(bn[t].y);
         }
-        V diag_top = splat<V>(j0 == 0 ? -2 * ge : (qs ? (j0 - 2) * ge : -go - ge));
-
-        uint2 rr_next = res[lane];
-        uint2 bn_next[4];
-        if (!first) {
+        V diag = st.diag_top;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
-        }
-        // profile slices of column 0
-        short8 sa[NS], sb[NS];
-        {
-            const short8 *na = row_ptr((half ? rr_next.y : rr_next.x) & 0xffu);
-            const short8 *nb = row_ptr(rr_next.y & 0xffu);
+        for (int k = 0; k < NS; ++k) {
 #pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                sa[k] = na[k];
-                if (PACKED) sb[k] = nb[k];
-            }
-        }
-        for (int b = 0; b < nblk; ++b) {
-            const uint2 rr = rr_next;
-            uint2 bn[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
-            if (b + 1 < nblk) {
-                rr_next = res[(size_t)(b + 1) * 64 + lane];
-                if (!first) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int i = b * 4 + t;
-                // residue codes of the next column (past the end: any code)
-                const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
-                const uint32_t wb = t < 3 ? rr.y : rr_next.y;
-                const int sh = 8 * ((t + 1) & 3);
-                const short8 *na = row_ptr((wa >> sh) & 0xffu);
-                const short8 *nb = row_ptr((wb >> sh) & 0xffu);
-                V htop, f;
-                if (first) {
-                    const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
-                    htop = splat<V>(ht);
-                    f = splat<V>(ht + cst); // F^(0,i)
+            for (int mm = 0; mm < 8; ++mm) {
+                const int m = k * 8 + mm;
+                V s;
+                if constexpr (PACKED) {
+                    s.x = sa[k][mm];
+                    s.y = sb[k][mm];
                 } else {
-                    htop = from_u32<V>(bn[t].x);
-                    f = from_u32<V>(bn[t].y);
+                    s = (int)sa[k][mm];
                 }
-                V diag = diag_top;
+                V h = vmax(vmax(diag + s, st.E[m]), f);
+                diag = st.H[m];
+                st.H[m] = h;
+                const V gg = h + vcst;
+                st.E[m] = vmax(st.E[m], gg);
+                f = vmax(f, gg);
+            }
+            // slice k is consumed: fetch it for the next column. The
+            // scheduling barrier keeps the fetch from being hoisted above
+            // the rows that still read the old slice, so only one extra
+            // slice is live at a time.
+            __builtin_amdgcn_sched_barrier(0);
+            sa[k] = na[k];
+            if (PACKED) sb[k] = nb[k];
+        }
+        st.diag_top = htop;
+        if (!last) {
+            uint2 o;
+            o.x = as_u32(st.H[K - 1]);
+            o.y = as_u32(f);
+            out[t * 64 + lane] = o;
+        } else {
+            // row Q-1 lives in register mstar of the last chunk
+            V hl = st.H[0];
 #pragma unroll
-                for (int k = 0; k < NS; ++k) {
-#pragma unroll
-                    for (int mm = 0; mm < 8; ++mm) {
-                        const int m = k * 8 + mm;
-                        V s;
-                        if constexpr (PACKED) {
-                            s.x = sa[k][mm];
-                            s.y = sb[k][mm];
-                        } else {
-                            s = (int)sa[k][mm];
-                        }
-                        V h = vmax(vmax(diag + s, E[m]), f);
-                        diag = H[m];
-                        H[m] = h;
-                        const V gg = h + vcst;
-                        E[m] = vmax(E[m], gg);
-                        f = vmax(f, gg);
+            for (int m = 1; m < K; ++m)
+                if (m == mstar) hl = st.H[m];
+            // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
+            const V u = hl - splat<V>(i * ge);
+            if constexpr (PACKED) {
+                const uint32_t uu = as_u32(u);
+                if (TE || QE) {
+                    if (i < RA) t16.te_a = max(t16.te_a, key_lo(uu, i));
+                    if (i < RB) t16.te_b = max(t16.te_b, key_hi(uu, i));
+                } else {
+                    if (i == RA - 1) t16.corner_a = (int)(short)(uu & 0xffffu);
+                    if (i == RB - 1) t16.corner_b = (int)(short)(uu >> 16);
+                }
+            } else {
+                const int uv = (int)as_u32(u);
+                if (TE || QE) {
+                    if (i < RA && uv > t32.te_v) {
+                        t32.te_v = uv;
+                        t32.te_i = i;
                     }
-                    // slice k is consumed: fetch it for the next column. The
-                    // scheduling barrier keeps the fetch from being hoisted
-                    // above the rows that still read the old slice, so only
-                    // one extra slice is live at a time.
-                    __builtin_amdgcn_sched_barrier(0);
-                    sa[k] = na[k];
-                    if (PACKED) sb[k] = nb[k];
+                } else if (i == RA - 1) {
+                    t32.corner = uv;
                 }
-                diag_top = htop;
-                if (!last) {
-                    uint2 o;
-                    o.x = as_u32(H[K - 1]);
-                    o.y = as_u32(f);
-                    bnd[(size_t)i * 64 + lane] = o;
-                } else {
-                    // row Q-1 lives in register mstar of the last chunk
-                    V hl = H[0];
+            }
+        }
+        if (qe_rt) {
+            const bool cap_a = i == RA - 1;
+            const bool cap_b = PACKED && i == RB - 1;
+            // wave-uniform test keeps this once-per-target path a real
+            // branch instead of predicated code on every column
+            if (__builtin_amdgcn_ballot_w64(cap_a || cap_b) != 0) {
 #pragma unroll
-                    for (int m = 1; m < K; ++m)
-                        if (m == mstar) hl = H[m];
-                    // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
-                    const V u = hl - splat<V>(i * ge);
+                for (int m = 0; m < K; ++m) {
+                    if (last && m > mstar) continue;
+                    const int j = j0 + m;
+                    // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
+                    const V wv = st.H[m] - splat<V>(j * ge);
                     if constexpr (PACKED) {
-                        const uint32_t uu = as_u32(u);
-                        if (TE || QE) {
-                            if (i < RA) t16.te_a = max(t16.te_a, key_lo(uu, i));
-                            if (i < RB) t16.te_b = max(t16.te_b, key_hi(uu, i));
-                        } else {
-                            if (i == RA - 1) t16.corner_a = (int)(short)(uu & 0xffffu);
-                            if (i == RB - 1) t16.corner_b = (int)(short)(uu >> 16);
-                        }
+                        const uint32_t ww = as_u32(wv);
+                        if (cap_a) t16.qe_a = max(t16.qe_a, key_lo(ww, j));
+                        if (cap_b) t16.qe_b = max(t16.qe_b, key_hi(ww, j));
                     } else {
-                        const int uv = (int)as_u32(u);
-                        if (TE || QE) {
-                            if (i < RA && uv > t32.te_v) {
-                                t32.te_v = uv;
-                                t32.te_i = i;
-                            }
-                        } else if (i == RA - 1) {
-                            t32.corner = uv;
-                        }
-                    }
-                }
-                if (qe_rt) {
-                    const bool cap_a = i == RA - 1;
-                    const bool cap_b = PACKED && i == RB - 1;
-                    // wave-uniform test keeps this once-per-target path a
-                    // real branch instead of predicated code on every column
-                    if (__builtin_amdgcn_ballot_w64(cap_a || cap_b) != 0) {
-#pragma unroll
-                        for (int m = 0; m < K; ++m) {
-                            if (last && m > mstar) continue;
-                            const int j = j0 + m;
-                            // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
-                            const V wv = H[m] - splat<V>(j * ge);
-                            if constexpr (PACKED) {
-                                const uint32_t ww = as_u32(wv);
-                                if (cap_a) t16.qe_a = max(t16.qe_a, key_lo(ww, j));
-                                if (cap_b) t16.qe_b = max(t16.qe_b, key_hi(ww, j));
-                            } else {
-                                const int wi = (int)as_u32(wv);
-                                if (cap_a && wi > t32.qe_v) {
-                                    t32.qe_v = wi;
-                                    t32.qe_j = j;
-                                }
-                            }
+                        const int wi = (int)as_u32(wv);
+                        if (cap_a && wi > t32.qe_v) {
+                            t32.qe_v = wi;
+                            t32.qe_j = j;
                         }
                     }
                 }
             }
         }
     }
+}
 
-    // results
+// Final end rules and the result writes of one lane's target(s).
+template <typename V>
+__device__ __forceinline__ void sg_finish(const ScoreParams &p, const QueryView &qv, int tA, int tB,
+                                          int RA, int RB, bool te_rt, bool qe_rt,
+                                          const TrackI16 &t16, const TrackI32 &t32) {
+    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    const int Q = qv.Q, ge = p.ge;
     const int qoff = (Q - 1) * ge;
     if constexpr (PACKED) {
         for (int which = 0; which < 2; ++which) {
@@ -342,6 +317,220 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
     }
 }
 
+// Scores one group (PACKED: 128 targets, two per lane) or one half group
+// (int32: 64 targets) against the query whose profile is staged in smem,
+// one wave through every query chunk in turn.
+// TE / QE say which end rules the variant can apply; the query's own flags
+// select among them at run time (a variant with TE or QE must only see
+// queries that have te or qe free, since it tracks no corner cell).
+template <typename V, int K, bool TE, bool QE>
+__device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &qv,
+                                         const uint8_t *smem, const uint16_t *slut, uint2 *bnd,
+                                         int g, int half, int lane) {
+    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
+    const int go = p.go, ge = p.ge, Q = qv.Q;
+    const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
+    const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
+    const GroupDesc gd = p.groups[g];
+    const int tA = gd.first + half * 64 + lane;
+    const int tB = gd.first + 64 + lane;
+    const int RA = p.lens[tA];
+    const int RB = PACKED ? p.lens[tB] : 0;
+    const uint2 *res = p.res + gd.res_off;
+    const int nblk = gd.ncols >> 2;
+
+    TrackI16 t16;
+    TrackI32 t32;
+    track_init(t16, t32);
+    for (int c = 0; c < qv.nchunks; ++c) {
+        const int j0 = c * K;
+        const bool first = c == 0;
+        const bool last = c == qv.nchunks - 1;
+        const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
+        // LDS address of this chunk's rows in profile row `code`
+        const uint8_t *chunk_base = smem + 2 * j0;
+        SgChunk<V, K> st;
+        sg_chunk_init(st, j0, qs, go, ge);
+        uint2 rr_next = res[lane];
+        uint2 bn_next[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) bn_next[t] = make_uint2(0u, 0u);
+        if (!first) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
+        }
+        // profile slices of column 0
+        short8 sa[NS], sb[NS];
+        {
+            const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[(half ? rr_next.y : rr_next.x) & 0xffu]);
+            const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                sa[k] = na[k];
+                sb[k] = PACKED ? nb[k] : na[k];
+            }
+        }
+        for (int b = 0; b < nblk; ++b) {
+            const uint2 rr = rr_next;
+            uint2 bn[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
+            if (b + 1 < nblk) {
+                rr_next = res[(size_t)(b + 1) * 64 + lane];
+                if (!first) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
+                }
+            }
+            sg_block<V, K, TE, QE>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
+                                   chunk_base, slut, half, RA, RB, ts, qe_rt, go, ge, t16, t32,
+                                   last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
+        }
+    }
+    sg_finish<V>(p, qv, tA, tB, RA, RB, te_rt, qe_rt, t16, t32);
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup pipeline (packed int16) for the longest groups: the four waves
+// of a workgroup share one group of 128 targets and split its query
+// chunks, wave w taking chunks w, w+4, ... Time advances in steps of one
+// 4-column block closed by a workgroup barrier; at step s wave w works on
+// block s - w - r*P of its round-r chunk (P = max(blocks, 5)), one block
+// behind the wave owning the chunk above. Chunk-bottom rows flow to the
+// next wave through a double-buffered LDS ring, and from wave 3 to wave 0
+// of the next round through a global boundary slot (written P-4 or more
+// steps before it is read, overwritten 3 steps after). The end-coordinate
+// keys of the waves combine by max in LDS (the last-row keys and the corner
+// come from the wave that owns the last chunk). Per-column code is
+// sg_block, so results are those of sg_group.
+// ---------------------------------------------------------------------------
+constexpr int kSgPipeWaves = 4;
+constexpr size_t kSgRingBytes = (size_t)(kSgPipeWaves - 1) * 2 * 4 * 64 * sizeof(uint2);
+constexpr size_t kSgCombBytes = (size_t)kSgPipeWaves * 64 * 6 * sizeof(int);
+
+size_t sg_pipe_extra_lds() { return kSgRingBytes + kSgCombBytes + 16; }
+
+template <int K, bool TE, bool QE>
+__device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryView &qv,
+                                              const uint8_t *smem, const uint16_t *slut,
+                                              uint2 *ring, int *comb, uint2 *gbnd, int g, int w,
+                                              int lane) {
+    using V = s16x2;
+    constexpr int NS = K / 8;
+    const int go = p.go, ge = p.ge, Q = qv.Q;
+    const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
+    const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
+    const GroupDesc gd = p.groups[g];
+    const int tA = gd.first + lane;
+    const int tB = gd.first + 64 + lane;
+    const int RA = p.lens[tA];
+    const int RB = p.lens[tB];
+    const uint2 *res = p.res + gd.res_off;
+    const int nblk = gd.ncols >> 2;
+    const int nch = qv.nchunks;
+    const int rounds = (nch + kSgPipeWaves - 1) / kSgPipeWaves;
+    const int P = nblk > 5 ? nblk : 5;
+    const int steps = (rounds - 1) * P + nblk + (kSgPipeWaves - 1);
+    const int last_w = (nch - 1) % kSgPipeWaves;
+
+    TrackI16 t16;
+    TrackI32 t32;
+    track_init(t16, t32);
+    SgChunk<V, K> st;
+    sg_chunk_init(st, 0, qs, go, ge);
+    uint2 rr_next = make_uint2(0u, 0u);
+    uint2 bn_next[4];
+    short8 sa[NS], sb[NS];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bn_next[t] = make_uint2(0u, 0u);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sa[k] = sb[k] = (short8)0;
+
+    int b = -w, r = 0; // block and round of this wave at the current step
+    for (int s = 0; s < steps; ++s) {
+        const int c = kSgPipeWaves * r + w;
+        if (b >= 0 && b < nblk && r < rounds && c < nch) {
+            const int j0 = c * K;
+            const bool first = c == 0;
+            const bool last = c == nch - 1;
+            const int mstar = Q - 1 - j0;
+            const int par = (r * P + b) & 1; // ring parity of this (round, block)
+            const uint8_t *chunk_base = smem + 2 * j0;
+            if (b == 0) {
+                sg_chunk_init(st, j0, qs, go, ge);
+                rr_next = res[lane];
+                if (!first && w == 0) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
+                }
+                const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.x & 0xffu]);
+                const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    sa[k] = na[k];
+                    sb[k] = nb[k];
+                }
+            }
+            const uint2 rr = rr_next;
+            uint2 bn[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bn[t] = make_uint2(0u, 0u);
+            if (!first) {
+                if (w == 0) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
+                    if (b + 1 < nblk) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            bn_next[t] = gbnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
+                    }
+                } else {
+                    const uint2 *src = ring + (size_t)((w - 1) * 2 + par) * 256;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) bn[t] = src[t * 64 + lane];
+                }
+            }
+            if (b + 1 < nblk) rr_next = res[(size_t)(b + 1) * 64 + lane];
+            uint2 *out = nullptr;
+            if (!last)
+                out = w < kSgPipeWaves - 1 ? ring + (size_t)(w * 2 + par) * 256
+                                           : gbnd + (size_t)b * 4 * 64;
+            sg_block<V, K, TE, QE>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
+                                   chunk_base, slut, 0, RA, RB, ts, qe_rt, go, ge, t16, t32, out,
+                                   lane);
+        }
+        __syncthreads();
+        if (++b == P) {
+            b = 0;
+            ++r;
+        }
+    }
+    int *cw = comb + (size_t)(w * 64 + lane) * 6;
+    cw[0] = t16.te_a;
+    cw[1] = t16.te_b;
+    cw[2] = t16.qe_a;
+    cw[3] = t16.qe_b;
+    cw[4] = t16.corner_a;
+    cw[5] = t16.corner_b;
+    __syncthreads();
+    if (w == 0) {
+        for (int v = 1; v < kSgPipeWaves; ++v) {
+            const int *cv = comb + (size_t)(v * 64 + lane) * 6;
+            t16.te_a = max(t16.te_a, cv[0]);
+            t16.te_b = max(t16.te_b, cv[1]);
+            t16.qe_a = max(t16.qe_a, cv[2]);
+            t16.qe_b = max(t16.qe_b, cv[3]);
+        }
+        // the corner cell belongs to the wave that ran the last chunk
+        const int *cl = comb + (size_t)(last_w * 64 + lane) * 6;
+        t16.corner_a = cl[4];
+        t16.corner_b = cl[5];
+        sg_finish<V>(p, qv, tA, tB, RA, RB, te_rt, qe_rt, t16, t32);
+    }
+    __syncthreads(); // ring and comb are reused by the next item
+}
+
 template <typename V, int K, bool TE, bool QE>
 __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
@@ -368,6 +557,53 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
         const int g = p.g_begin + (PACKED ? (int)w : (int)(w >> 1));
         const int half = PACKED ? 0 : (int)(w & 1);
         sg_group<V, K, TE, QE>(p, qv, smem, slut, bnd, g, half, lane);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid persistent kernel (packed int16): a workgroup-level queue whose
+// first p.n_long items are the longest groups, each scored by the four
+// waves of a workgroup together (sg_pipe_group), and whose other items are
+// tiles of four groups, one per wave (sg_group). Long groups then take a
+// quarter of the time one wave needs, so they no longer set the length of
+// the launch.
+// ---------------------------------------------------------------------------
+template <int K, bool TE, bool QE>
+__global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
+    using V = s16x2;
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ int s_item;
+    stage_profile(p.prof, p.lut, p.prof_words, smem);
+    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
+    uint8_t *tail = smem + (((size_t)p.prof_words * 4 + 512 + 15) & ~(size_t)15);
+    uint2 *ring = (uint2 *)tail;
+    int *comb = (int *)(tail + kSgRingBytes);
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int waves = blockDim.x >> 6;
+    uint2 *bnd = p.bnd + (size_t)(blockIdx.x * waves + w) * p.slot_stride;
+    uint2 *gbnd = p.bnd + (size_t)(blockIdx.x * waves) * p.slot_stride;
+    QueryView qv;
+    qv.Q = p.Q;
+    qv.nchunks = p.nchunks;
+    qv.neg_limit = p.neg_limit;
+    qv.flags = p.flags;
+    qv.out_off = 0;
+    const int ng = p.g_end - p.g_begin;
+    const int nlong = p.n_long < ng ? p.n_long : ng;
+    const int items = nlong + (ng - nlong + waves - 1) / waves;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads(); // s_item is rewritten by the next fetch
+        if (item >= items) break;
+        if (item < nlong) {
+            sg_pipe_group<K, TE, QE>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item, w, lane);
+        } else {
+            const int g = p.g_begin + nlong + (item - nlong) * waves + w;
+            if (g < p.g_end) sg_group<V, K, TE, QE>(p, qv, smem, slut, bnd, g, 0, lane);
+        }
     }
 }
 
@@ -638,6 +874,25 @@ hipError_t launch_sg_multi(const ScoreParams &p, bool packed, int chunk, int gri
                        : launch_multi<int, 32>(p, grid, block, lds, st);
 }
 
+template <int K>
+static hipError_t launch_hybrid(const ScoreParams &p, int grid, size_t lds, hipStream_t st) {
+    const bool te = p.flags & 8u, qe = p.flags & 2u;
+    if (te && qe)
+        hipLaunchKernelGGL((sg_hybrid<K, true, true>), dim3(grid), dim3(256), lds, st, p);
+    else if (te)
+        hipLaunchKernelGGL((sg_hybrid<K, true, false>), dim3(grid), dim3(256), lds, st, p);
+    else if (qe)
+        hipLaunchKernelGGL((sg_hybrid<K, false, true>), dim3(grid), dim3(256), lds, st, p);
+    else
+        hipLaunchKernelGGL((sg_hybrid<K, false, false>), dim3(grid), dim3(256), lds, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+                                hipStream_t st) {
+    return chunk == 16 ? launch_hybrid<16>(p, grid, lds, st) : launch_hybrid<32>(p, grid, lds, st);
+}
+
 hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid, int block,
                                  size_t lds, hipStream_t st) {
     if (chunk == 16) return launch_batched<s16x2, 16>(p, grid, block, lds, st);
@@ -707,6 +962,18 @@ int occupancy_sg_multi(bool packed, int chunk, int block, size_t lds, uint32_t f
     if (packed) return chunk == 16 ? occ_multi<s16x2, 16>(block, lds, flags)
                                    : occ_multi<s16x2, 32>(block, lds, flags);
     return chunk == 16 ? occ_multi<int, 16>(block, lds, flags) : occ_multi<int, 32>(block, lds, flags);
+}
+
+template <int K> static int occ_hybrid(size_t lds, uint32_t flags) {
+    const bool te = flags & 8u, qe = flags & 2u;
+    if (te && qe) return occ(sg_hybrid<K, true, true>, 256, lds);
+    if (te) return occ(sg_hybrid<K, true, false>, 256, lds);
+    if (qe) return occ(sg_hybrid<K, false, true>, 256, lds);
+    return occ(sg_hybrid<K, false, false>, 256, lds);
+}
+
+int occupancy_sg_hybrid_i16(int chunk, size_t lds, uint32_t flags) {
+    return chunk == 16 ? occ_hybrid<16>(lds, flags) : occ_hybrid<32>(lds, flags);
 }
 
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags) {

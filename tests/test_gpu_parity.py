@@ -38,17 +38,21 @@ def oracle_batch(q, targets, kind, go, ge, flags):
     return s, er, eq
 
 
-def check(eng, engine, q, targets, kind, go, ge, flags, label="", tile=32):
+def check(eng, engine, q, targets, kind, go, ge, flags, label="", tile=32, schedule="auto"):
     eng.set_engine(engine)
     eng.set_tile(tile)
+    eng.set_schedule(schedule)
     eng.set_query(q, kind, go, ge, flags, "striped-semi-global")
-    s, er, eq, st = eng.align(targets)
+    try:
+        s, er, eq, st = eng.align(targets)
+    finally:
+        eng.set_schedule("auto")
     os_, oer, oeq = oracle_batch(q, targets, kind, go, ge, flags)
     bad = np.nonzero((s != os_) | (er != oer) | (eq != oeq))[0]
     if len(bad):
         t = bad[0]
         raise AssertionError(
-            f"{label} engine={engine} {len(bad)}/{targets.n} differ; first t={t} "
+            f"{label} engine={engine} schedule={schedule} {len(bad)}/{targets.n} differ; first t={t} "
             f"R={targets.offsets[t + 1] - targets.offsets[t]} gpu=({s[t]},{er[t]},{eq[t]}) "
             f"oracle=({os_[t]},{oer[t]},{oeq[t]}) status={st[t]}")
     return s, er, eq, st
@@ -70,9 +74,11 @@ def test_golden_vectors(eng, engine):
         assert (s[0], er[0], eq[0]) == O.sg(q, t, m, c["go"], c["ge"], O.flags_from_str(c["flags"]))
 
 
-@pytest.mark.parametrize("engine,tile", [("batched", 32), ("batched", 16), ("plain", 32)])
+@pytest.mark.parametrize("engine,tile,schedule", [("batched", 32, "auto"), ("batched", 16, "auto"),
+                                                  ("batched", 32, "wave"), ("batched", 32, "workgroup"),
+                                                  ("batched", 16, "workgroup"), ("plain", 32, "auto")])
 @pytest.mark.parametrize("kind", ["actgn", "actgn0", "blosum62", "ascii"])
-def test_random_all_flags(eng, engine, tile, kind):
+def test_random_all_flags(eng, engine, tile, kind, schedule):
     rng = np.random.default_rng(zlib.crc32(f"{engine}-{kind}".encode()))
     alpha = {"actgn": 5, "actgn0": 5, "blosum62": 24, "ascii": 256}[kind]
     for flags in range(16):
@@ -91,7 +97,25 @@ def test_random_all_flags(eng, engine, tile, kind):
             seqs.append(r)
         go = int(rng.integers(1, 12))
         ge = int(rng.integers(0, go))
-        check(eng, engine, q, Targets.from_list(seqs), kind, go, ge, flags, f"flags={flags}", tile)
+        check(eng, engine, q, Targets.from_list(seqs), kind, go, ge, flags, f"flags={flags}", tile, schedule)
+
+
+@pytest.mark.parametrize("flags", ["FFTT", "TTTT", "FFFF", "TFTF", "FTFT"])
+def test_hybrid_long_groups_among_short(eng, flags):
+    """A few very long targets among many short ones: the automatic
+    schedule scores the long groups with whole workgroups and the rest one
+    group per wave, in one launch; workgroups take several items."""
+    rng = np.random.default_rng(zlib.crc32(flags.encode()))
+    q = rng.integers(0, 24, 200).astype(np.uint8)
+    seqs = [rng.integers(0, 24, int(rng.integers(0, 120))).astype(np.uint8) for _ in range(60000)]
+    for k in range(0, 60000, 211):
+        seqs[k] = rng.integers(0, 24, int(rng.integers(2000, 6000))).astype(np.uint8)
+        if k % 3 == 0:
+            seqs[k][100:300] = q
+    fl = O.flags_from_str(flags)
+    check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 3, 1, fl, "hybrid", 32, "auto")
+    check(eng, "batched", q, Targets.from_list(seqs[:3000]), "blosum62", 3, 1, fl, "all pipelined", 16,
+          "workgroup")
 
 
 @pytest.mark.parametrize("engine", ENGINES)
@@ -443,7 +467,7 @@ def test_autotune_keeps_results(eng):
         eng.set_query(w.query, "blosum62", 3, 1, "FFTT")
         ref = eng.align_db(db)
         tuned = eng.autotune(db)
-        assert len(tuned) == 4 and all(t in (16, 32) for _, t, _ in tuned)
+        assert tuned[0] in (16, 32) and tuned[1] in ("wave", "auto", "workgroup")
         got = eng.align_db(db)
         for a, b in zip(ref, got):
             assert np.array_equal(a, b)
@@ -453,7 +477,7 @@ def test_autotune_keeps_results(eng):
         eng.set_query(lr.query, "actgn0", 3, 1, "FFFF", "striped-local")
         ref = eng.align_db(db2)
         tuned = eng.autotune(db2, 1)
-        assert len(tuned) == 4
+        assert tuned is not None and eng.tuning(db2) == tuned
         got = eng.align_db(db2)
         for a, b in zip(ref, got):
             assert np.array_equal(a, b)
