@@ -68,19 +68,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(content_hash, tile, schedule=None):
+def measured_traffic(content_hash, tile, schedule):
     """HBM bytes per dispatch of the dominant kernel, as rocprofv3's
-    FETCH_SIZE / WRITE_SIZE counters measured them for this exact workload
-    (profiles/traffic.json, scaled by the copy-kernel calibration); None
-    when that workload and tile were not profiled."""
+    FETCH_SIZE / WRITE_SIZE counters measured them for this exact workload,
+    tile and schedule (profiles/traffic.json, scaled by the copy-kernel
+    calibration); None when that combination was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        entry = json.load(open(path)).get(content_hash)
+        entries = json.load(open(path)).get("entries", [])
     except (OSError, ValueError):
         return None
-    if not entry or entry.get("tile_rows") != tile or entry.get("schedule", "wave") != (schedule or "wave"):
-        return None
-    return entry["bytes"]
+    for e in entries:
+        if (e.get("content_hash"), e.get("tile_rows"), e.get("schedule")) == (content_hash, tile, schedule):
+            return e["bytes"]
+    return None
 
 
 def serve_stream(ish_amd, srv, reqs):
@@ -356,7 +357,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0,
-                    help="query rows per lane (16 or 32); 0 = pick the faster on this workload")
+                    help="query rows per lane (16 or 32); 0 = let the autotuner pick tile and schedule")
+    ap.add_argument("--schedule", choices=["auto", "wave", "workgroup"], default="auto",
+                    help="launch schedule used with --tile (ignored when the autotuner runs)")
     ap.add_argument("--requests", type=int, default=0,
                     help="serving requests per phase (0 = 64)")
     ap.add_argument("--no-serving", action="store_true",
@@ -398,17 +401,19 @@ def main():
     load_s = time.perf_counter() - t0
     log(f"db load (H2D + sort + pack): {load_s * 1e3:.1f} ms")
 
-    # launch configuration: the library's autotuner times tile heights (and
-    # for local alignment the schedule) per target-length bucket; results
-    # never depend on it. --tile fixes the tile instead.
+    # launch configuration: the library's autotuner times whole passes over
+    # tile heights x schedules and keeps the fastest; results never depend
+    # on it. --tile / --schedule fix both instead.
     t0 = time.perf_counter()
     if args.tile:
         eng.set_tile(args.tile)
+        eng.set_schedule(args.schedule)
         tuning = None
     else:
         tuning = eng.autotune(db)
         log(f"autotune ({time.perf_counter() - t0:.1f}s): tile {tuning[0]}, schedule {tuning[1]}")
     best_tile = args.tile or (tuning[0] if tuning else 0)
+    schedule = tuning[1] if tuning else args.schedule
     # repeated passes replay one captured HIP graph
     eng.set_graphs(True)
     for _ in range(args.warmup):
@@ -519,6 +524,7 @@ def main():
                    "cells": cells, "seed": args.seed, "content_hash": w.content_hash(),
                    "engine": "batched persistent, packed int16 (int32 for wide groups), HIP graph replay",
                    "tile_rows": best_tile or None,
+                   "schedule": schedule,
                    "tuning": {"tile": tuning[0], "schedule": tuning[1]} if tuning else None},
         "gcups_kernel": round(gcups_kernel, 2),
         "gcups_e2e": round(cells / e2e_s / 1e9, 2),
@@ -531,8 +537,7 @@ def main():
         "parity_sample": parity,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": measured_traffic(w.content_hash(), best_tile,
-                                                 tuning[1] if tuning else None),
+                     "traffic": measured_traffic(w.content_hash(), best_tile, schedule),
                      "ops_per_cell": ops_per_cell, "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }
