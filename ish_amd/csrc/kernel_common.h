@@ -34,6 +34,23 @@ __device__ __forceinline__ uint32_t wave_broadcast0(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Minimum of v over the 64 lanes of the wave, returned wave-uniform.
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// (a, b) in the two int16 halves; the int32 form keeps a only.
+template <typename V> __device__ __forceinline__ V pack2(int a, int b);
+template <> __device__ __forceinline__ s16x2 pack2<s16x2>(int a, int b) {
+    s16x2 r;
+    r.x = (short)a;
+    r.y = (short)b;
+    return r;
+}
+template <> __device__ __forceinline__ int pack2<int>(int a, int) { return a; }
+
 // Stage the query profile and the residue -> profile-row table into LDS.
 // The table holds each row's offset in dwords (256 x uint16), so a residue
 // code turns into an LDS address with one shift-add.

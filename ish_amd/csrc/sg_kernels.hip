@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
     uint2 *ring = (uint2 *)tail;
     int *comb = (int *)(tail + kSgRingBytes);
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = (int)wave_broadcast0(threadIdx.x >> 6); // wave-uniform
     const int waves = blockDim.x >> 6;
     uint2 *bnd = p.bnd + (size_t)(blockIdx.x * waves + w) * p.slot_stride;
     uint2 *gbnd = p.bnd + (size_t)(blockIdx.x * waves) * p.slot_stride;
@@ -619,7 +619,7 @@ __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
     __shared__ int s_tile[3]; // query, first wave item, items of that query
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = (int)wave_broadcast0(threadIdx.x >> 6); // wave-uniform
     const int waves = blockDim.x >> 6;
     const int slot = blockIdx.x * waves + wave;
     uint2 *bnd = p.bnd + (size_t)slot * p.slot_stride;

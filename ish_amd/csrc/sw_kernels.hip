@@ -87,14 +87,16 @@ __device__ __forceinline__ void sw_chunk_init(SwChunk<V, K> &st, int go, int ge)
 //   slices of the next column are fetched while a column is scored);
 //   bn: chunk-top rows (H~, F~ of the row above) when the chunk is not the
 //   first; out: where the chunk-bottom rows go (4 x 64 uint2, t*64 + lane),
-//   null for the last chunk. bestA / bestB: a lower bound of the maximum
-//   over the target's other chunks (filters the row scans).
+//   null for the last chunk. lbm1: one less than a lower bound of the
+//   maximum over the target's other chunks, per half (filters the row
+//   scans). rmin: the shortest target length of the wave (wave-uniform);
+//   columns below it need no end-of-target mask.
 template <typename V, int K>
 __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool first,
                                          uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
                                          short8 (&sa)[K / 8], short8 (&sb)[K / 8],
                                          const uint8_t *chunk_base, const uint16_t *slut, int half,
-                                         int RA, int RB, int bestA, int bestB, int go, int ge,
+                                         int RA, int RB, int rmin, V lbm1, int go, int ge,
                                          uint2 *out, int lane) {
     constexpr bool PACKED = !__is_same(V, int);
     constexpr int NS = K / 8;
@@ -133,9 +135,6 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             f = from_u32<V>(bn[t].y);
         }
         V diag = st.diag_top;
-        V cms[NS]; // column maximum, one partial per 8-row slice
-#pragma unroll
-        for (int k = 0; k < NS; ++k) cms[k] = vz;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
 #pragma unroll
@@ -148,13 +147,14 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
                 } else {
                     sv = (int)sa[k][mm];
                 }
-                const V h = vmax(vmax(vmax(diag + sv, st.E[m]), f), vz);
+                // the floor and E come from the previous column: only the
+                // last max waits on the F chain
+                const V h = vmax(vmax(vmax(diag + sv, st.E[m]), vz), f);
                 diag = st.H[m];
                 st.H[m] = h;
                 const V g = h + vcst;
                 st.E[m] = vmax(st.E[m], g);
                 f = vmax(f, g) - vge;
-                cms[k] = vmax(cms[k], h);
             }
             // slice k is consumed: fetch it for the next column (the
             // scheduling barrier keeps only one extra slice live)
@@ -169,35 +169,49 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             o.y = as_u32(f);
             out[t * 64 + lane] = o;
         }
+        // column maximum, one partial per 8-row slice, each a tree over its
+        // rows (every h already includes the floor vz)
+        V cms[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const V *hk = st.H + 8 * k;
+            cms[k] = vmax(vmax(vmax(hk[0], hk[1]), vmax(hk[2], hk[3])),
+                          vmax(vmax(hk[4], hk[5]), vmax(hk[6], hk[7])));
+        }
         V cm = cms[0];
 #pragma unroll
         for (int k = 1; k < NS; ++k) cm = vmax(cm, cms[k]);
         cm = cm - vz; // plain column maximum (>= 0)
-        // columns past a target's end do not count
-        if constexpr (PACKED) {
-            const uint32_t mask = (i < RA ? 0x0000FFFFu : 0u) | (i < RB ? 0xFFFF0000u : 0u);
-            cm = from_u32<V>(as_u32(cm) & mask);
-        } else {
-            cm = i < RA ? cm : 0;
+        // columns past a target's end do not count; only columns at or past
+        // the wave's shortest target need the mask
+        if (i >= rmin) {
+            if constexpr (PACKED) {
+                const uint32_t mask = (i < RA ? 0x0000FFFFu : 0u) | (i < RB ? 0xFFFF0000u : 0u);
+                cm = from_u32<V>(as_u32(cm) & mask);
+            } else {
+                cm = i < RA ? cm : 0;
+            }
         }
-        const V nr = vmax(st.run, cm);
         // A rise matters only when it reaches the target's best over the
         // other chunks: a chunk that ends below it cannot win, and one that
         // ends at or above it records its last rise here, which is its
-        // (M_c, i_c, r_c).
-        bool up_a, up_b;
-        if constexpr (PACKED) {
-            up_a = nr.x > st.run.x && (int)nr.x >= bestA;
-            up_b = nr.y > st.run.y && (int)nr.y >= bestB;
-        } else {
-            up_a = nr > st.run && nr >= bestA;
-            up_b = false;
-        }
+        // (M_c, i_c, r_c). Per half, "cm > max(run, lb - 1)" is exactly
+        // "the chunk maximum strictly rises, to at least lb"; one packed max
+        // and one 32-bit compare test both halves.
+        const V thr = vmax(st.run, lbm1);
         // wave-uniform tests keep these rare paths real branches
-        if (__builtin_amdgcn_ballot_w64(up_a || up_b) != 0) {
+        if (__builtin_amdgcn_ballot_w64(as_u32(vmax(cm, thr)) != as_u32(thr)) != 0) {
+            bool up_a, up_b;
+            if constexpr (PACKED) {
+                up_a = cm.x > thr.x;
+                up_b = cm.y > thr.y;
+            } else {
+                up_a = cm > thr;
+                up_b = false;
+            }
             // smallest row of this column holding the new maximum: the first
             // 8-row slice whose maximum is the key holds it
-            const V key = nr + vz;
+            const V key = cm + vz;
             int ra = K, rb = K;
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
@@ -231,7 +245,7 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
                 st.rcB = j0 + rb;
             }
         }
-        st.run = nr;
+        st.run = vmax(st.run, cm);
     }
 }
 
@@ -264,6 +278,7 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
         const int RB = PACKED ? p.lens[tB] : 0;
         const uint2 *res = p.res + gd.res_off;
         const int nblk = gd.ncols >> 2;
+        const int rmin = wave_min(PACKED ? min(RA, RB) : RA);
 
         int bestA = 0, erA = -1, eqA = 0, bestB = 0, erB = -1, eqB = 0;
         for (int c = 0; c < p.nchunks; ++c) {
@@ -271,6 +286,7 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
             const bool first = c == 0;
             const bool last = c == p.nchunks - 1;
             const uint8_t *chunk_base = smem + 2 * j0;
+            const V lbm1 = pack2<V>(bestA - 1, bestB - 1);
             SwChunk<V, K> st;
             sw_chunk_init(st, go, ge);
             uint2 rr_next = nblk > 0 ? res[lane] : make_uint2(0u, 0u);
@@ -300,7 +316,7 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
                     }
                 }
                 sw_block<V, K>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, half, RA,
-                               RB, bestA, bestB, go, ge, last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
+                               RB, rmin, lbm1, go, ge, last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
             }
             if constexpr (PACKED) {
                 sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
@@ -362,7 +378,7 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
 
     constexpr int NS = K / 8;
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = (int)wave_broadcast0(threadIdx.x >> 6); // wave-uniform
     uint2 *gbnd = p.bnd + (size_t)blockIdx.x * p.slot_stride;
     const int go = p.go, ge = p.ge;
     const int items = p.g_end - p.g_begin;
@@ -384,6 +400,7 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
         const int RB = p.lens[tB];
         const uint2 *res = p.res + gd.res_off;
         const int nblk = gd.ncols >> 2;
+        const int rmin = wave_min(min(RA, RB));
         const int P = nblk > 5 ? nblk : 5;
         const int steps = (rounds - 1) * P + nblk + (kPipeWaves - 1);
 
@@ -451,22 +468,16 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                 // bestA / bestB; the other waves publish their best so far
                 // (finished chunks and running chunk maxima, which only
                 // grow, so a stale entry is still a lower bound).
-                int lbA = bestA, lbB = bestB;
+                const V bestv = pack2<V>(bestA, bestB);
+                V lb = bestv;
 #pragma unroll
                 for (int v = 0; v < kPipeWaves; ++v) {
                     if (v == w) continue;
-                    const V o = from_u32<V>(pub[v * 64 + lane]);
-                    lbA = max(lbA, (int)o.x);
-                    lbB = max(lbB, (int)o.y);
+                    lb = vmax(lb, from_u32<V>(pub[v * 64 + lane]));
                 }
                 sw_block<V, K>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, 0, RA,
-                               RB, lbA, lbB, go, ge, out, lane);
-                {
-                    V mine;
-                    mine.x = (short)max(bestA, (int)st.run.x);
-                    mine.y = (short)max(bestB, (int)st.run.y);
-                    pub[w * 64 + lane] = as_u32(mine);
-                }
+                               RB, rmin, lb - splat<V>(1), go, ge, out, lane);
+                pub[w * 64 + lane] = as_u32(vmax(bestv, st.run));
                 if (b == nblk - 1) {
                     sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
                     sw_combine((int)st.run.y, st.icB, st.rcB, bestB, erB, eqB);
