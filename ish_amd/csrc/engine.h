@@ -78,6 +78,15 @@ struct ScoreParams {
     // hybrid launches: the first n_long groups of [g_begin, g_end) are
     // scored by whole workgroups
     int32_t n_long;
+
+    // paired local profile (small alphabets), used by the paired pipeline:
+    // pprof[pair * pqpad + j] = (S[A][j] + ge) | (S[B][j] + ge) << 16 for
+    // the profile classes A, B of pair = A * pnrows + B (0 past the query);
+    // plut = byte offsets of a class's pair rows in a wave's LDS copy, for
+    // tile 16 then tile 32, each as [A: 256][B: 256] uint16
+    const int32_t *pprof;
+    const uint16_t *plut;
+    int32_t pnrows, pqpad;
 };
 
 // Plain (one launch per batch) parameters: targets in input order,
@@ -116,10 +125,17 @@ hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, int grid, size_
 int occupancy_sg_hybrid_i16(int chunk, size_t lds, uint32_t flags);
 // Workgroup-pipelined local kernel (packed int16, 4 waves share a group):
 // extra LDS beyond the profile, launcher (block of 256) and occupancy.
+// `paired` selects the variant that reads the paired profile (p.pprof,
+// p.plut) instead of staging the int16 profile; its whole LDS size is
+// sw_pipe_paired_lds(chunk, profile classes).
 size_t sw_pipe_extra_lds();
-hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+size_t sw_pipe_paired_lds(int chunk, int nrows);
+size_t sw_pipe_paired_row_bytes(int chunk);
+// Profiles with at most this many classes get the paired local profile.
+constexpr int kPairMaxRows = 8;
+hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
                               hipStream_t st);
-int occupancy_sw_pipe_i16(int chunk, size_t lds);
+int occupancy_sw_pipe_i16(int chunk, bool paired, size_t lds);
 int occupancy_sw_batched_i32(int chunk, int block, size_t lds);
 hipError_t launch_sg_plain(const PlainParams &p, int grid, int block,
                            size_t lds, hipStream_t st);

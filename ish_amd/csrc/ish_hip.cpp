@@ -50,6 +50,9 @@ struct ish_hip_ctx {
     int32_t prof_words = 0;
     size_t lds = 0;
     DevBuf prof_t, prof_rt, prof_p, lut; // forward, reversed-query, plain profiles
+    // paired local profile and its tables (0 rows: none; see ScoreParams)
+    DevBuf pprof, plut;
+    int32_t pair_rows = 0, pqpad = 0;
     int engine = ISH_ENGINE_BATCHED;
     int chunk = kChunkDefault; // query rows per lane in the batched kernels
     int schedule = ISH_SCHEDULE_AUTO; // one wave or one workgroup per group
@@ -216,7 +219,8 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    for (DevBuf *b : {&ctx->prof_t, &ctx->prof_rt, &ctx->prof_p, &ctx->lut, &ctx->scratch, &ctx->counter,
+    for (DevBuf *b : {&ctx->prof_t, &ctx->prof_rt, &ctx->prof_p, &ctx->lut, &ctx->pprof, &ctx->plut,
+                      &ctx->scratch, &ctx->counter,
                       &ctx->in_res, &ctx->in_off, &ctx->out_score, &ctx->out_er, &ctx->out_eq,
                       &ctx->out_st})
         b->release();
@@ -402,6 +406,37 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_rt.p, pf.rev.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_p.p, pf.plain.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->lut.p, pf.lut, 512, hipMemcpyHostToDevice, ctx->stream));
+    // paired profile for local alignment over small alphabets: the packed
+    // operand of a lane's two targets is then one LDS word, with no
+    // per-cell pairing instruction in the kernel
+    std::vector<int32_t> pp;
+    std::vector<uint16_t> pl;
+    ctx->pair_rows = 0;
+    if (algo == ISH_ALGO_LOCAL && pf.nrows <= kPairMaxRows) {
+        const int32_t nr = pf.nrows, rw = pf.stride / 2;
+        pp.assign((size_t)nr * nr * pf.qpad, 0);
+        for (int32_t a = 0; a < nr; ++a)
+            for (int32_t b = 0; b < nr; ++b)
+                for (int32_t j = 0; j < pf.qpad; ++j)
+                    pp[((size_t)a * nr + b) * pf.qpad + j] =
+                        (int32_t)((uint32_t)(uint16_t)pf.fwd[(size_t)a * pf.stride + j] |
+                                  ((uint32_t)(uint16_t)pf.fwd[(size_t)b * pf.stride + j] << 16));
+        pl.assign(1024, 0);
+        for (int ti = 0; ti < 2; ++ti) {
+            const int32_t rowb = (int32_t)sw_pipe_paired_row_bytes(ti ? 32 : 16);
+            for (int c = 0; c < 256; ++c) {
+                const int32_t cls = pf.lut[c] / rw;
+                pl[(size_t)ti * 512 + c] = (uint16_t)(cls * nr * rowb);
+                pl[(size_t)ti * 512 + 256 + c] = (uint16_t)(cls * rowb);
+            }
+        }
+        HIP_TRY(ctx, ctx->pprof.ensure(pp.size() * 4));
+        HIP_TRY(ctx, ctx->plut.ensure(pl.size() * 2));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->pprof.p, pp.data(), pp.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->plut.p, pl.data(), pl.size() * 2, hipMemcpyHostToDevice, ctx->stream));
+        ctx->pair_rows = nr;
+        ctx->pqpad = pf.qpad;
+    }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
     ctx->Q = Q;
@@ -648,6 +683,7 @@ struct DbLaunch {
     int32_t g_begin, g_end;
     int32_t grid;
     int32_t n_long; // kSgHybrid: leading groups scored by whole workgroups
+    int32_t paired; // kSwPipe: the paired-profile variant
     int64_t slot_stride;
     uint64_t lds;
     ScoreParams p;
@@ -767,6 +803,12 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     // scratch between launches would free memory a queued launch uses)
     for (DbLaunch &L : out) {
         L.lds = L.kind == kSwPipe ? pipe_lds : L.kind == kSgHybrid ? hybrid_lds : ctx->lds;
+        // forward local passes over a small alphabet read the paired profile
+        if (L.kind == kSwPipe && prof == ctx->prof_t.p && ctx->pair_rows > 0 &&
+            sw_pipe_paired_lds(L.chunk, ctx->pair_rows) <= kLdsLimit) {
+            L.paired = 1;
+            L.lds = sw_pipe_paired_lds(L.chunk, ctx->pair_rows);
+        }
         int occ = 1;
         switch (L.kind) {
         case kSg32: occ = occupancy_sg_batched_i32(L.chunk, kBlock, L.lds, flags); break;
@@ -774,7 +816,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
         case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
         case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.lds, flags); break;
-        default: occ = occupancy_sw_pipe_i16(L.chunk, L.lds); break;
+        default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.lds); break;
         }
         int grid;
         int64_t ss;
@@ -809,6 +851,10 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         p.g_begin = L.g_begin;
         p.g_end = L.g_end;
         p.n_long = L.n_long;
+        p.pprof = L.paired ? (const int32_t *)ctx->pprof.p : nullptr;
+        p.plut = L.paired ? (const uint16_t *)ctx->plut.p : nullptr;
+        p.pnrows = L.paired ? ctx->pair_rows : 0;
+        p.pqpad = L.paired ? ctx->pqpad : 0;
     }
     return ISH_OK;
 }
@@ -823,7 +869,7 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
     case kSw32: e = launch_sw_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw16: e = launch_sw_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSgHybrid: e = launch_sg_hybrid_i16(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
-    default: e = launch_sw_pipe_i16(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
+    default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
     }
     if (e == hipSuccess && timed) e = timed_end(ctx);
     return e;

@@ -91,7 +91,12 @@ __device__ __forceinline__ void sw_chunk_init(SwChunk<V, K> &st, int go, int ge)
 //   maximum over the target's other chunks, per half (filters the row
 //   scans). rmin: the shortest target length of the wave (wave-uniform);
 //   columns below it need no end-of-target mask.
-template <typename V, int K>
+// PAIRED (packed lanes only): chunk_base is the wave's paired profile of
+//   this chunk, one row per pair of profile classes (class of target A,
+//   class of target B) holding K int32 words (S_A + ge, S_B + ge), and slut
+//   maps a residue code to a byte offset (A's at [0, 256), B's at
+//   [256, 512)), so the packed operand of a cell is loaded as it is used.
+template <typename V, int K, bool PAIRED = false>
 __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool first,
                                          uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
                                          short8 (&sa)[K / 8], short8 (&sb)[K / 8],
@@ -122,8 +127,15 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
         const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
         const uint32_t wb = t < 3 ? rr.y : rr_next.y;
         const int sh = 8 * ((t + 1) & 3);
-        const short8 *na = row_ptr((wa >> sh) & 0xffu);
-        const short8 *nb = row_ptr((wb >> sh) & 0xffu);
+        const short8 *na, *nb;
+        if constexpr (PAIRED) {
+            na = (const short8 *)(chunk_base + (uint32_t)slut[(wa >> sh) & 0xffu] +
+                                  (uint32_t)slut[256 + ((wb >> sh) & 0xffu)]);
+            nb = na;
+        } else {
+            na = row_ptr((wa >> sh) & 0xffu);
+            nb = row_ptr((wb >> sh) & 0xffu);
+        }
         const int zi = (i - st.ibase) * ge; // floor of column i (all rows)
         const V vz = splat<V>(zi);
         V htop, f;
@@ -141,7 +153,12 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             for (int mm = 0; mm < 8; ++mm) {
                 const int m = k * 8 + mm;
                 V sv;
-                if constexpr (PACKED) {
+                if constexpr (PAIRED) {
+                    // rows 8k..8k+3 in sa[k], 8k+4..8k+7 in sb[k]
+                    const short8 &src = mm < 4 ? sa[k] : sb[k];
+                    sv.x = src[2 * (mm & 3)];
+                    sv.y = src[2 * (mm & 3) + 1];
+                } else if constexpr (PACKED) {
                     sv.x = sa[k][mm];
                     sv.y = sb[k][mm];
                 } else {
@@ -159,8 +176,13 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             // slice k is consumed: fetch it for the next column (the
             // scheduling barrier keeps only one extra slice live)
             __builtin_amdgcn_sched_barrier(0);
-            sa[k] = na[k];
-            if (PACKED) sb[k] = nb[k];
+            if constexpr (PAIRED) {
+                sa[k] = na[2 * k];
+                sb[k] = na[2 * k + 1];
+            } else {
+                sa[k] = na[k];
+                if (PACKED) sb[k] = nb[k];
+            }
         }
         st.diag_top = htop;
         if (out) {
@@ -360,16 +382,41 @@ constexpr size_t kPipeRingBytes = (size_t)(kPipeWaves - 1) * 2 * 4 * 64 * sizeof
 constexpr size_t kPipeCombBytes = (size_t)kPipeWaves * 64 * 6 * sizeof(int);
 constexpr size_t kPipePubBytes = (size_t)kPipeWaves * 64 * sizeof(uint32_t);
 
-size_t sw_pipe_extra_lds() { return kPipeRingBytes + kPipeCombBytes + kPipePubBytes + 16; }
+constexpr size_t kPipeExtraBytes = kPipeRingBytes + kPipeCombBytes + kPipePubBytes + 16;
+size_t sw_pipe_extra_lds() { return kPipeExtraBytes; }
 
-template <int K>
+// Paired variant (small alphabets): bytes per pair row of a wave's paired
+// chunk profile (K int32 words plus 16 bytes, so that rows start 16 bytes
+// apart modulo 128 and the 16-byte slices of different rows spread over
+// the LDS banks), and the LDS a launch needs: the pair table (2 x 256
+// uint16), the pipeline's own buffers, and one paired chunk profile per
+// wave.
+constexpr int pair_row_bytes(int K) { return 4 * K + 16; }
+size_t sw_pipe_paired_row_bytes(int chunk) { return (size_t)pair_row_bytes(chunk); }
+size_t sw_pipe_paired_lds(int chunk, int nrows) {
+    return 1024 + sw_pipe_extra_lds() +
+           (size_t)kPipeWaves * nrows * nrows * pair_row_bytes(chunk);
+}
+
+template <int K, bool PAIRED>
 __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     using V = s16x2;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_item;
-    stage_profile(p.prof, p.lut, p.prof_words, smem);
-    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
-    uint8_t *tail = smem + (((size_t)p.prof_words * 4 + 512 + 15) & ~(size_t)15);
+    size_t head;
+    if constexpr (PAIRED) {
+        // the pair table of this tile height: byte offsets of a class's
+        // rows, for target A at [0, 256) and target B at [256, 512)
+        const uint32_t *src = (const uint32_t *)(p.plut + (K == 32 ? 512 : 0));
+        for (int x = threadIdx.x; x < 256; x += blockDim.x) ((uint32_t *)smem)[x] = src[x];
+        __syncthreads();
+        head = 1024;
+    } else {
+        stage_profile(p.prof, p.lut, p.prof_words, smem);
+        head = (size_t)p.prof_words * 4 + 512;
+    }
+    const uint16_t *slut = (const uint16_t *)(PAIRED ? smem : smem + (size_t)p.prof_words * 4);
+    uint8_t *tail = smem + ((head + 15) & ~(size_t)15);
     uint2 *ring = (uint2 *)tail;                // [wave 0..2][parity][t][lane]
     int *comb = (int *)(tail + kPipeRingBytes); // [wave][lane][6]
     // each wave's best so far per lane (packed A|B): the other waves read it
@@ -384,6 +431,9 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     const int items = p.g_end - p.g_begin;
     const int nch = p.nchunks;
     const int rounds = (nch + kPipeWaves - 1) / kPipeWaves;
+    // this wave's paired chunk profile (PAIRED only)
+    const int npairs = PAIRED ? p.pnrows * p.pnrows : 0;
+    uint8_t *pbuf = tail + kPipeExtraBytes + (size_t)w * npairs * pair_row_bytes(K);
 
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
@@ -423,7 +473,7 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                 const bool first = c == 0;
                 const bool last = c == nch - 1;
                 const int par = (r * P + b) & 1; // ring parity of this (round, block)
-                const uint8_t *chunk_base = smem + 2 * j0;
+                const uint8_t *chunk_base = PAIRED ? pbuf : smem + 2 * j0;
                 if (b == 0) {
                     sw_chunk_init(st, go, ge);
                     rr_next = res[lane];
@@ -431,12 +481,33 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
 #pragma unroll
                         for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
                     }
-                    const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.x & 0xffu]);
-                    const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
+                    if constexpr (PAIRED) {
+                        // copy rows j0 .. j0+K-1 of every pair row; the
+                        // previous chunk's reads of the buffer were issued
+                        // earlier, and a wave's LDS accesses complete in
+                        // order
+                        constexpr int parts = K / 4; // 16-byte parts per row
+                        for (int x = lane; x < npairs * parts; x += 64) {
+                            const int pr = x / parts, part = x % parts;
+                            const uint4 v = *(const uint4 *)(p.pprof + (size_t)pr * p.pqpad + j0 + part * 4);
+                            *(uint4 *)(pbuf + pr * pair_row_bytes(K) + part * 16) = v;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        const short8 *np = (const short8 *)(pbuf + (uint32_t)slut[rr_next.x & 0xffu] +
+                                                            (uint32_t)slut[256 + (rr_next.y & 0xffu)]);
 #pragma unroll
-                    for (int k = 0; k < NS; ++k) {
-                        sa[k] = na[k];
-                        sb[k] = nb[k];
+                        for (int k = 0; k < NS; ++k) {
+                            sa[k] = np[2 * k];
+                            sb[k] = np[2 * k + 1];
+                        }
+                    } else {
+                        const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.x & 0xffu]);
+                        const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
+#pragma unroll
+                        for (int k = 0; k < NS; ++k) {
+                            sa[k] = na[k];
+                            sb[k] = nb[k];
+                        }
                     }
                 }
                 const uint2 rr = rr_next;
@@ -475,8 +546,8 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                     if (v == w) continue;
                     lb = vmax(lb, from_u32<V>(pub[v * 64 + lane]));
                 }
-                sw_block<V, K>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, 0, RA,
-                               RB, rmin, lb - splat<V>(1), go, ge, out, lane);
+                sw_block<V, K, PAIRED>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, 0,
+                                       RA, RB, rmin, lb - splat<V>(1), go, ge, out, lane);
                 pub[w * 64 + lane] = as_u32(vmax(bestv, st.run));
                 if (b == nblk - 1) {
                     sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
@@ -518,12 +589,16 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     }
 }
 
-hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
                               hipStream_t st) {
-    if (chunk == 16)
-        hipLaunchKernelGGL((sw_pipe<16>), dim3(grid), dim3(64 * kPipeWaves), lds, st, p);
-    else
-        hipLaunchKernelGGL((sw_pipe<32>), dim3(grid), dim3(64 * kPipeWaves), lds, st, p);
+    const dim3 g(grid), b(64 * kPipeWaves);
+    if (paired) {
+        if (chunk == 16) hipLaunchKernelGGL((sw_pipe<16, true>), g, b, lds, st, p);
+        else hipLaunchKernelGGL((sw_pipe<32, true>), g, b, lds, st, p);
+    } else {
+        if (chunk == 16) hipLaunchKernelGGL((sw_pipe<16, false>), g, b, lds, st, p);
+        else hipLaunchKernelGGL((sw_pipe<32, false>), g, b, lds, st, p);
+    }
     return hipGetLastError();
 }
 
@@ -551,9 +626,12 @@ template <typename KernelT> static int occ_of(KernelT k, int block, size_t lds) 
     return nb > 0 ? nb : 1;
 }
 
-int occupancy_sw_pipe_i16(int chunk, size_t lds) {
-    return chunk == 16 ? occ_of(sw_pipe<16>, 64 * kPipeWaves, lds)
-                       : occ_of(sw_pipe<32>, 64 * kPipeWaves, lds);
+int occupancy_sw_pipe_i16(int chunk, bool paired, size_t lds) {
+    if (paired)
+        return chunk == 16 ? occ_of(sw_pipe<16, true>, 64 * kPipeWaves, lds)
+                           : occ_of(sw_pipe<32, true>, 64 * kPipeWaves, lds);
+    return chunk == 16 ? occ_of(sw_pipe<16, false>, 64 * kPipeWaves, lds)
+                       : occ_of(sw_pipe<32, false>, 64 * kPipeWaves, lds);
 }
 
 int occupancy_sw_batched_i16(int chunk, int block, size_t lds) {

@@ -274,6 +274,30 @@ def test_local_long_reads_and_windows(eng, schedule):
         assert (st == ish_amd.STATUS_I16).all()
 
 
+@pytest.mark.parametrize("tile", [32, 16])
+def test_local_paired_profile_wide_alphabet(eng, tile):
+    """A query over a few bytes of the 256-code ASCII matrix has a profile of
+    few classes, which the local pipeline reads as a paired profile. The
+    targets hold every byte value (most of them in the query's absent
+    class), stretches of the query's letters and copies of query pieces."""
+    rng = np.random.default_rng(770 + tile)
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    q = rng.choice(letters, 300)
+    seqs = []
+    for k in range(600):
+        L = int(rng.integers(200, 3000))
+        t = rng.integers(0, 256, L).astype(np.uint8)
+        if k % 3 == 0:
+            o = int(rng.integers(0, L - 100))
+            t[o:o + 100] = rng.choice(letters, 100)
+        if k % 7 == 0 and L > 400:
+            a = int(rng.integers(0, 150))
+            o = int(rng.integers(0, L - 150))
+            t[o:o + 150] = q[a:a + 150]
+        seqs.append(t)
+    check_local(eng, q, Targets.from_list(seqs), "ascii", 3, 1, tile, "paired ascii", "workgroup")
+
+
 def test_local_int32_path(eng):
     rng = np.random.default_rng(22)
     q = rng.integers(0, 24, 600).astype(np.uint8)
