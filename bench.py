@@ -134,19 +134,26 @@ def serve_stream(ish_amd, srv, reqs):
             "hits": int(hits)}
 
 
-def serving_phases(ish_amd, workloads, db, args, count):
+# share of serving requests that are windows of DB targets (so that they
+# have hits at the 0.8 threshold); the others are background-frequency
+# queries, which rarely reach it
+SERVING_HIT_FRACTION = 0.25
+
+
+def serving_phases(ish_amd, workloads, db, targets, args, count):
     """Phase 1: the stated 200 QPS open loop (beyond capacity: latency is
     queueing). Phase 2: the same queries at 70 % of the throughput phase 1
     sustained, where p50/p99 describe the service itself."""
     srv = ish_amd.Server(db, "blosum62", 3, 1, 0.8, max_batch=16, slots=2)
+    hf = dict(sources=targets, hit_fraction=SERVING_HIT_FRACTION)
     try:
         warm = workloads.serving_requests(4, seed=args.seed + 99, rate=1e6)
         serve_stream(ish_amd, srv, warm)
-        reqs = workloads.serving_requests(count, seed=args.seed, rate=200.0)
+        reqs = workloads.serving_requests(count, seed=args.seed, rate=200.0, **hf)
         sat = serve_stream(ish_amd, srv, reqs)
         log(f"serving @200 QPS offered: {sat}")
         rate = max(0.7 * sat["achieved_qps"], 0.1)
-        reqs2 = workloads.serving_requests(count, seed=args.seed, rate=rate)
+        reqs2 = workloads.serving_requests(count, seed=args.seed, rate=rate, **hf)
         sus = serve_stream(ish_amd, srv, reqs2)
         log(f"serving @{rate:.1f} QPS offered: {sus}")
     finally:
@@ -167,7 +174,7 @@ def run_serving(args, ish_amd, workloads):
     db = eng.load_db(w.targets)
     load_s = time.perf_counter() - t0
     count = args.requests or 64
-    sat, sus = serving_phases(ish_amd, workloads, db, args, count)
+    sat, sus = serving_phases(ish_amd, workloads, db, w.targets, args, count)
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -196,7 +203,8 @@ def run_serving(args, ish_amd, workloads):
         "config": {"workload": desc, "name": "serving", "targets": w.targets.n, "residues": nres,
                    "seed": args.seed, "content_hash": w.content_hash()},
         "db_load_ms": round(load_s * 1e3, 1),
-        "serving": {"at_offered_200qps": sat, "at_70pct_capacity": sus},
+        "serving": {"requests": f"{SERVING_HIT_FRACTION:.0%} windows of DB targets, the rest background-frequency queries",
+                    "at_offered_200qps": sat, "at_70pct_capacity": sus},
         "p50_ms": sus["p50_ms"], "p99_ms": sus["p99_ms"],
         # batches on two streams overlap on the device, so per-launch event
         # times double-count; the sustained wall-clock rate is the basis
@@ -501,9 +509,10 @@ def main():
 
     serving = None
     if args.workload == "protein" and not args.no_serving:
-        sat, sus = serving_phases(ish_amd, workloads, db, args, args.requests or 64)
-        serving = {"db": f"the {w.targets.n} resident targets above", "at_offered_200qps": sat,
-                   "at_70pct_capacity": sus}
+        sat, sus = serving_phases(ish_amd, workloads, db, w.targets, args, args.requests or 64)
+        serving = {"db": f"the {w.targets.n} resident targets above",
+                   "requests": f"{SERVING_HIT_FRACTION:.0%} windows of DB targets, the rest background-frequency queries",
+                   "at_offered_200qps": sat, "at_70pct_capacity": sus}
 
     achieved = cells * ops_per_cell / kern_s / 1e12
     out = {

@@ -118,19 +118,31 @@ class Request:
 
 def serving_requests(count: int, seed: int = 1, rate: float = 200.0, median: float = 300.0,
                      sigma: float = 0.5, lo: int = 20, hi: int = 2000,
-                     ttt_fraction: float = 0.5) -> list[Request]:
+                     ttt_fraction: float = 0.5, sources: Targets | None = None,
+                     hit_fraction: float = 0.0) -> list[Request]:
     """Open-loop query stream of the serving workload: Poisson arrivals at
     `rate` per second, ln Q ~ N(ln median, sigma^2) rounded and clipped to
     [lo, hi], ends-free TTTT with probability ttt_fraction, else FFTT.
     Query residues come from the workload generator (background
-    frequencies), one stream per request."""
+    frequencies), one stream per request. With `sources`, a hit_fraction
+    of the requests are instead a window of a random target of sources
+    (clipped to that target's length), so that they have at least that
+    target as a hit at any threshold."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, count)
     arrivals = np.cumsum(gaps) - gaps[0]
     lens = np.clip(np.rint(np.exp(np.log(median) + sigma * rng.standard_normal(count))), lo, hi).astype(int)
     ttt = rng.random(count) < ttt_fraction
+    hit = rng.random(count) < hit_fraction if sources is not None and sources.n > 0 else np.zeros(count, bool)
     out = []
     for k in range(count):
-        q = query(5, (seed << 20) + k + 1, int(lens[k]))
+        if hit[k]:
+            t = int(rng.integers(0, sources.n))
+            a, e = int(sources.offsets[t]), int(sources.offsets[t + 1])
+            L = min(int(lens[k]), e - a)
+            o = a + int(rng.integers(0, e - a - L + 1))
+            q = np.array(sources.residues[o:o + L], np.uint8)
+        else:
+            q = query(5, (seed << 20) + k + 1, int(lens[k]))
         out.append(Request(float(arrivals[k]), q, "TTTT" if ttt[k] else "FFTT"))
     return out
