@@ -280,6 +280,11 @@ typedef struct ish_hip_search_config {
     int32_t record_type;  /* ISH_RECORD_* */
     int64_t batch_bytes;  /* record bytes per GPU batch (0 = 256 MiB);
                              output never depends on it */
+    int32_t reader_threads;       /* parser threads for plain files (0 = up
+                                     to 16); gzip streams are read by one */
+    int32_t reader_segment_bytes; /* bytes per parallel parse segment
+                                     (0 = 8 MiB); output never depends on
+                                     either */
 } ish_hip_search_config;
 
 typedef struct ish_hip_search_stats {
@@ -306,6 +311,13 @@ typedef struct ish_hip_search_stats {
 typedef struct ish_hip_records ish_hip_records;
 int32_t ish_hip_records_read(const char *path, int32_t record_type,
                              ish_hip_records **out);
+/* The same records, read with `threads` parser threads (0 = up to 16) over
+ * segments of `segment_bytes` (0 = 8 MiB) when the file is not gzip: the
+ * records never depend on either. No counterpart in the reference (its
+ * kseq reader is serial). */
+int32_t ish_hip_records_read_ex(const char *path, int32_t record_type,
+                                int32_t threads, int64_t segment_bytes,
+                                ish_hip_records **out);
 int64_t ish_hip_records_count(const ish_hip_records *r);
 int32_t ish_hip_records_flags(const ish_hip_records *r);
 int32_t ish_hip_records_status(const ish_hip_records *r);

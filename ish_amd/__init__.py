@@ -38,7 +38,7 @@ ABI_SYMBOLS = [
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
     "ish_hip_server_clock", "ish_hip_server_submit", "ish_hip_server_wait", "ish_hip_server_hits",
     "ish_hip_server_scores", "ish_hip_server_release", "ish_hip_server_get_stats",
-    "ish_hip_search_file", "ish_hip_records_read", "ish_hip_records_count", "ish_hip_records_flags",
+    "ish_hip_search_file", "ish_hip_records_read", "ish_hip_records_read_ex", "ish_hip_records_count", "ish_hip_records_flags",
     "ish_hip_records_status", "ish_hip_records_get", "ish_hip_records_free",
 ]
 RECORD = {"line": 0, "fastx": 1}
@@ -47,7 +47,8 @@ RECORD = {"line": 0, "fastx": 1}
 class SearchConfig(ctypes.Structure):
     _fields_ = [("matrix_kind", ctypes.c_int32), ("algo", ctypes.c_int32), ("gap_open", ctypes.c_int32),
                 ("gap_extend", ctypes.c_int32), ("ends_free", ctypes.c_uint32), ("threshold", ctypes.c_float),
-                ("record_type", ctypes.c_int32), ("batch_bytes", ctypes.c_int64)]
+                ("record_type", ctypes.c_int32), ("batch_bytes", ctypes.c_int64),
+                ("reader_threads", ctypes.c_int32), ("reader_segment_bytes", ctypes.c_int32)]
 
 
 class SearchStats(ctypes.Structure):
@@ -145,6 +146,7 @@ def lib():
         "ish_hip_server_get_stats": ([vp, P(ServerStats)], i32),
         "ish_hip_search_file": ([vp, ctypes.c_char_p, u8p, i32, P(SearchConfig), i32, P(SearchStats)], i32),
         "ish_hip_records_read": ([ctypes.c_char_p, i32, P(vp)], i32),
+        "ish_hip_records_read_ex": ([ctypes.c_char_p, i32, i32, ctypes.c_int64, P(vp)], i32),
         "ish_hip_records_count": ([vp], i64),
         "ish_hip_records_flags": ([vp], i32),
         "ish_hip_records_status": ([vp], i32),
@@ -219,13 +221,15 @@ def flags_from_str(s: str) -> int:
     return sum(1 << k for k, c in enumerate(s) if c == "T")
 
 
-def read_records(path, record_type: str = "line"):
+def read_records(path, record_type: str = "line", threads: int = 0, segment_bytes: int = 0):
     """Every record of a file as the runner reads it (host only): returns
     (records, flags, status); records are (seq, name, qual) bytes, flags bit 0
-    = binary, bit 1 = FASTQ."""
+    = binary, bit 1 = FASTQ. threads / segment_bytes set the parallel parse of
+    plain files (0 = defaults); the records never depend on them."""
     L = lib()
     h = ctypes.c_void_p()
-    rc = L.ish_hip_records_read(os.fsencode(path), RECORD[record_type], ctypes.byref(h))
+    rc = L.ish_hip_records_read_ex(os.fsencode(path), RECORD[record_type], threads, segment_bytes,
+                                   ctypes.byref(h))
     if rc != 0:
         raise IshError(f"ish_hip_records_read failed: {ERRORS.get(rc, rc)}")
     try:
@@ -410,14 +414,16 @@ class Engine:
 
     def search_file(self, path, pattern: bytes, matrix_kind: str, algo="striped-semi-global",
                     gap_open: int = 3, gap_extend: int = 1, ends_free="FFTT", threshold: float = 0.8,
-                    record_type: str = "line", out_fd: int = 1, batch_bytes: int = 0):
+                    record_type: str = "line", out_fd: int = 1, batch_bytes: int = 0,
+                    reader_threads: int = 0, reader_segment_bytes: int = 0):
         """Searches one file (plain or gzip) and writes the emitted records to
         out_fd in the reference runners' format; returns the run's stats."""
         pat = np.frombuffer(bytes(pattern), np.uint8).copy()
         pb = pat if len(pat) else np.zeros(1, np.uint8)
         fl = flags_from_str(ends_free) if isinstance(ends_free, str) else int(ends_free)
         cfg = SearchConfig(MATRIX[matrix_kind], ALGO[algo] if isinstance(algo, str) else int(algo), gap_open,
-                           gap_extend, fl, threshold, RECORD[record_type], batch_bytes)
+                           gap_extend, fl, threshold, RECORD[record_type], batch_bytes, reader_threads,
+                           reader_segment_bytes)
         st = SearchStats()
         self._check(self._L.ish_hip_search_file(self._h, os.fsencode(path), _p(pb, ctypes.c_uint8), len(pat),
                                                 ctypes.byref(cfg), out_fd, ctypes.byref(st)))

@@ -15,6 +15,7 @@
 
 #include "engine.h"
 #include "host.h"
+#include "pieces.h"
 #include "tables.h"
 
 using namespace ish;
@@ -610,29 +611,24 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
 // Uploads residues[offsets[0] .. offsets[n]) and builds the database;
 // with `encode` the bytes are translated to residue codes on the device
 // first.
-static int32_t db_load_impl(ish_hip_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
-                            int64_t n, const EncodeTable *encode, ish_hip_db **out) {
-    if (!ctx || !out || n < 0 || (n > 0 && (!offsets || !residues)))
-        return fail(ctx, ISH_E_INVALID, "bad db arguments");
-    if (n > (int64_t)INT32_MAX - kGroup) return fail(ctx, ISH_E_RANGE, "too many targets");
-    *out = nullptr;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const uint64_t base = n > 0 ? offsets[0] : 0;
-    std::vector<uint64_t> h_off((size_t)n + 1, 0);
-    for (int64_t t = 0; t < n; ++t) {
-        if (offsets[t + 1] < offsets[t]) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
-        if (offsets[t + 1] - offsets[t] > (uint64_t)INT32_MAX / 2) return fail(ctx, ISH_E_RANGE, "target too long");
-        h_off[(size_t)t + 1] = offsets[t + 1] - base;
-    }
-    const uint64_t nres = h_off[(size_t)n];
+// Uploads the residues (the concatenation of pieces) and builds the db.
+// h_off: offsets from 0, already checked.
+static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t np,
+                         std::vector<uint64_t> h_off, const EncodeTable *encode, ish_hip_db **out) {
+    const size_t n = h_off.size() - 1;
+    const uint64_t nres = h_off[n];
     uint8_t *d_raw = nullptr;
     uint64_t *d_off = nullptr;
     hipError_t e = hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (size_t)(n + 1));
-    if (e == hipSuccess && nres > 0)
-        e = hipMemcpyAsync(d_raw, residues + base, nres, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (n + 1));
+    uint64_t at = 0;
+    for (size_t k = 0; k < np && e == hipSuccess; ++k) {
+        if (pieces[k].n > 0)
+            e = hipMemcpyAsync(d_raw + at, pieces[k].p, pieces[k].n, hipMemcpyHostToDevice, ctx->stream);
+        at += pieces[k].n;
+    }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, ctx->stream);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -641,6 +637,53 @@ static int32_t db_load_impl(ish_hip_ctx *ctx, const uint8_t *residues, const uin
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "db upload: %s", hipGetErrorString(e));
     }
     return db_build(ctx, d_raw, d_off, std::move(h_off), out);
+}
+
+// Checks the offsets (non-decreasing, targets within the int32 limit) and
+// returns them rebased to 0 in h_off.
+static int32_t db_offsets(ish_hip_ctx *ctx, const uint64_t *offsets, int64_t n,
+                          std::vector<uint64_t> &h_off) {
+    if (n > (int64_t)INT32_MAX - kGroup) return fail(ctx, ISH_E_RANGE, "too many targets");
+    const uint64_t base = n > 0 ? offsets[0] : 0;
+    h_off.assign((size_t)n + 1, 0);
+    for (int64_t t = 0; t < n; ++t) {
+        if (offsets[t + 1] < offsets[t]) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
+        if (offsets[t + 1] - offsets[t] > (uint64_t)INT32_MAX / 2) return fail(ctx, ISH_E_RANGE, "target too long");
+        h_off[(size_t)t + 1] = offsets[t + 1] - base;
+    }
+    return ISH_OK;
+}
+
+static int32_t db_load_impl(ish_hip_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
+                            int64_t n, const EncodeTable *encode, ish_hip_db **out) {
+    if (!ctx || !out || n < 0 || (n > 0 && (!offsets || !residues)))
+        return fail(ctx, ISH_E_INVALID, "bad db arguments");
+    *out = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<uint64_t> h_off;
+    const int32_t rc = db_offsets(ctx, offsets, n, h_off);
+    if (rc) return rc;
+    const ish::HostPiece one{n > 0 ? residues + offsets[0] : residues, h_off.back()};
+    return db_upload(ctx, &one, 1, std::move(h_off), encode, out);
+}
+
+int32_t ish::db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, size_t np,
+                                  const uint64_t *offsets, int64_t n, int32_t matrix_kind,
+                                  ish_hip_db **out) {
+    if (!ctx || !out || n < 0 || !offsets || (np > 0 && !pieces) || offsets[0] != 0)
+        return fail(ctx, ISH_E_INVALID, "bad db arguments");
+    *out = nullptr;
+    EncodeTable t;
+    if (ish_hip_encoding(matrix_kind, t.code) != ISH_OK)
+        return fail(ctx, ISH_E_INVALID, "unknown matrix kind %d", matrix_kind);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<uint64_t> h_off;
+    const int32_t rc = db_offsets(ctx, offsets, n, h_off);
+    if (rc) return rc;
+    uint64_t total = 0;
+    for (size_t k = 0; k < np; ++k) total += pieces[k].n;
+    if (total != h_off.back()) return fail(ctx, ISH_E_INVALID, "pieces do not match the offsets");
+    return db_upload(ctx, pieces, np, std::move(h_off), &t, out);
 }
 
 extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,

@@ -20,13 +20,31 @@
 // bytes (two in flight) while the calling thread uploads the previous batch
 // (the device translates the bytes to residue codes), scores it on the GPU
 // and writes its output.
+//
+// Plain (uncompressed) files are parsed in parallel: the file is mapped,
+// cut into segments, and each segment is parsed by a worker thread from its
+// first candidate record start (a line start; for FASTA/FASTQ a '>' or '@'
+// at a line start). A candidate need not be a record start of the serial
+// parse (a '>' line swallowed after an empty line, an '@' that begins a
+// quality line), so segment k's records are used only from the position
+// where the serial parse, continued from segment k-1, starts its next
+// record: records from that position on are the serial parse's own, since a
+// parse that has just read a record's header character is in the same
+// state however it got there. When segment k has no record there, it is
+// parsed again from that position. gzip streams are read serially.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
 #include <zlib.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <climits>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -35,6 +53,7 @@
 #include <vector>
 
 #include "../../include/ish_hip.h"
+#include "pieces.h"
 
 namespace {
 
@@ -45,19 +64,28 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Buffered reader over zlib with the reference's read_until semantics.
+// Buffered reader with the reference's read_until semantics, over zlib or
+// over bytes in memory (a mapped file: positions are then file offsets).
 struct Reader {
     gzFile f = nullptr;
-    std::vector<uint8_t> buf;
+    std::vector<uint8_t> own; // zlib reads land here
+    const uint8_t *base = nullptr;
     size_t start = 0, end = 0;
     bool eof = false;
     bool error = false;
 
-    explicit Reader(gzFile fh) : f(fh), buf(kReadBuf) {}
+    explicit Reader(gzFile fh) : f(fh), own(kReadBuf) { base = own.data(); }
+    Reader(const uint8_t *data, size_t from, size_t to)
+        : base(data), start(from), end(to), eof(true) {}
 
     bool fill() {
+        if (!f) {
+            eof = true;
+            return false;
+        }
         start = 0;
-        const int n = gzread(f, buf.data(), (unsigned)buf.size());
+        base = own.data();
+        const int n = gzread(f, own.data(), (unsigned)own.size());
         if (n < 0) {
             error = true;
             end = 0;
@@ -65,7 +93,7 @@ struct Reader {
             return false;
         }
         end = (size_t)n;
-        if (end < buf.size()) eof = true;
+        if (end < own.size()) eof = true;
         return end > 0;
     }
 
@@ -75,7 +103,7 @@ struct Reader {
             if (eof) return -1;
             if (!fill()) return -1;
         }
-        return buf[start++];
+        return base[start++];
     }
 
     // Appends bytes up to (not including) the next '\n' to out, consuming the
@@ -89,10 +117,10 @@ struct Reader {
                 if (!fill()) break;
             }
             got_any = true;
-            const uint8_t *p = buf.data() + start;
+            const uint8_t *p = base + start;
             const uint8_t *q = (const uint8_t *)memchr(p, '\n', end - start);
-            const size_t r = q ? (size_t)(q - buf.data()) : end;
-            out.insert(out.end(), buf.data() + start, buf.data() + r);
+            const size_t r = q ? (size_t)(q - base) : end;
+            out.insert(out.end(), base + start, base + r);
             start = r + 1;
             if (r < end) break;
         }
@@ -107,16 +135,23 @@ struct FastxParser {
     Reader &rd;
     int last_char = 0;
     std::vector<uint8_t> name, seq, qual;
+    // memory readers: position of the last record's header character, and
+    // a position at or after which read() stops before the record's body
+    int64_t rec_pos = -1;
+    int64_t limit = INT64_MAX;
 
     explicit FastxParser(Reader &r) : rd(r) {}
 
-    // >= 0: sequence length; -1 end of file; -2 truncated quality; -3 error
+    // >= 0: sequence length; -1 end of file; -2 truncated quality; -3 error;
+    // -4 the next record starts at or after `limit` (at rec_pos)
     int64_t read() {
         if (last_char == 0) {
             int c = rd.read_byte();
             while (c >= 0 && c != '>' && c != '@') c = rd.read_byte();
             if (c < 0) return -1;
         }
+        rec_pos = (int64_t)rd.start - 1;
+        if (rec_pos >= limit) return -4;
         seq.clear();
         qual.clear();
         name.clear();
@@ -146,16 +181,44 @@ struct FastxParser {
     }
 };
 
-struct Batch {
-    std::vector<uint8_t> res;       // encoded residues
+// Records as the parser produced them.
+struct Recs {
+    std::vector<uint8_t> res;       // record bytes (the device encodes them)
     std::vector<uint64_t> off{0};   // record offsets into res
     std::vector<uint8_t> names;     // fastx names
     std::vector<uint64_t> noff{0};
     std::vector<uint8_t> quals;     // fastq qualities
     std::vector<uint64_t> qoff{0};
+    size_t count() const { return off.size() - 1; }
+    uint64_t bytes(size_t i) const {
+        return off[i + 1] - off[i] + (noff.size() > i + 1 ? noff[i + 1] - noff[i] : 0) +
+               (qoff.size() > i + 1 ? qoff[i + 1] - qoff[i] : 0);
+    }
+};
+
+// Records parsed from one segment of a mapped file (or one run of a
+// serial read).
+struct Segment {
+    Recs recs;
+    std::vector<int64_t> start; // file offset where each record starts
+    int64_t stop = -1;          // start of the first record at or after the limit
+    bool ended = false;         // the file ends inside this segment
+    int32_t status = 0;         // the reader's code when it ended
+};
+
+// A batch: records [first, end) of each part's segment, in order. The
+// records are not copied into the batch; the upload gathers them.
+struct Batch {
+    struct Part {
+        std::shared_ptr<const Segment> seg;
+        size_t first, end;
+    };
+    std::vector<Part> parts;
+    size_t nrec = 0;
+    int64_t bytes = 0;
     bool last = false;
     int32_t error = 0;
-    size_t records() const { return off.size() - 1; }
+    size_t records() const { return nrec; }
 };
 
 struct Queue {
@@ -201,11 +264,7 @@ Peek peek_file(const char *path) {
     pk.binary = memchr(head.data(), 0, head.size()) != nullptr;
     // FASTQ if a record of the first bytes has a quality string (peek_file
     // parses only what the first read returned, with the same record rules)
-    Reader rd(nullptr);
-    rd.buf = head;
-    rd.start = 0;
-    rd.end = head.size();
-    rd.eof = true;
+    Reader rd(head.data(), 0, head.size());
     FastxParser fp(rd);
     while (fp.read() > 0) pk.fastq = pk.fastq || !fp.qual.empty();
     return pk;
@@ -244,6 +303,277 @@ struct RecordSource {
         return true;
     }
 };
+
+// ---------------------------------------------------------------------------
+// Batches: the records of a file, in order, cut into batches of about
+// batch_bytes (a batch is handed over once its bytes reach batch_bytes; the
+// last one, possibly empty, has last = true and the reader's status).
+// ---------------------------------------------------------------------------
+using BatchSink = std::function<void(std::unique_ptr<Batch>)>;
+
+// Parses records of m[from, n) as the serial reader does from a record
+// start, until the next record starts at or after `limit`.
+void parse_segment(const uint8_t *m, size_t n, int32_t type, size_t from, int64_t limit,
+                   Segment &sg) {
+    Reader rd(m, from, n);
+    Recs &b = sg.recs;
+    if (type == ISH_RECORD_LINE) {
+        std::vector<uint8_t> line;
+        for (;;) {
+            if ((int64_t)rd.start >= limit) {
+                sg.stop = (int64_t)rd.start;
+                return;
+            }
+            const int64_t at = (int64_t)rd.start;
+            line.clear();
+            if (rd.read_until_newline(line) < 0) {
+                sg.ended = true;
+                return;
+            }
+            sg.start.push_back(at);
+            b.res.insert(b.res.end(), line.begin(), line.end());
+            b.off.push_back(b.res.size());
+        }
+    }
+    FastxParser fp(rd);
+    fp.limit = limit;
+    for (;;) {
+        const int64_t r = fp.read();
+        if (r == -4) {
+            sg.stop = fp.rec_pos;
+            return;
+        }
+        if (r <= 0) { // end of file, an empty sequence, or a reader error
+            sg.ended = true;
+            sg.status = r < -1 ? (int32_t)r : 0;
+            return;
+        }
+        sg.start.push_back(fp.rec_pos);
+        b.res.insert(b.res.end(), fp.seq.begin(), fp.seq.end());
+        b.off.push_back(b.res.size());
+        b.names.insert(b.names.end(), fp.name.begin(), fp.name.end());
+        b.noff.push_back(b.names.size());
+        b.quals.insert(b.quals.end(), fp.qual.begin(), fp.qual.end());
+        b.qoff.push_back(b.quals.size());
+    }
+}
+
+// Cuts records into batches of about batch_bytes.
+struct Batcher {
+    const BatchSink &sink;
+    int64_t batch_bytes;
+    std::unique_ptr<Batch> b = std::make_unique<Batch>();
+    Batcher(const BatchSink &s, int64_t bb) : sink(s), batch_bytes(bb) {}
+    // records [i, count) of sg
+    void add(std::shared_ptr<const Segment> sg, size_t i) {
+        const Recs &r = sg->recs;
+        const size_t j = r.count();
+        while (i < j) {
+            // records up to the one that brings the batch to batch_bytes
+            size_t e = i;
+            while (e < j && b->bytes < batch_bytes) b->bytes += (int64_t)r.bytes(e++);
+            b->parts.push_back({sg, i, e});
+            b->nrec += e - i;
+            i = e;
+            if (b->bytes >= batch_bytes) {
+                sink(std::move(b));
+                b = std::make_unique<Batch>();
+            }
+        }
+    }
+    void finish(int32_t status) {
+        b->last = true;
+        b->error = status;
+        sink(std::move(b));
+    }
+};
+
+// Serial producer (gzip streams): records go into segments of about
+// batch_bytes, each handed over as it fills.
+void produce_serial(gzFile f, int32_t type, int64_t batch_bytes, const BatchSink &sink) {
+    RecordSource src(f, type);
+    const bool fastx = type == ISH_RECORD_FASTX;
+    Batcher bt(sink, batch_bytes);
+    auto sg = std::make_shared<Segment>();
+    int64_t bytes = 0;
+    const std::vector<uint8_t> *seq = nullptr, *name = nullptr, *qual = nullptr;
+    while (src.next(seq, name, qual)) {
+        Recs &r = sg->recs;
+        r.res.insert(r.res.end(), seq->begin(), seq->end());
+        r.off.push_back(r.res.size());
+        bytes += (int64_t)seq->size();
+        if (fastx) {
+            r.names.insert(r.names.end(), name->begin(), name->end());
+            r.noff.push_back(r.names.size());
+            r.quals.insert(r.quals.end(), qual->begin(), qual->end());
+            r.qoff.push_back(r.quals.size());
+            bytes += (int64_t)(name->size() + qual->size());
+        }
+        if (bytes >= batch_bytes) {
+            bt.add(std::move(sg), 0);
+            sg = std::make_shared<Segment>();
+            bytes = 0;
+        }
+    }
+    bt.add(std::move(sg), 0);
+    bt.finish(src.status);
+}
+
+struct Mapped {
+    int fd = -1;
+    const uint8_t *p = nullptr;
+    size_t n = 0;
+    bool open(const char *path) {
+        fd = ::open(path, O_RDONLY);
+        if (fd < 0) return false;
+        struct stat sb;
+        if (fstat(fd, &sb) != 0) return false;
+        n = (size_t)sb.st_size;
+        if (n == 0) return true;
+        void *m = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m == MAP_FAILED) return false;
+        p = (const uint8_t *)m;
+        (void)madvise(m, n, MADV_SEQUENTIAL);
+        return true;
+    }
+    ~Mapped() {
+        if (p) munmap((void *)p, n);
+        if (fd >= 0) ::close(fd);
+    }
+};
+
+// First candidate record start at or after s (0 at the start of the file,
+// n when there is none).
+size_t candidate(const uint8_t *m, size_t n, int32_t type, size_t s) {
+    if (s == 0) return 0;
+    size_t pos = s - 1; // a line starts at s when m[s-1] is '\n'
+    while (pos < n) {
+        const uint8_t *q = (const uint8_t *)memchr(m + pos, '\n', n - pos);
+        if (!q) return n;
+        const size_t y = (size_t)(q - m) + 1;
+        if (y >= n) return n;
+        if (type == ISH_RECORD_LINE || m[y] == '>' || m[y] == '@') return y;
+        pos = y;
+    }
+    return n;
+}
+
+// Parallel producer over a mapped plain file (see the top of the file).
+void produce_parallel(const Mapped &mf, int32_t type, int64_t batch_bytes, int threads,
+                      int64_t seg_bytes, const BatchSink &sink) {
+    const uint8_t *m = mf.p;
+    const size_t n = mf.n;
+    const size_t nseg = n == 0 ? 1 : (size_t)((n + (size_t)seg_bytes - 1) / (size_t)seg_bytes);
+    // candidate starts, each at or after the previous one (segments may be
+    // empty); cand[nseg] = n
+    std::vector<size_t> cand(nseg + 1, n);
+    cand[0] = 0;
+    for (size_t k = 1; k < nseg; ++k) {
+        const size_t s = k * (size_t)seg_bytes;
+        cand[k] = cand[k - 1] >= s ? cand[k - 1] : candidate(m, n, type, s);
+    }
+    auto limit_of = [&](size_t k) -> int64_t { return k + 1 < nseg ? (int64_t)cand[k + 1] : INT64_MAX; };
+
+    const size_t window = 2 * (size_t)threads; // segments parsed ahead of the stitcher
+    std::vector<std::unique_ptr<Segment>> seg(nseg);
+    std::vector<char> ready(nseg, 0);
+    std::mutex mu;
+    std::condition_variable cv;
+    size_t consumed = 0;
+    bool stop = false;
+    std::atomic<size_t> next{0};
+    auto worker = [&] {
+        for (;;) {
+            const size_t k = next.fetch_add(1);
+            if (k >= nseg) return;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || k < consumed + window; });
+                if (stop) return;
+            }
+            auto sg = std::make_unique<Segment>();
+            parse_segment(m, n, type, cand[k], limit_of(k), *sg);
+            std::lock_guard<std::mutex> lk(mu);
+            seg[k] = std::move(sg);
+            ready[k] = 1;
+            cv.notify_all();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
+
+    Batcher bt(sink, batch_bytes);
+    int32_t status = 0;
+    int64_t expected = 0; // where the serial parse starts its next record
+    for (size_t k = 0; k < nseg; ++k) {
+        std::unique_ptr<Segment> sg;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return ready[k] != 0; });
+            sg = std::move(seg[k]);
+        }
+        size_t first = 0;
+        if (expected != (int64_t)cand[k]) {
+            // the serial parse continued past this candidate: use the
+            // segment from its record at `expected`, or parse it again
+            // from there
+            const auto it = std::lower_bound(sg->start.begin(), sg->start.end(), expected);
+            if (it != sg->start.end() && *it == expected) {
+                first = (size_t)(it - sg->start.begin());
+            } else {
+                sg = std::make_unique<Segment>();
+                parse_segment(m, n, type, (size_t)expected, limit_of(k), *sg);
+            }
+        }
+        const bool ended = sg->ended;
+        status = sg->status;
+        expected = sg->stop;
+        bt.add(std::shared_ptr<const Segment>(std::move(sg)), first);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            consumed = k + 1;
+            cv.notify_all();
+        }
+        if (ended) break;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+        cv.notify_all();
+    }
+    for (auto &t : pool) t.join();
+    bt.finish(status);
+}
+
+bool is_gzip(const char *path) {
+    uint8_t h[2] = {0, 0};
+    const int fd = ::open(path, O_RDONLY);
+    if (fd < 0) return false;
+    const ssize_t r = ::read(fd, h, 2);
+    ::close(fd);
+    return r == 2 && h[0] == 0x1f && h[1] == 0x8b;
+}
+
+// Every record of path, in batches. Returns 0 or ISH_E_INVALID when the
+// file cannot be opened.
+int32_t produce_batches(const char *path, int32_t type, int64_t batch_bytes, int threads,
+                        int64_t seg_bytes, const BatchSink &sink) {
+    if (threads <= 0) threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (seg_bytes <= 0) seg_bytes = (int64_t)8 << 20;
+    if (!is_gzip(path)) {
+        Mapped mf;
+        if (mf.open(path)) {
+            produce_parallel(mf, type, batch_bytes, threads, seg_bytes, sink);
+            return ISH_OK;
+        }
+    }
+    gzFile f = gzopen(path, "rb");
+    if (!f) return ISH_E_INVALID;
+    gzbuffer(f, 1 << 20);
+    produce_serial(f, type, batch_bytes, sink);
+    gzclose(f);
+    return ISH_OK;
+}
 
 struct Writer {
     int fd;
@@ -292,40 +622,24 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         return ISH_OK; // binary files are skipped
     }
     const bool fastq = pk.fastq;
-    gzFile f = gzopen(path, "rb");
-    if (!f) return ISH_E_INVALID;
-    gzbuffer(f, 1 << 20);
+    {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return ISH_E_INVALID;
+        ::close(fd);
+    }
     const int64_t batch_bytes = cfg->batch_bytes > 0 ? cfg->batch_bytes : ((int64_t)256 << 20);
 
     Queue queue;
     double t_parse = 0;
     std::thread parser([&] {
         const double t0 = now_s();
-        RecordSource src(f, cfg->record_type);
-        auto b = std::make_unique<Batch>();
-        bool more = true;
-        while (more) {
-            const std::vector<uint8_t> *seq = nullptr, *name = nullptr, *qual = nullptr;
-            if (!src.next(seq, name, qual)) {
-                more = false;
-                b->error = src.status;
-            } else {
-                // raw bytes: the device encodes them after the upload
-                b->res.insert(b->res.end(), seq->begin(), seq->end());
-                b->off.push_back(b->res.size());
-                if (name) {
-                    b->names.insert(b->names.end(), name->begin(), name->end());
-                    b->noff.push_back(b->names.size());
-                    b->quals.insert(b->quals.end(), qual->begin(), qual->end());
-                    b->qoff.push_back(b->quals.size());
-                }
-            }
-            const int64_t bytes = (int64_t)(b->res.size() + b->names.size() + b->quals.size());
-            if (!more || bytes >= batch_bytes) {
-                b->last = !more;
-                queue.push(std::move(b));
-                b = std::make_unique<Batch>();
-            }
+        const BatchSink sink = [&](std::unique_ptr<Batch> b) { queue.push(std::move(b)); };
+        if (produce_batches(path, cfg->record_type, batch_bytes, cfg->reader_threads,
+                            cfg->reader_segment_bytes, sink) != ISH_OK) {
+            auto b = std::make_unique<Batch>();
+            b->last = true;
+            b->error = -3;
+            queue.push(std::move(b));
         }
         t_parse = now_s() - t0;
     });
@@ -339,9 +653,21 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
             const double tg = now_s();
             std::vector<int32_t> score((size_t)n), er((size_t)n), eq((size_t)n);
             std::vector<uint8_t> emit((size_t)n);
+            // the batch's record bytes, piece by piece, and their offsets
+            std::vector<ish::HostPiece> pieces;
+            std::vector<uint64_t> off((size_t)n + 1, 0);
+            uint64_t acc = 0;
+            size_t k = 0;
+            for (const Batch::Part &pt : b->parts) {
+                const Recs &r = pt.seg->recs;
+                const uint64_t a = r.off[pt.first];
+                for (size_t x = pt.first; x < pt.end; ++x) off[++k] = acc + (r.off[x + 1] - a);
+                pieces.push_back({r.res.data() + a, r.off[pt.end] - a});
+                acc += r.off[pt.end] - a;
+            }
             ish_hip_db *db = nullptr;
-            if (b->res.empty()) b->res.push_back(0);
-            rc = ish_hip_db_load_ascii(ctx, b->res.data(), b->off.data(), n, cfg->matrix_kind, &db);
+            rc = ish::db_load_ascii_pieces(ctx, pieces.data(), pieces.size(), off.data(), n,
+                                           cfg->matrix_kind, &db);
             if (rc == ISH_OK) rc = ish_hip_align_db(ctx, db, score.data(), er.data(), eq.data(), nullptr);
             if (db) ish_hip_db_free(db);
             if (rc == ISH_OK)
@@ -351,31 +677,35 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
             st.seconds_gpu += tw - tg;
             if (rc == ISH_OK) {
                 std::vector<uint8_t> tmp;
-                for (int64_t k = 0; k < n; ++k) {
-                    if (!emit[(size_t)k]) continue;
-                    st.matches++;
-                    const uint64_t a = b->off[(size_t)k], e = b->off[(size_t)k + 1];
-                    tmp.resize(e - a);
-                    for (uint64_t x = a; x < e; ++x) tmp[x - a] = dec[enc[b->res[x]]];
-                    if (cfg->record_type == ISH_RECORD_LINE) {
-                        out.put(tmp.data(), tmp.size());
-                        out.put("\n");
-                    } else {
-                        out.put(">");
-                        out.put(b->names.data() + b->noff[(size_t)k], b->noff[(size_t)k + 1] - b->noff[(size_t)k]);
-                        out.put("\n");
-                        out.put(tmp.data(), tmp.size());
-                        if (fastq) {
-                            out.put("\n+\n");
-                            out.put(b->quals.data() + b->qoff[(size_t)k], b->qoff[(size_t)k + 1] - b->qoff[(size_t)k]);
+                k = 0;
+                for (const Batch::Part &pt : b->parts) {
+                    const Recs &r = pt.seg->recs;
+                    for (size_t x = pt.first; x < pt.end; ++x, ++k) {
+                        if (!emit[k]) continue;
+                        st.matches++;
+                        const uint64_t a = r.off[x], e = r.off[x + 1];
+                        tmp.resize(e - a);
+                        for (uint64_t y = a; y < e; ++y) tmp[y - a] = dec[enc[r.res[y]]];
+                        if (cfg->record_type == ISH_RECORD_LINE) {
+                            out.put(tmp.data(), tmp.size());
+                            out.put("\n");
+                        } else {
+                            out.put(">");
+                            out.put(r.names.data() + r.noff[x], r.noff[x + 1] - r.noff[x]);
+                            out.put("\n");
+                            out.put(tmp.data(), tmp.size());
+                            if (fastq) {
+                                out.put("\n+\n");
+                                out.put(r.quals.data() + r.qoff[x], r.qoff[x + 1] - r.qoff[x]);
+                            }
+                            out.put("\n");
                         }
-                        out.put("\n");
                     }
                 }
             }
             st.seconds_write += now_s() - tw;
             st.records += n;
-            st.residues += (int64_t)(b->off.back());
+            st.residues += (int64_t)acc;
             st.batches++;
         }
         // like the reference's runners, a reader error ends the file
@@ -384,7 +714,6 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         if (b->last) break;
     }
     parser.join();
-    gzclose(f);
     out.flush();
     if (out.failed && rc == ISH_OK) rc = ISH_E_INVALID;
     st.cells = (double)st.residues * (double)plen;
@@ -400,30 +729,43 @@ struct ish_hip_records {
     int32_t flags = 0, status = 0;
 };
 
-extern "C" int32_t ish_hip_records_read(const char *path, int32_t record_type, ish_hip_records **out) {
+extern "C" int32_t ish_hip_records_read_ex(const char *path, int32_t record_type, int32_t threads,
+                                           int64_t segment_bytes, ish_hip_records **out) {
     if (!path || !out || (record_type != ISH_RECORD_LINE && record_type != ISH_RECORD_FASTX))
         return ISH_E_INVALID;
     *out = nullptr;
     const Peek pk = peek_file(path);
     if (!pk.ok) return ISH_E_INVALID;
-    gzFile f = gzopen(path, "rb");
-    if (!f) return ISH_E_INVALID;
     auto r = std::make_unique<ish_hip_records>();
     r->flags = (pk.binary ? 1 : 0) | (pk.fastq ? 2 : 0);
-    RecordSource src(f, record_type);
-    const std::vector<uint8_t> *seq = nullptr, *name = nullptr, *qual = nullptr;
-    while (src.next(seq, name, qual)) {
-        r->seqs.insert(r->seqs.end(), seq->begin(), seq->end());
-        r->soff.push_back(r->seqs.size());
-        if (name) r->names.insert(r->names.end(), name->begin(), name->end());
-        r->noff.push_back(r->names.size());
-        if (qual) r->quals.insert(r->quals.end(), qual->begin(), qual->end());
-        r->qoff.push_back(r->quals.size());
-    }
-    r->status = src.status;
-    gzclose(f);
+    const bool fastx = record_type == ISH_RECORD_FASTX;
+    const BatchSink sink = [&](std::unique_ptr<Batch> b) {
+        for (const Batch::Part &pt : b->parts) {
+            const Recs &s = pt.seg->recs;
+            for (size_t x = pt.first; x < pt.end; ++x) {
+                r->seqs.insert(r->seqs.end(), s.res.begin() + (ptrdiff_t)s.off[x],
+                               s.res.begin() + (ptrdiff_t)s.off[x + 1]);
+                r->soff.push_back(r->seqs.size());
+                if (fastx) {
+                    r->names.insert(r->names.end(), s.names.begin() + (ptrdiff_t)s.noff[x],
+                                    s.names.begin() + (ptrdiff_t)s.noff[x + 1]);
+                    r->quals.insert(r->quals.end(), s.quals.begin() + (ptrdiff_t)s.qoff[x],
+                                    s.quals.begin() + (ptrdiff_t)s.qoff[x + 1]);
+                }
+                r->noff.push_back(r->names.size());
+                r->qoff.push_back(r->quals.size());
+            }
+        }
+        if (b->last) r->status = b->error;
+    };
+    if (produce_batches(path, record_type, (int64_t)64 << 20, threads, segment_bytes, sink) != ISH_OK)
+        return ISH_E_INVALID;
     *out = r.release();
     return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_records_read(const char *path, int32_t record_type, ish_hip_records **out) {
+    return ish_hip_records_read_ex(path, record_type, 0, 0, out);
 }
 
 extern "C" int64_t ish_hip_records_count(const ish_hip_records *r) {

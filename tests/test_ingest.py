@@ -239,3 +239,70 @@ def test_reader_large_file_buffer_boundaries(tmp_path):
 def test_reader_missing_file(tmp_path):
     with pytest.raises(ish_amd.IshError):
         ish_amd.read_records(str(tmp_path / "nope.fa"), "fastx")
+
+
+# The parallel reader of plain files (segments parsed by worker threads and
+# stitched at the serial parse's record starts) must give the records of the
+# serial restatement whatever the segment size and thread count; tiny
+# segments put a boundary inside every kind of line.
+PARALLEL = [(3, 1), (4, 7), (2, 64), (8, 4096)]
+
+
+def _adversarial(rng, kind):
+    """Candidate record starts that are not record starts: '@' at the start
+    of quality lines, '>' lines swallowed after an empty or '\\r' line, '+'
+    and '>' inside qualities, truncated records."""
+    parts = []
+    for k in range(int(rng.integers(1, 60))):
+        if kind == "fastq":
+            L = int(rng.integers(0, 20))
+            seq = _b(rng, b"ACGT", L)
+            q = _b(rng, b"@>+I", L)
+            if rng.random() < 0.5 and L > 2:
+                q = q[:L // 2] + b"\n" + q[L // 2:]  # quality over two lines
+            parts.append(b"@q%d\n" % k + seq + b"\n+\n" + q + b"\n")
+            if rng.random() < 0.1:
+                parts.append(b"@" + _b(rng, b"I@", int(rng.integers(0, 6))) + b"\n")
+        else:
+            body = b"\n".join(_b(rng, b"ACGT", int(rng.integers(0, 15))) for _ in range(int(rng.integers(0, 3))))
+            sep = [b"\n", b"\n\n", b"\n\r\n", b"\r\n", b"\n\r\r\n"][int(rng.integers(0, 5))]
+            parts.append(b">f%d c\n" % k + body + sep)
+    data = b"".join(parts)
+    if rng.random() < 0.3:
+        data = data[:int(rng.integers(0, len(data) + 1))]
+    return data
+
+
+@pytest.mark.parametrize("threads,segment", PARALLEL)
+def test_parallel_reader_edge_and_random(tmp_path, threads, segment):
+    rng = np.random.default_rng(threads * 1000 + segment)
+    files = list(EDGE)
+    files += [_random_file(rng, kind) for kind in ("fasta", "fastq") for _ in range(40)]
+    files += [_adversarial(rng, kind) for kind in ("fasta", "fastq") for _ in range(40)]
+    for k, data in enumerate(files):
+        path = _write(tmp_path, f"p{k}", data, False)
+        for rtype in ("fastx", "line"):
+            got = ish_amd.read_records(path, rtype, threads=threads, segment_bytes=segment)
+            assert got == expected_records(data, rtype), (k, rtype, threads, segment, data[:200])
+
+
+def test_parallel_reader_many_segments(tmp_path):
+    """A file of a few thousand records over hundreds of segments and
+    several threads, FASTA and FASTQ, against the serial restatement."""
+    rng = np.random.default_rng(11)
+    for kind in ("fasta", "fastq"):
+        parts = []
+        for k in range(4000):
+            L = int(rng.integers(0, 200))
+            s = _b(rng, b"ACGT", L)
+            if kind == "fastq":
+                parts.append(b"@r%d\n" % k + s + b"\n+\n" + _b(rng, b"@I#", L) + b"\n")
+            else:
+                parts.append(b">r%d\n" % k + b"\n".join(s[i:i + 60] for i in range(0, L, 60)) + b"\n")
+        data = b"".join(parts)
+        path = _write(tmp_path, kind, data, False)
+        exp = {rt: expected_records(data, rt) for rt in ("fastx", "line")}
+        for threads, segment in ((8, 1000), (3, 777), (16, 50000)):
+            for rtype in ("fastx", "line"):
+                got = ish_amd.read_records(path, rtype, threads=threads, segment_bytes=segment)
+                assert got == exp[rtype], (kind, rtype, threads, segment)
