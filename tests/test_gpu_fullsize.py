@@ -1,0 +1,148 @@
+"""Parity at the full sizes of the workload configurations: the targets and
+queries that the bench lines run. Each check uses a property that does not
+shrink with the size: two independent GPU paths equal on every target, and
+the oracle on every target or on a large even sample.
+
+- config 1: 200k nucleotide records, actgn, FFTT.
+  - Every target's score and end coordinates against the oracle.
+  - The plain int32 engine equal to the batched one.
+  - The emitted set at 0.8 equal to the oracle's emission rule.
+- config 2: 2M protein targets, blosum62, FFTT.
+  - The batched launch (packed int16, hybrid) equal to the plain int32
+    engine on every target.
+  - Every target against the oracle's striped restatement.
+- config 3: 64 queries x 4M records.
+  - The multi-query launch equal, on every target, to single-query passes
+    for four of the queries.
+  - Those four against the oracle on an even sample.
+- config 4: 500k long reads, local.
+  - The workgroup pipeline over the paired profile equal to the one-wave
+    kernel over the int16 profile on every read.
+  - Both against the oracle on every 25th read.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ish_amd
+from ish_amd import Targets, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    if ish_amd.device_count() == 0:
+        pytest.fail("no gfx950 device visible to the HIP library")
+    e = ish_amd.Engine(0)
+    yield e
+    e.close()
+
+
+def _sample(t: Targets, idx):
+    lens = (t.offsets[idx + 1] - t.offsets[idx]).astype(np.int64)
+    off = np.zeros(len(idx) + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    res = np.concatenate([t.residues[int(t.offsets[i]):int(t.offsets[i + 1])] for i in idx])
+    return res, off
+
+
+def _same(label, a, b):
+    for name, x, y in zip(("score", "end_ref", "end_query"), a, b):
+        bad = np.flatnonzero(np.asarray(x) != np.asarray(y))
+        assert len(bad) == 0, f"{label}: {name} differs on {len(bad)} targets, first {bad[0]}: {x[bad[0]]} vs {y[bad[0]]}"
+
+
+def _scores(eng, db):
+    eng.set_engine("batched")
+    eng.set_schedule("auto")
+    s, er, eq, _ = eng.align_db(db)
+    return s, er, eq
+
+
+def test_config1_nucleotide_full(eng):
+    w = workloads.generate(1, 200_000, seed=1)
+    flags = O.flags_from_str("FFTT")
+    eng.set_query(w.query, "actgn", 3, 1, "FFTT")
+    db = eng.load_db(w.targets)
+    try:
+        got = _scores(eng, db)
+    finally:
+        db.close()
+    eng.set_engine("plain")
+    try:
+        plain = eng.align(w.targets)[:3]
+    finally:
+        eng.set_engine("batched")
+    _same("batched vs plain", got, plain)
+    m = O.Matrix("actgn")
+    ref = O.batch(0, w.query, w.targets.residues, w.targets.offsets, m, 3, 1, flags, nthreads=THREADS)[:3]
+    _same("batched vs oracle", got, ref)
+    ms = int(m.values.reshape(m.A, m.A)[w.query, w.query].sum())
+    emit = ish_amd.emit("striped-semi-global", *got, ms, 0.8)
+    keys, inv = np.unique(ref[0], return_inverse=True)
+    emit_ref = np.array([O.sg_emit(int(k), ms, 0.8) for k in keys], bool)[inv]
+    assert np.array_equal(emit, emit_ref)
+    assert emit.sum() > 0  # the planted copies are found
+
+
+def test_config2_protein_full(eng):
+    w = workloads.generate(2, 2_000_000, seed=1)
+    eng.set_query(w.query, "blosum62", 3, 1, "FFTT")
+    db = eng.load_db(w.targets)
+    try:
+        got = _scores(eng, db)
+    finally:
+        db.close()
+    eng.set_engine("plain")
+    try:
+        plain = eng.align(w.targets)[:3]
+    finally:
+        eng.set_engine("batched")
+    _same("batched vs plain", got, plain)
+    ref = O.batch(0, w.query, w.targets.residues, w.targets.offsets, O.Matrix("blosum62"), 3, 1,
+                  O.flags_from_str("FFTT"), nthreads=THREADS, striped=True)[:3]
+    _same("batched vs oracle", got, ref)
+
+
+def test_config3_multiquery_full(eng):
+    w = workloads.generate(3, 4_000_000, seed=1)
+    rng = np.random.default_rng(1)
+    qlens = rng.integers(50, 1001, 64)
+    queries = [workloads.query(3, 1000 + k, int(L)) for k, L in enumerate(qlens)]
+    db = eng.load_db(w.targets)
+    try:
+        ms, mer, meq, _ = eng.align_db_multi(db, queries, "blosum62", 3, 1, "FFTT")
+        idx = np.linspace(0, w.targets.n - 1, 20_000).astype(np.int64)
+        res, off = _sample(w.targets, idx)
+        for k in (0, 21, 42, 63):
+            eng.set_query(queries[k], "blosum62", 3, 1, "FFTT")
+            one = _scores(eng, db)
+            _same(f"multi vs single, query {k}", (ms[k], mer[k], meq[k]), one)
+            ref = O.batch(0, queries[k], res, off, O.Matrix("blosum62"), 3, 1, O.flags_from_str("FFTT"),
+                          nthreads=THREADS, striped=True)[:3]
+            _same(f"query {k} vs oracle", tuple(a[idx] for a in one), ref)
+    finally:
+        db.close()
+
+
+def test_config4_longread_full(eng):
+    w = workloads.generate(4, 500_000, seed=1)
+    eng.set_query(w.query, "actgn0", 3, 1, "FFFF", "striped-local")
+    db = eng.load_db(w.targets)
+    try:
+        eng.set_tile(32)
+        pipe = _scores(eng, db)
+        eng.set_schedule("wave")
+        wave = eng.align_db(db)[:3]
+    finally:
+        eng.set_schedule("auto")
+        db.close()
+    _same("workgroup pipeline vs one wave per group", pipe, wave)
+    idx = np.arange(0, w.targets.n, 25, dtype=np.int64)
+    res, off = _sample(w.targets, idx)
+    ref = O.batch(1, w.query, res, off, O.Matrix("actgn0"), 3, 1, 0, nthreads=THREADS, striped=True)[:3]
+    _same("pipeline vs oracle", tuple(a[idx] for a in pipe), ref)
