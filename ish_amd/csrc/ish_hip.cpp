@@ -608,11 +608,9 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     return ISH_OK;
 }
 
-// Uploads residues[offsets[0] .. offsets[n]) and builds the database;
+// Uploads the residues (the concatenation of pieces) and builds the db;
 // with `encode` the bytes are translated to residue codes on the device
-// first.
-// Uploads the residues (the concatenation of pieces) and builds the db.
-// h_off: offsets from 0, already checked.
+// first. h_off: offsets from 0, already checked.
 static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t np,
                          std::vector<uint64_t> h_off, const EncodeTable *encode, ish_hip_db **out) {
     const size_t n = h_off.size() - 1;

@@ -274,6 +274,21 @@ def test_local_long_reads_and_windows(eng, schedule):
         assert (st == ish_amd.STATUS_I16).all()
 
 
+@pytest.mark.parametrize("schedule", ["wave", "workgroup"])
+@pytest.mark.parametrize("tile", [32, 16])
+def test_local_degenerate_inputs(eng, tile, schedule):
+    """Queries of 0, 1 and 2 residues against empty, one-residue and short
+    reads and one long read; with the workgroup schedule the (one-class or
+    few-class) paired profile is used."""
+    rng = np.random.default_rng(21 + tile)
+    seqs = [np.zeros(0, np.uint8), np.array([0], np.uint8), np.array([3, 3], np.uint8)]
+    seqs += [rng.integers(0, 5, int(L)).astype(np.uint8) for L in rng.integers(1, 12, 200)]
+    seqs += [rng.integers(0, 5, 5000).astype(np.uint8)]
+    for Q in (0, 1, 2):
+        q = (np.arange(Q) % 4).astype(np.uint8)
+        check_local(eng, q, Targets.from_list(seqs), "actgn0", 3, 1, tile, f"Q={Q}", schedule)
+
+
 @pytest.mark.parametrize("tile", [32, 16])
 def test_local_paired_profile_wide_alphabet(eng, tile):
     """A query over a few bytes of the 256-code ASCII matrix has a profile of
