@@ -34,6 +34,10 @@ struct QueryDesc {
     int32_t reserved;
     int64_t tile_base;      // first work tile of this query (prefix sum)
     int64_t out_off;        // offset of this query's results in the outputs
+    // paired profile of this query (paired launches; see ScoreParams)
+    const int32_t *pprof;
+    const uint16_t *plut;
+    int32_t pnrows, pqpad;
 };
 
 // Everything a semi-global / local scoring launch needs.
@@ -117,12 +121,20 @@ hipError_t launch_sw_batched_i16(const ScoreParams &p, int chunk, int grid,
 hipError_t launch_sw_batched_i32(const ScoreParams &p, int chunk, int grid,
                                  int block, size_t lds, hipStream_t st);
 int occupancy_sw_batched_i16(int chunk, int block, size_t lds);
+// Paired profiles (both algorithms): the bytes of one pair row of a wave's
+// LDS copy of a chunk (K int32 words plus 16 bytes), and the largest class
+// count that gets one.
+constexpr int pair_row_bytes(int K) { return 4 * K + 16; }
+constexpr int kPairMaxRows = 8;
 // Hybrid semi-global launch (packed int16, block of 256): the first n_long
-// groups are pipelined across a workgroup's four waves.
+// groups are pipelined across a workgroup's four waves. `paired` selects
+// the variant over the paired profile (p.pprof, p.plut), whose whole LDS
+// size is sg_hybrid_paired_lds(chunk, classes).
 size_t sg_pipe_extra_lds();
-hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+size_t sg_hybrid_paired_lds(int chunk, int nrows);
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
                                 hipStream_t st);
-int occupancy_sg_hybrid_i16(int chunk, size_t lds, uint32_t flags);
+int occupancy_sg_hybrid_i16(int chunk, bool paired, size_t lds, uint32_t flags);
 // Workgroup-pipelined local kernel (packed int16, 4 waves share a group):
 // extra LDS beyond the profile, launcher (block of 256) and occupancy.
 // `paired` selects the variant that reads the paired profile (p.pprof,
@@ -130,9 +142,6 @@ int occupancy_sg_hybrid_i16(int chunk, size_t lds, uint32_t flags);
 // sw_pipe_paired_lds(chunk, profile classes).
 size_t sw_pipe_extra_lds();
 size_t sw_pipe_paired_lds(int chunk, int nrows);
-size_t sw_pipe_paired_row_bytes(int chunk);
-// Profiles with at most this many classes get the paired local profile.
-constexpr int kPairMaxRows = 8;
 hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
                               hipStream_t st);
 int occupancy_sw_pipe_i16(int chunk, bool paired, size_t lds);
@@ -149,9 +158,12 @@ hipError_t launch_gather_reversed(const uint8_t *raw, const uint64_t *src_off,
 // Multi-query launch. p.flags selects the kernel variant: its te / qe bits
 // must be the union over the launch's queries, and either every query has
 // te or qe free or none has (the per-query bits come from QueryDesc.flags).
-hipError_t launch_sg_multi(const ScoreParams &p, bool packed, int chunk, int grid,
+// `paired` (packed only): every query reads its paired profile
+// (QueryDesc.pprof / plut) and p.pnrows is the largest class count; the
+// LDS is then 1 KB plus 4 waves x pnrows^2 pair rows.
+hipError_t launch_sg_multi(const ScoreParams &p, bool packed, bool paired, int chunk, int grid,
                            int block, size_t lds, hipStream_t st);
-int occupancy_sg_multi(bool packed, int chunk, int block, size_t lds, uint32_t flags);
+int occupancy_sg_multi(bool packed, bool paired, int chunk, int block, size_t lds, uint32_t flags);
 // Emission at the --score threshold for the results of nq queries (query
 // k's results at k * n): hit (target, score, end_ref, end_query) records go
 // to hits[k * cap ...] and count[k] counts every emitted target (it may
@@ -175,6 +187,9 @@ struct EncodeTable {
 };
 // p[i] = t.code[p[i]] for i < n (p 16-byte aligned).
 hipError_t launch_encode_bytes(uint8_t *p, uint64_t n, const EncodeTable &t, hipStream_t st);
+// Sets bit c of mask[c >> 5] (8 words, cleared first) for every residue code
+// c that occurs in p[0, n).
+hipError_t launch_code_mask(const uint8_t *p, uint64_t n, uint32_t *mask, hipStream_t st);
 
 // Occupancy helpers: blocks per CU of each kernel at the given LDS size.
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags);

@@ -58,6 +58,19 @@ struct Profile {
 // query are 0.
 void build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
                    int32_t bias, bool want_rev_plain, Profile &pf);
+// Paired profile of a query over the profile classes of the residue codes
+// a database holds (the layout is described with ScoreParams.pprof):
+// `classes` classes, words[(a * classes + b) * qpad + j], and the pair
+// tables of tile heights 16 then 32, each [A: 256][B: 256].
+struct PairProfile {
+    int32_t classes = 0, qpad = 0;
+    std::vector<int32_t> words;
+    std::vector<uint16_t> table;
+};
+// Builds it for the codes set in present[8] (code 0, the padding code past
+// targets' ends, always counts). False when those codes fall into more
+// than kPairMaxRows classes.
+bool build_pairs(const Profile &pf, const uint32_t present[8], PairProfile &out);
 // Alphabet size of a built-in matrix kind (-1 when unknown).
 int32_t alphabet_of(int32_t kind);
 // The A*A matrix to use: the caller's, or the built-in one of `kind` when
@@ -92,6 +105,8 @@ struct ish_hip_db {
     uint8_t *d_raw = nullptr;
     uint64_t *d_off = nullptr;
     std::vector<uint64_t> h_off;
+    // residue codes present: bit c of code_mask[c >> 5]
+    uint32_t code_mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace ish {

@@ -51,8 +51,16 @@ struct ish_hip_ctx {
     int32_t prof_words = 0;
     size_t lds = 0;
     DevBuf prof_t, prof_rt, prof_p, lut; // forward, reversed-query, plain profiles
-    // paired local profile and its tables (0 rows: none; see ScoreParams)
+    // host copy of the query profile (paired profiles are built from it per
+    // database) and a counter of set_query calls
+    Profile hprof;
+    uint64_t query_gen = 0;
+    // paired profile of the current query for the code set pair_mask
+    // (pair_rows classes; 0: the codes need more than kPairMaxRows)
     DevBuf pprof, plut;
+    PairProfile pairs;
+    uint64_t pair_gen = 0; // query_gen it was built for (0: none)
+    uint32_t pair_mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int32_t pair_rows = 0, pqpad = 0;
     int engine = ISH_ENGINE_BATCHED;
     int chunk = kChunkDefault; // query rows per lane in the batched kernels
@@ -335,6 +343,43 @@ void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &
         }
 }
 
+bool ish::build_pairs(const Profile &pf, const uint32_t present[8], PairProfile &out) {
+    const int32_t rw = pf.stride / 2;
+    std::vector<int32_t> remap((size_t)pf.nrows, -1), cls_of;
+    for (int c = 0; c < 256; ++c) {
+        if (c != 0 && !((present[c >> 5] >> (c & 31)) & 1u)) continue;
+        const int32_t cls = pf.lut[c] / rw;
+        if (remap[(size_t)cls] < 0) {
+            remap[(size_t)cls] = (int32_t)cls_of.size();
+            cls_of.push_back(cls);
+            if ((int32_t)cls_of.size() > kPairMaxRows) return false;
+        }
+    }
+    const int32_t C = (int32_t)cls_of.size();
+    out.classes = C;
+    out.qpad = pf.qpad;
+    out.words.assign((size_t)C * C * pf.qpad, 0);
+    for (int32_t a = 0; a < C; ++a)
+        for (int32_t b = 0; b < C; ++b) {
+            const int16_t *ra = pf.fwd.data() + (size_t)cls_of[(size_t)a] * pf.stride;
+            const int16_t *rb = pf.fwd.data() + (size_t)cls_of[(size_t)b] * pf.stride;
+            int32_t *w = out.words.data() + ((size_t)a * C + b) * pf.qpad;
+            for (int32_t j = 0; j < pf.qpad; ++j)
+                w[j] = (int32_t)((uint32_t)(uint16_t)ra[j] | ((uint32_t)(uint16_t)rb[j] << 16));
+        }
+    // codes the database does not hold take class 0's rows (never read)
+    out.table.assign(1024, 0);
+    for (int ti = 0; ti < 2; ++ti) {
+        const int32_t rowb = pair_row_bytes(ti ? 32 : 16);
+        for (int c = 0; c < 256; ++c) {
+            const int32_t r = std::max(remap[(size_t)(pf.lut[c] / rw)], 0);
+            out.table[(size_t)ti * 512 + c] = (uint16_t)(r * C * rowb);
+            out.table[(size_t)ti * 512 + 256 + c] = (uint16_t)(r * rowb);
+        }
+    }
+    return true;
+}
+
 int32_t ish::matrix_of(const int8_t *matrix, int32_t &A, int32_t kind, std::vector<int8_t> &mat,
                        std::string &err) {
     if (!matrix) {
@@ -407,37 +452,6 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_rt.p, pf.rev.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->prof_p.p, pf.plain.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->lut.p, pf.lut, 512, hipMemcpyHostToDevice, ctx->stream));
-    // paired profile for local alignment over small alphabets: the packed
-    // operand of a lane's two targets is then one LDS word, with no
-    // per-cell pairing instruction in the kernel
-    std::vector<int32_t> pp;
-    std::vector<uint16_t> pl;
-    ctx->pair_rows = 0;
-    if (algo == ISH_ALGO_LOCAL && pf.nrows <= kPairMaxRows) {
-        const int32_t nr = pf.nrows, rw = pf.stride / 2;
-        pp.assign((size_t)nr * nr * pf.qpad, 0);
-        for (int32_t a = 0; a < nr; ++a)
-            for (int32_t b = 0; b < nr; ++b)
-                for (int32_t j = 0; j < pf.qpad; ++j)
-                    pp[((size_t)a * nr + b) * pf.qpad + j] =
-                        (int32_t)((uint32_t)(uint16_t)pf.fwd[(size_t)a * pf.stride + j] |
-                                  ((uint32_t)(uint16_t)pf.fwd[(size_t)b * pf.stride + j] << 16));
-        pl.assign(1024, 0);
-        for (int ti = 0; ti < 2; ++ti) {
-            const int32_t rowb = (int32_t)sw_pipe_paired_row_bytes(ti ? 32 : 16);
-            for (int c = 0; c < 256; ++c) {
-                const int32_t cls = pf.lut[c] / rw;
-                pl[(size_t)ti * 512 + c] = (uint16_t)(cls * nr * rowb);
-                pl[(size_t)ti * 512 + 256 + c] = (uint16_t)(cls * rowb);
-            }
-        }
-        HIP_TRY(ctx, ctx->pprof.ensure(pp.size() * 4));
-        HIP_TRY(ctx, ctx->plut.ensure(pl.size() * 2));
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->pprof.p, pp.data(), pp.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->plut.p, pl.data(), pl.size() * 2, hipMemcpyHostToDevice, ctx->stream));
-        ctx->pair_rows = nr;
-        ctx->pqpad = pf.qpad;
-    }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
     ctx->Q = Q;
@@ -456,6 +470,8 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     ctx->lds = lds;
     ctx->r16max = algo == ISH_ALGO_LOCAL ? (local_fits_int16(pf.qpad, go, ge, mx, mn) ? INT64_MAX : -1)
                                          : int16_max_len(Q, go, ge, mx, mn);
+    ctx->hprof = std::move(pf);
+    ctx->query_gen++;
     ctx->have_query = true;
     return ISH_OK;
 }
@@ -578,7 +594,10 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     }
     db->h_off = std::move(h_off);
     const uint64_t res_words = off + 64; // one spare block: empty groups read block 0
+    uint32_t *d_mask = nullptr;
     auto cleanup = [&](int32_t code) {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (d_mask) (void)hipFree(d_mask);
         ish_hip_db_free(db);
         return code;
     };
@@ -596,6 +615,10 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     DB_TRY(hipMalloc((void **)&db->d_end_q, sizeof(int32_t) * std::max<int64_t>(n, 1)));
     DB_TRY(hipMalloc((void **)&db->d_status, std::max<int64_t>(n, 1)));
     DB_TRY(hipMemsetAsync(db->d_res, 0, sizeof(uint2) * res_words, ctx->stream));
+    // residue codes present (paired profiles are built over their classes)
+    DB_TRY(hipMalloc((void **)&d_mask, 8 * sizeof(uint32_t)));
+    DB_TRY(launch_code_mask(db->d_raw, (uint64_t)db->nres, d_mask, ctx->stream));
+    DB_TRY(hipMemcpyAsync(db->code_mask, d_mask, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     if (ng > 0) {
         DB_TRY(hipMemcpyAsync(db->d_groups, groups.data(), sizeof(GroupDesc) * ng, hipMemcpyHostToDevice, ctx->stream));
         DB_TRY(hipMemcpyAsync(db->d_lens, lens.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
@@ -604,6 +627,7 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     }
     DB_TRY(hipStreamSynchronize(ctx->stream));
 #undef DB_TRY
+    (void)hipFree(d_mask);
     *out = db;
     return ISH_OK;
 }
@@ -724,7 +748,7 @@ struct DbLaunch {
     int32_t g_begin, g_end;
     int32_t grid;
     int32_t n_long; // kSgHybrid: leading groups scored by whole workgroups
-    int32_t paired; // kSwPipe: the paired-profile variant
+    int32_t paired; // kSwPipe / kSgHybrid: the paired-profile variant
     int64_t slot_stride;
     uint64_t lds;
     ScoreParams p;
@@ -767,11 +791,34 @@ static int32_t long_groups(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t
                            int32_t chunk, uint32_t flags) {
     double cols = 0;
     for (int32_t g = g0; g < g1; ++g) cols += db->group_maxlen[g] + 3;
-    const int occ = occupancy_sg_hybrid_i16(chunk, ctx->lds + sg_pipe_extra_lds(), flags);
+    const int occ = occupancy_sg_hybrid_i16(chunk, false, ctx->lds + sg_pipe_extra_lds(), flags);
     const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
     int32_t n = 0;
     while (g0 + n < g1 && db->group_maxlen[g0 + n] > 0.5 * per_wave) ++n;
     return n;
+}
+
+// Paired profile of the current query for db's residue codes, built and
+// uploaded when the query or the code set changed. Returns its class count
+// (0: those codes need more than kPairMaxRows classes) or an ISH_E_* code.
+static int32_t ensure_pairs(ish_hip_ctx *ctx, const ish_hip_db *db) {
+    if (ctx->pair_gen == ctx->query_gen && memcmp(ctx->pair_mask, db->code_mask, sizeof db->code_mask) == 0)
+        return ctx->pair_rows;
+    // a queued launch may still read the buffers
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pair_gen = ctx->query_gen;
+    memcpy(ctx->pair_mask, db->code_mask, sizeof db->code_mask);
+    ctx->pair_rows = 0;
+    if (!build_pairs(ctx->hprof, db->code_mask, ctx->pairs)) return 0;
+    const size_t wb = ctx->pairs.words.size() * 4, tb = ctx->pairs.table.size() * 2;
+    HIP_TRY(ctx, ctx->pprof.ensure(wb));
+    HIP_TRY(ctx, ctx->plut.ensure(tb));
+    if (wb) HIP_TRY(ctx, hipMemcpyAsync(ctx->pprof.p, ctx->pairs.words.data(), wb, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->plut.p, ctx->pairs.table.data(), tb, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pair_rows = ctx->pairs.classes;
+    ctx->pqpad = ctx->pairs.qpad;
+    return ctx->pair_rows;
 }
 
 // Plans the launches that score groups [g_lo, g_hi) of db (int32 groups
@@ -808,19 +855,29 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     }
     const size_t pipe_lds = ctx->lds + sw_pipe_extra_lds();
     const size_t hybrid_lds = ctx->lds + sg_pipe_extra_lds();
+    // forward passes over a database whose residue codes fall into few
+    // profile classes read the paired profile (pipelined local launches and
+    // hybrid semi-global launches, which then also serve the one-wave
+    // schedule with no pipelined groups)
+    int32_t pc = 0;
+    if (prof == ctx->prof_t.p) {
+        pc = ensure_pairs(ctx, db);
+        if (pc < 0) return pc;
+    }
     auto add16 = [&](int32_t g0, int32_t g1, int chunk, int sched) {
         if (g1 <= g0) return;
         DbLaunch L{};
         bool pipe = false;
         int32_t n_long = 0;
+        const bool sg_paired = !local && pc > 0 && sg_hybrid_paired_lds(chunk, pc) <= kLdsLimit;
         if (local && pipe_lds <= kLdsLimit) {
             if (sched == ISH_SCHEDULE_WORKGROUP) pipe = true;
             else if (sched == ISH_SCHEDULE_AUTO) pipe = pipe_heuristic(ctx, db, g0, g1, chunk);
-        } else if (!local && hybrid_lds <= kLdsLimit) {
+        } else if (!local && (hybrid_lds <= kLdsLimit || sg_paired)) {
             if (sched == ISH_SCHEDULE_WORKGROUP) n_long = g1 - g0;
             else if (sched == ISH_SCHEDULE_AUTO) n_long = long_groups(ctx, db, g0, g1, chunk, flags);
         }
-        L.kind = pipe ? kSwPipe : local ? kSw16 : n_long > 0 ? kSgHybrid : kSg16;
+        L.kind = pipe ? kSwPipe : local ? kSw16 : (n_long > 0 || sg_paired) ? kSgHybrid : kSg16;
         L.chunk = chunk;
         L.g_begin = g0;
         L.g_end = g1;
@@ -844,11 +901,12 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     // scratch between launches would free memory a queued launch uses)
     for (DbLaunch &L : out) {
         L.lds = L.kind == kSwPipe ? pipe_lds : L.kind == kSgHybrid ? hybrid_lds : ctx->lds;
-        // forward local passes over a small alphabet read the paired profile
-        if (L.kind == kSwPipe && prof == ctx->prof_t.p && ctx->pair_rows > 0 &&
-            sw_pipe_paired_lds(L.chunk, ctx->pair_rows) <= kLdsLimit) {
+        if (L.kind == kSwPipe && pc > 0 && sw_pipe_paired_lds(L.chunk, pc) <= kLdsLimit) {
             L.paired = 1;
-            L.lds = sw_pipe_paired_lds(L.chunk, ctx->pair_rows);
+            L.lds = sw_pipe_paired_lds(L.chunk, pc);
+        } else if (L.kind == kSgHybrid && pc > 0 && sg_hybrid_paired_lds(L.chunk, pc) <= kLdsLimit) {
+            L.paired = 1;
+            L.lds = sg_hybrid_paired_lds(L.chunk, pc);
         }
         int occ = 1;
         switch (L.kind) {
@@ -856,7 +914,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         case kSg16: occ = occupancy_sg_batched_i16(L.chunk, kBlock, L.lds, flags); break;
         case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
         case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
-        case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.lds, flags); break;
+        case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.lds, flags); break;
         default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.lds); break;
         }
         int grid;
@@ -909,7 +967,7 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
     case kSg16: e = launch_sg_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw32: e = launch_sw_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw16: e = launch_sw_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
-    case kSgHybrid: e = launch_sg_hybrid_i16(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
+    case kSgHybrid: e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
     default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
     }
     if (e == hipSuccess && timed) e = timed_end(ctx);
@@ -1181,12 +1239,32 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
         return ISH_E_RANGE;
     }
     const size_t luts_off = pofs[nq];
+    // paired profiles when every query's classes over the database's codes
+    // are few: the packed launches then read those instead
+    std::vector<PairProfile> pairs(nq);
+    bool paired = true;
+    int32_t max_classes = 0;
+    for (size_t k = 0; k < nq && paired; ++k) {
+        paired = build_pairs(*items[k].pf, db->code_mask, pairs[k]);
+        max_classes = std::max(max_classes, pairs[k].classes);
+    }
+    const size_t paired_lds = 1024 + (size_t)(kBlock / 64) * max_classes * max_classes * pair_row_bytes(chunk);
+    if (paired && paired_lds > kLdsLimit) paired = false;
+    std::vector<size_t> pp_ofs(nq + 1, luts_off + nq * 512);
+    if (paired)
+        for (size_t k = 0; k < nq; ++k)
+            pp_ofs[k + 1] = pp_ofs[k] + ((pairs[k].words.size() * 4 + pairs[k].table.size() * 2 + 255) & ~(size_t)255);
     std::vector<uint8_t> &blob = *mb.host_blob;
-    blob.assign(luts_off + nq * 512, 0);
+    blob.assign(pp_ofs[nq], 0);
     for (size_t k = 0; k < nq; ++k) {
         const Profile &pf = *items[k].pf;
         memcpy(blob.data() + pofs[k], pf.fwd.data(), pf.fwd.size() * 2);
         memcpy(blob.data() + luts_off + k * 512, pf.lut, 512);
+        if (paired) {
+            memcpy(blob.data() + pp_ofs[k], pairs[k].words.data(), pairs[k].words.size() * 4);
+            memcpy(blob.data() + pp_ofs[k] + pairs[k].words.size() * 4, pairs[k].table.data(),
+                   pairs[k].table.size() * 2);
+        }
     }
     hipError_t e = mb.blob->ensure(blob.size());
     if (e == hipSuccess) e = mb.desc->ensure(sizeof(QueryDesc) * (nq + 1) * 4);
@@ -1200,7 +1278,7 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
     const int waves = kBlock / 64;
     struct Launch {
         ScoreParams p;
-        bool packed;
+        bool packed, paired;
         int32_t maxcols, ntab;
     };
     std::vector<Launch> launches;
@@ -1242,6 +1320,14 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
                 d.flags = it.flags;
                 d.tile_base = tiles;
                 d.out_off = it.out_off;
+                if (paired && packed) {
+                    const PairProfile &pq = pairs[members[m]];
+                    uint8_t *base = (uint8_t *)mb.blob->p + pp_ofs[members[m]];
+                    d.pprof = (const int32_t *)base;
+                    d.plut = (const uint16_t *)(base + pq.words.size() * 4);
+                    d.pnrows = pq.classes;
+                    d.pqpad = pq.qpad;
+                }
                 const int64_t witems = (int64_t)(d.g_end - d.g_begin) * (packed ? 1 : 2);
                 tiles += (witems + waves - 1) / waves;
                 if (d.g_end > d.g_begin) maxcols = std::max(maxcols, db->group_maxlen[d.g_begin] + 3);
@@ -1264,7 +1350,9 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
             p.status = status;
             p.queries = (QueryDesc *)mb.desc->p + (size_t)table * (nq + 1);
             p.nqueries = (int32_t)members.size();
+            p.pnrows = paired && packed ? max_classes : 0;
             L.packed = packed;
+            L.paired = paired && packed;
             L.maxcols = maxcols;
             L.ntab = (int32_t)members.size() + 1;
             launches.push_back(L);
@@ -1275,7 +1363,9 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
     std::vector<int> grids(launches.size());
     for (size_t l = 0; l < launches.size(); ++l) {
         int64_t ss;
-        const int occ = occupancy_sg_multi(launches[l].packed, chunk, kBlock, lds, launches[l].p.flags);
+        const size_t llds = launches[l].paired ? paired_lds : lds;
+        const int occ = occupancy_sg_multi(launches[l].packed, launches[l].paired, chunk, kBlock, llds,
+                                           launches[l].p.flags);
         e = persistent_grid(*mb.scratch, occ, cus, launches[l].maxcols, &grids[l], &ss);
         if (e != hipSuccess) return hip_fail(e, "boundary scratch");
         launches[l].p.slot_stride = ss;
@@ -1287,7 +1377,9 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
                            sizeof(QueryDesc) * (size_t)launches[l].ntab, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_reset(p.counter, st);
         if (e == hipSuccess) e = on_launch(true);
-        if (e == hipSuccess) e = launch_sg_multi(p, launches[l].packed, chunk, grids[l], kBlock, lds, st);
+        if (e == hipSuccess)
+            e = launch_sg_multi(p, launches[l].packed, launches[l].paired, chunk, grids[l], kBlock,
+                                launches[l].paired ? paired_lds : lds, st);
         if (e == hipSuccess) e = on_launch(false);
         if (e != hipSuccess) return hip_fail(e, "multi-query launch");
     }

@@ -67,4 +67,49 @@ __device__ __forceinline__ void stage_profile(const int16_t *prof, const uint16_
 
 typedef short short8 __attribute__((ext_vector_type(8)));
 
+// Paired profile (small alphabets): one row per pair of profile classes
+// (class of a lane's target A, class of its target B) holding, per query
+// row j, the int32 word (S_A[j] + bias) | (S_B[j] + bias) << 16, so that a
+// packed cell's operand is one LDS word. A wave copies the K words of its
+// current chunk of every pair row into its own LDS buffer, rows
+// pair_row_bytes(K) apart (16 bytes over K words, so that rows start 16
+// bytes apart modulo 128 and the 16-byte slices of different rows spread
+// over the LDS banks). The pair table maps a residue code to the byte
+// offset of its class's rows: for target A at [0, 256), for target B at
+// [256, 512). pair_row_bytes is in engine.h.
+struct PairView {
+    const int32_t *pprof; // npairs rows of pqpad words
+    int32_t npairs, pqpad;
+    uint8_t *pbuf;        // this wave's LDS copy of one chunk
+};
+
+// Copies rows j0 .. j0+K-1 of every pair row into the wave's buffer. The
+// previous chunk's reads of the buffer were issued earlier, and a wave's
+// LDS accesses complete in order.
+template <int K>
+__device__ __forceinline__ void stage_pairs(const PairView &pv, int j0, int lane) {
+    constexpr int parts = K / 4; // 16-byte parts per row
+    for (int x = lane; x < pv.npairs * parts; x += 64) {
+        const int pr = x / parts, part = x % parts;
+        const uint4 v = *(const uint4 *)(pv.pprof + (size_t)pr * pv.pqpad + j0 + part * 4);
+        *(uint4 *)(pv.pbuf + pr * pair_row_bytes(K) + part * 16) = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// Pair row of the residue codes (a, b) in a wave's buffer.
+__device__ __forceinline__ const short8 *pair_row(const uint8_t *pbuf, const uint16_t *plut,
+                                                  uint32_t a, uint32_t b) {
+    return (const short8 *)(pbuf + (uint32_t)plut[a] + (uint32_t)plut[256 + b]);
+}
+
+// Stages the pair table of tile height K (2 x 256 uint16, 1 KB) into smem
+// and waits for the workgroup.
+template <int K>
+__device__ __forceinline__ void stage_pair_table(const uint16_t *plut, uint8_t *smem) {
+    const uint32_t *src = (const uint32_t *)(plut + (K == 32 ? 512 : 0));
+    for (int x = threadIdx.x; x < 256; x += blockDim.x) ((uint32_t *)smem)[x] = src[x];
+    __syncthreads();
+}
+
 } // namespace ish

@@ -74,6 +74,46 @@ __global__ void __launch_bounds__(256) encode_bytes(uint8_t *p, uint64_t n, Enco
     if (tail < n && tail < nv * 16 + 16) p[tail] = t.code[p[tail]];
 }
 
+// Which residue codes occur in p[0, n): bit c of mask[c >> 5] is set when
+// code c does (mask zeroed by the caller). Block of 256 threads.
+__global__ void __launch_bounds__(256) code_mask(const uint8_t *p, uint64_t n, uint32_t *mask) {
+    __shared__ uint8_t seen[256];
+    seen[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t nv = n / 16;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint4 *v = (const uint4 *)p;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+        const uint4 x = v[i];
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) seen[(w[k] >> (8 * b)) & 0xffu] = 1;
+    }
+    for (uint64_t t = nv * 16 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+        seen[p[t]] = 1;
+    __syncthreads();
+    // thread t reports code t: one 64-bit ballot per wave covers 64 codes
+    const uint64_t bits = __builtin_amdgcn_ballot_w64(seen[threadIdx.x] != 0);
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        if ((uint32_t)bits) atomicOr(&mask[2 * w], (uint32_t)bits);
+        if ((uint32_t)(bits >> 32)) atomicOr(&mask[2 * w + 1], (uint32_t)(bits >> 32));
+    }
+}
+
+hipError_t launch_code_mask(const uint8_t *p, uint64_t n, uint32_t *mask, hipStream_t st) {
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, mask, (int64_t)8, 0u);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n == 0) return e;
+    uint64_t b = (n / 16 + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > 2048) b = 2048;
+    hipLaunchKernelGGL(code_mask, dim3((unsigned)b), dim3(256), 0, st, p, n, mask);
+    return hipGetLastError();
+}
+
 hipError_t launch_encode_bytes(uint8_t *p, uint64_t n, const EncodeTable &t, hipStream_t st) {
     if (n == 0) return hipSuccess;
     uint64_t b = (n / 16 + 255) / 256;

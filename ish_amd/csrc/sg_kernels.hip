@@ -148,7 +148,9 @@ __device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
 //   out: where the chunk-bottom rows go (t*64 + lane), null for the last
 //   chunk, which tracks row Q-1 (register mstar) instead.
 // TE / QE say which end rules the variant can apply; qe_rt is the query's.
-template <typename V, int K, bool TE, bool QE>
+// PAIRED (packed lanes): chunk_base is the wave's paired chunk profile and
+// slut the pair table (see PairView).
+template <typename V, int K, bool TE, bool QE, bool PAIRED = false>
 __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool first, bool last,
                                          int mstar, uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
                                          short8 (&sa)[K / 8], short8 (&sb)[K / 8],
@@ -170,8 +172,14 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
         const uint32_t wa = t < 3 ? (half ? rr.y : rr.x) : (half ? rr_next.y : rr_next.x);
         const uint32_t wb = t < 3 ? rr.y : rr_next.y;
         const int sh = 8 * ((t + 1) & 3);
-        const short8 *na = row_ptr((wa >> sh) & 0xffu);
-        const short8 *nb = row_ptr((wb >> sh) & 0xffu);
+        const short8 *na, *nb;
+        if constexpr (PAIRED) {
+            na = pair_row(chunk_base, slut, (wa >> sh) & 0xffu, (wb >> sh) & 0xffu);
+            nb = na;
+        } else {
+            na = row_ptr((wa >> sh) & 0xffu);
+            nb = row_ptr((wb >> sh) & 0xffu);
+        }
         V htop, f;
         if (first) {
             const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
@@ -188,7 +196,12 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             for (int mm = 0; mm < 8; ++mm) {
                 const int m = k * 8 + mm;
                 V s;
-                if constexpr (PACKED) {
+                if constexpr (PAIRED) {
+                    // rows 8k..8k+3 in sa[k], 8k+4..8k+7 in sb[k]
+                    const short8 &src = mm < 4 ? sa[k] : sb[k];
+                    s.x = src[2 * (mm & 3)];
+                    s.y = src[2 * (mm & 3) + 1];
+                } else if constexpr (PACKED) {
                     s.x = sa[k][mm];
                     s.y = sb[k][mm];
                 } else {
@@ -206,8 +219,13 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             // the rows that still read the old slice, so only one extra
             // slice is live at a time.
             __builtin_amdgcn_sched_barrier(0);
-            sa[k] = na[k];
-            if (PACKED) sb[k] = nb[k];
+            if constexpr (PAIRED) {
+                sa[k] = na[2 * k];
+                sb[k] = na[2 * k + 1];
+            } else {
+                sa[k] = na[k];
+                if (PACKED) sb[k] = nb[k];
+            }
         }
         st.diag_top = htop;
         if (!last) {
@@ -323,10 +341,11 @@ __device__ __forceinline__ void sg_finish(const ScoreParams &p, const QueryView 
 // TE / QE say which end rules the variant can apply; the query's own flags
 // select among them at run time (a variant with TE or QE must only see
 // queries that have te or qe free, since it tracks no corner cell).
-template <typename V, int K, bool TE, bool QE>
+// PAIRED: the profile is the paired one of pv, slut the pair table.
+template <typename V, int K, bool TE, bool QE, bool PAIRED = false>
 __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &qv,
                                          const uint8_t *smem, const uint16_t *slut, uint2 *bnd,
-                                         int g, int half, int lane) {
+                                         int g, int half, int lane, const PairView *pv = nullptr) {
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
     constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
     const int go = p.go, ge = p.ge, Q = qv.Q;
@@ -349,7 +368,8 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
         const bool last = c == qv.nchunks - 1;
         const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
         // LDS address of this chunk's rows in profile row `code`
-        const uint8_t *chunk_base = smem + 2 * j0;
+        const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * j0;
+        if constexpr (PAIRED) stage_pairs<K>(*pv, j0, lane);
         SgChunk<V, K> st;
         sg_chunk_init(st, j0, qs, go, ge);
         uint2 rr_next = res[lane];
@@ -362,7 +382,14 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
         }
         // profile slices of column 0
         short8 sa[NS], sb[NS];
-        {
+        if constexpr (PAIRED) {
+            const short8 *np = pair_row(chunk_base, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                sa[k] = np[2 * k];
+                sb[k] = np[2 * k + 1];
+            }
+        } else {
             const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[(half ? rr_next.y : rr_next.x) & 0xffu]);
             const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
 #pragma unroll
@@ -383,9 +410,9 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
                     for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
                 }
             }
-            sg_block<V, K, TE, QE>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
-                                   chunk_base, slut, half, RA, RB, ts, qe_rt, go, ge, t16, t32,
-                                   last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
+            sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
+                                           chunk_base, slut, half, RA, RB, ts, qe_rt, go, ge, t16,
+                                           t32, last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
         }
     }
     sg_finish<V>(p, qv, tA, tB, RA, RB, te_rt, qe_rt, t16, t32);
@@ -409,13 +436,19 @@ constexpr int kSgPipeWaves = 4;
 constexpr size_t kSgRingBytes = (size_t)(kSgPipeWaves - 1) * 2 * 4 * 64 * sizeof(uint2);
 constexpr size_t kSgCombBytes = (size_t)kSgPipeWaves * 64 * 6 * sizeof(int);
 
-size_t sg_pipe_extra_lds() { return kSgRingBytes + kSgCombBytes + 16; }
+constexpr size_t kSgExtraBytes = kSgRingBytes + kSgCombBytes + 16;
+size_t sg_pipe_extra_lds() { return kSgExtraBytes; }
+// LDS of a paired hybrid launch: the pair table (1 KB), the pipeline's
+// buffers, and one paired chunk profile per wave.
+size_t sg_hybrid_paired_lds(int chunk, int nrows) {
+    return 1024 + kSgExtraBytes + (size_t)kSgPipeWaves * nrows * nrows * pair_row_bytes(chunk);
+}
 
-template <int K, bool TE, bool QE>
+template <int K, bool TE, bool QE, bool PAIRED = false>
 __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryView &qv,
                                               const uint8_t *smem, const uint16_t *slut,
                                               uint2 *ring, int *comb, uint2 *gbnd, int g, int w,
-                                              int lane) {
+                                              int lane, const PairView *pv = nullptr) {
     using V = s16x2;
     constexpr int NS = K / 8;
     const int go = p.go, ge = p.ge, Q = qv.Q;
@@ -456,7 +489,7 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
             const bool last = c == nch - 1;
             const int mstar = Q - 1 - j0;
             const int par = (r * P + b) & 1; // ring parity of this (round, block)
-            const uint8_t *chunk_base = smem + 2 * j0;
+            const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * j0;
             if (b == 0) {
                 sg_chunk_init(st, j0, qs, go, ge);
                 rr_next = res[lane];
@@ -464,12 +497,22 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
 #pragma unroll
                     for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
                 }
-                const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.x & 0xffu]);
-                const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
+                if constexpr (PAIRED) {
+                    stage_pairs<K>(*pv, j0, lane);
+                    const short8 *np = pair_row(chunk_base, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
 #pragma unroll
-                for (int k = 0; k < NS; ++k) {
-                    sa[k] = na[k];
-                    sb[k] = nb[k];
+                    for (int k = 0; k < NS; ++k) {
+                        sa[k] = np[2 * k];
+                        sb[k] = np[2 * k + 1];
+                    }
+                } else {
+                    const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.x & 0xffu]);
+                    const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+                        sa[k] = na[k];
+                        sb[k] = nb[k];
+                    }
                 }
             }
             const uint2 rr = rr_next;
@@ -496,9 +539,9 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
             if (!last)
                 out = w < kSgPipeWaves - 1 ? ring + (size_t)(w * 2 + par) * 256
                                            : gbnd + (size_t)b * 4 * 64;
-            sg_block<V, K, TE, QE>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
-                                   chunk_base, slut, 0, RA, RB, ts, qe_rt, go, ge, t16, t32, out,
-                                   lane);
+            sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
+                                           chunk_base, slut, 0, RA, RB, ts, qe_rt, go, ge, t16,
+                                           t32, out, lane);
         }
         __syncthreads();
         if (++b == P) {
@@ -568,19 +611,31 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
 // quarter of the time one wave needs, so they no longer set the length of
 // the launch.
 // ---------------------------------------------------------------------------
-template <int K, bool TE, bool QE>
+template <int K, bool TE, bool QE, bool PAIRED>
 __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
     using V = s16x2;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_item;
-    stage_profile(p.prof, p.lut, p.prof_words, smem);
-    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
-    uint8_t *tail = smem + (((size_t)p.prof_words * 4 + 512 + 15) & ~(size_t)15);
+    size_t head;
+    if constexpr (PAIRED) {
+        stage_pair_table<K>(p.plut, smem);
+        head = 1024;
+    } else {
+        stage_profile(p.prof, p.lut, p.prof_words, smem);
+        head = (size_t)p.prof_words * 4 + 512;
+    }
+    const uint16_t *slut = (const uint16_t *)(PAIRED ? smem : smem + (size_t)p.prof_words * 4);
+    uint8_t *tail = smem + ((head + 15) & ~(size_t)15);
     uint2 *ring = (uint2 *)tail;
     int *comb = (int *)(tail + kSgRingBytes);
     const int lane = threadIdx.x & 63;
     const int w = (int)wave_broadcast0(threadIdx.x >> 6); // wave-uniform
     const int waves = blockDim.x >> 6;
+    PairView pv; // this wave's paired chunk profile (PAIRED only)
+    pv.pprof = p.pprof;
+    pv.npairs = PAIRED ? p.pnrows * p.pnrows : 0;
+    pv.pqpad = p.pqpad;
+    pv.pbuf = tail + kSgExtraBytes + (size_t)w * pv.npairs * pair_row_bytes(K);
     uint2 *bnd = p.bnd + (size_t)(blockIdx.x * waves + w) * p.slot_stride;
     uint2 *gbnd = p.bnd + (size_t)(blockIdx.x * waves) * p.slot_stride;
     QueryView qv;
@@ -599,10 +654,11 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
         __syncthreads(); // s_item is rewritten by the next fetch
         if (item >= items) break;
         if (item < nlong) {
-            sg_pipe_group<K, TE, QE>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item, w, lane);
+            sg_pipe_group<K, TE, QE, PAIRED>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item,
+                                             w, lane, &pv);
         } else {
             const int g = p.g_begin + nlong + (item - nlong) * waves + w;
-            if (g < p.g_end) sg_group<V, K, TE, QE>(p, qv, smem, slut, bnd, g, 0, lane);
+            if (g < p.g_end) sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, 0, lane, &pv);
         }
     }
 }
@@ -613,7 +669,7 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
 // its next tile belongs to another query. Wave item = group (PACKED) or
 // half group (int32).
 // ---------------------------------------------------------------------------
-template <typename V, int K, bool TE, bool QE>
+template <typename V, int K, bool TE, bool QE, bool PAIRED>
 __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_tile[3]; // query, first wave item, items of that query
@@ -652,8 +708,14 @@ __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
         if (q < 0) break;
         const QueryDesc qd = p.queries[q];
         if (q != cur_q) {
-            stage_profile(qd.prof, qd.lut, qd.prof_words, smem); // ends in a barrier
-            slut = (const uint16_t *)(smem + (size_t)qd.prof_words * 4);
+            // both end in a barrier
+            if constexpr (PAIRED) {
+                stage_pair_table<K>(qd.plut, smem);
+                slut = (const uint16_t *)smem;
+            } else {
+                stage_profile(qd.prof, qd.lut, qd.prof_words, smem);
+                slut = (const uint16_t *)(smem + (size_t)qd.prof_words * 4);
+            }
             cur_q = q;
         }
         const int item = s_tile[1] + wave;
@@ -666,7 +728,14 @@ __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
             qv.out_off = qd.out_off;
             const int g = qd.g_begin + (PACKED ? item : (item >> 1));
             const int half = PACKED ? 0 : (item & 1);
-            sg_group<V, K, TE, QE>(p, qv, smem, slut, bnd, g, half, lane);
+            // this wave's paired chunk profile (PAIRED only); p.pnrows is
+            // the largest class count of the launch's queries
+            PairView pv;
+            pv.pprof = qd.pprof;
+            pv.npairs = qd.pnrows * qd.pnrows;
+            pv.pqpad = qd.pqpad;
+            pv.pbuf = smem + 1024 + (size_t)wave * p.pnrows * p.pnrows * pair_row_bytes(K);
+            sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, half, lane, &pv);
         }
         __syncthreads(); // everyone is done with s_tile and the profile
     }
@@ -852,45 +921,49 @@ static hipError_t launch_batched(const ScoreParams &p, int grid, int block, size
     return hipGetLastError();
 }
 
-template <typename V, int K>
+template <typename V, int K, bool PAIRED>
 static hipError_t launch_multi(const ScoreParams &p, int grid, int block, size_t lds, hipStream_t st) {
     const bool te = p.flags & 8u, qe = p.flags & 2u;
     if (te && qe)
-        hipLaunchKernelGGL((sg_multi<V, K, true, true>), dim3(grid), dim3(block), lds, st, p);
+        hipLaunchKernelGGL((sg_multi<V, K, true, true, PAIRED>), dim3(grid), dim3(block), lds, st, p);
     else if (te)
-        hipLaunchKernelGGL((sg_multi<V, K, true, false>), dim3(grid), dim3(block), lds, st, p);
+        hipLaunchKernelGGL((sg_multi<V, K, true, false, PAIRED>), dim3(grid), dim3(block), lds, st, p);
     else if (qe)
-        hipLaunchKernelGGL((sg_multi<V, K, false, true>), dim3(grid), dim3(block), lds, st, p);
+        hipLaunchKernelGGL((sg_multi<V, K, false, true, PAIRED>), dim3(grid), dim3(block), lds, st, p);
     else
-        hipLaunchKernelGGL((sg_multi<V, K, false, false>), dim3(grid), dim3(block), lds, st, p);
+        hipLaunchKernelGGL((sg_multi<V, K, false, false, PAIRED>), dim3(grid), dim3(block), lds, st, p);
     return hipGetLastError();
 }
 
-hipError_t launch_sg_multi(const ScoreParams &p, bool packed, int chunk, int grid, int block,
-                           size_t lds, hipStream_t st) {
-    if (packed) return chunk == 16 ? launch_multi<s16x2, 16>(p, grid, block, lds, st)
-                                   : launch_multi<s16x2, 32>(p, grid, block, lds, st);
-    return chunk == 16 ? launch_multi<int, 16>(p, grid, block, lds, st)
-                       : launch_multi<int, 32>(p, grid, block, lds, st);
+hipError_t launch_sg_multi(const ScoreParams &p, bool packed, bool paired, int chunk, int grid,
+                           int block, size_t lds, hipStream_t st) {
+    if (packed && paired) return chunk == 16 ? launch_multi<s16x2, 16, true>(p, grid, block, lds, st)
+                                             : launch_multi<s16x2, 32, true>(p, grid, block, lds, st);
+    if (packed) return chunk == 16 ? launch_multi<s16x2, 16, false>(p, grid, block, lds, st)
+                                   : launch_multi<s16x2, 32, false>(p, grid, block, lds, st);
+    return chunk == 16 ? launch_multi<int, 16, false>(p, grid, block, lds, st)
+                       : launch_multi<int, 32, false>(p, grid, block, lds, st);
 }
 
-template <int K>
+template <int K, bool PAIRED>
 static hipError_t launch_hybrid(const ScoreParams &p, int grid, size_t lds, hipStream_t st) {
     const bool te = p.flags & 8u, qe = p.flags & 2u;
     if (te && qe)
-        hipLaunchKernelGGL((sg_hybrid<K, true, true>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, true, true, PAIRED>), dim3(grid), dim3(256), lds, st, p);
     else if (te)
-        hipLaunchKernelGGL((sg_hybrid<K, true, false>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, true, false, PAIRED>), dim3(grid), dim3(256), lds, st, p);
     else if (qe)
-        hipLaunchKernelGGL((sg_hybrid<K, false, true>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, false, true, PAIRED>), dim3(grid), dim3(256), lds, st, p);
     else
-        hipLaunchKernelGGL((sg_hybrid<K, false, false>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, false, false, PAIRED>), dim3(grid), dim3(256), lds, st, p);
     return hipGetLastError();
 }
 
-hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, int grid, size_t lds,
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
                                 hipStream_t st) {
-    return chunk == 16 ? launch_hybrid<16>(p, grid, lds, st) : launch_hybrid<32>(p, grid, lds, st);
+    if (paired)
+        return chunk == 16 ? launch_hybrid<16, true>(p, grid, lds, st) : launch_hybrid<32, true>(p, grid, lds, st);
+    return chunk == 16 ? launch_hybrid<16, false>(p, grid, lds, st) : launch_hybrid<32, false>(p, grid, lds, st);
 }
 
 hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid, int block,
@@ -951,29 +1024,33 @@ template <typename V, int K> static int occ_batched(int block, size_t lds, uint3
     if (qe) return occ(sg_batched<V, K, false, true>, block, lds);
     return occ(sg_batched<V, K, false, false>, block, lds);
 }
-template <typename V, int K> static int occ_multi(int block, size_t lds, uint32_t flags) {
+template <typename V, int K, bool PAIRED> static int occ_multi(int block, size_t lds, uint32_t flags) {
     const bool te = flags & 8u, qe = flags & 2u;
-    if (te && qe) return occ(sg_multi<V, K, true, true>, block, lds);
-    if (te) return occ(sg_multi<V, K, true, false>, block, lds);
-    if (qe) return occ(sg_multi<V, K, false, true>, block, lds);
-    return occ(sg_multi<V, K, false, false>, block, lds);
+    if (te && qe) return occ(sg_multi<V, K, true, true, PAIRED>, block, lds);
+    if (te) return occ(sg_multi<V, K, true, false, PAIRED>, block, lds);
+    if (qe) return occ(sg_multi<V, K, false, true, PAIRED>, block, lds);
+    return occ(sg_multi<V, K, false, false, PAIRED>, block, lds);
 }
-int occupancy_sg_multi(bool packed, int chunk, int block, size_t lds, uint32_t flags) {
-    if (packed) return chunk == 16 ? occ_multi<s16x2, 16>(block, lds, flags)
-                                   : occ_multi<s16x2, 32>(block, lds, flags);
-    return chunk == 16 ? occ_multi<int, 16>(block, lds, flags) : occ_multi<int, 32>(block, lds, flags);
+int occupancy_sg_multi(bool packed, bool paired, int chunk, int block, size_t lds, uint32_t flags) {
+    if (packed && paired) return chunk == 16 ? occ_multi<s16x2, 16, true>(block, lds, flags)
+                                             : occ_multi<s16x2, 32, true>(block, lds, flags);
+    if (packed) return chunk == 16 ? occ_multi<s16x2, 16, false>(block, lds, flags)
+                                   : occ_multi<s16x2, 32, false>(block, lds, flags);
+    return chunk == 16 ? occ_multi<int, 16, false>(block, lds, flags)
+                       : occ_multi<int, 32, false>(block, lds, flags);
 }
 
-template <int K> static int occ_hybrid(size_t lds, uint32_t flags) {
+template <int K, bool PAIRED> static int occ_hybrid(size_t lds, uint32_t flags) {
     const bool te = flags & 8u, qe = flags & 2u;
-    if (te && qe) return occ(sg_hybrid<K, true, true>, 256, lds);
-    if (te) return occ(sg_hybrid<K, true, false>, 256, lds);
-    if (qe) return occ(sg_hybrid<K, false, true>, 256, lds);
-    return occ(sg_hybrid<K, false, false>, 256, lds);
+    if (te && qe) return occ(sg_hybrid<K, true, true, PAIRED>, 256, lds);
+    if (te) return occ(sg_hybrid<K, true, false, PAIRED>, 256, lds);
+    if (qe) return occ(sg_hybrid<K, false, true, PAIRED>, 256, lds);
+    return occ(sg_hybrid<K, false, false, PAIRED>, 256, lds);
 }
 
-int occupancy_sg_hybrid_i16(int chunk, size_t lds, uint32_t flags) {
-    return chunk == 16 ? occ_hybrid<16>(lds, flags) : occ_hybrid<32>(lds, flags);
+int occupancy_sg_hybrid_i16(int chunk, bool paired, size_t lds, uint32_t flags) {
+    if (paired) return chunk == 16 ? occ_hybrid<16, true>(lds, flags) : occ_hybrid<32, true>(lds, flags);
+    return chunk == 16 ? occ_hybrid<16, false>(lds, flags) : occ_hybrid<32, false>(lds, flags);
 }
 
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags) {

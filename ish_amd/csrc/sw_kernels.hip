@@ -129,8 +129,7 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
         const int sh = 8 * ((t + 1) & 3);
         const short8 *na, *nb;
         if constexpr (PAIRED) {
-            na = (const short8 *)(chunk_base + (uint32_t)slut[(wa >> sh) & 0xffu] +
-                                  (uint32_t)slut[256 + ((wb >> sh) & 0xffu)]);
+            na = pair_row(chunk_base, slut, (wa >> sh) & 0xffu, (wb >> sh) & 0xffu);
             nb = na;
         } else {
             na = row_ptr((wa >> sh) & 0xffu);
@@ -385,14 +384,8 @@ constexpr size_t kPipePubBytes = (size_t)kPipeWaves * 64 * sizeof(uint32_t);
 constexpr size_t kPipeExtraBytes = kPipeRingBytes + kPipeCombBytes + kPipePubBytes + 16;
 size_t sw_pipe_extra_lds() { return kPipeExtraBytes; }
 
-// Paired variant (small alphabets): bytes per pair row of a wave's paired
-// chunk profile (K int32 words plus 16 bytes, so that rows start 16 bytes
-// apart modulo 128 and the 16-byte slices of different rows spread over
-// the LDS banks), and the LDS a launch needs: the pair table (2 x 256
-// uint16), the pipeline's own buffers, and one paired chunk profile per
-// wave.
-constexpr int pair_row_bytes(int K) { return 4 * K + 16; }
-size_t sw_pipe_paired_row_bytes(int chunk) { return (size_t)pair_row_bytes(chunk); }
+// Paired variant: the LDS a launch needs is the pair table (1 KB), the
+// pipeline's own buffers, and one paired chunk profile per wave.
 size_t sw_pipe_paired_lds(int chunk, int nrows) {
     return 1024 + sw_pipe_extra_lds() +
            (size_t)kPipeWaves * nrows * nrows * pair_row_bytes(chunk);
@@ -405,11 +398,7 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     __shared__ int s_item;
     size_t head;
     if constexpr (PAIRED) {
-        // the pair table of this tile height: byte offsets of a class's
-        // rows, for target A at [0, 256) and target B at [256, 512)
-        const uint32_t *src = (const uint32_t *)(p.plut + (K == 32 ? 512 : 0));
-        for (int x = threadIdx.x; x < 256; x += blockDim.x) ((uint32_t *)smem)[x] = src[x];
-        __syncthreads();
+        stage_pair_table<K>(p.plut, smem);
         head = 1024;
     } else {
         stage_profile(p.prof, p.lut, p.prof_words, smem);
@@ -432,8 +421,12 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     const int nch = p.nchunks;
     const int rounds = (nch + kPipeWaves - 1) / kPipeWaves;
     // this wave's paired chunk profile (PAIRED only)
-    const int npairs = PAIRED ? p.pnrows * p.pnrows : 0;
-    uint8_t *pbuf = tail + kPipeExtraBytes + (size_t)w * npairs * pair_row_bytes(K);
+    PairView pv;
+    pv.pprof = p.pprof;
+    pv.npairs = PAIRED ? p.pnrows * p.pnrows : 0;
+    pv.pqpad = p.pqpad;
+    pv.pbuf = tail + kPipeExtraBytes + (size_t)w * pv.npairs * pair_row_bytes(K);
+    uint8_t *pbuf = pv.pbuf;
 
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
@@ -482,19 +475,8 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                         for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
                     }
                     if constexpr (PAIRED) {
-                        // copy rows j0 .. j0+K-1 of every pair row; the
-                        // previous chunk's reads of the buffer were issued
-                        // earlier, and a wave's LDS accesses complete in
-                        // order
-                        constexpr int parts = K / 4; // 16-byte parts per row
-                        for (int x = lane; x < npairs * parts; x += 64) {
-                            const int pr = x / parts, part = x % parts;
-                            const uint4 v = *(const uint4 *)(p.pprof + (size_t)pr * p.pqpad + j0 + part * 4);
-                            *(uint4 *)(pbuf + pr * pair_row_bytes(K) + part * 16) = v;
-                        }
-                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        const short8 *np = (const short8 *)(pbuf + (uint32_t)slut[rr_next.x & 0xffu] +
-                                                            (uint32_t)slut[256 + (rr_next.y & 0xffu)]);
+                        stage_pairs<K>(pv, j0, lane);
+                        const short8 *np = pair_row(pbuf, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
 #pragma unroll
                         for (int k = 0; k < NS; ++k) {
                             sa[k] = np[2 * k];

@@ -17,6 +17,10 @@
 //    (each position substituted with probability 0.30 by b != a drawn with
 //    weight p_b * 2^(S62(a,b)/2)) at an offset U[0, R-Q] (first R residues
 //    of the homolog when R < Q).
+//  3 multi-query targets: the records of 1 (including its planted copies of
+//    the 150 nt query of 1), with the letters A, C, G, T, N encoded as the
+//    BLOSUM62 residues of those letters (A0 C4 G7 T16 N2); the queries are
+//    protein (background frequencies).
 //  4 long reads: query 1000 nt iid ACGT; ln R ~ N(ln 8000, 0.5^2) clipped to
 //    [1000, 50000]; per base 3 % substitution, 3 % insertion, 2 % deletion,
 //    2 % N; 2 % of reads carry the query passed through the same error
@@ -174,6 +178,7 @@ struct Job {
     const uint64_t *offsets;
     uint8_t *res;
     uint8_t *planted;
+    const uint8_t *recode; // code translation applied to every target, or null
     int64_t next;
     pthread_mutex_t mu;
 };
@@ -229,7 +234,13 @@ void *worker(void *arg) {
         pthread_mutex_unlock(&jb->mu);
         if (lo >= jb->n) break;
         const int64_t hi = std::min(jb->n, lo + 256);
-        for (int64_t t = lo; t < hi; ++t) gen_target(*jb, t);
+        for (int64_t t = lo; t < hi; ++t) {
+            gen_target(*jb, t);
+            if (jb->recode) {
+                uint8_t *d = jb->res + jb->offsets[t];
+                for (int64_t k = 0; k < jb->lens[t]; ++k) d[k] = jb->recode[d[k]];
+            }
+        }
     }
     return nullptr;
 }
@@ -274,6 +285,17 @@ int32_t ish_wl_targets(int32_t kind, uint64_t seed, const uint8_t *query, int32_
     jb.n = n;
     jb.query = query;
     jb.qlen = qlen;
+    // kind 3: the records of kind 1 with its own query planted, letters
+    // recoded to BLOSUM62 (ACTGN A0 C1 G2 T3 N4 -> A0 C4 G7 T16 N2)
+    uint8_t nt_query[150];
+    static const uint8_t kNtToBlosum[256] = {0, 4, 7, 16, 2};
+    jb.recode = nullptr;
+    if (kind == 3) {
+        ish_wl_query(1, seed, 150, nt_query);
+        jb.query = nt_query;
+        jb.qlen = 150;
+        jb.recode = kNtToBlosum;
+    }
     jb.lens = lens;
     jb.offsets = offsets;
     jb.res = residues;

@@ -90,8 +90,9 @@ def query(kind: int, seed: int, qlen: int | None = None) -> np.ndarray:
 
 
 def generate(kind: int, n: int, seed: int = 1, qlen: int | None = None, threads: int = 0) -> Workload:
-    """kind 1: nucleotide lines, 2: protein FASTA, 3: nucleotide targets with
-    protein queries, 4: long reads, 5: serving database (see workload.cpp)."""
+    """kind 1: nucleotide lines, 2: protein FASTA, 3: the records of 1 as
+    BLOSUM62 letters (A C G T N) with protein queries, 4: long reads, 5:
+    serving database (see workload.cpp)."""
     L = _lib()
     q = query(kind, seed, qlen)
     lens = np.zeros(max(n, 1), np.int64)
