@@ -29,6 +29,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS on 1 MI355X (bit-exact scores), batched vs plain; p50/p99 query latency"
 PEAK_INT16_TOPS = 78.6  # 256 CU x 2.4 GHz x 128 packed int16 lane-ops/clk
+# ops_per_cell is the textbook recurrence's count (8 semi-global, 9 local):
+# the kernels issue fewer per cell (offset forms, and no pairing
+# instruction with a paired profile), so `frac` measures algorithmic
+# throughput against the packed-int16 issue peak and can exceed 1
+ROOFLINE_NOTE = ("achieved = cells x textbook ops per cell / kernel time; the kernels issue fewer "
+                 "VALU ops per cell than the textbook recurrence, so frac can exceed 1")
 PEAK_INT32_TOPS = 39.3  # 256 CU x 2.4 GHz x 64 int32 lane-ops/clk
 OPS_PER_CELL_SG = 8     # semi-global Gotoh: 1 add + 2 max (H), 2 sub + 1 max (E), 1 sub + 1 max (F)
 OPS_PER_CELL_SW = 9     # local: the same plus the max with 0
@@ -208,7 +214,8 @@ def run_serving(args, ish_amd, workloads):
         "p50_ms": sus["p50_ms"], "p99_ms": sus["p99_ms"],
         # batches on two streams overlap on the device, so per-launch event
         # times double-count; the sustained wall-clock rate is the basis
-        "roofline": {"bound": "valu", "achieved": round(sat["gcups_sustained"] * OPS_PER_CELL_SG / 1e3, 3),
+        "roofline": {"bound": "valu", "note": ROOFLINE_NOTE,
+                     "achieved": round(sat["gcups_sustained"] * OPS_PER_CELL_SG / 1e3, 3),
                      "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
                      "frac": round(sat["gcups_sustained"] * OPS_PER_CELL_SG / 1e3 / PEAK_INT16_TOPS, 4),
                      "traffic": None, "ops_per_cell": OPS_PER_CELL_SG, "basis": "sustained wall clock"},
@@ -342,7 +349,8 @@ def run_multiquery(args, ish_amd, workloads):
                    "content_hash": w.content_hash(), "tile_rows": args.tile or 32},
         "gcups_kernel": round(cells / kern_s / 1e9, 2), "db_load_ms": round(load_s * 1e3, 1),
         "parity_sample": parity,
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
+        "roofline": {"bound": "valu", "note": ROOFLINE_NOTE, "achieved": round(achieved, 3),
+                     "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
                      "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": None,
                      "ops_per_cell": OPS_PER_CELL_SG, "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
@@ -544,7 +552,8 @@ def main():
         "p50_ms": serving["at_70pct_capacity"]["p50_ms"] if serving else None,
         "p99_ms": serving["at_70pct_capacity"]["p99_ms"] if serving else None,
         "parity_sample": parity,
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
+        "roofline": {"bound": "valu", "note": ROOFLINE_NOTE, "achieved": round(achieved, 3),
+                     "peak": PEAK_INT16_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
                      "traffic": measured_traffic(w.content_hash(), best_tile, schedule),
                      "ops_per_cell": ops_per_cell, "kernel_ms": round(kern_s * 1e3, 3)},
