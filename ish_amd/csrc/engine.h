@@ -135,6 +135,15 @@ size_t sg_hybrid_paired_lds(int chunk, int nrows);
 hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
                                 hipStream_t st);
 int occupancy_sg_hybrid_i16(int chunk, bool paired, size_t lds, uint32_t flags);
+// Hybrid launch over a workgroup-shared paired profile (up to
+// kPairMaxShared classes; tiles of four groups in lockstep, long groups over
+// the int16 profile). Its LDS is sg_hybrid_lock_lds(chunk, classes, the int16
+// profile's LDS). The pair table holds uint16 byte offsets: with 21 classes
+// and K = 32 the last pair row starts at (21 * 21 - 1) * 144 = 63,360 bytes.
+constexpr int kPairMaxShared = 21;
+size_t sg_hybrid_lock_lds(int chunk, int nrows, size_t profile_lds);
+hipError_t launch_sg_hybrid_lock(const ScoreParams &p, int chunk, int grid, size_t lds, hipStream_t st);
+int occupancy_sg_hybrid_lock(int chunk, size_t lds, uint32_t flags);
 // Workgroup-pipelined local kernel (packed int16, 4 waves share a group):
 // extra LDS beyond the profile, launcher (block of 256) and occupancy.
 // `paired` selects the variant that reads the paired profile (p.pprof,

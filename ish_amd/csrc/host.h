@@ -69,8 +69,9 @@ struct PairProfile {
 };
 // Builds it for the codes set in present[8] (code 0, the padding code past
 // targets' ends, always counts). False when those codes fall into more
-// than kPairMaxRows classes.
-bool build_pairs(const Profile &pf, const uint32_t present[8], PairProfile &out);
+// than max_classes classes.
+bool build_pairs(const Profile &pf, const uint32_t present[8], int32_t max_classes,
+                 PairProfile &out);
 // Alphabet size of a built-in matrix kind (-1 when unknown).
 int32_t alphabet_of(int32_t kind);
 // The A*A matrix to use: the caller's, or the built-in one of `kind` when

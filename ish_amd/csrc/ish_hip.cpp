@@ -343,7 +343,8 @@ void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &
         }
 }
 
-bool ish::build_pairs(const Profile &pf, const uint32_t present[8], PairProfile &out) {
+bool ish::build_pairs(const Profile &pf, const uint32_t present[8], int32_t max_classes,
+                      PairProfile &out) {
     const int32_t rw = pf.stride / 2;
     std::vector<int32_t> remap((size_t)pf.nrows, -1), cls_of;
     for (int c = 0; c < 256; ++c) {
@@ -352,10 +353,12 @@ bool ish::build_pairs(const Profile &pf, const uint32_t present[8], PairProfile 
         if (remap[(size_t)cls] < 0) {
             remap[(size_t)cls] = (int32_t)cls_of.size();
             cls_of.push_back(cls);
-            if ((int32_t)cls_of.size() > kPairMaxRows) return false;
+            if ((int32_t)cls_of.size() > max_classes) return false;
         }
     }
     const int32_t C = (int32_t)cls_of.size();
+    // the table's uint16 offsets must reach the last pair row (tile 32)
+    if ((int64_t)(C * C - 1) * pair_row_bytes(32) > 65535) return false;
     out.classes = C;
     out.qpad = pf.qpad;
     out.words.assign((size_t)C * C * pf.qpad, 0);
@@ -743,17 +746,17 @@ extern "C" int32_t ish_hip_db_size(const ish_hip_db *db, int64_t *n, int64_t *nr
 // profile (forward or reversed query) and free-end flags.
 // One persistent launch of a db pass.
 struct DbLaunch {
-    int32_t kind; // kSg16, kSg32, kSw16, kSw32, kSwPipe, kSgHybrid
+    int32_t kind; // kSg16, kSg32, kSw16, kSw32, kSwPipe, kSgHybrid, kSgLock
     int32_t chunk;
     int32_t g_begin, g_end;
     int32_t grid;
-    int32_t n_long; // kSgHybrid: leading groups scored by whole workgroups
-    int32_t paired; // kSwPipe / kSgHybrid: the paired-profile variant
+    int32_t n_long; // kSgHybrid / kSgLock: leading groups scored by whole workgroups
+    int32_t paired; // kSwPipe / kSgHybrid: the paired-profile variant (kSgLock: always)
     int64_t slot_stride;
     uint64_t lds;
     ScoreParams p;
 };
-enum { kSg16 = 0, kSg32 = 1, kSw16 = 2, kSw32 = 3, kSwPipe = 4, kSgHybrid = 5 };
+enum { kSg16 = 0, kSg32 = 1, kSw16 = 2, kSw32 = 3, kSwPipe = 4, kSgHybrid = 5, kSgLock = 6 };
 
 static int32_t len_bucket(int32_t maxlen) {
     for (int32_t b = 0; b < kLenBuckets - 1; ++b)
@@ -800,7 +803,9 @@ static int32_t long_groups(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t
 
 // Paired profile of the current query for db's residue codes, built and
 // uploaded when the query or the code set changed. Returns its class count
-// (0: those codes need more than kPairMaxRows classes) or an ISH_E_* code.
+// (0: those codes need more than kPairMaxShared classes) or an ISH_E_*
+// code. Per-wave paired launches take at most kPairMaxRows classes, the
+// workgroup-shared one (kSgLock) up to kPairMaxShared.
 static int32_t ensure_pairs(ish_hip_ctx *ctx, const ish_hip_db *db) {
     if (ctx->pair_gen == ctx->query_gen && memcmp(ctx->pair_mask, db->code_mask, sizeof db->code_mask) == 0)
         return ctx->pair_rows;
@@ -809,7 +814,7 @@ static int32_t ensure_pairs(ish_hip_ctx *ctx, const ish_hip_db *db) {
     ctx->pair_gen = ctx->query_gen;
     memcpy(ctx->pair_mask, db->code_mask, sizeof db->code_mask);
     ctx->pair_rows = 0;
-    if (!build_pairs(ctx->hprof, db->code_mask, ctx->pairs)) return 0;
+    if (!build_pairs(ctx->hprof, db->code_mask, kPairMaxShared, ctx->pairs)) return 0;
     const size_t wb = ctx->pairs.words.size() * 4, tb = ctx->pairs.table.size() * 2;
     HIP_TRY(ctx, ctx->pprof.ensure(wb));
     HIP_TRY(ctx, ctx->plut.ensure(tb));
@@ -869,15 +874,18 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         DbLaunch L{};
         bool pipe = false;
         int32_t n_long = 0;
-        const bool sg_paired = !local && pc > 0 && sg_hybrid_paired_lds(chunk, pc) <= kLdsLimit;
+        const bool sg_paired = !local && pc > 0 && pc <= kPairMaxRows &&
+                               sg_hybrid_paired_lds(chunk, pc) <= kLdsLimit;
+        const bool sg_lock = !local && pc > kPairMaxRows &&
+                             sg_hybrid_lock_lds(chunk, pc, ctx->lds) <= kLdsLimit;
         if (local && pipe_lds <= kLdsLimit) {
             if (sched == ISH_SCHEDULE_WORKGROUP) pipe = true;
             else if (sched == ISH_SCHEDULE_AUTO) pipe = pipe_heuristic(ctx, db, g0, g1, chunk);
-        } else if (!local && (hybrid_lds <= kLdsLimit || sg_paired)) {
+        } else if (!local && (hybrid_lds <= kLdsLimit || sg_paired || sg_lock)) {
             if (sched == ISH_SCHEDULE_WORKGROUP) n_long = g1 - g0;
             else if (sched == ISH_SCHEDULE_AUTO) n_long = long_groups(ctx, db, g0, g1, chunk, flags);
         }
-        L.kind = pipe ? kSwPipe : local ? kSw16 : (n_long > 0 || sg_paired) ? kSgHybrid : kSg16;
+        L.kind = pipe ? kSwPipe : local ? kSw16 : sg_lock ? kSgLock : (n_long > 0 || sg_paired) ? kSgHybrid : kSg16;
         L.chunk = chunk;
         L.g_begin = g0;
         L.g_end = g1;
@@ -901,10 +909,15 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     // scratch between launches would free memory a queued launch uses)
     for (DbLaunch &L : out) {
         L.lds = L.kind == kSwPipe ? pipe_lds : L.kind == kSgHybrid ? hybrid_lds : ctx->lds;
-        if (L.kind == kSwPipe && pc > 0 && sw_pipe_paired_lds(L.chunk, pc) <= kLdsLimit) {
+        if (L.kind == kSgLock) {
+            L.paired = 1;
+            L.lds = sg_hybrid_lock_lds(L.chunk, pc, ctx->lds);
+        } else if (L.kind == kSwPipe && pc > 0 && pc <= kPairMaxRows &&
+                   sw_pipe_paired_lds(L.chunk, pc) <= kLdsLimit) {
             L.paired = 1;
             L.lds = sw_pipe_paired_lds(L.chunk, pc);
-        } else if (L.kind == kSgHybrid && pc > 0 && sg_hybrid_paired_lds(L.chunk, pc) <= kLdsLimit) {
+        } else if (L.kind == kSgHybrid && pc > 0 && pc <= kPairMaxRows &&
+                   sg_hybrid_paired_lds(L.chunk, pc) <= kLdsLimit) {
             L.paired = 1;
             L.lds = sg_hybrid_paired_lds(L.chunk, pc);
         }
@@ -915,6 +928,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
         case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
         case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.lds, flags); break;
+        case kSgLock: occ = occupancy_sg_hybrid_lock(L.chunk, L.lds, flags); break;
         default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.lds); break;
         }
         int grid;
@@ -968,6 +982,7 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
     case kSw32: e = launch_sw_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw16: e = launch_sw_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSgHybrid: e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
+    case kSgLock: e = launch_sg_hybrid_lock(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
     default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
     }
     if (e == hipSuccess && timed) e = timed_end(ctx);
@@ -1245,7 +1260,7 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
     bool paired = true;
     int32_t max_classes = 0;
     for (size_t k = 0; k < nq && paired; ++k) {
-        paired = build_pairs(*items[k].pf, db->code_mask, pairs[k]);
+        paired = build_pairs(*items[k].pf, db->code_mask, kPairMaxRows, pairs[k]);
         max_classes = std::max(max_classes, pairs[k].classes);
     }
     const size_t paired_lds = 1024 + (size_t)(kBlock / 64) * max_classes * max_classes * pair_row_bytes(chunk);

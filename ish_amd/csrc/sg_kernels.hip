@@ -335,21 +335,83 @@ __device__ __forceinline__ void sg_finish(const ScoreParams &p, const QueryView 
     }
 }
 
+// One query chunk c (rows j0 .. j0+K-1) of a group: every column of the
+// lane's target(s). chunk_base / slut: the chunk's profile rows and their
+// table (int16 profile: smem + 2*j0 and the residue lut; PAIRED: the
+// chunk's pair rows and the pair table).
+template <typename V, int K, bool TE, bool QE, bool PAIRED>
+__device__ __forceinline__ void sg_chunk(const ScoreParams &p, const QueryView &qv,
+                                         const uint8_t *chunk_base, const uint16_t *slut,
+                                         uint2 *bnd, const uint2 *res, int nblk, int c, int half,
+                                         int lane, int RA, int RB, TrackI16 &t16, TrackI32 &t32) {
+    constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
+    const int go = p.go, ge = p.ge, Q = qv.Q;
+    const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
+    const bool qe_rt = QE && (qv.flags & 2u);
+    const int j0 = c * K;
+    const bool first = c == 0;
+    const bool last = c == qv.nchunks - 1;
+    const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
+    SgChunk<V, K> st;
+    sg_chunk_init(st, j0, qs, go, ge);
+    uint2 rr_next = res[lane];
+    uint2 bn_next[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bn_next[t] = make_uint2(0u, 0u);
+    if (!first) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
+    }
+    // profile slices of column 0
+    short8 sa[NS], sb[NS];
+    if constexpr (PAIRED) {
+        const short8 *np = pair_row(chunk_base, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            sa[k] = np[2 * k];
+            sb[k] = np[2 * k + 1];
+        }
+    } else {
+        const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[(half ? rr_next.y : rr_next.x) & 0xffu]);
+        const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            sa[k] = na[k];
+            sb[k] = PACKED ? nb[k] : na[k];
+        }
+    }
+    for (int b = 0; b < nblk; ++b) {
+        const uint2 rr = rr_next;
+        uint2 bn[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
+        if (b + 1 < nblk) {
+            rr_next = res[(size_t)(b + 1) * 64 + lane];
+            if (!first) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
+            }
+        }
+        sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
+                                       chunk_base, slut, half, RA, RB, ts, qe_rt, go, ge, t16, t32,
+                                       last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
+    }
+}
+
 // Scores one group (PACKED: 128 targets, two per lane) or one half group
 // (int32: 64 targets) against the query whose profile is staged in smem,
 // one wave through every query chunk in turn.
 // TE / QE say which end rules the variant can apply; the query's own flags
 // select among them at run time (a variant with TE or QE must only see
 // queries that have te or qe free, since it tracks no corner cell).
-// PAIRED: the profile is the paired one of pv, slut the pair table.
+// PAIRED: the profile is the paired one of pv (the wave copies each
+// chunk's rows into its own buffer), slut the pair table.
 template <typename V, int K, bool TE, bool QE, bool PAIRED = false>
 __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &qv,
                                          const uint8_t *smem, const uint16_t *slut, uint2 *bnd,
                                          int g, int half, int lane, const PairView *pv = nullptr) {
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
-    constexpr int NS = K / 8; // profile slices of 8 rows (one 16-byte LDS read each)
-    const int go = p.go, ge = p.ge, Q = qv.Q;
-    const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
     const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
     const GroupDesc gd = p.groups[g];
     const int tA = gd.first + half * 64 + lane;
@@ -364,58 +426,59 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
     track_init(t16, t32);
     for (int c = 0; c < qv.nchunks; ++c) {
         const int j0 = c * K;
-        const bool first = c == 0;
-        const bool last = c == qv.nchunks - 1;
-        const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
         // LDS address of this chunk's rows in profile row `code`
         const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * j0;
         if constexpr (PAIRED) stage_pairs<K>(*pv, j0, lane);
-        SgChunk<V, K> st;
-        sg_chunk_init(st, j0, qs, go, ge);
-        uint2 rr_next = res[lane];
-        uint2 bn_next[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) bn_next[t] = make_uint2(0u, 0u);
-        if (!first) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) bn_next[t] = bnd[t * 64 + lane];
-        }
-        // profile slices of column 0
-        short8 sa[NS], sb[NS];
-        if constexpr (PAIRED) {
-            const short8 *np = pair_row(chunk_base, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                sa[k] = np[2 * k];
-                sb[k] = np[2 * k + 1];
-            }
-        } else {
-            const short8 *na = (const short8 *)(chunk_base + 4u * (uint32_t)slut[(half ? rr_next.y : rr_next.x) & 0xffu]);
-            const short8 *nb = (const short8 *)(chunk_base + 4u * (uint32_t)slut[rr_next.y & 0xffu]);
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                sa[k] = na[k];
-                sb[k] = PACKED ? nb[k] : na[k];
-            }
-        }
-        for (int b = 0; b < nblk; ++b) {
-            const uint2 rr = rr_next;
-            uint2 bn[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) bn[t] = bn_next[t];
-            if (b + 1 < nblk) {
-                rr_next = res[(size_t)(b + 1) * 64 + lane];
-                if (!first) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
-                }
-            }
-            sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
-                                           chunk_base, slut, half, RA, RB, ts, qe_rt, go, ge, t16,
-                                           t32, last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
-        }
+        sg_chunk<V, K, TE, QE, PAIRED>(p, qv, chunk_base, slut, bnd, res, nblk, c, half, lane, RA,
+                                       RB, t16, t32);
     }
     sg_finish<V>(p, qv, tA, tB, RA, RB, te_rt, qe_rt, t16, t32);
+}
+
+// A tile of groups g0 + w (one per wave of the workgroup) over a paired
+// profile shared by the workgroup: the waves step through the query chunks
+// together, and before each chunk the workgroup copies that chunk's rows of
+// every pair row into pbuf. Groups in a tile are neighbours in length
+// order, so the waves reach each barrier at nearly the same time. A wave
+// whose group is past the range only joins the barriers and the copies.
+template <int K, bool TE, bool QE>
+__device__ __forceinline__ void sg_tile_lockstep(const ScoreParams &p, const QueryView &qv,
+                                                 const uint16_t *ptab, uint8_t *pbuf, uint2 *bnd,
+                                                 int g, int lane) {
+    using V = s16x2;
+    const bool valid = g < p.g_end; // wave-uniform
+    const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
+    GroupDesc gd;
+    gd.res_off = 0;
+    gd.ncols = 0;
+    gd.first = 0;
+    if (valid) gd = p.groups[g];
+    const int tA = gd.first + lane;
+    const int tB = gd.first + 64 + lane;
+    const int RA = valid ? p.lens[tA] : 0;
+    const int RB = valid ? p.lens[tB] : 0;
+    const uint2 *res = p.res + gd.res_off;
+    const int nblk = gd.ncols >> 2;
+    const int npairs = p.pnrows * p.pnrows;
+    constexpr int parts = K / 4; // 16-byte parts per pair row
+
+    TrackI16 t16;
+    TrackI32 t32;
+    track_init(t16, t32);
+    for (int c = 0; c < qv.nchunks; ++c) {
+        const int j0 = c * K;
+        __syncthreads(); // every wave is done with the previous chunk's rows
+        for (int x = threadIdx.x; x < npairs * parts; x += blockDim.x) {
+            const int pr = x / parts, part = x % parts;
+            const uint4 v = *(const uint4 *)(p.pprof + (size_t)pr * p.pqpad + j0 + part * 4);
+            *(uint4 *)(pbuf + pr * pair_row_bytes(K) + part * 16) = v;
+        }
+        __syncthreads();
+        if (valid)
+            sg_chunk<V, K, TE, QE, true>(p, qv, pbuf, ptab, bnd, res, nblk, c, 0, lane, RA, RB, t16,
+                                         t32);
+    }
+    if (valid) sg_finish<V>(p, qv, tA, tB, RA, RB, te_rt, qe_rt, t16, t32);
 }
 
 // ---------------------------------------------------------------------------
@@ -659,6 +722,70 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
         } else {
             const int g = p.g_begin + nlong + (item - nlong) * waves + w;
             if (g < p.g_end) sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, 0, lane, &pv);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid launch over a workgroup-shared paired profile (databases whose
+// codes fall into more profile classes than per-wave copies of a chunk's
+// pair rows can hold in LDS, e.g. protein): the long groups run as in
+// sg_hybrid over the int16 profile; the other items are tiles of four
+// groups scored in lockstep (sg_tile_lockstep). After the pair table
+// (1 KB), one LDS area holds either the int16 profile with the pipeline's
+// buffers (long groups, which the queue hands out first) or the shared
+// pair rows of one chunk (tiles).
+// ---------------------------------------------------------------------------
+size_t sg_hybrid_lock_lds(int chunk, int nrows, size_t profile_lds) {
+    const size_t pairs = (size_t)nrows * nrows * pair_row_bytes(chunk);
+    const size_t prof = ((profile_lds + 15) & ~(size_t)15) + kSgExtraBytes;
+    return 1024 + (pairs > prof ? pairs : prof);
+}
+
+template <int K, bool TE, bool QE>
+__global__ void __launch_bounds__(256) sg_hybrid_lock(ScoreParams p) {
+    using V = s16x2;
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ int s_item;
+    stage_pair_table<K>(p.plut, smem);
+    const uint16_t *ptab = (const uint16_t *)smem;
+    uint8_t *area = smem + 1024;
+    const uint16_t *slut = (const uint16_t *)(area + (size_t)p.prof_words * 4);
+    uint8_t *tail = area + (((size_t)p.prof_words * 4 + 512 + 15) & ~(size_t)15);
+    uint2 *ring = (uint2 *)tail;
+    int *comb = (int *)(tail + kSgRingBytes);
+    const int lane = threadIdx.x & 63;
+    const int w = (int)wave_broadcast0(threadIdx.x >> 6); // wave-uniform
+    const int waves = blockDim.x >> 6;
+    uint2 *bnd = p.bnd + (size_t)(blockIdx.x * waves + w) * p.slot_stride;
+    uint2 *gbnd = p.bnd + (size_t)(blockIdx.x * waves) * p.slot_stride;
+    QueryView qv;
+    qv.Q = p.Q;
+    qv.nchunks = p.nchunks;
+    qv.neg_limit = p.neg_limit;
+    qv.flags = p.flags;
+    qv.out_off = 0;
+    const int ng = p.g_end - p.g_begin;
+    const int nlong = p.n_long < ng ? p.n_long : ng;
+    const int items = nlong + (ng - nlong + waves - 1) / waves;
+    bool have_profile = false; // the int16 profile is in `area`
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads(); // s_item is rewritten by the next fetch
+        if (item >= items) break;
+        if (item < nlong) {
+            if (!have_profile) {
+                stage_profile(p.prof, p.lut, p.prof_words, area); // ends in a barrier
+                have_profile = true;
+            }
+            sg_pipe_group<K, TE, QE, false>(p, qv, area, slut, ring, comb, gbnd, p.g_begin + item, w,
+                                            lane);
+        } else {
+            have_profile = false; // the tile's pair rows overwrite it
+            const int g = p.g_begin + nlong + (item - nlong) * waves + w;
+            sg_tile_lockstep<K, TE, QE>(p, qv, ptab, area, bnd, g, lane);
         }
     }
 }
@@ -966,6 +1093,24 @@ hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, in
     return chunk == 16 ? launch_hybrid<16, false>(p, grid, lds, st) : launch_hybrid<32, false>(p, grid, lds, st);
 }
 
+template <int K>
+static hipError_t launch_lock(const ScoreParams &p, int grid, size_t lds, hipStream_t st) {
+    const bool te = p.flags & 8u, qe = p.flags & 2u;
+    if (te && qe)
+        hipLaunchKernelGGL((sg_hybrid_lock<K, true, true>), dim3(grid), dim3(256), lds, st, p);
+    else if (te)
+        hipLaunchKernelGGL((sg_hybrid_lock<K, true, false>), dim3(grid), dim3(256), lds, st, p);
+    else if (qe)
+        hipLaunchKernelGGL((sg_hybrid_lock<K, false, true>), dim3(grid), dim3(256), lds, st, p);
+    else
+        hipLaunchKernelGGL((sg_hybrid_lock<K, false, false>), dim3(grid), dim3(256), lds, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sg_hybrid_lock(const ScoreParams &p, int chunk, int grid, size_t lds, hipStream_t st) {
+    return chunk == 16 ? launch_lock<16>(p, grid, lds, st) : launch_lock<32>(p, grid, lds, st);
+}
+
 hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid, int block,
                                  size_t lds, hipStream_t st) {
     if (chunk == 16) return launch_batched<s16x2, 16>(p, grid, block, lds, st);
@@ -1051,6 +1196,18 @@ template <int K, bool PAIRED> static int occ_hybrid(size_t lds, uint32_t flags) 
 int occupancy_sg_hybrid_i16(int chunk, bool paired, size_t lds, uint32_t flags) {
     if (paired) return chunk == 16 ? occ_hybrid<16, true>(lds, flags) : occ_hybrid<32, true>(lds, flags);
     return chunk == 16 ? occ_hybrid<16, false>(lds, flags) : occ_hybrid<32, false>(lds, flags);
+}
+
+template <int K> static int occ_lock(size_t lds, uint32_t flags) {
+    const bool te = flags & 8u, qe = flags & 2u;
+    if (te && qe) return occ(sg_hybrid_lock<K, true, true>, 256, lds);
+    if (te) return occ(sg_hybrid_lock<K, true, false>, 256, lds);
+    if (qe) return occ(sg_hybrid_lock<K, false, true>, 256, lds);
+    return occ(sg_hybrid_lock<K, false, false>, 256, lds);
+}
+
+int occupancy_sg_hybrid_lock(int chunk, size_t lds, uint32_t flags) {
+    return chunk == 16 ? occ_lock<16>(lds, flags) : occ_lock<32>(lds, flags);
 }
 
 int occupancy_sg_batched_i16(int chunk, int block, size_t lds, uint32_t flags) {
