@@ -100,6 +100,32 @@ def test_random_all_flags(eng, engine, tile, kind, schedule):
         check(eng, engine, q, Targets.from_list(seqs), kind, go, ge, flags, f"flags={flags}", tile, schedule)
 
 
+@pytest.mark.parametrize("schedule", ["auto", "wave", "workgroup"])
+@pytest.mark.parametrize("tile", [32, 16])
+@pytest.mark.parametrize("ncodes", [9, 14, 20, 21, 22])
+def test_shared_paired_profile_class_counts(eng, tile, schedule, ncodes):
+    """Databases whose codes fall into 9..21 profile classes run over the
+    workgroup-shared paired profile (tiles of four groups in lockstep, long
+    groups pipelined over the int16 profile in the same launch); 22 classes
+    fall back to the int16 profile. A BLOSUM62 query holding every residue
+    gives each database code its own class."""
+    rng = np.random.default_rng(ncodes * 100 + tile)
+    q = np.concatenate([np.arange(24), rng.integers(0, 24, 176)]).astype(np.uint8)
+    rng.shuffle(q)
+    lens = [int(x) for x in rng.integers(3000, 6000, 24)] + [int(x) for x in rng.integers(0, 400, 700)]
+    seqs = [rng.integers(0, ncodes, L).astype(np.uint8) for L in lens]
+    for k in range(0, len(seqs), 9):  # plant part of the query (its codes below ncodes)
+        r = seqs[k]
+        if len(r) > 60:
+            piece = q[q < ncodes][:60]
+            o = int(rng.integers(0, len(r) - len(piece) + 1))
+            r[o:o + len(piece)] = piece
+    T = Targets.from_list(seqs)
+    for flags in ("FFTT", "TTTT", "FFFF", "TFTF"):
+        check(eng, "batched", q, T, "blosum62", 3, 1, O.flags_from_str(flags), f"codes={ncodes} {flags}",
+              tile, schedule)
+
+
 @pytest.mark.parametrize("flags", ["FFTT", "TTTT", "FFFF", "TFTF", "FTFT"])
 def test_hybrid_long_groups_among_short(eng, flags):
     """A few very long targets among many short ones: the automatic
