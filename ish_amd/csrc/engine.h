@@ -173,6 +173,11 @@ hipError_t launch_gather_reversed(const uint8_t *raw, const uint64_t *src_off,
 hipError_t launch_sg_multi(const ScoreParams &p, bool packed, bool paired, int chunk, int grid,
                            int block, size_t lds, hipStream_t st);
 int occupancy_sg_multi(bool packed, bool paired, int chunk, int block, size_t lds, uint32_t flags);
+// Multi-query launch over workgroup-shared paired profiles (packed, queries
+// of up to kPairMaxShared classes, p.pnrows the largest): LDS 1 KB plus
+// pnrows^2 pair rows of the tile height; block of 256.
+hipError_t launch_sg_multi_lock(const ScoreParams &p, int chunk, int grid, size_t lds, hipStream_t st);
+int occupancy_sg_multi_lock(int chunk, size_t lds, uint32_t flags);
 // Emission at the --score threshold for the results of nq queries (query
 // k's results at k * n): hit (target, score, end_ref, end_query) records go
 // to hits[k * cap ...] and count[k] counts every emitted target (it may

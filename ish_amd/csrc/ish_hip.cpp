@@ -1255,16 +1255,24 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
     }
     const size_t luts_off = pofs[nq];
     // paired profiles when every query's classes over the database's codes
-    // are few: the packed launches then read those instead
+    // are few: the packed launches then read those instead, one copy of a
+    // chunk's pair rows per wave (at most kPairMaxRows classes) or one per
+    // workgroup with the tile's groups in lockstep (at most kPairMaxShared)
     std::vector<PairProfile> pairs(nq);
     bool paired = true;
     int32_t max_classes = 0;
     for (size_t k = 0; k < nq && paired; ++k) {
-        paired = build_pairs(*items[k].pf, db->code_mask, kPairMaxRows, pairs[k]);
+        paired = build_pairs(*items[k].pf, db->code_mask, kPairMaxShared, pairs[k]);
         max_classes = std::max(max_classes, pairs[k].classes);
     }
-    const size_t paired_lds = 1024 + (size_t)(kBlock / 64) * max_classes * max_classes * pair_row_bytes(chunk);
-    if (paired && paired_lds > kLdsLimit) paired = false;
+    const size_t wave_pair_lds = 1024 + (size_t)(kBlock / 64) * max_classes * max_classes * pair_row_bytes(chunk);
+    const size_t lock_lds = 1024 + (size_t)max_classes * max_classes * pair_row_bytes(chunk);
+    enum { kUnpaired = 0, kPerWave = 1, kShared = 2 };
+    int pmode = kUnpaired;
+    if (paired && max_classes <= kPairMaxRows && wave_pair_lds <= kLdsLimit) pmode = kPerWave;
+    else if (paired && lock_lds <= kLdsLimit) pmode = kShared;
+    paired = pmode != kUnpaired;
+    const size_t paired_lds = pmode == kPerWave ? wave_pair_lds : lock_lds;
     std::vector<size_t> pp_ofs(nq + 1, luts_off + nq * 512);
     if (paired)
         for (size_t k = 0; k < nq; ++k)
@@ -1293,7 +1301,7 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
     const int waves = kBlock / 64;
     struct Launch {
         ScoreParams p;
-        bool packed, paired;
+        bool packed, paired; // paired: pmode's variant
         int32_t maxcols, ntab;
     };
     std::vector<Launch> launches;
@@ -1379,8 +1387,10 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
     for (size_t l = 0; l < launches.size(); ++l) {
         int64_t ss;
         const size_t llds = launches[l].paired ? paired_lds : lds;
-        const int occ = occupancy_sg_multi(launches[l].packed, launches[l].paired, chunk, kBlock, llds,
-                                           launches[l].p.flags);
+        const int occ = launches[l].paired && pmode == kShared
+                            ? occupancy_sg_multi_lock(chunk, llds, launches[l].p.flags)
+                            : occupancy_sg_multi(launches[l].packed, launches[l].paired, chunk, kBlock,
+                                                 llds, launches[l].p.flags);
         e = persistent_grid(*mb.scratch, occ, cus, launches[l].maxcols, &grids[l], &ss);
         if (e != hipSuccess) return hip_fail(e, "boundary scratch");
         launches[l].p.slot_stride = ss;
@@ -1392,9 +1402,13 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
                            sizeof(QueryDesc) * (size_t)launches[l].ntab, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_reset(p.counter, st);
         if (e == hipSuccess) e = on_launch(true);
-        if (e == hipSuccess)
-            e = launch_sg_multi(p, launches[l].packed, launches[l].paired, chunk, grids[l], kBlock,
-                                launches[l].paired ? paired_lds : lds, st);
+        if (e == hipSuccess) {
+            if (launches[l].paired && pmode == kShared)
+                e = launch_sg_multi_lock(p, chunk, grids[l], paired_lds, st);
+            else
+                e = launch_sg_multi(p, launches[l].packed, launches[l].paired, chunk, grids[l], kBlock,
+                                    launches[l].paired ? paired_lds : lds, st);
+        }
         if (e == hipSuccess) e = on_launch(false);
         if (e != hipSuccess) return hip_fail(e, "multi-query launch");
     }

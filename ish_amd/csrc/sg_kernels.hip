@@ -441,12 +441,14 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
 // every pair row into pbuf. Groups in a tile are neighbours in length
 // order, so the waves reach each barrier at nearly the same time. A wave
 // whose group is past the range only joins the barriers and the copies.
+// pprof / npairs / pqpad: the query's paired profile; valid: the wave has
+// a group (wave-uniform).
 template <int K, bool TE, bool QE>
 __device__ __forceinline__ void sg_tile_lockstep(const ScoreParams &p, const QueryView &qv,
-                                                 const uint16_t *ptab, uint8_t *pbuf, uint2 *bnd,
-                                                 int g, int lane) {
+                                                 const uint16_t *ptab, uint8_t *pbuf,
+                                                 const int32_t *pprof, int npairs, int pqpad,
+                                                 uint2 *bnd, int g, bool valid, int lane) {
     using V = s16x2;
-    const bool valid = g < p.g_end; // wave-uniform
     const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
     GroupDesc gd;
     gd.res_off = 0;
@@ -459,7 +461,6 @@ __device__ __forceinline__ void sg_tile_lockstep(const ScoreParams &p, const Que
     const int RB = valid ? p.lens[tB] : 0;
     const uint2 *res = p.res + gd.res_off;
     const int nblk = gd.ncols >> 2;
-    const int npairs = p.pnrows * p.pnrows;
     constexpr int parts = K / 4; // 16-byte parts per pair row
 
     TrackI16 t16;
@@ -470,7 +471,7 @@ __device__ __forceinline__ void sg_tile_lockstep(const ScoreParams &p, const Que
         __syncthreads(); // every wave is done with the previous chunk's rows
         for (int x = threadIdx.x; x < npairs * parts; x += blockDim.x) {
             const int pr = x / parts, part = x % parts;
-            const uint4 v = *(const uint4 *)(p.pprof + (size_t)pr * p.pqpad + j0 + part * 4);
+            const uint4 v = *(const uint4 *)(pprof + (size_t)pr * pqpad + j0 + part * 4);
             *(uint4 *)(pbuf + pr * pair_row_bytes(K) + part * 16) = v;
         }
         __syncthreads();
@@ -785,7 +786,8 @@ __global__ void __launch_bounds__(256) sg_hybrid_lock(ScoreParams p) {
         } else {
             have_profile = false; // the tile's pair rows overwrite it
             const int g = p.g_begin + nlong + (item - nlong) * waves + w;
-            sg_tile_lockstep<K, TE, QE>(p, qv, ptab, area, bnd, g, lane);
+            sg_tile_lockstep<K, TE, QE>(p, qv, ptab, area, p.pprof, p.pnrows * p.pnrows, p.pqpad, bnd,
+                                        g, g < p.g_end, lane);
         }
     }
 }
@@ -865,6 +867,65 @@ __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
             sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, half, lane, &pv);
         }
         __syncthreads(); // everyone is done with s_tile and the profile
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-query kernel over workgroup-shared paired profiles (packed int16):
+// as sg_multi, but a tile's four groups step through its query's chunks in
+// lockstep over one LDS copy of the chunk's pair rows (sg_tile_lockstep).
+// LDS: the query's pair table (1 KB), then the pair rows of one chunk
+// (p.pnrows is the largest class count of the launch's queries).
+// ---------------------------------------------------------------------------
+template <int K, bool TE, bool QE>
+__global__ void __launch_bounds__(256) sg_multi_lock(ScoreParams p) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    __shared__ int s_tile[3]; // query, first wave item, items of that query
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)wave_broadcast0(threadIdx.x >> 6); // wave-uniform
+    const int waves = blockDim.x >> 6;
+    uint2 *bnd = p.bnd + (size_t)(blockIdx.x * waves + wave) * p.slot_stride;
+    const uint16_t *ptab = (const uint16_t *)smem;
+    uint8_t *area = smem + 1024;
+    int cur_q = -1;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int64_t tile = (int64_t)atomicAdd(p.counter, 1u);
+            int q = -1;
+            if (tile < p.queries[p.nqueries].tile_base) {
+                int lo = 0, hi = p.nqueries - 1; // last query whose tile_base <= tile
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (p.queries[mid].tile_base <= tile) lo = mid;
+                    else hi = mid - 1;
+                }
+                q = lo;
+            }
+            s_tile[0] = q;
+            if (q >= 0) {
+                const QueryDesc &qd = p.queries[q];
+                s_tile[1] = (int)((tile - qd.tile_base) * waves);
+                s_tile[2] = qd.g_end - qd.g_begin;
+            }
+        }
+        __syncthreads();
+        const int q = s_tile[0];
+        if (q < 0) break;
+        const QueryDesc qd = p.queries[q];
+        if (q != cur_q) {
+            stage_pair_table<K>(qd.plut, smem); // ends in a barrier
+            cur_q = q;
+        }
+        QueryView qv;
+        qv.Q = qd.Q;
+        qv.nchunks = qd.nchunks;
+        qv.neg_limit = qd.neg_limit;
+        qv.flags = qd.flags;
+        qv.out_off = qd.out_off;
+        const int item = s_tile[1] + wave;
+        sg_tile_lockstep<K, TE, QE>(p, qv, ptab, area, qd.pprof, qd.pnrows * qd.pnrows, qd.pqpad,
+                                    bnd, qd.g_begin + item, item < s_tile[2], lane);
+        __syncthreads(); // everyone is done with s_tile and the pair rows
     }
 }
 
@@ -1062,6 +1123,24 @@ static hipError_t launch_multi(const ScoreParams &p, int grid, int block, size_t
     return hipGetLastError();
 }
 
+template <int K>
+static hipError_t launch_multi_lock(const ScoreParams &p, int grid, size_t lds, hipStream_t st) {
+    const bool te = p.flags & 8u, qe = p.flags & 2u;
+    if (te && qe)
+        hipLaunchKernelGGL((sg_multi_lock<K, true, true>), dim3(grid), dim3(256), lds, st, p);
+    else if (te)
+        hipLaunchKernelGGL((sg_multi_lock<K, true, false>), dim3(grid), dim3(256), lds, st, p);
+    else if (qe)
+        hipLaunchKernelGGL((sg_multi_lock<K, false, true>), dim3(grid), dim3(256), lds, st, p);
+    else
+        hipLaunchKernelGGL((sg_multi_lock<K, false, false>), dim3(grid), dim3(256), lds, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sg_multi_lock(const ScoreParams &p, int chunk, int grid, size_t lds, hipStream_t st) {
+    return chunk == 16 ? launch_multi_lock<16>(p, grid, lds, st) : launch_multi_lock<32>(p, grid, lds, st);
+}
+
 hipError_t launch_sg_multi(const ScoreParams &p, bool packed, bool paired, int chunk, int grid,
                            int block, size_t lds, hipStream_t st) {
     if (packed && paired) return chunk == 16 ? launch_multi<s16x2, 16, true>(p, grid, block, lds, st)
@@ -1176,6 +1255,17 @@ template <typename V, int K, bool PAIRED> static int occ_multi(int block, size_t
     if (qe) return occ(sg_multi<V, K, false, true, PAIRED>, block, lds);
     return occ(sg_multi<V, K, false, false, PAIRED>, block, lds);
 }
+template <int K> static int occ_multi_lock(size_t lds, uint32_t flags) {
+    const bool te = flags & 8u, qe = flags & 2u;
+    if (te && qe) return occ(sg_multi_lock<K, true, true>, 256, lds);
+    if (te) return occ(sg_multi_lock<K, true, false>, 256, lds);
+    if (qe) return occ(sg_multi_lock<K, false, true>, 256, lds);
+    return occ(sg_multi_lock<K, false, false>, 256, lds);
+}
+int occupancy_sg_multi_lock(int chunk, size_t lds, uint32_t flags) {
+    return chunk == 16 ? occ_multi_lock<16>(lds, flags) : occ_multi_lock<32>(lds, flags);
+}
+
 int occupancy_sg_multi(bool packed, bool paired, int chunk, int block, size_t lds, uint32_t flags) {
     if (packed && paired) return chunk == 16 ? occ_multi<s16x2, 16, true>(block, lds, flags)
                                              : occ_multi<s16x2, 32, true>(block, lds, flags);

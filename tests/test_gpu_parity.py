@@ -520,6 +520,37 @@ def test_multi_query(eng, flags):
     db.close()
 
 
+@pytest.mark.parametrize("tile", [32, 16])
+@pytest.mark.parametrize("ncodes", [5, 12, 20, 21, 22])
+def test_multi_query_paired_class_counts(eng, tile, ncodes):
+    """Multi-query launches over databases of 5 codes (per-wave paired
+    profiles), 12 to 21 (workgroup-shared paired profiles, tiles in
+    lockstep) and 22 (int16 profile), with queries of every length class,
+    int32 groups and per-query free ends (FFTT and TTTT in one launch
+    through the serving-style per-query flags are covered by the server
+    tests; here one set per launch)."""
+    rng = np.random.default_rng(ncodes * 10 + tile)
+    m = O.Matrix("blosum62")
+    seqs = [rng.integers(0, ncodes, int(rng.integers(0, 500))).astype(np.uint8) for _ in range(900)]
+    seqs += [rng.integers(0, ncodes, L).astype(np.uint8) for L in (9000, 4000)]
+    T = Targets.from_list(seqs)
+    queries = [np.concatenate([np.arange(24), rng.integers(0, 24, L)]).astype(np.uint8)
+               for L in (0, 9, 100, 333, 700)]
+    eng.set_engine("batched")
+    eng.set_tile(tile)
+    db = eng.load_db(T)
+    try:
+        for flags in ("FFTT", "TTTT", "FFFF"):
+            s, er, eq, st = eng.align_db_multi(db, queries, "blosum62", 3, 1, flags)
+            fl = O.flags_from_str(flags)
+            for k, q in enumerate(queries):
+                os_, oer, oeq, _ = O.batch(0, q, T.residues, T.offsets, m, 3, 1, fl)
+                bad = np.flatnonzero((s[k] != os_) | (er[k] != oer) | (eq[k] != oeq))
+                assert len(bad) == 0, (ncodes, tile, flags, k, len(q), bad[:5])
+    finally:
+        db.close()
+
+
 # ---- autotuner and HIP graphs ------------------------------------------------
 def test_autotune_keeps_results(eng):
     """The tuned launches (per length bucket) give the same results as the
