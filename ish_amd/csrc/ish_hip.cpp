@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -76,6 +77,10 @@ struct ish_hip_ctx {
     std::vector<uint8_t> graph_sig;
     // work buffers
     DevBuf scratch, counter, in_res, in_off, out_score, out_er, out_eq, out_st;
+    // pinned staging of database uploads: two buffers, each with the event
+    // of its last copy to the device
+    uint8_t *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_done[2] = {nullptr, nullptr};
     // timing
     std::vector<hipEvent_t> ev;
     int launches = 0;
@@ -226,6 +231,10 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
+        if (ctx->stage_done[k]) (void)hipEventDestroy(ctx->stage_done[k]);
+    }
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (DevBuf *b : {&ctx->prof_t, &ctx->prof_rt, &ctx->prof_p, &ctx->lut, &ctx->pprof, &ctx->plut,
@@ -635,6 +644,78 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     return ISH_OK;
 }
 
+// Host-to-device copy of the concatenation of pieces into dst, through two
+// pinned staging buffers: host threads copy one part into a buffer while the
+// other buffer's part is on its way to the device, so the upload runs at
+// the pinned DMA rate instead of the pageable path's (which stages small
+// parts one after the other). Returns once every part is enqueued on the
+// stream (the last copies may still run).
+constexpr size_t kStageBytes = (size_t)64 << 20;
+static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiece *pieces, size_t np) {
+    for (int k = 0; k < 2; ++k) {
+        if (!ctx->stage[k]) {
+            hipError_t e = hipHostMalloc((void **)&ctx->stage[k], kStageBytes, hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+        }
+        if (!ctx->stage_done[k]) {
+            hipError_t e = hipEventCreateWithFlags(&ctx->stage_done[k], hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+            e = hipEventRecord(ctx->stage_done[k], ctx->stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int threads = (int)std::min(8u, hw);
+    int buf = 0;
+    uint64_t at = 0;
+    size_t pk = 0;
+    uint64_t in_piece = 0; // bytes of pieces[pk] already staged
+    while (pk < np) {
+        // fill one staging buffer with up to kStageBytes of the pieces
+        hipError_t e = hipEventSynchronize(ctx->stage_done[buf]);
+        if (e != hipSuccess) return e;
+        struct Span {
+            const uint8_t *src;
+            size_t off, n;
+        };
+        std::vector<Span> spans;
+        size_t fill = 0;
+        while (pk < np && fill < kStageBytes) {
+            const uint64_t left = pieces[pk].n - in_piece;
+            const size_t take = (size_t)std::min<uint64_t>(left, kStageBytes - fill);
+            if (take) spans.push_back({pieces[pk].p + in_piece, fill, take});
+            fill += take;
+            in_piece += take;
+            if (in_piece == pieces[pk].n) {
+                ++pk;
+                in_piece = 0;
+            }
+        }
+        if (fill == 0) continue;
+        // the host copy, split over threads by byte range
+        uint8_t *stage = ctx->stage[buf];
+        auto copy_range = [&](size_t lo, size_t hi) {
+            for (const Span &sp : spans) {
+                const size_t a = std::max(lo, sp.off), b = std::min(hi, sp.off + sp.n);
+                if (a < b) memcpy(stage + a, sp.src + (a - sp.off), b - a);
+            }
+        };
+        const int tn = fill >= ((size_t)1 << 22) ? threads : 1;
+        const size_t per = (fill + tn - 1) / tn;
+        std::vector<std::thread> pool;
+        for (int t = 1; t < tn; ++t)
+            pool.emplace_back(copy_range, std::min(fill, (size_t)t * per), std::min(fill, (size_t)(t + 1) * per));
+        copy_range(0, std::min(fill, per));
+        for (auto &t : pool) t.join();
+        e = hipMemcpyAsync(dst + at, stage, fill, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->stage_done[buf], ctx->stream);
+        if (e != hipSuccess) return e;
+        at += fill;
+        buf ^= 1;
+    }
+    return hipSuccess;
+}
+
 // Uploads the residues (the concatenation of pieces) and builds the db;
 // with `encode` the bytes are translated to residue codes on the device
 // first. h_off: offsets from 0, already checked.
@@ -646,12 +727,7 @@ static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t 
     uint64_t *d_off = nullptr;
     hipError_t e = hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (n + 1));
-    uint64_t at = 0;
-    for (size_t k = 0; k < np && e == hipSuccess; ++k) {
-        if (pieces[k].n > 0)
-            e = hipMemcpyAsync(d_raw + at, pieces[k].p, pieces[k].n, hipMemcpyHostToDevice, ctx->stream);
-        at += pieces[k].n;
-    }
+    if (e == hipSuccess) e = staged_h2d(ctx, d_raw, pieces, np);
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, ctx->stream);
