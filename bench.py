@@ -499,7 +499,7 @@ def main():
         sample_cells = Q * int(off[-1])
         cpu = {"value": round(sample_cells / cpu_s / 1e9, 3), "unit": "GCUPS", "cores": int(used),
                "kind": "port",
-               "sample": f"{n_s} of the {w.targets.n} targets (every {w.targets.n // n_s}th), "
+               "sample": f"{n_s} of the {w.targets.n} targets, evenly spaced in input order, "
                          f"{sample_cells / 1e9:.2f} Gcells, {CPU_KIND}, "
                          f"{os.cpu_count()} host threads"}
         parity = bool(np.array_equal(os_, s[idx]) and np.array_equal(oer, er[idx]) and np.array_equal(oeq, eq[idx]))
