@@ -454,13 +454,12 @@ def main():
         f"widened targets {widened}")
 
     eng.set_graphs(False)
-    # end-to-end: upload + pack + score + fetch, for one query
+    # end-to-end: one synchronous batch call from host buffers (upload, sort
+    # and pack, score, results to the host), in slices whose uploads
+    # overlap the scoring of the slice before
     t0 = time.perf_counter()
-    db2 = eng.load_db(w.targets)
-    eng.align_db_async(db2)
-    eng.fetch(db2)
+    eng.align(w.targets)
     e2e_s = time.perf_counter() - t0
-    db2.close()
 
     plain = None
     if not args.no_plain and not local:

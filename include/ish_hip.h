@@ -164,11 +164,20 @@ int32_t ish_hip_set_graphs(ish_hip_ctx *ctx, int32_t on);
 /* One synchronous batch (the batch_match_coarse + copy_outputs_to_host
  * step). Targets are residues[offsets[t] .. offsets[t+1]) for t < n.
  * Outputs (n entries each): score, 0-based end_ref / end_query, status.
- * status may be NULL. */
+ * status may be NULL. With the batched engine a batch of more residues
+ * than the slice size is cut into input-order slices: each slice is
+ * uploaded while the slice before it is scored. */
 int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
                             const uint64_t *offsets, int64_t n, int32_t *score,
                             int32_t *end_ref, int32_t *end_query,
                             uint8_t *status);
+/* Slice size of ish_hip_align_batch in residue bytes: 0 (the default)
+ * sizes slices automatically (at least 64 MiB, and enough work per slice
+ * that its longest target does not set the launch's length), a positive
+ * value is a fixed size, a negative value turns slicing off. No
+ * counterpart in the reference, whose batches are sized by the host
+ * (searcher_settings.mojo:44-50). */
+int32_t ish_hip_set_slice_bytes(ish_hip_ctx *ctx, int64_t bytes);
 
 /* Device-resident target database: uploaded once, sorted by length and
  * packed into coalesced groups on the device, then scored against any
@@ -191,7 +200,9 @@ int32_t ish_hip_align_db(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
                          int32_t *end_ref, int32_t *end_query,
                          uint8_t *status);
 /* Asynchronous form: enqueue on the context's stream; results stay on the
- * device until ish_hip_db_fetch. ish_hip_sync waits for the stream. */
+ * device until ish_hip_db_fetch. ish_hip_sync waits for the stream. A
+ * database load while a pass is in flight uploads on the context's copy
+ * stream during that pass; its build runs after the pass. */
 int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db);
 int32_t ish_hip_sync(ish_hip_ctx *ctx);
 int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,

@@ -32,7 +32,7 @@ ABI_SYMBOLS = [
     "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
     "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_autotune", "ish_hip_tuning",
-    "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_db_load", "ish_hip_db_load_ascii", "ish_hip_db_free", "ish_hip_db_size",
+    "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_set_slice_bytes", "ish_hip_db_load", "ish_hip_db_load_ascii", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
@@ -120,6 +120,7 @@ def lib():
         "ish_hip_autotune": ([vp, vp, i32], i32),
         "ish_hip_tuning": ([vp, vp, i32p, i32p], i32),
         "ish_hip_set_graphs": ([vp, i32], i32),
+        "ish_hip_set_slice_bytes": ([vp, ctypes.c_int64], i32),
         "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
         "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
         "ish_hip_db_load_ascii": ([vp, u8p, u64p, i64, i32, P(vp)], i32),
@@ -305,6 +306,13 @@ class Engine:
 
     def set_graphs(self, on: bool):
         self._check(self._L.ish_hip_set_graphs(self._h, int(bool(on))))
+
+    def set_slice_bytes(self, nbytes: int):
+        """Slice size of align() in residue bytes (0: automatic, the
+        default; < 0: never slice): a larger batch is uploaded and scored
+        slice by slice, each upload overlapping the scoring of the slice
+        before."""
+        self._check(self._L.ish_hip_set_slice_bytes(self._h, int(nbytes)))
 
     def autotune(self, db: "Database", reps: int = 2):
         """Times whole passes over db with every launch configuration for

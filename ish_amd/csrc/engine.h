@@ -157,9 +157,13 @@ int occupancy_sw_pipe_i16(int chunk, bool paired, size_t lds);
 int occupancy_sw_batched_i32(int chunk, int block, size_t lds);
 hipError_t launch_sg_plain(const PlainParams &p, int grid, int block,
                            size_t lds, hipStream_t st);
+// Packs the length-sorted targets into the group layout (total_blocks =
+// the blocks of all groups together). raw must hold kRawPad bytes past the
+// last residue: the packer reads whole aligned dwords.
+constexpr uint64_t kRawPad = 16;
 hipError_t launch_pack(const uint8_t *raw, const uint64_t *off,
                        const int32_t *sorted_idx, const GroupDesc *groups,
-                       int32_t ngroups, int64_t n, uint2 *res,
+                       int32_t ngroups, int64_t n, uint64_t total_blocks, uint2 *res,
                        hipStream_t st);
 hipError_t launch_gather_reversed(const uint8_t *raw, const uint64_t *src_off,
                                   const int64_t *sel, const uint64_t *dst_off,
@@ -216,5 +220,9 @@ int occupancy_sg_plain(int block, size_t lds, uint32_t flags);
 constexpr int kChunkAlign = 32;
 constexpr int kChunkDefault = 32;
 constexpr int kChunkPlain = 32;
+
+// Smallest slice of a one-shot batch cut into input-order slices (each
+// uploaded while the slice before is scored).
+constexpr int64_t kSliceMin = (int64_t)64 << 20;
 
 } // namespace ish

@@ -81,6 +81,18 @@ struct ish_hip_ctx {
     // of its last copy to the device
     uint8_t *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_done[2] = {nullptr, nullptr};
+    // uploads run on a stream of their own, so that one database's upload
+    // overlaps the scoring already queued on `stream`; copy_done marks the
+    // end of the last upload, and the database build waits for it
+    hipStream_t copy = nullptr;
+    hipEvent_t copy_done = nullptr;
+    // pinned landing buffer of fetched results (grow-only)
+    uint8_t *res_host = nullptr;
+    size_t res_host_bytes = 0;
+    // one-shot batches are uploaded and scored in slices, each slice's
+    // upload overlapping the scoring of the slice before: 0 sizes them
+    // automatically, > 0 is a fixed size in residues, < 0 never slices
+    int64_t slice_bytes = 0;
     // timing
     std::vector<hipEvent_t> ev;
     int launches = 0;
@@ -222,6 +234,13 @@ extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
         delete c;
         return ISH_E_HIP;
     }
+    if (hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->copy_done, hipEventDisableTiming) != hipSuccess) {
+        if (c->copy) (void)hipStreamDestroy(c->copy);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return ISH_E_HIP;
+    }
     *out = c;
     return ISH_OK;
 }
@@ -230,6 +249,7 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->copy);
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k) {
         if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
@@ -242,6 +262,9 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
                       &ctx->in_res, &ctx->in_off, &ctx->out_score, &ctx->out_er, &ctx->out_eq,
                       &ctx->out_st})
         b->release();
+    if (ctx->res_host) (void)hipHostFree(ctx->res_host);
+    (void)hipEventDestroy(ctx->copy_done);
+    (void)hipStreamDestroy(ctx->copy);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -559,22 +582,22 @@ static int32_t size_grid(ish_hip_ctx *ctx, int occ_blocks, int64_t cols, int *gr
 // ---------------------------------------------------------------------------
 // device-resident database
 // ---------------------------------------------------------------------------
-// Builds the packed, length-sorted layout from residues already on the
-// device (d_raw, offsets d_off rebased to 0). Takes ownership of both
-// buffers, which the database keeps for start-coordinate passes.
-static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
-                        std::vector<uint64_t> h_off, ish_hip_db **out) {
+// The host half of a database build: targets sorted by length (counting
+// sort, descending, stable in input order) into groups of kGroup slots.
+// It needs only the offsets, so it can run while the residues upload.
+struct SortPlan {
+    std::vector<int32_t> sorted, lens; // per slot: target index (-1: empty), length
+    std::vector<GroupDesc> groups;
+    std::vector<int32_t> group_maxlen;
+    int32_t maxcols = 0;
+    uint64_t words = 0; // uint2 words of all groups' blocks
+};
+
+static void plan_sort(const std::vector<uint64_t> &h_off, SortPlan &sp) {
     const int64_t n = (int64_t)h_off.size() - 1;
-    ish_hip_db *db = new ish_hip_db();
-    db->device = ctx->device;
-    db->n = n;
-    db->nres = n > 0 ? (int64_t)h_off[(size_t)n] : 0;
-    db->d_raw = d_raw;
-    db->d_off = d_off;
     int64_t maxlen = 0;
     for (int64_t t = 0; t < n; ++t)
         maxlen = std::max<int64_t>(maxlen, (int64_t)(h_off[(size_t)t + 1] - h_off[(size_t)t]));
-    // counting sort by length, descending, stable in input order
     std::vector<int64_t> hist((size_t)maxlen + 2, 0);
     for (int64_t t = 0; t < n; ++t) hist[(size_t)(maxlen - (int64_t)(h_off[(size_t)t + 1] - h_off[(size_t)t]))]++;
     int64_t acc = 0;
@@ -585,25 +608,50 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     }
     const int32_t ng = (int32_t)((n + kGroup - 1) / kGroup);
     const int64_t nslots = (int64_t)ng * kGroup;
-    std::vector<int32_t> sorted((size_t)nslots, -1), lens((size_t)nslots, 0);
+    sp.sorted.assign((size_t)nslots, -1);
+    sp.lens.assign((size_t)nslots, 0);
     for (int64_t t = 0; t < n; ++t) {
         const int64_t L = (int64_t)(h_off[(size_t)t + 1] - h_off[(size_t)t]);
         const int64_t pos = hist[(size_t)(maxlen - L)]++;
-        sorted[(size_t)pos] = (int32_t)t;
-        lens[(size_t)pos] = (int32_t)L;
+        sp.sorted[(size_t)pos] = (int32_t)t;
+        sp.lens[(size_t)pos] = (int32_t)L;
     }
-    db->ngroups = ng;
-    std::vector<GroupDesc> groups((size_t)ng);
+    sp.groups.resize((size_t)ng);
+    sp.group_maxlen.resize((size_t)ng);
     uint64_t off = 0;
     for (int32_t g = 0; g < ng; ++g) {
-        const int32_t L = lens[(size_t)g * kGroup];
-        groups[g].res_off = off;
-        groups[g].ncols = (L + 3) & ~3;
-        groups[g].first = g * kGroup;
-        db->group_maxlen.push_back(L);
-        db->maxcols = std::max(db->maxcols, groups[g].ncols);
-        off += (uint64_t)(groups[g].ncols / 4) * 64;
+        const int32_t L = sp.lens[(size_t)g * kGroup];
+        sp.groups[g].res_off = off;
+        sp.groups[g].ncols = (L + 3) & ~3;
+        sp.groups[g].first = g * kGroup;
+        sp.group_maxlen[g] = L;
+        sp.maxcols = std::max(sp.maxcols, sp.groups[g].ncols);
+        off += (uint64_t)(sp.groups[g].ncols / 4) * 64;
     }
+    sp.words = off;
+}
+
+// Builds the packed, length-sorted layout from residues already on the
+// device (d_raw, offsets d_off rebased to 0) and the sort plan of h_off.
+// Takes ownership of both buffers, which the database keeps for
+// start-coordinate passes.
+static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
+                        std::vector<uint64_t> h_off, SortPlan &sp, ish_hip_db **out) {
+    const int64_t n = (int64_t)h_off.size() - 1;
+    ish_hip_db *db = new ish_hip_db();
+    db->device = ctx->device;
+    db->n = n;
+    db->nres = n > 0 ? (int64_t)h_off[(size_t)n] : 0;
+    db->d_raw = d_raw;
+    db->d_off = d_off;
+    const int32_t ng = (int32_t)sp.groups.size();
+    const int64_t nslots = (int64_t)ng * kGroup;
+    db->ngroups = ng;
+    db->group_maxlen = std::move(sp.group_maxlen);
+    db->maxcols = sp.maxcols;
+    const std::vector<GroupDesc> &groups = sp.groups;
+    const std::vector<int32_t> &sorted = sp.sorted, &lens = sp.lens;
+    const uint64_t off = sp.words;
     db->h_off = std::move(h_off);
     const uint64_t res_words = off + 64; // one spare block: empty groups read block 0
     uint32_t *d_mask = nullptr;
@@ -626,7 +674,8 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     DB_TRY(hipMalloc((void **)&db->d_end_ref, sizeof(int32_t) * std::max<int64_t>(n, 1)));
     DB_TRY(hipMalloc((void **)&db->d_end_q, sizeof(int32_t) * std::max<int64_t>(n, 1)));
     DB_TRY(hipMalloc((void **)&db->d_status, std::max<int64_t>(n, 1)));
-    DB_TRY(hipMemsetAsync(db->d_res, 0, sizeof(uint2) * res_words, ctx->stream));
+    // the packer writes every block of every group; the spare block is zero
+    DB_TRY(hipMemsetAsync(db->d_res + off, 0, sizeof(uint2) * 64, ctx->stream));
     // residue codes present (paired profiles are built over their classes)
     DB_TRY(hipMalloc((void **)&d_mask, 8 * sizeof(uint32_t)));
     DB_TRY(launch_code_mask(db->d_raw, (uint64_t)db->nres, d_mask, ctx->stream));
@@ -635,7 +684,8 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
         DB_TRY(hipMemcpyAsync(db->d_groups, groups.data(), sizeof(GroupDesc) * ng, hipMemcpyHostToDevice, ctx->stream));
         DB_TRY(hipMemcpyAsync(db->d_lens, lens.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
         DB_TRY(hipMemcpyAsync(db->d_orig, sorted.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
-        DB_TRY(launch_pack(db->d_raw, db->d_off, db->d_orig, db->d_groups, ng, n, db->d_res, ctx->stream));
+        DB_TRY(launch_pack(db->d_raw, db->d_off, db->d_orig, db->d_groups, ng, n, off / 64, db->d_res,
+                           ctx->stream));
     }
     DB_TRY(hipStreamSynchronize(ctx->stream));
 #undef DB_TRY
@@ -648,8 +698,9 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
 // pinned staging buffers: host threads copy one part into a buffer while the
 // other buffer's part is on its way to the device, so the upload runs at
 // the pinned DMA rate instead of the pageable path's (which stages small
-// parts one after the other). Returns once every part is enqueued on the
-// stream (the last copies may still run).
+// parts one after the other). The copies run on the context's copy stream;
+// it returns once every part is enqueued there (the last copies may still
+// run, but the pieces are no longer read).
 constexpr size_t kStageBytes = (size_t)64 << 20;
 static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiece *pieces, size_t np) {
     for (int k = 0; k < 2; ++k) {
@@ -660,7 +711,7 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
         if (!ctx->stage_done[k]) {
             hipError_t e = hipEventCreateWithFlags(&ctx->stage_done[k], hipEventDisableTiming);
             if (e != hipSuccess) return e;
-            e = hipEventRecord(ctx->stage_done[k], ctx->stream);
+            e = hipEventRecord(ctx->stage_done[k], ctx->copy);
             if (e != hipSuccess) return e;
         }
     }
@@ -707,8 +758,8 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
             pool.emplace_back(copy_range, std::min(fill, (size_t)t * per), std::min(fill, (size_t)(t + 1) * per));
         copy_range(0, std::min(fill, per));
         for (auto &t : pool) t.join();
-        e = hipMemcpyAsync(dst + at, stage, fill, hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(ctx->stage_done[buf], ctx->stream);
+        e = hipMemcpyAsync(dst + at, stage, fill, hipMemcpyHostToDevice, ctx->copy);
+        if (e == hipSuccess) e = hipEventRecord(ctx->stage_done[buf], ctx->copy);
         if (e != hipSuccess) return e;
         at += fill;
         buf ^= 1;
@@ -718,26 +769,35 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
 
 // Uploads the residues (the concatenation of pieces) and builds the db;
 // with `encode` the bytes are translated to residue codes on the device
-// first. h_off: offsets from 0, already checked.
+// first. h_off: offsets from 0, already checked. The upload runs on the
+// copy stream, so it overlaps a scoring pass already queued on the
+// context's stream; the build is queued behind that pass and behind the
+// upload (copy_done), and returns when it is done.
 static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t np,
                          std::vector<uint64_t> h_off, const EncodeTable *encode, ish_hip_db **out) {
     const size_t n = h_off.size() - 1;
     const uint64_t nres = h_off[n];
     uint8_t *d_raw = nullptr;
     uint64_t *d_off = nullptr;
-    hipError_t e = hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1));
+    SortPlan sp;
+    std::thread planner([&] { plan_sort(h_off, sp); }); // beside the upload
+    hipError_t e = hipMalloc((void **)&d_raw, nres + kRawPad);
     if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (n + 1));
     if (e == hipSuccess) e = staged_h2d(ctx, d_raw, pieces, np);
+    planner.join();
     if (e == hipSuccess)
-        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->copy);
+    if (e == hipSuccess) e = hipEventRecord(ctx->copy_done, ctx->copy);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->copy_done, 0);
     if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, ctx->stream);
     if (e != hipSuccess) {
+        (void)hipStreamSynchronize(ctx->copy);
         (void)hipStreamSynchronize(ctx->stream);
         if (d_raw) (void)hipFree(d_raw);
         if (d_off) (void)hipFree(d_off);
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "db upload: %s", hipGetErrorString(e));
     }
-    return db_build(ctx, d_raw, d_off, std::move(h_off), out);
+    return db_build(ctx, d_raw, d_off, std::move(h_off), sp, out);
 }
 
 // Checks the offsets (non-decreasing, targets within the int32 limit) and
@@ -1192,21 +1252,59 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     return launch_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags);
 }
 
+int32_t ish::db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db) {
+    if (!ctx || !db) return ISH_E_INVALID;
+    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
+    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return launch_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, false);
+}
+
 extern "C" int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
                                     int32_t *end_ref, int32_t *end_q, uint8_t *status) {
     if (!ctx || !db || (db->n > 0 && (!score || !end_ref || !end_q)))
         return ISH_E_INVALID;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const int64_t n = db->n;
-    if (n > 0) {
-        HIP_TRY(ctx, hipMemcpyAsync(score, db->d_score, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(end_ref, db->d_end_ref, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(end_q, db->d_end_q, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
-        if (status)
-            HIP_TRY(ctx, hipMemcpyAsync(status, db->d_status, n, hipMemcpyDeviceToHost, ctx->stream));
+    if (n <= 0) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        collect_timing(ctx);
+        return ISH_OK;
     }
+    // results land in pinned memory (the DMA rate; pageable copies are
+    // staged in small parts) and are copied out, one thread per array
+    const size_t need = (size_t)n * 13;
+    if (ctx->res_host_bytes < need) {
+        if (ctx->res_host) (void)hipHostFree(ctx->res_host);
+        ctx->res_host = nullptr;
+        ctx->res_host_bytes = 0;
+        HIP_TRY(ctx, hipHostMalloc((void **)&ctx->res_host, need, hipHostMallocDefault));
+        ctx->res_host_bytes = need;
+    }
+    uint8_t *h = ctx->res_host;
+    const size_t w = sizeof(int32_t) * (size_t)n;
+    HIP_TRY(ctx, hipMemcpyAsync(h, db->d_score, w, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h + w, db->d_end_ref, w, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h + 2 * w, db->d_end_q, w, hipMemcpyDeviceToHost, ctx->stream));
+    if (status) HIP_TRY(ctx, hipMemcpyAsync(h + 3 * w, db->d_status, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     collect_timing(ctx);
+    struct Out {
+        void *dst;
+        const uint8_t *src;
+        size_t bytes;
+    };
+    std::vector<Out> outs = {{score, h, w}, {end_ref, h + w, w}, {end_q, h + 2 * w, w}};
+    if (status) outs.push_back({status, h + 3 * w, (size_t)n});
+    if (n < ((int64_t)1 << 16)) {
+        for (const Out &o : outs) memcpy(o.dst, o.src, o.bytes);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t k = 1; k < outs.size(); ++k)
+            pool.emplace_back([o = outs[k]] { memcpy(o.dst, o.src, o.bytes); });
+        memcpy(outs[0].dst, outs[0].src, outs[0].bytes);
+        for (auto &t : pool) t.join();
+    }
     return ISH_OK;
 }
 
@@ -1295,12 +1393,84 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
             return fail(ctx, ISH_E_INVALID, "the plain engine scores semi-global alignments only");
         return align_plain(ctx, residues, offsets, n, score, end_ref, end_q, status);
     }
-    ish_hip_db *db = nullptr;
-    int32_t rc = ish_hip_db_load(ctx, residues, offsets, n, &db);
-    if (rc) return rc;
-    rc = ish_hip_align_db(ctx, db, score, end_ref, end_q, status);
-    ish_hip_db_free(db);
+    // slices of about equal residues, cut at target boundaries
+    const uint64_t base = offsets[0];
+    const uint64_t total = offsets[n] - base;
+    int64_t ns = 1;
+    if (ctx->slice_bytes > 0) {
+        ns = (int64_t)((total + (uint64_t)ctx->slice_bytes - 1) / (uint64_t)ctx->slice_bytes);
+    } else if (ctx->slice_bytes == 0 && total >= 2 * (uint64_t)kSliceMin) {
+        // Each launch ends with its longest group, pipelined over a
+        // workgroup's four waves; a slice pays for that tail only if the
+        // device's other waves have as much work: about cus x 256 residues
+        // per residue of the longest target (8 resident waves per CU, 128
+        // targets per group).
+        uint64_t maxlen = 0;
+        for (int64_t t = 0; t < n; ++t) maxlen = std::max<uint64_t>(maxlen, offsets[t + 1] - offsets[t]);
+        const uint64_t need = std::max<uint64_t>((uint64_t)kSliceMin, (uint64_t)ctx->cus * 256 * maxlen);
+        ns = (int64_t)(total / need);
+    }
+    std::vector<int64_t> cut{0};
+    if (ns > 1) {
+        for (int64_t s = 1; s < ns; ++s) {
+            const uint64_t want = base + total / (uint64_t)ns * (uint64_t)s;
+            const int64_t t = (int64_t)(std::lower_bound(offsets, offsets + n + 1, want) - offsets);
+            if (t > cut.back() && t < n) cut.push_back(t);
+        }
+    }
+    cut.push_back(n);
+    // Slice s+1 is loaded (its upload on the copy stream, its build queued
+    // behind slice s's scoring) while slice s is scored; then slice s's
+    // results are fetched and slice s+1 is launched. No graphs: every
+    // slice is a new database.
+    float kernel_ms = 0.f;
+    int32_t launches = 0;
+    ish_hip_db *cur = nullptr;
+    int64_t cur_t0 = 0;
+    int32_t rc = ISH_OK;
+    auto finish = [&](ish_hip_db *db, int64_t t0) {
+        int32_t r = ish_hip_db_fetch(ctx, db, score + t0, end_ref + t0, end_q + t0,
+                                     status ? status + t0 : nullptr);
+        kernel_ms += ctx->last_ms;
+        launches += ctx->launches;
+        ish_hip_db_free(db);
+        return r;
+    };
+    for (size_t s = 0; s + 1 < cut.size() && rc == ISH_OK; ++s) {
+        const int64_t t0 = cut[s], t1 = cut[s + 1];
+        ish_hip_db *db = nullptr;
+        rc = ish_hip_db_load(ctx, residues, offsets + t0, t1 - t0, &db);
+        if (cur) {
+            const int32_t r = finish(cur, cur_t0);
+            if (rc == ISH_OK) rc = r;
+            cur = nullptr;
+        }
+        if (rc != ISH_OK) {
+            if (db) ish_hip_db_free(db);
+            break;
+        }
+        rc = ish::db_score_once_async(ctx, db);
+        if (rc != ISH_OK) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ish_hip_db_free(db);
+            break;
+        }
+        cur = db;
+        cur_t0 = t0;
+    }
+    if (cur) {
+        const int32_t r = finish(cur, cur_t0);
+        if (rc == ISH_OK) rc = r;
+    }
+    ctx->last_ms = kernel_ms;
+    ctx->launches = launches;
     return rc;
+}
+
+extern "C" int32_t ish_hip_set_slice_bytes(ish_hip_ctx *ctx, int64_t bytes) {
+    if (!ctx) return ISH_E_INVALID;
+    ctx->slice_bytes = bytes;
+    return ISH_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1601,7 +1771,7 @@ extern "C" int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db, const i
     uint8_t *d_raw = nullptr;
     uint64_t *d_off = nullptr;
     int64_t *d_sel = nullptr;
-    hipError_t e = hipMalloc((void **)&d_raw, std::max<uint64_t>(nres, 1));
+    hipError_t e = hipMalloc((void **)&d_raw, nres + kRawPad);
     if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (size_t)(n_sel + 1));
     if (e == hipSuccess) e = hipMalloc((void **)&d_sel, sizeof(int64_t) * (size_t)n_sel);
     if (e == hipSuccess)
@@ -1618,7 +1788,9 @@ extern "C" int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db, const i
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "reverse gather: %s", hipGetErrorString(e));
     }
     ish_hip_db *rdb = nullptr;
-    int32_t rc = db_build(ctx, d_raw, d_off, std::move(off), &rdb);
+    SortPlan sp;
+    plan_sort(off, sp);
+    int32_t rc = db_build(ctx, d_raw, d_off, std::move(off), sp, &rdb);
     if (rc) return rc;
     // free ends swap for the reverse pass (striped.mojo:165-183)
     const uint32_t f = ctx->flags;

@@ -22,4 +22,7 @@ int32_t db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, size_t n
                              const uint64_t *offsets, int64_t n, int32_t matrix_kind,
                              ish_hip_db **out);
 
+// ish_hip_align_db_async without graph capture, for databases scored once.
+int32_t db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db);
+
 } // namespace ish

@@ -646,13 +646,71 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
 
     Writer out(out_fd);
     rc = ISH_OK;
+    // Batch k+1 is loaded (its upload on the copy stream during batch k's
+    // scoring, its build after it) before batch k's results are fetched;
+    // batch k+1 is then launched, and batch k's matches are written while
+    // it is scored.
+    std::unique_ptr<Batch> cur; // the batch whose database is being scored
+    ish_hip_db *cur_db = nullptr;
+    std::vector<int32_t> score, er, eq;
+    std::vector<uint8_t> emit;
+    // fetches cur's results and frees its database; false when no matches
+    // are to be written
+    auto fetch = [&]() {
+        if (!cur_db) return false;
+        int64_t n = 0;
+        (void)ish_hip_db_size(cur_db, &n, nullptr);
+        score.resize((size_t)n);
+        er.resize((size_t)n);
+        eq.resize((size_t)n);
+        emit.resize((size_t)n);
+        const double tg = now_s();
+        int32_t r = ish_hip_db_fetch(ctx, cur_db, score.data(), er.data(), eq.data(), nullptr);
+        ish_hip_db_free(cur_db);
+        cur_db = nullptr;
+        if (r == ISH_OK)
+            r = ish_hip_emit(cfg->algo, score.data(), er.data(), eq.data(), n, max_score,
+                             cfg->threshold, emit.data());
+        st.seconds_gpu += now_s() - tg;
+        if (r != ISH_OK && rc == ISH_OK) rc = r;
+        return r == ISH_OK;
+    };
+    auto write = [&](const Batch &b) {
+        const double tw = now_s();
+        std::vector<uint8_t> tmp;
+        size_t k = 0;
+        for (const Batch::Part &pt : b.parts) {
+            const Recs &r = pt.seg->recs;
+            for (size_t x = pt.first; x < pt.end; ++x, ++k) {
+                if (!emit[k]) continue;
+                st.matches++;
+                const uint64_t a = r.off[x], e = r.off[x + 1];
+                tmp.resize(e - a);
+                for (uint64_t y = a; y < e; ++y) tmp[y - a] = dec[enc[r.res[y]]];
+                if (cfg->record_type == ISH_RECORD_LINE) {
+                    out.put(tmp.data(), tmp.size());
+                    out.put("\n");
+                } else {
+                    out.put(">");
+                    out.put(r.names.data() + r.noff[x], r.noff[x + 1] - r.noff[x]);
+                    out.put("\n");
+                    out.put(tmp.data(), tmp.size());
+                    if (fastq) {
+                        out.put("\n+\n");
+                        out.put(r.quals.data() + r.qoff[x], r.qoff[x + 1] - r.qoff[x]);
+                    }
+                    out.put("\n");
+                }
+            }
+        }
+        st.seconds_write += now_s() - tw;
+    };
     for (;;) {
         std::unique_ptr<Batch> b = queue.pop();
         const int64_t n = (int64_t)b->records();
+        ish_hip_db *db = nullptr;
         if (rc == ISH_OK && n > 0) {
             const double tg = now_s();
-            std::vector<int32_t> score((size_t)n), er((size_t)n), eq((size_t)n);
-            std::vector<uint8_t> emit((size_t)n);
             // the batch's record bytes, piece by piece, and their offsets
             std::vector<ish::HostPiece> pieces;
             std::vector<uint64_t> off((size_t)n + 1, 0);
@@ -665,54 +723,33 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
                 pieces.push_back({r.res.data() + a, r.off[pt.end] - a});
                 acc += r.off[pt.end] - a;
             }
-            ish_hip_db *db = nullptr;
             rc = ish::db_load_ascii_pieces(ctx, pieces.data(), pieces.size(), off.data(), n,
                                            cfg->matrix_kind, &db);
-            if (rc == ISH_OK) rc = ish_hip_align_db(ctx, db, score.data(), er.data(), eq.data(), nullptr);
-            if (db) ish_hip_db_free(db);
-            if (rc == ISH_OK)
-                rc = ish_hip_emit(cfg->algo, score.data(), er.data(), eq.data(), n, max_score,
-                                  cfg->threshold, emit.data());
-            const double tw = now_s();
-            st.seconds_gpu += tw - tg;
-            if (rc == ISH_OK) {
-                std::vector<uint8_t> tmp;
-                k = 0;
-                for (const Batch::Part &pt : b->parts) {
-                    const Recs &r = pt.seg->recs;
-                    for (size_t x = pt.first; x < pt.end; ++x, ++k) {
-                        if (!emit[k]) continue;
-                        st.matches++;
-                        const uint64_t a = r.off[x], e = r.off[x + 1];
-                        tmp.resize(e - a);
-                        for (uint64_t y = a; y < e; ++y) tmp[y - a] = dec[enc[r.res[y]]];
-                        if (cfg->record_type == ISH_RECORD_LINE) {
-                            out.put(tmp.data(), tmp.size());
-                            out.put("\n");
-                        } else {
-                            out.put(">");
-                            out.put(r.names.data() + r.noff[x], r.noff[x + 1] - r.noff[x]);
-                            out.put("\n");
-                            out.put(tmp.data(), tmp.size());
-                            if (fastq) {
-                                out.put("\n+\n");
-                                out.put(r.quals.data() + r.qoff[x], r.qoff[x + 1] - r.qoff[x]);
-                            }
-                            out.put("\n");
-                        }
-                    }
-                }
-            }
-            st.seconds_write += now_s() - tw;
+            st.seconds_gpu += now_s() - tg;
             st.records += n;
             st.residues += (int64_t)acc;
             st.batches++;
         }
+        const bool have = fetch();
+        if (db && rc == ISH_OK) {
+            const double tg = now_s();
+            rc = ish::db_score_once_async(ctx, db);
+            st.seconds_gpu += now_s() - tg;
+        }
+        if (rc != ISH_OK && db) {
+            (void)ish_hip_sync(ctx);
+            ish_hip_db_free(db);
+            db = nullptr;
+        }
+        if (have && rc == ISH_OK) write(*cur);
+        cur = std::move(b);
+        cur_db = db;
         // like the reference's runners, a reader error ends the file
         // without failing the search; its code is reported in the stats
-        if (b->error) st.reader_status = b->error;
-        if (b->last) break;
+        if (cur->error) st.reader_status = cur->error;
+        if (cur->last) break;
     }
+    if (fetch() && rc == ISH_OK) write(*cur);
     parser.join();
     out.flush();
     if (out.failed && rc == ISH_OK) rc = ISH_E_INVALID;

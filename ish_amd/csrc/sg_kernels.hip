@@ -1035,35 +1035,82 @@ __global__ void __launch_bounds__(256) sg_plain(PlainParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// Group packing: sorted targets -> coalesced uint2 blocks. One thread per
-// (group, block, lane); a thread gathers 4 residues of each of its two
-// targets (zero past the target's end).
+// Group packing: sorted targets -> coalesced uint2 blocks. Block b of group g
+// (columns 4b .. 4b+3) is 64 uint2 words: lane l holds the four residues of
+// target slot first+l in .x and of slot first+64+l in .y, zero past a
+// target's end and in empty slots.
+//
+// The blocks of all groups form one sequence (group g starts at block
+// res_off/64). Each wave packs kPackSpan consecutive blocks of it, four per
+// step: a lane reads five aligned dwords of each of its two targets, shifts
+// them into place (v_alignbyte) and writes four words, so the 512-byte rows
+// are written whole and a lane walks its targets front to back. The last
+// dword read can lie up to 3 bytes past the buffer's last residue; callers
+// pad raw by kRawPad bytes.
 // ---------------------------------------------------------------------------
+constexpr int kPackSpan = 64;
+
 __global__ void __launch_bounds__(256) pack_groups(const uint8_t *raw, const uint64_t *off,
                                                    const int32_t *sorted_idx,
                                                    const GroupDesc *groups, int32_t ngroups,
-                                                   int64_t n, uint2 *res) {
-    const int g = blockIdx.y;
-    if (g >= ngroups) return;
-    const GroupDesc gd = groups[g];
-    const int nblk = gd.ncols >> 2;
+                                                   int64_t n, uint64_t total_blocks, uint2 *res) {
     const int lane = threadIdx.x & 63;
-    for (int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); b < nblk;
-         b += gridDim.x * (blockDim.x >> 6)) {
-        uint32_t word[2];
+    const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint64_t gb = wave * kPackSpan;
+    if (gb >= total_blocks) return;
+    const uint64_t gend = gb + kPackSpan < total_blocks ? gb + kPackSpan : total_blocks;
+    // the group holding block gb: the last one starting at or before it
+    // (groups without blocks start where the next one does)
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (groups[mid].res_off / 64 <= gb) lo = mid;
+        else hi = mid - 1;
+    }
+    for (int g = lo; g < ngroups && gb < gend; ++g) {
+        const GroupDesc gd = groups[g];
+        const uint64_t gstart = gd.res_off / 64;
+        const uint64_t stop = min(gstart + (uint64_t)(gd.ncols >> 2), gend);
+        if (gb >= stop) continue;
+        uint64_t a[2], e[2]; // this lane's two targets: first byte, end
+#pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int64_t si = (int64_t)gd.first + h * 64 + lane;
-            uint32_t wv = 0;
+            a[h] = e[h] = 0;
             if (si < n) {
                 const int32_t t = sorted_idx[si];
-                const uint64_t o0 = off[t], o1 = off[t + 1];
-                const uint64_t c0 = o0 + (uint64_t)b * 4;
-                for (int k = 0; k < 4; ++k)
-                    if (c0 + k < o1) wv |= (uint32_t)raw[c0 + k] << (8 * k);
+                a[h] = off[t];
+                e[h] = off[t + 1];
             }
-            word[h] = wv;
         }
-        res[gd.res_off + (size_t)b * 64 + lane] = make_uint2(word[0], word[1]);
+        while (gb < stop) {
+            const uint64_t lb = gb - gstart; // block inside the group
+            const int u = stop - gb < 4 ? (int)(stop - gb) : 4;
+            uint32_t w[2][4];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint64_t s0 = a[h] + lb * 4; // first byte of block lb
+                const uint64_t base = s0 & ~(uint64_t)3;
+                const uint32_t sh = (uint32_t)(s0 & 3);
+                uint32_t d[5];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const uint64_t q = base + 4 * k;
+                    d[k] = q < e[h] ? *(const uint32_t *)(raw + q) : 0u;
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t v = __builtin_amdgcn_alignbyte(d[j + 1], d[j], sh);
+                    const int64_t left = (int64_t)e[h] - (int64_t)(s0 + 4 * j); // bytes of the target left
+                    w[h][j] = left >= 4 ? v : left <= 0 ? 0u : v & ((1u << (8 * (uint32_t)left)) - 1u);
+                }
+            }
+            uint2 *dst = res + gd.res_off + lb * 64 + lane;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j < u) dst[(size_t)j * 64] = make_uint2(w[0][j], w[1][j]);
+            gb += (uint64_t)u;
+        }
     }
 }
 
@@ -1217,17 +1264,15 @@ hipError_t launch_sg_plain(const PlainParams &p, int grid, int block, size_t lds
 }
 
 hipError_t launch_pack(const uint8_t *raw, const uint64_t *off, const int32_t *sorted_idx,
-                       const GroupDesc *groups, int32_t ngroups, int64_t n, uint2 *res,
-                       hipStream_t st) {
-    // grid.y = groups (<= 65535 per launch slice), grid.x covers blocks
-    for (int32_t g0 = 0; g0 < ngroups; g0 += 65535) {
-        const int32_t ng = ngroups - g0 < 65535 ? ngroups - g0 : 65535;
-        hipLaunchKernelGGL(pack_groups, dim3(8, ng), dim3(256), 0, st, raw, off, sorted_idx,
-                           groups + g0, ng, n, res);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+                       const GroupDesc *groups, int32_t ngroups, int64_t n, uint64_t total_blocks,
+                       uint2 *res, hipStream_t st) {
+    if (ngroups <= 0 || total_blocks == 0) return hipSuccess;
+    const uint64_t waves = (total_blocks + kPackSpan - 1) / kPackSpan;
+    const uint64_t grid = (waves + 3) / 4;
+    if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pack_groups, dim3((unsigned)grid), dim3(256), 0, st, raw, off, sorted_idx,
+                       groups, ngroups, n, total_blocks, res);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset(uint32_t *counter, hipStream_t st) {

@@ -207,6 +207,39 @@ def test_protein_workload(eng, engine, tile):
         check(eng, engine, w.query, w.targets, "blosum62", 3, 1, O.flags_from_str(flags), "wl2", tile)
 
 
+@pytest.mark.parametrize("slice_bytes", [1, 40_000, 1_000_000])
+def test_sliced_batches(eng, slice_bytes):
+    """A batch cut into input-order slices (each uploaded while the one
+    before is scored) gives the unsliced results, semi-global and local,
+    with empty targets at slice edges."""
+    w = workloads.generate(2, 3000 if slice_bytes > 1 else 40, seed=9)
+    seqs = [w.targets.residues[w.targets.offsets[t]:w.targets.offsets[t + 1]] for t in range(w.targets.n)]
+    for t in range(0, len(seqs), 7):
+        seqs[t] = seqs[t][:0]
+    targets = Targets.from_list(seqs)
+    try:
+        for algo, flags in (("striped-semi-global", "FFTT"), ("striped-semi-global", "TTTT"),
+                            ("striped-local", "FFFF")):
+            eng.set_engine("batched")
+            eng.set_tile(32)
+            eng.set_query(w.query, "blosum62", 3, 1, flags, algo)
+            eng.set_slice_bytes(-1)
+            whole = eng.align(targets)
+            eng.set_slice_bytes(slice_bytes)
+            sliced = eng.align(targets)
+            for a, b in zip(whole, sliced):
+                assert np.array_equal(a, b), (algo, flags)
+            m = O.Matrix("blosum62")
+            if algo == "striped-local":
+                ref = O.batch(1, w.query, targets.residues, targets.offsets, m, 3, 1, 0)
+            else:
+                ref = O.batch(0, w.query, targets.residues, targets.offsets, m, 3, 1, O.flags_from_str(flags))
+            for a, b in zip(sliced[:3], ref[:3]):
+                assert np.array_equal(a, b), (algo, flags)
+    finally:
+        eng.set_slice_bytes(0)
+
+
 def test_database_reuse_across_queries(eng):
     w = workloads.generate(2, 5000, seed=8)
     eng.set_engine("batched")
