@@ -89,11 +89,25 @@ int32_t neg_limit_of(int32_t go, int32_t mn);
 hipError_t persistent_grid(DevBuf &scratch, int occ_blocks, int cus, int64_t cols, int *grid,
                            int64_t *slot_stride);
 
+// Device buffers of a database, kept by a context and reused (grow-only)
+// by the databases of one-shot passes, so that those allocate nothing once
+// warm: hipFree waits for every stream of the device, which would stall an
+// upload overlapping a pass.
+struct DbBuffers {
+    DevBuf groups, res, lens, orig, score, er, eq, st, raw, off;
+    void release() {
+        for (DevBuf *b : {&groups, &res, &lens, &orig, &score, &er, &eq, &st, &raw, &off}) b->release();
+    }
+};
+
 } // namespace ish
 
 // Device-resident target database (ish_hip_db of the C ABI).
 struct ish_hip_db {
     int device = 0;
+    // buffers borrowed from a context (one-shot passes): not freed with
+    // the database
+    ish::DbBuffers *borrowed = nullptr;
     int64_t n = 0, nres = 0;
     int32_t ngroups = 0, maxcols = 0;
     std::vector<int32_t> group_maxlen; // per group, descending

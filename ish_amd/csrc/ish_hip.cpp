@@ -81,14 +81,21 @@ struct ish_hip_ctx {
     // of its last copy to the device
     uint8_t *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_done[2] = {nullptr, nullptr};
-    // uploads run on a stream of their own, so that one database's upload
-    // overlaps the scoring already queued on `stream`; copy_done marks the
-    // end of the last upload, and the database build waits for it
+    // database uploads and builds run on a stream of their own, beside the
+    // scoring queued on `stream`; one-shot passes record pass_done[slot]
+    // after their launches, and their results are fetched on `copy` once
+    // it fires; span brackets a sliced batch's launches
     hipStream_t copy = nullptr;
-    hipEvent_t copy_done = nullptr;
+    hipEvent_t pass_done[2] = {nullptr, nullptr};
+    hipEvent_t span[2] = {nullptr, nullptr};
     // pinned landing buffer of fetched results (grow-only)
     uint8_t *res_host = nullptr;
     size_t res_host_bytes = 0;
+    // buffers of one-shot databases (sliced batches, file batches): slot k
+    // serves every other database, so one is loaded while the other's
+    // pass runs; `mask` receives a build's residue-code mask
+    ish::DbBuffers once[2];
+    DevBuf mask;
     // one-shot batches are uploaded and scored in slices, each slice's
     // upload overlapping the scoring of the slice before: 0 sizes them
     // automatically, > 0 is a fixed size in residues, < 0 never slices
@@ -234,8 +241,15 @@ extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
         delete c;
         return ISH_E_HIP;
     }
-    if (hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->copy_done, hipEventDisableTiming) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) == hipSuccess;
+    for (int k = 0; k < 2 && ok; ++k)
+        ok = hipEventCreateWithFlags(&c->pass_done[k], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreate(&c->span[k]) == hipSuccess;
+    if (!ok) {
+        for (int k = 0; k < 2; ++k) {
+            if (c->pass_done[k]) (void)hipEventDestroy(c->pass_done[k]);
+            if (c->span[k]) (void)hipEventDestroy(c->span[k]);
+        }
         if (c->copy) (void)hipStreamDestroy(c->copy);
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -257,13 +271,18 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     }
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->once[0].release();
+    ctx->once[1].release();
     for (DevBuf *b : {&ctx->prof_t, &ctx->prof_rt, &ctx->prof_p, &ctx->lut, &ctx->pprof, &ctx->plut,
-                      &ctx->scratch, &ctx->counter,
+                      &ctx->scratch, &ctx->counter, &ctx->mask,
                       &ctx->in_res, &ctx->in_off, &ctx->out_score, &ctx->out_er, &ctx->out_eq,
                       &ctx->out_st})
         b->release();
     if (ctx->res_host) (void)hipHostFree(ctx->res_host);
-    (void)hipEventDestroy(ctx->copy_done);
+    for (int k = 0; k < 2; ++k) {
+        (void)hipEventDestroy(ctx->pass_done[k]);
+        (void)hipEventDestroy(ctx->span[k]);
+    }
     (void)hipStreamDestroy(ctx->copy);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -632,13 +651,25 @@ static void plan_sort(const std::vector<uint64_t> &h_off, SortPlan &sp) {
 }
 
 // Builds the packed, length-sorted layout from residues already on the
-// device (d_raw, offsets d_off rebased to 0) and the sort plan of h_off.
-// Takes ownership of both buffers, which the database keeps for
-// start-coordinate passes.
-static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
-                        std::vector<uint64_t> h_off, SortPlan &sp, ish_hip_db **out) {
+// device (d_raw, offsets d_off rebased to 0) and the sort plan of h_off,
+// on stream st, and returns when it is done. Takes ownership of both
+// buffers, which the database keeps for start-coordinate passes.
+// A device buffer of `bytes`: from the reused slot b when given, else
+// allocated.
+template <typename T>
+static hipError_t db_alloc(ish::DevBuf *b, T **p, size_t bytes) {
+    if (!b) return hipMalloc((void **)p, bytes);
+    const hipError_t e = b->ensure(bytes);
+    *p = e == hipSuccess ? (T *)b->p : nullptr;
+    return e;
+}
+
+static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64_t *d_off,
+                        std::vector<uint64_t> h_off, SortPlan &sp, ish::DbBuffers *reuse,
+                        ish_hip_db **out) {
     const int64_t n = (int64_t)h_off.size() - 1;
     ish_hip_db *db = new ish_hip_db();
+    db->borrowed = reuse;
     db->device = ctx->device;
     db->n = n;
     db->nres = n > 0 ? (int64_t)h_off[(size_t)n] : 0;
@@ -656,8 +687,7 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     const uint64_t res_words = off + 64; // one spare block: empty groups read block 0
     uint32_t *d_mask = nullptr;
     auto cleanup = [&](int32_t code) {
-        (void)hipStreamSynchronize(ctx->stream);
-        if (d_mask) (void)hipFree(d_mask);
+        (void)hipStreamSynchronize(st);
         ish_hip_db_free(db);
         return code;
     };
@@ -666,30 +696,29 @@ static int32_t db_build(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off,
     if ((e = (expr)) != hipSuccess)                                                           \
         return cleanup(fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "%s: %s", \
                             #expr, hipGetErrorString(e)));
-    DB_TRY(hipMalloc((void **)&db->d_groups, sizeof(GroupDesc) * std::max(ng, 1)));
-    DB_TRY(hipMalloc((void **)&db->d_res, sizeof(uint2) * res_words));
-    DB_TRY(hipMalloc((void **)&db->d_lens, sizeof(int32_t) * std::max<int64_t>(nslots, 1)));
-    DB_TRY(hipMalloc((void **)&db->d_orig, sizeof(int32_t) * std::max<int64_t>(nslots, 1)));
-    DB_TRY(hipMalloc((void **)&db->d_score, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-    DB_TRY(hipMalloc((void **)&db->d_end_ref, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-    DB_TRY(hipMalloc((void **)&db->d_end_q, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-    DB_TRY(hipMalloc((void **)&db->d_status, std::max<int64_t>(n, 1)));
+    DB_TRY(db_alloc(reuse ? &reuse->groups : nullptr, &db->d_groups, sizeof(GroupDesc) * std::max(ng, 1)));
+    DB_TRY(db_alloc(reuse ? &reuse->res : nullptr, &db->d_res, sizeof(uint2) * res_words));
+    DB_TRY(db_alloc(reuse ? &reuse->lens : nullptr, &db->d_lens, sizeof(int32_t) * std::max<int64_t>(nslots, 1)));
+    DB_TRY(db_alloc(reuse ? &reuse->orig : nullptr, &db->d_orig, sizeof(int32_t) * std::max<int64_t>(nslots, 1)));
+    DB_TRY(db_alloc(reuse ? &reuse->score : nullptr, &db->d_score, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    DB_TRY(db_alloc(reuse ? &reuse->er : nullptr, &db->d_end_ref, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    DB_TRY(db_alloc(reuse ? &reuse->eq : nullptr, &db->d_end_q, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    DB_TRY(db_alloc(reuse ? &reuse->st : nullptr, &db->d_status, std::max<int64_t>(n, 1)));
     // the packer writes every block of every group; the spare block is zero
-    DB_TRY(hipMemsetAsync(db->d_res + off, 0, sizeof(uint2) * 64, ctx->stream));
+    DB_TRY(hipMemsetAsync(db->d_res + off, 0, sizeof(uint2) * 64, st));
     // residue codes present (paired profiles are built over their classes)
-    DB_TRY(hipMalloc((void **)&d_mask, 8 * sizeof(uint32_t)));
-    DB_TRY(launch_code_mask(db->d_raw, (uint64_t)db->nres, d_mask, ctx->stream));
-    DB_TRY(hipMemcpyAsync(db->code_mask, d_mask, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    DB_TRY(db_alloc(&ctx->mask, &d_mask, 8 * sizeof(uint32_t)));
+    DB_TRY(launch_code_mask(db->d_raw, (uint64_t)db->nres, d_mask, st));
+    DB_TRY(hipMemcpyAsync(db->code_mask, d_mask, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (ng > 0) {
-        DB_TRY(hipMemcpyAsync(db->d_groups, groups.data(), sizeof(GroupDesc) * ng, hipMemcpyHostToDevice, ctx->stream));
-        DB_TRY(hipMemcpyAsync(db->d_lens, lens.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
-        DB_TRY(hipMemcpyAsync(db->d_orig, sorted.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, ctx->stream));
+        DB_TRY(hipMemcpyAsync(db->d_groups, groups.data(), sizeof(GroupDesc) * ng, hipMemcpyHostToDevice, st));
+        DB_TRY(hipMemcpyAsync(db->d_lens, lens.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, st));
+        DB_TRY(hipMemcpyAsync(db->d_orig, sorted.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, st));
         DB_TRY(launch_pack(db->d_raw, db->d_off, db->d_orig, db->d_groups, ng, n, off / 64, db->d_res,
-                           ctx->stream));
+                           st));
     }
-    DB_TRY(hipStreamSynchronize(ctx->stream));
+    DB_TRY(hipStreamSynchronize(st));
 #undef DB_TRY
-    (void)hipFree(d_mask);
     *out = db;
     return ISH_OK;
 }
@@ -721,6 +750,9 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
     uint64_t at = 0;
     size_t pk = 0;
     uint64_t in_piece = 0; // bytes of pieces[pk] already staged
+    // parts of 16, 32, then 64 MB: the first copy to the device starts
+    // after a short host copy
+    size_t part = kStageBytes / 4;
     while (pk < np) {
         // fill one staging buffer with up to kStageBytes of the pieces
         hipError_t e = hipEventSynchronize(ctx->stage_done[buf]);
@@ -731,9 +763,9 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
         };
         std::vector<Span> spans;
         size_t fill = 0;
-        while (pk < np && fill < kStageBytes) {
+        while (pk < np && fill < part) {
             const uint64_t left = pieces[pk].n - in_piece;
-            const size_t take = (size_t)std::min<uint64_t>(left, kStageBytes - fill);
+            const size_t take = (size_t)std::min<uint64_t>(left, part - fill);
             if (take) spans.push_back({pieces[pk].p + in_piece, fill, take});
             fill += take;
             in_piece += take;
@@ -763,74 +795,106 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
         if (e != hipSuccess) return e;
         at += fill;
         buf ^= 1;
+        part = std::min(kStageBytes, part * 2);
     }
     return hipSuccess;
 }
 
 // Uploads the residues (the concatenation of pieces) and builds the db;
 // with `encode` the bytes are translated to residue codes on the device
-// first. h_off: offsets from 0, already checked. The upload runs on the
-// copy stream, so it overlaps a scoring pass already queued on the
-// context's stream; the build is queued behind that pass and behind the
-// upload (copy_done), and returns when it is done.
+// first. offsets: n + 1 entries, already checked; the pieces hold
+// offsets[n] - offsets[0] bytes. Upload and build run on the copy stream,
+// beside any pass queued on the context's stream, and the call returns
+// when the database is complete.
 static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t np,
-                         std::vector<uint64_t> h_off, const EncodeTable *encode, ish_hip_db **out) {
-    const size_t n = h_off.size() - 1;
-    const uint64_t nres = h_off[n];
+                         const uint64_t *offsets, int64_t nt, const EncodeTable *encode,
+                         ish::DbBuffers *reuse, ish_hip_db **out) {
+    const size_t n = (size_t)nt;
+    const uint64_t nres = n > 0 ? offsets[n] - offsets[0] : 0;
     uint8_t *d_raw = nullptr;
     uint64_t *d_off = nullptr;
+    // the offsets rebased to 0 and the length sort, beside the upload
+    std::vector<uint64_t> h_off;
     SortPlan sp;
-    std::thread planner([&] { plan_sort(h_off, sp); }); // beside the upload
-    hipError_t e = hipMalloc((void **)&d_raw, nres + kRawPad);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_off, sizeof(uint64_t) * (n + 1));
+    std::thread planner([&] {
+        h_off.resize(n + 1);
+        const uint64_t base = n > 0 ? offsets[0] : 0;
+        h_off[0] = 0;
+        for (size_t t = 1; t <= n; ++t) h_off[t] = offsets[t] - base;
+        plan_sort(h_off, sp);
+    });
+    hipError_t e = db_alloc(reuse ? &reuse->raw : nullptr, &d_raw, nres + kRawPad);
+    if (e == hipSuccess) e = db_alloc(reuse ? &reuse->off : nullptr, &d_off, sizeof(uint64_t) * (n + 1));
     if (e == hipSuccess) e = staged_h2d(ctx, d_raw, pieces, np);
     planner.join();
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->copy);
-    if (e == hipSuccess) e = hipEventRecord(ctx->copy_done, ctx->copy);
-    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->copy_done, 0);
-    if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, ctx->stream);
+    if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, ctx->copy);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(ctx->copy);
-        (void)hipStreamSynchronize(ctx->stream);
-        if (d_raw) (void)hipFree(d_raw);
-        if (d_off) (void)hipFree(d_off);
+        if (!reuse && d_raw) (void)hipFree(d_raw);
+        if (!reuse && d_off) (void)hipFree(d_off);
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "db upload: %s", hipGetErrorString(e));
     }
-    return db_build(ctx, d_raw, d_off, std::move(h_off), sp, out);
+    return db_build(ctx, ctx->copy, d_raw, d_off, std::move(h_off), sp, reuse, out);
 }
 
 // Checks the offsets (non-decreasing, targets within the int32 limit) and
-// returns them rebased to 0 in h_off.
+// returns the longest target in *maxlen; large arrays are split over up to
+// 8 host threads.
 static int32_t db_offsets(ish_hip_ctx *ctx, const uint64_t *offsets, int64_t n,
-                          std::vector<uint64_t> &h_off) {
+                          uint64_t *maxlen = nullptr) {
     if (n > (int64_t)INT32_MAX - kGroup) return fail(ctx, ISH_E_RANGE, "too many targets");
-    const uint64_t base = n > 0 ? offsets[0] : 0;
-    h_off.assign((size_t)n + 1, 0);
-    for (int64_t t = 0; t < n; ++t) {
-        if (offsets[t + 1] < offsets[t]) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
-        if (offsets[t + 1] - offsets[t] > (uint64_t)INT32_MAX / 2) return fail(ctx, ISH_E_RANGE, "target too long");
-        h_off[(size_t)t + 1] = offsets[t + 1] - base;
+    struct Part {
+        uint64_t maxlen = 0;
+        int bad = 0; // 1: decreasing, 2: too long
+    };
+    auto scan = [offsets](int64_t lo, int64_t hi, Part &r) {
+        uint64_t m = 0;
+        bool dec = false;
+        for (int64_t t = lo; t < hi; ++t) {
+            dec |= offsets[t + 1] < offsets[t];
+            m = std::max<uint64_t>(m, offsets[t + 1] - offsets[t]);
+        }
+        r.bad = dec ? 1 : m > (uint64_t)INT32_MAX / 2 ? 2 : 0;
+        r.maxlen = m;
+    };
+    const int nt = n >= ((int64_t)1 << 20) ? (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency())) : 1;
+    std::vector<Part> parts((size_t)nt);
+    std::vector<std::thread> pool;
+    const int64_t per = (n + nt - 1) / nt;
+    for (int k = 1; k < nt; ++k)
+        pool.emplace_back(scan, std::min(n, k * per), std::min(n, (k + 1) * per), std::ref(parts[(size_t)k]));
+    scan(0, std::min(n, per), parts[0]);
+    for (auto &t : pool) t.join();
+    uint64_t m = 0;
+    for (const Part &r : parts) {
+        if (r.bad == 1) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
+        if (r.bad == 2) return fail(ctx, ISH_E_RANGE, "target too long");
+        m = std::max(m, r.maxlen);
     }
+    if (maxlen) *maxlen = m;
     return ISH_OK;
 }
 
+// slot >= 0: the database borrows the context's one-shot buffers
+// once[slot]; checked: the offsets were checked by the caller
 static int32_t db_load_impl(ish_hip_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
-                            int64_t n, const EncodeTable *encode, ish_hip_db **out) {
+                            int64_t n, const EncodeTable *encode, ish_hip_db **out, int slot = -1,
+                            bool checked = false) {
     if (!ctx || !out || n < 0 || (n > 0 && (!offsets || !residues)))
         return fail(ctx, ISH_E_INVALID, "bad db arguments");
     *out = nullptr;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    std::vector<uint64_t> h_off;
-    const int32_t rc = db_offsets(ctx, offsets, n, h_off);
+    const int32_t rc = checked ? ISH_OK : db_offsets(ctx, offsets, n);
     if (rc) return rc;
-    const ish::HostPiece one{n > 0 ? residues + offsets[0] : residues, h_off.back()};
-    return db_upload(ctx, &one, 1, std::move(h_off), encode, out);
+    const ish::HostPiece one{n > 0 ? residues + offsets[0] : residues, n > 0 ? offsets[n] - offsets[0] : 0};
+    return db_upload(ctx, &one, 1, offsets, n, encode, slot >= 0 ? &ctx->once[slot & 1] : nullptr, out);
 }
 
 int32_t ish::db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, size_t np,
                                   const uint64_t *offsets, int64_t n, int32_t matrix_kind,
-                                  ish_hip_db **out) {
+                                  ish_hip_db **out, int slot) {
     if (!ctx || !out || n < 0 || !offsets || (np > 0 && !pieces) || offsets[0] != 0)
         return fail(ctx, ISH_E_INVALID, "bad db arguments");
     *out = nullptr;
@@ -838,13 +902,12 @@ int32_t ish::db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, siz
     if (ish_hip_encoding(matrix_kind, t.code) != ISH_OK)
         return fail(ctx, ISH_E_INVALID, "unknown matrix kind %d", matrix_kind);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    std::vector<uint64_t> h_off;
-    const int32_t rc = db_offsets(ctx, offsets, n, h_off);
+    const int32_t rc = db_offsets(ctx, offsets, n);
     if (rc) return rc;
     uint64_t total = 0;
     for (size_t k = 0; k < np; ++k) total += pieces[k].n;
-    if (total != h_off.back()) return fail(ctx, ISH_E_INVALID, "pieces do not match the offsets");
-    return db_upload(ctx, pieces, np, std::move(h_off), &t, out);
+    if (total != offsets[n]) return fail(ctx, ISH_E_INVALID, "pieces do not match the offsets");
+    return db_upload(ctx, pieces, np, offsets, n, &t, slot >= 0 ? &ctx->once[slot & 1] : nullptr, out);
 }
 
 extern "C" int32_t ish_hip_db_load(ish_hip_ctx *ctx, const uint8_t *residues,
@@ -863,6 +926,10 @@ extern "C" int32_t ish_hip_db_load_ascii(ish_hip_ctx *ctx, const uint8_t *ascii,
 
 extern "C" void ish_hip_db_free(ish_hip_db *db) {
     if (!db) return;
+    if (db->borrowed) { // the context keeps the buffers
+        delete db;
+        return;
+    }
     (void)hipSetDevice(db->device);
     for (void *p : {(void *)db->d_groups, (void *)db->d_res, (void *)db->d_lens, (void *)db->d_orig,
                     (void *)db->d_score, (void *)db->d_end_ref, (void *)db->d_end_q,
@@ -1252,27 +1319,26 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     return launch_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags);
 }
 
-int32_t ish::db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db) {
+int32_t ish::db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db, int slot) {
     if (!ctx || !db) return ISH_E_INVALID;
     if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
     if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    return launch_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, false);
+    const int32_t rc = launch_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, false);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->pass_done[slot & 1], ctx->stream));
+    return ISH_OK;
 }
 
-extern "C" int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
-                                    int32_t *end_ref, int32_t *end_q, uint8_t *status) {
-    if (!ctx || !db || (db->n > 0 && (!score || !end_ref || !end_q)))
-        return ISH_E_INVALID;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
+// Copies db's results to the host arrays on stream st (pinned landing
+// buffer, then one host thread per array) and returns when they are there.
+static int32_t fetch_results(ish_hip_ctx *ctx, ish_hip_db *db, hipStream_t st, int32_t *score,
+                             int32_t *end_ref, int32_t *end_q, uint8_t *status) {
     const int64_t n = db->n;
     if (n <= 0) {
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        collect_timing(ctx);
+        HIP_TRY(ctx, hipStreamSynchronize(st));
         return ISH_OK;
     }
-    // results land in pinned memory (the DMA rate; pageable copies are
-    // staged in small parts) and are copied out, one thread per array
     const size_t need = (size_t)n * 13;
     if (ctx->res_host_bytes < need) {
         if (ctx->res_host) (void)hipHostFree(ctx->res_host);
@@ -1283,12 +1349,11 @@ extern "C" int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *s
     }
     uint8_t *h = ctx->res_host;
     const size_t w = sizeof(int32_t) * (size_t)n;
-    HIP_TRY(ctx, hipMemcpyAsync(h, db->d_score, w, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(h + w, db->d_end_ref, w, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(h + 2 * w, db->d_end_q, w, hipMemcpyDeviceToHost, ctx->stream));
-    if (status) HIP_TRY(ctx, hipMemcpyAsync(h + 3 * w, db->d_status, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    collect_timing(ctx);
+    HIP_TRY(ctx, hipMemcpyAsync(h, db->d_score, w, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(h + w, db->d_end_ref, w, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(h + 2 * w, db->d_end_q, w, hipMemcpyDeviceToHost, st));
+    if (status) HIP_TRY(ctx, hipMemcpyAsync(h + 3 * w, db->d_status, (size_t)n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
     struct Out {
         void *dst;
         const uint8_t *src;
@@ -1306,6 +1371,24 @@ extern "C" int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *s
         for (auto &t : pool) t.join();
     }
     return ISH_OK;
+}
+
+int32_t ish::db_fetch_after(ish_hip_ctx *ctx, ish_hip_db *db, int slot, int32_t *score,
+                            int32_t *end_ref, int32_t *end_q, uint8_t *status) {
+    if (!ctx || !db || (db->n > 0 && (!score || !end_ref || !end_q))) return ISH_E_INVALID;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->copy, ctx->pass_done[slot & 1], 0));
+    return fetch_results(ctx, db, ctx->copy, score, end_ref, end_q, status);
+}
+
+extern "C" int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
+                                    int32_t *end_ref, int32_t *end_q, uint8_t *status) {
+    if (!ctx || !db || (db->n > 0 && (!score || !end_ref || !end_q)))
+        return ISH_E_INVALID;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int32_t rc = fetch_results(ctx, db, ctx->stream, score, end_ref, end_q, status);
+    collect_timing(ctx);
+    return rc;
 }
 
 extern "C" int32_t ish_hip_align_db(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
@@ -1394,6 +1477,9 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
         return align_plain(ctx, residues, offsets, n, score, end_ref, end_q, status);
     }
     // slices of about equal residues, cut at target boundaries
+    uint64_t maxlen = 0;
+    const int32_t checked = db_offsets(ctx, offsets, n, &maxlen);
+    if (checked) return checked;
     const uint64_t base = offsets[0];
     const uint64_t total = offsets[n] - base;
     int64_t ns = 1;
@@ -1405,8 +1491,6 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
         // device's other waves have as much work: about cus x 256 residues
         // per residue of the longest target (8 resident waves per CU, 128
         // targets per group).
-        uint64_t maxlen = 0;
-        for (int64_t t = 0; t < n; ++t) maxlen = std::max<uint64_t>(maxlen, offsets[t + 1] - offsets[t]);
         const uint64_t need = std::max<uint64_t>((uint64_t)kSliceMin, (uint64_t)ctx->cus * 256 * maxlen);
         ns = (int64_t)(total / need);
     }
@@ -1419,50 +1503,55 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
         }
     }
     cut.push_back(n);
-    // Slice s+1 is loaded (its upload on the copy stream, its build queued
-    // behind slice s's scoring) while slice s is scored; then slice s's
-    // results are fetched and slice s+1 is launched. No graphs: every
-    // slice is a new database.
-    float kernel_ms = 0.f;
-    int32_t launches = 0;
-    ish_hip_db *cur = nullptr;
-    int64_t cur_t0 = 0;
+    // Slice s is uploaded and built on the copy stream while slice s-1 is
+    // scored, then queued behind it; slice s-1's results are fetched on the
+    // copy stream once its pass ends, while slice s scores. The slices use
+    // the context's two one-shot buffer slots in turn, so nothing is
+    // allocated or freed between passes. No graphs: every slice is a new
+    // database.
     int32_t rc = ISH_OK;
-    auto finish = [&](ish_hip_db *db, int64_t t0) {
-        int32_t r = ish_hip_db_fetch(ctx, db, score + t0, end_ref + t0, end_q + t0,
-                                     status ? status + t0 : nullptr);
-        kernel_ms += ctx->last_ms;
-        launches += ctx->launches;
-        ish_hip_db_free(db);
-        return r;
-    };
+    int32_t launches = 0;
+    ish_hip_db *prev = nullptr;
+    int64_t prev_t0 = 0;
     for (size_t s = 0; s + 1 < cut.size() && rc == ISH_OK; ++s) {
         const int64_t t0 = cut[s], t1 = cut[s + 1];
+        const int slot = (int)(s & 1);
         ish_hip_db *db = nullptr;
-        rc = ish_hip_db_load(ctx, residues, offsets + t0, t1 - t0, &db);
-        if (cur) {
-            const int32_t r = finish(cur, cur_t0);
+        rc = db_load_impl(ctx, residues, offsets + t0, t1 - t0, nullptr, &db, slot, true);
+        if (rc == ISH_OK && s == 0) {
+            const hipError_t e = hipEventRecord(ctx->span[0], ctx->stream);
+            if (e != hipSuccess) rc = fail(ctx, ISH_E_HIP, "span event: %s", hipGetErrorString(e));
+        }
+        if (rc == ISH_OK) {
+            rc = ish::db_score_once_async(ctx, db, slot);
+            launches += ctx->launches;
+        }
+        if (prev) {
+            const int32_t r = ish::db_fetch_after(ctx, prev, slot ^ 1, score + prev_t0, end_ref + prev_t0,
+                                                  end_q + prev_t0, status ? status + prev_t0 : nullptr);
             if (rc == ISH_OK) rc = r;
-            cur = nullptr;
+            ish_hip_db_free(prev);
         }
-        if (rc != ISH_OK) {
-            if (db) ish_hip_db_free(db);
-            break;
-        }
-        rc = ish::db_score_once_async(ctx, db);
-        if (rc != ISH_OK) {
-            (void)hipStreamSynchronize(ctx->stream);
-            ish_hip_db_free(db);
-            break;
-        }
-        cur = db;
-        cur_t0 = t0;
+        prev = db;
+        prev_t0 = t0;
     }
-    if (cur) {
-        const int32_t r = finish(cur, cur_t0);
-        if (rc == ISH_OK) rc = r;
+    if (prev && rc == ISH_OK) {
+        const int slot = (int)((cut.size() - 2) & 1);
+        hipError_t e = hipEventRecord(ctx->span[1], ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, ISH_E_HIP, "span event: %s", hipGetErrorString(e));
+        if (rc == ISH_OK)
+            rc = ish::db_fetch_after(ctx, prev, slot, score + prev_t0, end_ref + prev_t0, end_q + prev_t0,
+                                     status ? status + prev_t0 : nullptr);
+        float ms = 0.f;
+        if (rc == ISH_OK && hipEventSynchronize(ctx->span[1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, ctx->span[0], ctx->span[1]) == hipSuccess)
+            ctx->last_ms = ms; // first launch to the end of the last one
     }
-    ctx->last_ms = kernel_ms;
+    if (rc != ISH_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->copy);
+    }
+    if (prev) ish_hip_db_free(prev);
     ctx->launches = launches;
     return rc;
 }
@@ -1790,7 +1879,7 @@ extern "C" int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db, const i
     ish_hip_db *rdb = nullptr;
     SortPlan sp;
     plan_sort(off, sp);
-    int32_t rc = db_build(ctx, d_raw, d_off, std::move(off), sp, &rdb);
+    int32_t rc = db_build(ctx, ctx->stream, d_raw, d_off, std::move(off), sp, nullptr, &rdb);
     if (rc) return rc;
     // free ends swap for the reverse pass (striped.mojo:165-183)
     const uint32_t f = ctx->flags;

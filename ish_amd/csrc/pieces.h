@@ -17,12 +17,21 @@ struct HostPiece {
 
 // ish_hip_db_load_ascii over the concatenation of pieces[0..np): offsets
 // (n + 1 entries, offsets[0] = 0) index that concatenation, whose length
-// offsets[n] must equal the pieces' total.
+// offsets[n] must equal the pieces' total. With slot 0 or 1 the database
+// borrows the context's one-shot buffers of that slot (grow-only, never
+// freed mid-run); it must be done with (scored and fetched) before the
+// slot is loaded again.
 int32_t db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, size_t np,
                              const uint64_t *offsets, int64_t n, int32_t matrix_kind,
-                             ish_hip_db **out);
+                             ish_hip_db **out, int slot = -1);
 
-// ish_hip_align_db_async without graph capture, for databases scored once.
-int32_t db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db);
+// One-shot passes: db_score_once_async enqueues a pass (no graph capture)
+// behind whatever the context's stream holds and marks its end with the
+// event of `slot`; db_fetch_after copies its results on the copy stream
+// once that event fires, so a pass queued after it keeps the device busy
+// meanwhile. Returns when the results are in the host arrays.
+int32_t db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db, int slot);
+int32_t db_fetch_after(ish_hip_ctx *ctx, ish_hip_db *db, int slot, int32_t *score,
+                       int32_t *end_ref, int32_t *end_q, uint8_t *status);
 
 } // namespace ish

@@ -646,15 +646,17 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
 
     Writer out(out_fd);
     rc = ISH_OK;
-    // Batch k+1 is loaded (its upload on the copy stream during batch k's
-    // scoring, its build after it) before batch k's results are fetched;
-    // batch k+1 is then launched, and batch k's matches are written while
-    // it is scored.
+    // Batch k+1 is uploaded and built (copy stream) while batch k is
+    // scored, and queued behind it; batch k's results are then fetched on
+    // the copy stream as its pass ends, and its matches written while
+    // batch k+1 scores. Batches use the context's two one-shot buffer
+    // slots in turn, so nothing is allocated or freed between passes.
     std::unique_ptr<Batch> cur; // the batch whose database is being scored
     ish_hip_db *cur_db = nullptr;
+    int cur_slot = 0, next_slot = 0;
     std::vector<int32_t> score, er, eq;
     std::vector<uint8_t> emit;
-    // fetches cur's results and frees its database; false when no matches
+    // fetches cur's results and drops its database; false when no matches
     // are to be written
     auto fetch = [&]() {
         if (!cur_db) return false;
@@ -665,7 +667,7 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         eq.resize((size_t)n);
         emit.resize((size_t)n);
         const double tg = now_s();
-        int32_t r = ish_hip_db_fetch(ctx, cur_db, score.data(), er.data(), eq.data(), nullptr);
+        int32_t r = ish::db_fetch_after(ctx, cur_db, cur_slot, score.data(), er.data(), eq.data(), nullptr);
         ish_hip_db_free(cur_db);
         cur_db = nullptr;
         if (r == ISH_OK)
@@ -724,18 +726,18 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
                 acc += r.off[pt.end] - a;
             }
             rc = ish::db_load_ascii_pieces(ctx, pieces.data(), pieces.size(), off.data(), n,
-                                           cfg->matrix_kind, &db);
+                                           cfg->matrix_kind, &db, next_slot);
             st.seconds_gpu += now_s() - tg;
             st.records += n;
             st.residues += (int64_t)acc;
             st.batches++;
         }
-        const bool have = fetch();
         if (db && rc == ISH_OK) {
             const double tg = now_s();
-            rc = ish::db_score_once_async(ctx, db);
+            rc = ish::db_score_once_async(ctx, db, next_slot);
             st.seconds_gpu += now_s() - tg;
         }
+        const bool have = fetch(); // the pass before this one
         if (rc != ISH_OK && db) {
             (void)ish_hip_sync(ctx);
             ish_hip_db_free(db);
@@ -744,6 +746,10 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         if (have && rc == ISH_OK) write(*cur);
         cur = std::move(b);
         cur_db = db;
+        if (db) {
+            cur_slot = next_slot;
+            next_slot ^= 1;
+        }
         // like the reference's runners, a reader error ends the file
         // without failing the search; its code is reported in the stats
         if (cur->error) st.reader_status = cur->error;

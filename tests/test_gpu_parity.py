@@ -240,6 +240,25 @@ def test_sliced_batches(eng, slice_bytes):
         eng.set_slice_bytes(0)
 
 
+def test_bad_offsets_rejected_before_any_read(eng):
+    """Decreasing offsets and over-long targets are errors of the call, found
+    before any residue is read (the residue buffer here is 8 bytes)."""
+    res = np.arange(8, dtype=np.uint8)
+    eng.set_engine("batched")
+    eng.set_query(np.array([1, 2, 3], np.uint8), "blosum62", 3, 1, "FFTT")
+    cases = [(np.array([0, 5, 3, 8], np.uint64), "INVALID"),
+             (np.array([0, 4, 1 << 31], np.uint64), "RANGE")]
+    for off, code in cases:
+        t = Targets(res, off)
+        with pytest.raises(ish_amd.IshError, match=code):
+            eng.align(t)
+        with pytest.raises(ish_amd.IshError, match=code):
+            eng.load_db(t)
+    # the context still works afterwards
+    s, _, _, _ = eng.align(Targets(res, np.array([0, 3, 8], np.uint64)))
+    assert len(s) == 2
+
+
 def test_database_reuse_across_queries(eng):
     w = workloads.generate(2, 5000, seed=8)
     eng.set_engine("batched")
