@@ -1319,6 +1319,8 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
     return launch_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags);
 }
 
+int32_t ish::ctx_cus(const ish_hip_ctx *ctx) { return ctx ? ctx->cus : 0; }
+
 int32_t ish::db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db, int slot) {
     if (!ctx || !db) return ISH_E_INVALID;
     if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");

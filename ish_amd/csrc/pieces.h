@@ -25,6 +25,9 @@ int32_t db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, size_t n
                              const uint64_t *offsets, int64_t n, int32_t matrix_kind,
                              ish_hip_db **out, int slot = -1);
 
+// Compute units of the context's device.
+int32_t ctx_cus(const ish_hip_ctx *ctx);
+
 // One-shot passes: db_score_once_async enqueues a pass (no graph capture)
 // behind whatever the context's stream holds and marks its end with the
 // event of `slot`; db_fetch_after copies its results on the copy stream

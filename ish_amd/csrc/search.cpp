@@ -359,23 +359,38 @@ void parse_segment(const uint8_t *m, size_t n, int32_t type, size_t from, int64_
 }
 
 // Cuts records into batches of about batch_bytes.
+// Largest batch the adaptive rule asks for (the reference's default batch
+// size, searcher_settings.mojo:138).
+constexpr int64_t kMaxAdaptiveBatch = (int64_t)4 << 30;
+
 struct Batcher {
     const BatchSink &sink;
     int64_t batch_bytes;
+    // > 0: a batch also holds at least this many residues per residue of
+    // the longest record seen so far (up to kMaxAdaptiveBatch)
+    int64_t per_longest;
+    int64_t longest = 0;
     std::unique_ptr<Batch> b = std::make_unique<Batch>();
-    Batcher(const BatchSink &s, int64_t bb) : sink(s), batch_bytes(bb) {}
+    Batcher(const BatchSink &s, int64_t bb, int64_t pl) : sink(s), batch_bytes(bb), per_longest(pl) {}
+    int64_t limit() const {
+        return std::max(batch_bytes, std::min(per_longest * longest, kMaxAdaptiveBatch));
+    }
     // records [i, count) of sg
     void add(std::shared_ptr<const Segment> sg, size_t i) {
         const Recs &r = sg->recs;
         const size_t j = r.count();
         while (i < j) {
-            // records up to the one that brings the batch to batch_bytes
+            // records up to the one that brings the batch to its limit
             size_t e = i;
-            while (e < j && b->bytes < batch_bytes) b->bytes += (int64_t)r.bytes(e++);
+            while (e < j && b->bytes < limit()) {
+                const int64_t L = (int64_t)r.bytes(e++);
+                b->bytes += L;
+                longest = std::max(longest, L);
+            }
             b->parts.push_back({sg, i, e});
             b->nrec += e - i;
             i = e;
-            if (b->bytes >= batch_bytes) {
+            if (b->bytes >= limit()) {
                 sink(std::move(b));
                 b = std::make_unique<Batch>();
             }
@@ -390,10 +405,11 @@ struct Batcher {
 
 // Serial producer (gzip streams): records go into segments of about
 // batch_bytes, each handed over as it fills.
-void produce_serial(gzFile f, int32_t type, int64_t batch_bytes, const BatchSink &sink) {
+void produce_serial(gzFile f, int32_t type, int64_t batch_bytes, int64_t per_longest,
+                    const BatchSink &sink) {
     RecordSource src(f, type);
     const bool fastx = type == ISH_RECORD_FASTX;
-    Batcher bt(sink, batch_bytes);
+    Batcher bt(sink, batch_bytes, per_longest);
     auto sg = std::make_shared<Segment>();
     int64_t bytes = 0;
     const std::vector<uint8_t> *seq = nullptr, *name = nullptr, *qual = nullptr;
@@ -459,7 +475,7 @@ size_t candidate(const uint8_t *m, size_t n, int32_t type, size_t s) {
 }
 
 // Parallel producer over a mapped plain file (see the top of the file).
-void produce_parallel(const Mapped &mf, int32_t type, int64_t batch_bytes, int threads,
+void produce_parallel(const Mapped &mf, int32_t type, int64_t batch_bytes, int64_t per_longest, int threads,
                       int64_t seg_bytes, const BatchSink &sink) {
     const uint8_t *m = mf.p;
     const size_t n = mf.n;
@@ -502,7 +518,7 @@ void produce_parallel(const Mapped &mf, int32_t type, int64_t batch_bytes, int t
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
 
-    Batcher bt(sink, batch_bytes);
+    Batcher bt(sink, batch_bytes, per_longest);
     int32_t status = 0;
     int64_t expected = 0; // where the serial parse starts its next record
     for (size_t k = 0; k < nseg; ++k) {
@@ -556,21 +572,21 @@ bool is_gzip(const char *path) {
 
 // Every record of path, in batches. Returns 0 or ISH_E_INVALID when the
 // file cannot be opened.
-int32_t produce_batches(const char *path, int32_t type, int64_t batch_bytes, int threads,
+int32_t produce_batches(const char *path, int32_t type, int64_t batch_bytes, int64_t per_longest, int threads,
                         int64_t seg_bytes, const BatchSink &sink) {
     if (threads <= 0) threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
     if (seg_bytes <= 0) seg_bytes = (int64_t)8 << 20;
     if (!is_gzip(path)) {
         Mapped mf;
         if (mf.open(path)) {
-            produce_parallel(mf, type, batch_bytes, threads, seg_bytes, sink);
+            produce_parallel(mf, type, batch_bytes, per_longest, threads, seg_bytes, sink);
             return ISH_OK;
         }
     }
     gzFile f = gzopen(path, "rb");
     if (!f) return ISH_E_INVALID;
     gzbuffer(f, 1 << 20);
-    produce_serial(f, type, batch_bytes, sink);
+    produce_serial(f, type, batch_bytes, per_longest, sink);
     gzclose(f);
     return ISH_OK;
 }
@@ -628,13 +644,18 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         ::close(fd);
     }
     const int64_t batch_bytes = cfg->batch_bytes > 0 ? cfg->batch_bytes : ((int64_t)256 << 20);
+    // With the default size, a batch grows to hold at least cus x 256
+    // residues per residue of the longest record seen: each batch's pass
+    // ends with its longest group pipelined over four waves, and a smaller
+    // batch would be mostly that tail.
+    const int64_t per_longest = cfg->batch_bytes > 0 ? 0 : (int64_t)ish::ctx_cus(ctx) * 256;
 
     Queue queue;
     double t_parse = 0;
     std::thread parser([&] {
         const double t0 = now_s();
         const BatchSink sink = [&](std::unique_ptr<Batch> b) { queue.push(std::move(b)); };
-        if (produce_batches(path, cfg->record_type, batch_bytes, cfg->reader_threads,
+        if (produce_batches(path, cfg->record_type, batch_bytes, per_longest, cfg->reader_threads,
                             cfg->reader_segment_bytes, sink) != ISH_OK) {
             auto b = std::make_unique<Batch>();
             b->last = true;
@@ -801,7 +822,7 @@ extern "C" int32_t ish_hip_records_read_ex(const char *path, int32_t record_type
         }
         if (b->last) r->status = b->error;
     };
-    if (produce_batches(path, record_type, (int64_t)64 << 20, threads, segment_bytes, sink) != ISH_OK)
+    if (produce_batches(path, record_type, (int64_t)64 << 20, 0, threads, segment_bytes, sink) != ISH_OK)
         return ISH_E_INVALID;
     *out = r.release();
     return ISH_OK;
