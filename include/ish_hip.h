@@ -200,9 +200,7 @@ int32_t ish_hip_align_db(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
                          int32_t *end_ref, int32_t *end_query,
                          uint8_t *status);
 /* Asynchronous form: enqueue on the context's stream; results stay on the
- * device until ish_hip_db_fetch. ish_hip_sync waits for the stream. A
- * database load while a pass is in flight uploads and builds on the
- * context's copy stream during that pass. */
+ * device until ish_hip_db_fetch. ish_hip_sync waits for the stream. */
 int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db);
 int32_t ish_hip_sync(ish_hip_ctx *ctx);
 int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,

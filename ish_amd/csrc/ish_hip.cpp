@@ -81,10 +81,14 @@ struct ish_hip_ctx {
     // of its last copy to the device
     uint8_t *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_done[2] = {nullptr, nullptr};
-    // database uploads and builds run on a stream of their own, beside the
-    // scoring queued on `stream`; one-shot passes record pass_done[slot]
-    // after their launches, and their results are fetched on `copy` once
-    // it fires; span brackets a sliced batch's launches
+    // One-shot calls (sliced batches, file batches) load databases on a
+    // stream of their own, beside the scoring queued on `stream`: passes
+    // record pass_done[slot] after their launches, and their results are
+    // fetched on `copy` once it fires; span brackets a sliced batch's
+    // launches. `copy` exists only during such a call: a process gets
+    // few hardware queues (4 by default), and a stream held between calls
+    // would make a server's launch streams share one, which serializes
+    // the batches they overlap.
     hipStream_t copy = nullptr;
     hipEvent_t pass_done[2] = {nullptr, nullptr};
     hipEvent_t span[2] = {nullptr, nullptr};
@@ -241,7 +245,7 @@ extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
         delete c;
         return ISH_E_HIP;
     }
-    bool ok = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) == hipSuccess;
+    bool ok = true;
     for (int k = 0; k < 2 && ok; ++k)
         ok = hipEventCreateWithFlags(&c->pass_done[k], hipEventDisableTiming) == hipSuccess &&
              hipEventCreate(&c->span[k]) == hipSuccess;
@@ -250,7 +254,6 @@ extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
             if (c->pass_done[k]) (void)hipEventDestroy(c->pass_done[k]);
             if (c->span[k]) (void)hipEventDestroy(c->span[k]);
         }
-        if (c->copy) (void)hipStreamDestroy(c->copy);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return ISH_E_HIP;
@@ -263,7 +266,7 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamSynchronize(ctx->copy);
+    ish::one_shot_end(ctx);
     for (auto e : ctx->ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k) {
         if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
@@ -283,7 +286,6 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
         (void)hipEventDestroy(ctx->pass_done[k]);
         (void)hipEventDestroy(ctx->span[k]);
     }
-    (void)hipStreamDestroy(ctx->copy);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -727,11 +729,12 @@ static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64
 // pinned staging buffers: host threads copy one part into a buffer while the
 // other buffer's part is on its way to the device, so the upload runs at
 // the pinned DMA rate instead of the pageable path's (which stages small
-// parts one after the other). The copies run on the context's copy stream;
-// it returns once every part is enqueued there (the last copies may still
-// run, but the pieces are no longer read).
+// parts one after the other). The copies run on stream st; it returns once
+// every part is enqueued there (the last copies may still run, but the
+// pieces are no longer read).
 constexpr size_t kStageBytes = (size_t)64 << 20;
-static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiece *pieces, size_t np) {
+static hipError_t staged_h2d(ish_hip_ctx *ctx, hipStream_t st, uint8_t *dst, const ish::HostPiece *pieces,
+                             size_t np) {
     for (int k = 0; k < 2; ++k) {
         if (!ctx->stage[k]) {
             hipError_t e = hipHostMalloc((void **)&ctx->stage[k], kStageBytes, hipHostMallocDefault);
@@ -740,7 +743,7 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
         if (!ctx->stage_done[k]) {
             hipError_t e = hipEventCreateWithFlags(&ctx->stage_done[k], hipEventDisableTiming);
             if (e != hipSuccess) return e;
-            e = hipEventRecord(ctx->stage_done[k], ctx->copy);
+            e = hipEventRecord(ctx->stage_done[k], st);
             if (e != hipSuccess) return e;
         }
     }
@@ -790,8 +793,8 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
             pool.emplace_back(copy_range, std::min(fill, (size_t)t * per), std::min(fill, (size_t)(t + 1) * per));
         copy_range(0, std::min(fill, per));
         for (auto &t : pool) t.join();
-        e = hipMemcpyAsync(dst + at, stage, fill, hipMemcpyHostToDevice, ctx->copy);
-        if (e == hipSuccess) e = hipEventRecord(ctx->stage_done[buf], ctx->copy);
+        e = hipMemcpyAsync(dst + at, stage, fill, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipEventRecord(ctx->stage_done[buf], st);
         if (e != hipSuccess) return e;
         at += fill;
         buf ^= 1;
@@ -803,14 +806,17 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, uint8_t *dst, const ish::HostPiec
 // Uploads the residues (the concatenation of pieces) and builds the db;
 // with `encode` the bytes are translated to residue codes on the device
 // first. offsets: n + 1 entries, already checked; the pieces hold
-// offsets[n] - offsets[0] bytes. Upload and build run on the copy stream,
-// beside any pass queued on the context's stream, and the call returns
-// when the database is complete.
+// offsets[n] - offsets[0] bytes. A one-shot database (reuse given) is
+// uploaded and built on the copy stream, beside the pass queued on the
+// context's stream; any other is loaded on the context's stream. The call
+// returns when the database is complete.
 static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t np,
                          const uint64_t *offsets, int64_t nt, const EncodeTable *encode,
                          ish::DbBuffers *reuse, ish_hip_db **out) {
     const size_t n = (size_t)nt;
     const uint64_t nres = n > 0 ? offsets[n] - offsets[0] : 0;
+    hipStream_t st = reuse ? ctx->copy : ctx->stream;
+    if (!st) return fail(ctx, ISH_E_INVALID, "one-shot load outside a one-shot call");
     uint8_t *d_raw = nullptr;
     uint64_t *d_off = nullptr;
     // the offsets rebased to 0 and the length sort, beside the upload
@@ -825,18 +831,18 @@ static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t 
     });
     hipError_t e = db_alloc(reuse ? &reuse->raw : nullptr, &d_raw, nres + kRawPad);
     if (e == hipSuccess) e = db_alloc(reuse ? &reuse->off : nullptr, &d_off, sizeof(uint64_t) * (n + 1));
-    if (e == hipSuccess) e = staged_h2d(ctx, d_raw, pieces, np);
+    if (e == hipSuccess) e = staged_h2d(ctx, st, d_raw, pieces, np);
     planner.join();
     if (e == hipSuccess)
-        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, ctx->copy);
-    if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, ctx->copy);
+        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, st);
     if (e != hipSuccess) {
-        (void)hipStreamSynchronize(ctx->copy);
+        (void)hipStreamSynchronize(st);
         if (!reuse && d_raw) (void)hipFree(d_raw);
         if (!reuse && d_off) (void)hipFree(d_off);
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "db upload: %s", hipGetErrorString(e));
     }
-    return db_build(ctx, ctx->copy, d_raw, d_off, std::move(h_off), sp, reuse, out);
+    return db_build(ctx, st, d_raw, d_off, std::move(h_off), sp, reuse, out);
 }
 
 // Checks the offsets (non-decreasing, targets within the int32 limit) and
@@ -1321,6 +1327,19 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
 
 int32_t ish::ctx_cus(const ish_hip_ctx *ctx) { return ctx ? ctx->cus : 0; }
 
+int32_t ish::one_shot_begin(ish_hip_ctx *ctx) {
+    if (ctx->copy) return ISH_OK;
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking));
+    return ISH_OK;
+}
+
+void ish::one_shot_end(ish_hip_ctx *ctx) {
+    if (!ctx->copy) return;
+    (void)hipStreamSynchronize(ctx->copy);
+    (void)hipStreamDestroy(ctx->copy);
+    ctx->copy = nullptr;
+}
+
 int32_t ish::db_score_once_async(ish_hip_ctx *ctx, ish_hip_db *db, int slot) {
     if (!ctx || !db) return ISH_E_INVALID;
     if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
@@ -1378,6 +1397,7 @@ static int32_t fetch_results(ish_hip_ctx *ctx, ish_hip_db *db, hipStream_t st, i
 int32_t ish::db_fetch_after(ish_hip_ctx *ctx, ish_hip_db *db, int slot, int32_t *score,
                             int32_t *end_ref, int32_t *end_q, uint8_t *status) {
     if (!ctx || !db || (db->n > 0 && (!score || !end_ref || !end_q))) return ISH_E_INVALID;
+    if (!ctx->copy) return fail(ctx, ISH_E_INVALID, "one-shot fetch outside a one-shot call");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamWaitEvent(ctx->copy, ctx->pass_done[slot & 1], 0));
     return fetch_results(ctx, db, ctx->copy, score, end_ref, end_q, status);
@@ -1511,7 +1531,7 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
     // the context's two one-shot buffer slots in turn, so nothing is
     // allocated or freed between passes. No graphs: every slice is a new
     // database.
-    int32_t rc = ISH_OK;
+    int32_t rc = ish::one_shot_begin(ctx);
     int32_t launches = 0;
     ish_hip_db *prev = nullptr;
     int64_t prev_t0 = 0;
@@ -1551,9 +1571,10 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
     }
     if (rc != ISH_OK) {
         (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamSynchronize(ctx->copy);
+        if (ctx->copy) (void)hipStreamSynchronize(ctx->copy);
     }
     if (prev) ish_hip_db_free(prev);
+    ish::one_shot_end(ctx);
     ctx->launches = launches;
     return rc;
 }

@@ -28,6 +28,13 @@ int32_t db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, size_t n
 // Compute units of the context's device.
 int32_t ctx_cus(const ish_hip_ctx *ctx);
 
+// Bracket a one-shot call (slot loads, db_score_once_async,
+// db_fetch_after): begin creates the context's copy stream, end waits for
+// it and destroys it, so that the context holds one hardware queue between
+// calls.
+int32_t one_shot_begin(ish_hip_ctx *ctx);
+void one_shot_end(ish_hip_ctx *ctx);
+
 // One-shot passes: db_score_once_async enqueues a pass (no graph capture)
 // behind whatever the context's stream holds and marks its end with the
 // event of `slot`; db_fetch_after copies its results on the copy stream

@@ -728,6 +728,7 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         }
         st.seconds_write += now_s() - tw;
     };
+    rc = ish::one_shot_begin(ctx);
     for (;;) {
         std::unique_ptr<Batch> b = queue.pop();
         const int64_t n = (int64_t)b->records();
@@ -777,6 +778,7 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         if (cur->last) break;
     }
     if (fetch() && rc == ISH_OK) write(*cur);
+    ish::one_shot_end(ctx);
     parser.join();
     out.flush();
     if (out.failed && rc == ISH_OK) rc = ISH_E_INVALID;
