@@ -806,17 +806,17 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, hipStream_t st, uint8_t *dst, con
 // Uploads the residues (the concatenation of pieces) and builds the db;
 // with `encode` the bytes are translated to residue codes on the device
 // first. offsets: n + 1 entries, already checked; the pieces hold
-// offsets[n] - offsets[0] bytes. A one-shot database (reuse given) is
-// uploaded and built on the copy stream, beside the pass queued on the
-// context's stream; any other is loaded on the context's stream. The call
-// returns when the database is complete.
+// offsets[n] - offsets[0] bytes. During a one-shot call (the copy stream
+// exists) a one-shot database (reuse given) is uploaded and built on the
+// copy stream, beside the pass queued on the context's stream; any other
+// load runs on the context's stream. The call returns when the database
+// is complete.
 static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t np,
                          const uint64_t *offsets, int64_t nt, const EncodeTable *encode,
                          ish::DbBuffers *reuse, ish_hip_db **out) {
     const size_t n = (size_t)nt;
     const uint64_t nres = n > 0 ? offsets[n] - offsets[0] : 0;
-    hipStream_t st = reuse ? ctx->copy : ctx->stream;
-    if (!st) return fail(ctx, ISH_E_INVALID, "one-shot load outside a one-shot call");
+    hipStream_t st = reuse && ctx->copy ? ctx->copy : ctx->stream;
     uint8_t *d_raw = nullptr;
     uint64_t *d_off = nullptr;
     // the offsets rebased to 0 and the length sort, beside the upload
@@ -1531,6 +1531,20 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
     // the context's two one-shot buffer slots in turn, so nothing is
     // allocated or freed between passes. No graphs: every slice is a new
     // database.
+    if (cut.size() == 2) {
+        // one slice: nothing to overlap, so no copy stream
+        ish_hip_db *db = nullptr;
+        int32_t rc = db_load_impl(ctx, residues, offsets, n, nullptr, &db, 0, true);
+        if (rc == ISH_OK) rc = ish::db_score_once_async(ctx, db, 0);
+        if (rc == ISH_OK) {
+            rc = fetch_results(ctx, db, ctx->stream, score, end_ref, end_q, status);
+            collect_timing(ctx);
+        } else {
+            (void)hipStreamSynchronize(ctx->stream);
+        }
+        if (db) ish_hip_db_free(db);
+        return rc;
+    }
     int32_t rc = ish::one_shot_begin(ctx);
     int32_t launches = 0;
     ish_hip_db *prev = nullptr;
