@@ -16,7 +16,8 @@ namespace ish {
 constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
 constexpr int kBlock = 256;              // 4 waves per workgroup
 
-// Grow-only device buffer (power-of-two capacity).
+// Grow-only device buffer. Growing frees the old buffer (hipFree waits for
+// the whole device) and loses its contents.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
@@ -27,9 +28,14 @@ struct DevBuf {
             p = nullptr;
             cap = 0;
         }
-        size_t c = 1;
-        while (c < bytes) c <<= 1;
-        if (c < 256) c = 256;
+        // powers of two up to 64 MiB, then 1/8 headroom in 2 MiB steps
+        size_t c = 256;
+        if (bytes <= ((size_t)64 << 20)) {
+            while (c < bytes) c <<= 1;
+        } else {
+            const size_t step = (size_t)2 << 20;
+            c = (bytes + bytes / 8 + step - 1) / step * step;
+        }
         hipError_t e = hipMalloc(&p, c);
         if (e == hipSuccess) cap = c;
         return e;

@@ -652,10 +652,6 @@ static void plan_sort(const std::vector<uint64_t> &h_off, SortPlan &sp) {
     sp.words = off;
 }
 
-// Builds the packed, length-sorted layout from residues already on the
-// device (d_raw, offsets d_off rebased to 0) and the sort plan of h_off,
-// on stream st, and returns when it is done. Takes ownership of both
-// buffers, which the database keeps for start-coordinate passes.
 // A device buffer of `bytes`: from the reused slot b when given, else
 // allocated.
 template <typename T>
@@ -666,6 +662,11 @@ static hipError_t db_alloc(ish::DevBuf *b, T **p, size_t bytes) {
     return e;
 }
 
+// Builds the packed, length-sorted layout from residues already on the
+// device (d_raw, offsets d_off rebased to 0) and the sort plan of h_off,
+// on stream st, and returns when it is done. Takes ownership of both
+// buffers (or borrows them from reuse), which the database keeps for
+// start-coordinate passes.
 static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64_t *d_off,
                         std::vector<uint64_t> h_off, SortPlan &sp, ish::DbBuffers *reuse,
                         ish_hip_db **out) {
