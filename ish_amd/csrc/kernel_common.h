@@ -7,6 +7,7 @@
 namespace ish {
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) {
     return __builtin_elementwise_max(a, b);

@@ -230,28 +230,47 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
                 up_a = cm > thr;
                 up_b = false;
             }
-            // smallest row of this column holding the new maximum: the first
-            // 8-row slice whose maximum is the key holds it
+            // smallest row of this column holding the new maximum
             const V key = cm + vz;
             int ra = K, rb = K;
+            if constexpr (PACKED) {
+                // Both halves at once, branch-free per row: x = (H ^ key) | nu
+                // is zero in a half exactly where that half rises and H
+                // equals the key, so min(x, 255) * 256 + m is the row m
+                // there and at least 256 elsewhere; the running minimum is
+                // the first such row. A slice whose maximum is below the
+                // key (for every rising half of every lane) holds no such
+                // row and is skipped.
+                const uint32_t ku = as_u32(key);
+                const uint32_t nu = (up_a ? 0u : 0x0000FFFFu) | (up_b ? 0u : 0xFFFF0000u);
+                const u16x2 one = {1, 1}, cap = {255, 255}, scale = {256, 256};
+                u16x2 r = {0xFFFF, 0xFFFF};
 #pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                bool ha, hb;
-                if constexpr (PACKED) {
-                    ha = up_a && ra == K && cms[k].x == key.x;
-                    hb = up_b && rb == K && cms[k].y == key.y;
-                } else {
-                    ha = up_a && ra == K && cms[k] == key;
-                    hb = false;
+                for (int k = 0; k < NS; ++k) {
+                    const u16x2 t = __builtin_elementwise_min(
+                        __builtin_bit_cast(u16x2, (as_u32(cms[k]) ^ ku) | nu), one);
+                    if (__builtin_amdgcn_ballot_w64(__builtin_bit_cast(uint32_t, t) != 0x00010001u) != 0) {
+#pragma unroll
+                        for (int mm = 0; mm < 8; ++mm) {
+                            const int m = k * 8 + mm;
+                            const u16x2 x = __builtin_elementwise_min(
+                                __builtin_bit_cast(u16x2, (as_u32(st.H[m]) ^ ku) | nu), cap);
+                            r = __builtin_elementwise_min(
+                                r, x * scale + (u16x2){(unsigned short)m, (unsigned short)m});
+                        }
+                    }
                 }
-                if (__builtin_amdgcn_ballot_w64(ha || hb) != 0) {
+                ra = r.x < 256 ? (int)r.x : K;
+                rb = r.y < 256 ? (int)r.y : K;
+            } else {
+                // the first 8-row slice whose maximum is the key holds it
 #pragma unroll
-                    for (int mm = 0; mm < 8; ++mm) {
-                        const int m = k * 8 + mm;
-                        if constexpr (PACKED) {
-                            if (ha && ra == K && st.H[m].x == key.x) ra = m;
-                            if (hb && rb == K && st.H[m].y == key.y) rb = m;
-                        } else {
+                for (int k = 0; k < NS; ++k) {
+                    const bool ha = up_a && ra == K && cms[k] == key;
+                    if (__builtin_amdgcn_ballot_w64(ha) != 0) {
+#pragma unroll
+                        for (int mm = 0; mm < 8; ++mm) {
+                            const int m = k * 8 + mm;
                             if (ha && ra == K && st.H[m] == key) ra = m;
                         }
                     }
