@@ -95,6 +95,12 @@ struct ish_hip_ctx {
     // pinned landing buffer of fetched results (grow-only)
     uint8_t *res_host = nullptr;
     size_t res_host_bytes = 0;
+    // pinned staging of a build's offsets and group tables (grow-only), apart
+    // from the residue staging so that they need not wait for its last
+    // copies; tab_done marks the end of the last table copies
+    uint8_t *tab_host = nullptr;
+    size_t tab_bytes = 0;
+    hipEvent_t tab_done = nullptr;
     // buffers of one-shot databases (sliced batches, file batches): slot k
     // serves every other database, so one is loaded while the other's
     // pass runs; `mask` receives a build's residue-code mask
@@ -282,6 +288,9 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
                       &ctx->out_st})
         b->release();
     if (ctx->res_host) (void)hipHostFree(ctx->res_host);
+    if (ctx->tab_done) (void)hipEventSynchronize(ctx->tab_done);
+    if (ctx->tab_host) (void)hipHostFree(ctx->tab_host);
+    if (ctx->tab_done) (void)hipEventDestroy(ctx->tab_done);
     for (int k = 0; k < 2; ++k) {
         (void)hipEventDestroy(ctx->pass_done[k]);
         (void)hipEventDestroy(ctx->span[k]);
@@ -652,6 +661,56 @@ static void plan_sort(const std::vector<uint64_t> &h_off, SortPlan &sp) {
     sp.words = off;
 }
 
+// Host-to-device copies of a build's small host arrays (offsets, group
+// tables) through the context's table staging buffer, each at its own
+// offset: a pageable copy blocks the host for longer than the copy takes.
+struct TableCopy {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+static hipError_t staged_tables(ish_hip_ctx *ctx, hipStream_t st, const TableCopy *c, size_t nc) {
+    size_t total = 0;
+    for (size_t k = 0; k < nc; ++k) total += (c[k].bytes + 255) & ~(size_t)255;
+    if (total == 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (!ctx->tab_done) {
+        e = hipEventCreateWithFlags(&ctx->tab_done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+        e = hipEventRecord(ctx->tab_done, st);
+        if (e != hipSuccess) return e;
+    }
+    e = hipEventSynchronize(ctx->tab_done); // the last table copies have been read
+    if (e != hipSuccess) return e;
+    if (ctx->tab_bytes < total) {
+        if (ctx->tab_host) (void)hipHostFree(ctx->tab_host);
+        ctx->tab_host = nullptr;
+        ctx->tab_bytes = 0;
+        e = hipHostMalloc((void **)&ctx->tab_host, total + total / 4, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        ctx->tab_bytes = total + total / 4;
+    }
+    // host copies (one thread per array when there is much to copy), then
+    // the copies to the device
+    std::vector<size_t> at(nc, 0);
+    for (size_t k = 1; k < nc; ++k) at[k] = at[k - 1] + ((c[k - 1].bytes + 255) & ~(size_t)255);
+    if (total >= ((size_t)4 << 20)) {
+        std::vector<std::thread> pool;
+        for (size_t k = 1; k < nc; ++k)
+            if (c[k].bytes) pool.emplace_back([&, k] { memcpy(ctx->tab_host + at[k], c[k].src, c[k].bytes); });
+        if (c[0].bytes) memcpy(ctx->tab_host + at[0], c[0].src, c[0].bytes);
+        for (auto &t : pool) t.join();
+    } else {
+        for (size_t k = 0; k < nc; ++k)
+            if (c[k].bytes) memcpy(ctx->tab_host + at[k], c[k].src, c[k].bytes);
+    }
+    for (size_t k = 0; k < nc && e == hipSuccess; ++k)
+        if (c[k].bytes)
+            e = hipMemcpyAsync(c[k].dst, ctx->tab_host + at[k], c[k].bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(ctx->tab_done, st);
+    return e;
+}
+
 // A device buffer of `bytes`: from the reused slot b when given, else
 // allocated.
 template <typename T>
@@ -669,7 +728,8 @@ static hipError_t db_alloc(ish::DevBuf *b, T **p, size_t bytes) {
 // start-coordinate passes.
 static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64_t *d_off,
                         std::vector<uint64_t> h_off, SortPlan &sp, ish::DbBuffers *reuse,
-                        ish_hip_db **out) {
+                        ish_hip_db **out, bool copy_off = false) {
+    uint64_t *const off_dst = copy_off ? d_off : nullptr; // offsets still on the host
     const int64_t n = (int64_t)h_off.size() - 1;
     ish_hip_db *db = new ish_hip_db();
     db->borrowed = reuse;
@@ -713,13 +773,17 @@ static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64
     DB_TRY(db_alloc(&ctx->mask, &d_mask, 8 * sizeof(uint32_t)));
     DB_TRY(launch_code_mask(db->d_raw, (uint64_t)db->nres, d_mask, st));
     DB_TRY(hipMemcpyAsync(db->code_mask, d_mask, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    if (ng > 0) {
-        DB_TRY(hipMemcpyAsync(db->d_groups, groups.data(), sizeof(GroupDesc) * ng, hipMemcpyHostToDevice, st));
-        DB_TRY(hipMemcpyAsync(db->d_lens, lens.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, st));
-        DB_TRY(hipMemcpyAsync(db->d_orig, sorted.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, st));
+    {
+        const TableCopy tc[4] = {
+            {db->d_groups, groups.data(), sizeof(GroupDesc) * (size_t)ng},
+            {db->d_lens, lens.data(), sizeof(int32_t) * (size_t)nslots},
+            {db->d_orig, sorted.data(), sizeof(int32_t) * (size_t)nslots},
+            {off_dst, db->h_off.data(), off_dst ? sizeof(uint64_t) * (size_t)(n + 1) : 0}};
+        DB_TRY(staged_tables(ctx, st, tc, 4));
+    }
+    if (ng > 0)
         DB_TRY(launch_pack(db->d_raw, db->d_off, db->d_orig, db->d_groups, ng, n, off / 64, db->d_res,
                            st));
-    }
     DB_TRY(hipStreamSynchronize(st));
 #undef DB_TRY
     *out = db;
@@ -754,9 +818,9 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, hipStream_t st, uint8_t *dst, con
     uint64_t at = 0;
     size_t pk = 0;
     uint64_t in_piece = 0; // bytes of pieces[pk] already staged
-    // parts of 16, 32, then 64 MB: the first copy to the device starts
-    // after a short host copy
-    size_t part = kStageBytes / 4;
+    // parts of 4, 8, 16, 32, then 64 MB: the first copy to the device
+    // starts after a short host copy
+    size_t part = kStageBytes / 16;
     while (pk < np) {
         // fill one staging buffer with up to kStageBytes of the pieces
         hipError_t e = hipEventSynchronize(ctx->stage_done[buf]);
@@ -834,8 +898,6 @@ static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t 
     if (e == hipSuccess) e = db_alloc(reuse ? &reuse->off : nullptr, &d_off, sizeof(uint64_t) * (n + 1));
     if (e == hipSuccess) e = staged_h2d(ctx, st, d_raw, pieces, np);
     planner.join();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(d_off, h_off.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, st);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(st);
@@ -843,7 +905,8 @@ static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t 
         if (!reuse && d_off) (void)hipFree(d_off);
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "db upload: %s", hipGetErrorString(e));
     }
-    return db_build(ctx, st, d_raw, d_off, std::move(h_off), sp, reuse, out);
+    // the offsets go up with the group tables
+    return db_build(ctx, st, d_raw, d_off, std::move(h_off), sp, reuse, out, true);
 }
 
 // Checks the offsets (non-decreasing, targets within the int32 limit) and
@@ -1337,6 +1400,14 @@ int32_t ish::one_shot_begin(ish_hip_ctx *ctx) {
 void ish::one_shot_end(ish_hip_ctx *ctx) {
     if (!ctx->copy) return;
     (void)hipStreamSynchronize(ctx->copy);
+    // Events must not outlive the stream they were last recorded on: HIP
+    // looks at that stream when an event is waited for. The staging events
+    // may have been recorded on the copy stream; they are complete now, and
+    // are created again (and recorded) at their next use.
+    for (hipEvent_t *e : {&ctx->stage_done[0], &ctx->stage_done[1], &ctx->tab_done}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
     (void)hipStreamDestroy(ctx->copy);
     ctx->copy = nullptr;
 }

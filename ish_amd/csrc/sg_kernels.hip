@@ -745,7 +745,6 @@ size_t sg_hybrid_lock_lds(int chunk, int nrows, size_t profile_lds) {
 
 template <int K, bool TE, bool QE>
 __global__ void __launch_bounds__(256) sg_hybrid_lock(ScoreParams p) {
-    using V = s16x2;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_item;
     stage_pair_table<K>(p.plut, smem);
