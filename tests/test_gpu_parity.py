@@ -240,6 +240,31 @@ def test_sliced_batches(eng, slice_bytes):
         eng.set_slice_bytes(0)
 
 
+def test_resident_loads_after_one_shot_calls():
+    """A one-shot call's copy stream ends with the call; the staging it used
+    must still serve later resident loads, in the same context and in a new
+    one after the first is destroyed (a wait on an event left on the ended
+    stream made the next load fail)."""
+    w = workloads.generate(2, 4000, seed=12)
+    m = O.Matrix("blosum62")
+    ref = O.batch(0, w.query, w.targets.residues, w.targets.offsets, m, 3, 1, O.flags_from_str("FFTT"))
+    for round_ in range(2):
+        e = ish_amd.Engine(0)
+        try:
+            e.set_engine("batched")
+            e.set_query(w.query, "blosum62", 3, 1, "FFTT")
+            e.set_slice_bytes(100_000)  # several slices: the copy stream is used
+            one = e.align(w.targets)
+            db = e.load_db(w.targets)
+            res = e.align_db(db)
+            db.close()
+            for got in (one, res):
+                for a, b in zip(got[:3], ref[:3]):
+                    assert np.array_equal(a, b), round_
+        finally:
+            e.close()
+
+
 def test_bad_offsets_rejected_before_any_read(eng):
     """Decreasing offsets and over-long targets are errors of the call, found
     before any residue is read (the residue buffer here is 8 bytes)."""
