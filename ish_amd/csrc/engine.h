@@ -126,15 +126,16 @@ int occupancy_sw_batched_i16(int chunk, int block, size_t lds);
 // count that gets one.
 constexpr int pair_row_bytes(int K) { return 4 * K + 16; }
 constexpr int kPairMaxRows = 8;
-// Hybrid semi-global launch (packed int16, block of 256): the first n_long
-// groups are pipelined across a workgroup's four waves. `paired` selects
+// Hybrid semi-global launch (packed int16, `waves` = 2 or 4 waves per
+// workgroup): the first n_long groups are pipelined across a workgroup's
+// waves, the others run as tiles of one group per wave. `paired` selects
 // the variant over the paired profile (p.pprof, p.plut), whose whole LDS
-// size is sg_hybrid_paired_lds(chunk, classes).
+// size is sg_hybrid_paired_lds(chunk, classes, waves).
 size_t sg_pipe_extra_lds();
-size_t sg_hybrid_paired_lds(int chunk, int nrows);
-hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
-                                hipStream_t st);
-int occupancy_sg_hybrid_i16(int chunk, bool paired, size_t lds, uint32_t flags);
+size_t sg_hybrid_paired_lds(int chunk, int nrows, int waves);
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int waves, int grid,
+                                size_t lds, hipStream_t st);
+int occupancy_sg_hybrid_i16(int chunk, bool paired, int waves, size_t lds, uint32_t flags);
 // Hybrid launch over a workgroup-shared paired profile (up to
 // kPairMaxShared classes; tiles of four groups in lockstep, long groups over
 // the int16 profile). Its LDS is sg_hybrid_lock_lds(chunk, classes, the int16

@@ -93,7 +93,7 @@ int32_t neg_limit_of(int32_t go, int32_t mn);
 // per-wave boundary scratch for groups of up to `cols` columns; grows
 // `scratch` and may shrink the grid to stay within the memory budget.
 hipError_t persistent_grid(DevBuf &scratch, int occ_blocks, int cus, int64_t cols, int *grid,
-                           int64_t *slot_stride);
+                           int64_t *slot_stride, int waves_per_block = 4);
 
 // Device buffers of a database, kept by a context and reused (grow-only)
 // by the databases of one-shot passes, so that those allocate nothing once

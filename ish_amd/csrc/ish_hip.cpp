@@ -588,9 +588,9 @@ extern "C" int32_t ish_hip_sync(ish_hip_ctx *ctx) {
 // bytes per resident wave. The grid is capped so that scratch stays within
 // half of the free device memory.
 hipError_t ish::persistent_grid(DevBuf &scratch, int occ_blocks, int cus, int64_t cols,
-                                int *grid_out, int64_t *slot_stride) {
+                                int *grid_out, int64_t *slot_stride, int waves_per_block) {
     int64_t grid = (int64_t)occ_blocks * cus;
-    const int64_t per_block = (int64_t)(kBlock / 64) * std::max<int64_t>(cols, 4) * 64 * 8;
+    const int64_t per_block = (int64_t)waves_per_block * std::max<int64_t>(cols, 4) * 64 * 8;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
     const int64_t budget = std::min<int64_t>((int64_t)(free_b / 2) + (int64_t)scratch.cap,
@@ -604,8 +604,9 @@ hipError_t ish::persistent_grid(DevBuf &scratch, int occ_blocks, int cus, int64_
 }
 
 static int32_t size_grid(ish_hip_ctx *ctx, int occ_blocks, int64_t cols, int *grid_out,
-                         int64_t *slot_stride) {
-    HIP_TRY(ctx, persistent_grid(ctx->scratch, occ_blocks, ctx->cus, cols, grid_out, slot_stride));
+                         int64_t *slot_stride, int waves_per_block = kBlock / 64) {
+    HIP_TRY(ctx, persistent_grid(ctx->scratch, occ_blocks, ctx->cus, cols, grid_out, slot_stride,
+                                 waves_per_block));
     return ISH_OK;
 }
 
@@ -1025,6 +1026,7 @@ struct DbLaunch {
     int32_t grid;
     int32_t n_long; // kSgHybrid / kSgLock: leading groups scored by whole workgroups
     int32_t paired; // kSwPipe / kSgHybrid: the paired-profile variant (kSgLock: always)
+    int32_t waves;  // waves per workgroup (kSgHybrid: 2 or 4; the others: 4)
     int64_t slot_stride;
     uint64_t lds;
     ScoreParams p;
@@ -1067,7 +1069,7 @@ static int32_t long_groups(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t
                            int32_t chunk, uint32_t flags) {
     double cols = 0;
     for (int32_t g = g0; g < g1; ++g) cols += db->group_maxlen[g] + 3;
-    const int occ = occupancy_sg_hybrid_i16(chunk, false, ctx->lds + sg_pipe_extra_lds(), flags);
+    const int occ = occupancy_sg_hybrid_i16(chunk, false, 4, ctx->lds + sg_pipe_extra_lds(), flags);
     const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
     int32_t n = 0;
     while (g0 + n < g1 && db->group_maxlen[g0 + n] > 0.5 * per_wave) ++n;
@@ -1148,7 +1150,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         bool pipe = false;
         int32_t n_long = 0;
         const bool sg_paired = !local && pc > 0 && pc <= kPairMaxRows &&
-                               sg_hybrid_paired_lds(chunk, pc) <= kLdsLimit;
+                               sg_hybrid_paired_lds(chunk, pc, 4) <= kLdsLimit;
         const bool sg_lock = !local && pc > kPairMaxRows &&
                              sg_hybrid_lock_lds(chunk, pc, ctx->lds) <= kLdsLimit;
         if (local && pipe_lds <= kLdsLimit) {
@@ -1159,6 +1161,15 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
             else if (sched == ISH_SCHEDULE_AUTO) n_long = long_groups(ctx, db, g0, g1, chunk, flags);
         }
         L.kind = pipe ? kSwPipe : local ? kSw16 : sg_lock ? kSgLock : (n_long > 0 || sg_paired) ? kSgHybrid : kSg16;
+        L.waves = 4;
+        if (L.kind == kSgHybrid && n_long > 0) {
+            // a pipelined group runs its chunks round by round over the
+            // workgroup's waves: two waves leave fewer idle chunk slots in
+            // the last round when the chunk count is 1 or 2 modulo 4
+            // (ten chunks: 10 of 10 slots busy against 10 of 12)
+            const int nch = (ctx->Q + chunk - 1) / chunk;
+            if (2 * ((nch + 1) / 2) < 4 * ((nch + 3) / 4)) L.waves = 2;
+        }
         L.chunk = chunk;
         L.g_begin = g0;
         L.g_end = g1;
@@ -1190,9 +1201,9 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
             L.paired = 1;
             L.lds = sw_pipe_paired_lds(L.chunk, pc);
         } else if (L.kind == kSgHybrid && pc > 0 && pc <= kPairMaxRows &&
-                   sg_hybrid_paired_lds(L.chunk, pc) <= kLdsLimit) {
+                   sg_hybrid_paired_lds(L.chunk, pc, 4) <= kLdsLimit) {
             L.paired = 1;
-            L.lds = sg_hybrid_paired_lds(L.chunk, pc);
+            L.lds = sg_hybrid_paired_lds(L.chunk, pc, L.waves);
         }
         int occ = 1;
         switch (L.kind) {
@@ -1200,13 +1211,14 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         case kSg16: occ = occupancy_sg_batched_i16(L.chunk, kBlock, L.lds, flags); break;
         case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
         case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
-        case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.lds, flags); break;
+        case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.waves, L.lds, flags); break;
         case kSgLock: occ = occupancy_sg_hybrid_lock(L.chunk, L.lds, flags); break;
         default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.lds); break;
         }
         int grid;
         int64_t ss;
-        int32_t rc = size_grid(ctx, occ, db->group_maxlen[L.g_begin] + 3, &grid, &ss);
+        int32_t rc = size_grid(ctx, occ, db->group_maxlen[L.g_begin] + 3, &grid, &ss,
+                               L.kind == kSgHybrid ? L.waves : kBlock / 64);
         if (rc) return rc;
         L.grid = grid;
         L.slot_stride = ss;
@@ -1254,7 +1266,9 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
     case kSg16: e = launch_sg_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw32: e = launch_sw_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw16: e = launch_sw_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
-    case kSgHybrid: e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
+    case kSgHybrid:
+        e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.waves, L.grid, L.lds, ctx->stream);
+        break;
     case kSgLock: e = launch_sg_hybrid_lock(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
     default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
     }

@@ -502,13 +502,15 @@ constexpr size_t kSgCombBytes = (size_t)kSgPipeWaves * 64 * 6 * sizeof(int);
 
 constexpr size_t kSgExtraBytes = kSgRingBytes + kSgCombBytes + 16;
 size_t sg_pipe_extra_lds() { return kSgExtraBytes; }
-// LDS of a paired hybrid launch: the pair table (1 KB), the pipeline's
-// buffers, and one paired chunk profile per wave.
-size_t sg_hybrid_paired_lds(int chunk, int nrows) {
-    return 1024 + kSgExtraBytes + (size_t)kSgPipeWaves * nrows * nrows * pair_row_bytes(chunk);
+// LDS of a paired hybrid launch of `waves` waves per workgroup: the pair
+// table (1 KB), the pipeline's buffers, and one paired chunk profile per
+// wave.
+size_t sg_hybrid_paired_lds(int chunk, int nrows, int waves) {
+    return 1024 + kSgExtraBytes + (size_t)waves * nrows * nrows * pair_row_bytes(chunk);
 }
 
-template <int K, bool TE, bool QE, bool PAIRED = false>
+// W: the waves of the workgroup, which share the group's query chunks.
+template <int K, bool TE, bool QE, bool PAIRED = false, int W = kSgPipeWaves>
 __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryView &qv,
                                               const uint8_t *smem, const uint16_t *slut,
                                               uint2 *ring, int *comb, uint2 *gbnd, int g, int w,
@@ -526,10 +528,10 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
     const uint2 *res = p.res + gd.res_off;
     const int nblk = gd.ncols >> 2;
     const int nch = qv.nchunks;
-    const int rounds = (nch + kSgPipeWaves - 1) / kSgPipeWaves;
+    const int rounds = (nch + W - 1) / W;
     const int P = nblk > 5 ? nblk : 5;
-    const int steps = (rounds - 1) * P + nblk + (kSgPipeWaves - 1);
-    const int last_w = (nch - 1) % kSgPipeWaves;
+    const int steps = (rounds - 1) * P + nblk + (W - 1);
+    const int last_w = (nch - 1) % W;
 
     TrackI16 t16;
     TrackI32 t32;
@@ -546,7 +548,7 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
 
     int b = -w, r = 0; // block and round of this wave at the current step
     for (int s = 0; s < steps; ++s) {
-        const int c = kSgPipeWaves * r + w;
+        const int c = W * r + w;
         if (b >= 0 && b < nblk && r < rounds && c < nch) {
             const int j0 = c * K;
             const bool first = c == 0;
@@ -601,7 +603,7 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
             if (b + 1 < nblk) rr_next = res[(size_t)(b + 1) * 64 + lane];
             uint2 *out = nullptr;
             if (!last)
-                out = w < kSgPipeWaves - 1 ? ring + (size_t)(w * 2 + par) * 256
+                out = w < W - 1 ? ring + (size_t)(w * 2 + par) * 256
                                            : gbnd + (size_t)b * 4 * 64;
             sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
                                            chunk_base, slut, 0, RA, RB, ts, qe_rt, go, ge, t16,
@@ -622,7 +624,7 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
     cw[5] = t16.corner_b;
     __syncthreads();
     if (w == 0) {
-        for (int v = 1; v < kSgPipeWaves; ++v) {
+        for (int v = 1; v < W; ++v) {
             const int *cv = comb + (size_t)(v * 64 + lane) * 6;
             t16.te_a = max(t16.te_a, cv[0]);
             t16.te_b = max(t16.te_b, cv[1]);
@@ -675,7 +677,7 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
 // quarter of the time one wave needs, so they no longer set the length of
 // the launch.
 // ---------------------------------------------------------------------------
-template <int K, bool TE, bool QE, bool PAIRED>
+template <int K, bool TE, bool QE, bool PAIRED, int W>
 __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
     using V = s16x2;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
@@ -718,8 +720,8 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
         __syncthreads(); // s_item is rewritten by the next fetch
         if (item >= items) break;
         if (item < nlong) {
-            sg_pipe_group<K, TE, QE, PAIRED>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item,
-                                             w, lane, &pv);
+            sg_pipe_group<K, TE, QE, PAIRED, W>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item,
+                                                w, lane, &pv);
         } else {
             const int g = p.g_begin + nlong + (item - nlong) * waves + w;
             if (g < p.g_end) sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, 0, lane, &pv);
@@ -1197,25 +1199,33 @@ hipError_t launch_sg_multi(const ScoreParams &p, bool packed, bool paired, int c
                        : launch_multi<int, 32, false>(p, grid, block, lds, st);
 }
 
-template <int K, bool PAIRED>
+template <int K, bool PAIRED, int W>
 static hipError_t launch_hybrid(const ScoreParams &p, int grid, size_t lds, hipStream_t st) {
     const bool te = p.flags & 8u, qe = p.flags & 2u;
+    const dim3 block(64 * W);
     if (te && qe)
-        hipLaunchKernelGGL((sg_hybrid<K, true, true, PAIRED>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, true, true, PAIRED, W>), dim3(grid), block, lds, st, p);
     else if (te)
-        hipLaunchKernelGGL((sg_hybrid<K, true, false, PAIRED>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, true, false, PAIRED, W>), dim3(grid), block, lds, st, p);
     else if (qe)
-        hipLaunchKernelGGL((sg_hybrid<K, false, true, PAIRED>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, false, true, PAIRED, W>), dim3(grid), block, lds, st, p);
     else
-        hipLaunchKernelGGL((sg_hybrid<K, false, false, PAIRED>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, false, false, PAIRED, W>), dim3(grid), block, lds, st, p);
     return hipGetLastError();
 }
 
-hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
-                                hipStream_t st) {
+template <int K, bool PAIRED>
+static hipError_t launch_hybrid_w(const ScoreParams &p, int waves, int grid, size_t lds, hipStream_t st) {
+    return waves == 2 ? launch_hybrid<K, PAIRED, 2>(p, grid, lds, st) : launch_hybrid<K, PAIRED, 4>(p, grid, lds, st);
+}
+
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int waves, int grid,
+                                size_t lds, hipStream_t st) {
     if (paired)
-        return chunk == 16 ? launch_hybrid<16, true>(p, grid, lds, st) : launch_hybrid<32, true>(p, grid, lds, st);
-    return chunk == 16 ? launch_hybrid<16, false>(p, grid, lds, st) : launch_hybrid<32, false>(p, grid, lds, st);
+        return chunk == 16 ? launch_hybrid_w<16, true>(p, waves, grid, lds, st)
+                           : launch_hybrid_w<32, true>(p, waves, grid, lds, st);
+    return chunk == 16 ? launch_hybrid_w<16, false>(p, waves, grid, lds, st)
+                       : launch_hybrid_w<32, false>(p, waves, grid, lds, st);
 }
 
 template <int K>
@@ -1319,17 +1329,22 @@ int occupancy_sg_multi(bool packed, bool paired, int chunk, int block, size_t ld
                        : occ_multi<int, 32, false>(block, lds, flags);
 }
 
-template <int K, bool PAIRED> static int occ_hybrid(size_t lds, uint32_t flags) {
+template <int K, bool PAIRED, int W> static int occ_hybrid(size_t lds, uint32_t flags) {
     const bool te = flags & 8u, qe = flags & 2u;
-    if (te && qe) return occ(sg_hybrid<K, true, true, PAIRED>, 256, lds);
-    if (te) return occ(sg_hybrid<K, true, false, PAIRED>, 256, lds);
-    if (qe) return occ(sg_hybrid<K, false, true, PAIRED>, 256, lds);
-    return occ(sg_hybrid<K, false, false, PAIRED>, 256, lds);
+    if (te && qe) return occ(sg_hybrid<K, true, true, PAIRED, W>, 64 * W, lds);
+    if (te) return occ(sg_hybrid<K, true, false, PAIRED, W>, 64 * W, lds);
+    if (qe) return occ(sg_hybrid<K, false, true, PAIRED, W>, 64 * W, lds);
+    return occ(sg_hybrid<K, false, false, PAIRED, W>, 64 * W, lds);
 }
 
-int occupancy_sg_hybrid_i16(int chunk, bool paired, size_t lds, uint32_t flags) {
-    if (paired) return chunk == 16 ? occ_hybrid<16, true>(lds, flags) : occ_hybrid<32, true>(lds, flags);
-    return chunk == 16 ? occ_hybrid<16, false>(lds, flags) : occ_hybrid<32, false>(lds, flags);
+template <int K, bool PAIRED> static int occ_hybrid_w(int waves, size_t lds, uint32_t flags) {
+    return waves == 2 ? occ_hybrid<K, PAIRED, 2>(lds, flags) : occ_hybrid<K, PAIRED, 4>(lds, flags);
+}
+
+int occupancy_sg_hybrid_i16(int chunk, bool paired, int waves, size_t lds, uint32_t flags) {
+    if (paired)
+        return chunk == 16 ? occ_hybrid_w<16, true>(waves, lds, flags) : occ_hybrid_w<32, true>(waves, lds, flags);
+    return chunk == 16 ? occ_hybrid_w<16, false>(waves, lds, flags) : occ_hybrid_w<32, false>(waves, lds, flags);
 }
 
 template <int K> static int occ_lock(size_t lds, uint32_t flags) {
