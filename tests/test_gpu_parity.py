@@ -144,6 +144,22 @@ def test_hybrid_long_groups_among_short(eng, flags):
           "workgroup")
 
 
+@pytest.mark.parametrize("tile", [16, 32])
+def test_two_wave_pipelines(eng, tile):
+    """Chunk counts of 1 or 2 modulo 4 run pipelined groups over two-wave
+    workgroups (the workgroup schedule pipelines every group); each must
+    equal the oracle under every end-free setting the kernel distinguishes."""
+    rng = np.random.default_rng(31 + tile)
+    seqs = [rng.integers(0, 5, int(rng.integers(1, 700))).astype(np.uint8) for _ in range(700)]
+    targets = Targets.from_list(seqs)
+    for nch in (1, 2, 5, 6, 9, 10):
+        Q = max(1, nch * tile - int(rng.integers(0, tile)))
+        q = rng.integers(0, 5, Q).astype(np.uint8)
+        for flags in ("FFFF", "FFTT", "TTFF", "FTFT", "TFTF", "TTTT"):
+            check(eng, "batched", q, targets, "actgn", 3, 1, O.flags_from_str(flags),
+                  f"nch={nch} Q={Q} {flags}", tile, "workgroup")
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 def test_gap_open_not_above_extend(eng, engine):
     """go <= ge: the engine computes textbook Gotoh (the oracle)."""
