@@ -888,17 +888,23 @@ static int32_t db_upload(ish_hip_ctx *ctx, const ish::HostPiece *pieces, size_t 
     // the offsets rebased to 0 and the length sort, beside the upload
     std::vector<uint64_t> h_off;
     SortPlan sp;
+    bool plan_ok = true; // an exception must not leave the thread
     std::thread planner([&] {
-        h_off.resize(n + 1);
-        const uint64_t base = n > 0 ? offsets[0] : 0;
-        h_off[0] = 0;
-        for (size_t t = 1; t <= n; ++t) h_off[t] = offsets[t] - base;
-        plan_sort(h_off, sp);
+        try {
+            h_off.resize(n + 1);
+            const uint64_t base = n > 0 ? offsets[0] : 0;
+            h_off[0] = 0;
+            for (size_t t = 1; t <= n; ++t) h_off[t] = offsets[t] - base;
+            plan_sort(h_off, sp);
+        } catch (...) {
+            plan_ok = false;
+        }
     });
     hipError_t e = db_alloc(reuse ? &reuse->raw : nullptr, &d_raw, nres + kRawPad);
     if (e == hipSuccess) e = db_alloc(reuse ? &reuse->off : nullptr, &d_off, sizeof(uint64_t) * (n + 1));
     if (e == hipSuccess) e = staged_h2d(ctx, st, d_raw, pieces, np);
     planner.join();
+    if (e == hipSuccess && !plan_ok) e = hipErrorOutOfMemory; // the host plan could not be allocated
     if (e == hipSuccess && encode) e = launch_encode_bytes(d_raw, nres, *encode, st);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(st);
