@@ -168,7 +168,7 @@ def serving_phases(ish_amd, workloads, db, targets, args, count):
 
 
 def run_serving(args, ish_amd, workloads):
-    """Serving loop over a 10M-target resident DB (config 5)."""
+    """Serving loop over a 10M-target resident protein DB."""
     kind, n_default, mkind, algo, flags, desc = WORKLOADS["serving"]
     n_targets = args.targets or n_default
     t0 = time.perf_counter()
@@ -283,7 +283,7 @@ def run_file(args, ish_amd, workloads):
 
 def run_multiquery(args, ish_amd, workloads):
     """64 queries, lengths U[50,1000], against 4M records in one persistent
-    launch per cell width (config 3)."""
+    launch per cell width."""
     kind, n_default, mkind, algo, flags, desc = WORKLOADS["multiquery"]
     n_targets = args.targets or n_default
     t0 = time.perf_counter()

@@ -1,21 +1,21 @@
-"""Parity at the full sizes of the workload configurations: the targets and
+"""Parity at the full sizes of the benchmark workloads: the targets and
 queries that the bench lines run. Each check uses a property that does not
 shrink with the size: two independent GPU paths equal on every target, and
 the oracle on every target or on a large even sample.
 
-- config 1: 200k nucleotide records, actgn, FFTT.
+- 200k nucleotide line records, 150 nt query, actgn, FFTT.
   - Every target's score and end coordinates against the oracle.
   - The plain int32 engine equal to the batched one.
   - The emitted set at 0.8 equal to the oracle's emission rule.
-- config 2: 2M protein targets, blosum62, FFTT.
+- 2M protein targets, 350 aa query, blosum62, FFTT.
   - The batched launch (packed int16, hybrid) equal to the plain int32
     engine on every target.
   - Every target against the oracle's striped restatement.
-- config 3: 64 queries x 4M records.
+- 64 protein queries x 4M records.
   - The multi-query launch equal, on every target, to single-query passes
     for four of the queries.
   - Those four against the oracle on an even sample.
-- config 4: 500k long reads, local.
+- 500k long reads, 1,000 nt query, local.
   - The workgroup pipeline over the paired profile equal to the one-wave
     kernel over the int16 profile on every read.
   - Both against the oracle on every 25th read.
@@ -63,7 +63,7 @@ def _scores(eng, db):
     return s, er, eq
 
 
-def test_config1_nucleotide_full(eng):
+def test_nucleotide_200k_full(eng):
     w = workloads.generate(1, 200_000, seed=1)
     flags = O.flags_from_str("FFTT")
     eng.set_query(w.query, "actgn", 3, 1, "FFTT")
@@ -89,7 +89,7 @@ def test_config1_nucleotide_full(eng):
     assert emit.sum() > 0  # the planted copies are found
 
 
-def test_config2_protein_full(eng):
+def test_protein_2M_full(eng):
     w = workloads.generate(2, 2_000_000, seed=1)
     eng.set_query(w.query, "blosum62", 3, 1, "FFTT")
     db = eng.load_db(w.targets)
@@ -108,7 +108,7 @@ def test_config2_protein_full(eng):
     _same("batched vs oracle", got, ref)
 
 
-def test_config3_multiquery_full(eng):
+def test_multiquery_64x4M_full(eng):
     w = workloads.generate(3, 4_000_000, seed=1)
     rng = np.random.default_rng(1)
     qlens = rng.integers(50, 1001, 64)
@@ -129,7 +129,7 @@ def test_config3_multiquery_full(eng):
         db.close()
 
 
-def test_config4_longread_full(eng):
+def test_longread_500k_full(eng):
     w = workloads.generate(4, 500_000, seed=1)
     eng.set_query(w.query, "actgn0", 3, 1, "FFFF", "striped-local")
     db = eng.load_db(w.targets)
