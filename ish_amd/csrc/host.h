@@ -4,9 +4,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <cstdint>
 #include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -94,6 +99,75 @@ int32_t neg_limit_of(int32_t go, int32_t mn);
 // `scratch` and may shrink the grid to stay within the memory budget.
 hipError_t persistent_grid(DevBuf &scratch, int occ_blocks, int cus, int64_t cols, int *grid,
                            int64_t *slot_stride, int waves_per_block = 4);
+
+// Host worker threads that run one fork-join job at a time (a context is
+// used from one thread): run(n, fn) calls fn(0) .. fn(n - 1) on the workers
+// and the calling thread and returns when all have returned. The workers
+// live as long as the pool, so the host copies of a call do not pay for
+// creating threads.
+class HostPool {
+  public:
+    explicit HostPool(int workers) {
+        for (int k = 0; k < workers; ++k) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    int size() const { return (int)th_.size() + 1; }
+    void run(int n, const std::function<void(int)> &fn) {
+        if (n <= 0) return;
+        if (n == 1 || th_.empty()) {
+            for (int k = 0; k < n; ++k) fn(k);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &fn;
+            njobs_ = n;
+            next_.store(0);
+            busy_ = (int)th_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        work(fn, n);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return busy_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    void work(const std::function<void(int)> &fn, int n) {
+        for (int k; (k = next_.fetch_add(1)) < n;) fn(k);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            const std::function<void(int)> *fn = job_;
+            const int n = njobs_;
+            lk.unlock();
+            work(*fn, n);
+            lk.lock();
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *job_ = nullptr;
+    int njobs_ = 0, busy_ = 0;
+    std::atomic<int> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
 
 // Device buffers of a database, kept by a context and reused (grow-only)
 // by the databases of one-shot passes, so that those allocate nothing once

@@ -92,6 +92,8 @@ struct ish_hip_ctx {
     hipStream_t copy = nullptr;
     hipEvent_t pass_done[2] = {nullptr, nullptr};
     hipEvent_t span[2] = {nullptr, nullptr};
+    // host workers for the parallel host copies (created at first use)
+    std::unique_ptr<ish::HostPool> pool;
     // pinned landing buffer of fetched results (grow-only)
     uint8_t *res_host = nullptr;
     size_t res_host_bytes = 0;
@@ -115,6 +117,15 @@ struct ish_hip_ctx {
     int launches = 0;
     float last_ms = 0.f;
 };
+
+// The context's host worker pool: up to 8 threads in all, with the caller.
+static ish::HostPool &host_pool(ish_hip_ctx *ctx) {
+    if (!ctx->pool) {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        ctx->pool = std::make_unique<ish::HostPool>((int)std::min(8u, hw) - 1);
+    }
+    return *ctx->pool;
+}
 
 static int32_t fail(ish_hip_ctx *ctx, int32_t code, const char *fmt, ...) {
     if (ctx) {
@@ -696,11 +707,9 @@ static hipError_t staged_tables(ish_hip_ctx *ctx, hipStream_t st, const TableCop
     std::vector<size_t> at(nc, 0);
     for (size_t k = 1; k < nc; ++k) at[k] = at[k - 1] + ((c[k - 1].bytes + 255) & ~(size_t)255);
     if (total >= ((size_t)4 << 20)) {
-        std::vector<std::thread> pool;
-        for (size_t k = 1; k < nc; ++k)
-            if (c[k].bytes) pool.emplace_back([&, k] { memcpy(ctx->tab_host + at[k], c[k].src, c[k].bytes); });
-        if (c[0].bytes) memcpy(ctx->tab_host + at[0], c[0].src, c[0].bytes);
-        for (auto &t : pool) t.join();
+        host_pool(ctx).run((int)nc, [&](int k) {
+            if (c[k].bytes) memcpy(ctx->tab_host + at[k], c[k].src, c[k].bytes);
+        });
     } else {
         for (size_t k = 0; k < nc; ++k)
             if (c[k].bytes) memcpy(ctx->tab_host + at[k], c[k].src, c[k].bytes);
@@ -813,8 +822,7 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, hipStream_t st, uint8_t *dst, con
             if (e != hipSuccess) return e;
         }
     }
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int threads = (int)std::min(8u, hw);
+    const int threads = host_pool(ctx).size();
     int buf = 0;
     uint64_t at = 0;
     size_t pk = 0;
@@ -854,11 +862,9 @@ static hipError_t staged_h2d(ish_hip_ctx *ctx, hipStream_t st, uint8_t *dst, con
         };
         const int tn = fill >= ((size_t)1 << 22) ? threads : 1;
         const size_t per = (fill + tn - 1) / tn;
-        std::vector<std::thread> pool;
-        for (int t = 1; t < tn; ++t)
-            pool.emplace_back(copy_range, std::min(fill, (size_t)t * per), std::min(fill, (size_t)(t + 1) * per));
-        copy_range(0, std::min(fill, per));
-        for (auto &t : pool) t.join();
+        host_pool(ctx).run(tn, [&](int t) {
+            copy_range(std::min(fill, (size_t)t * per), std::min(fill, (size_t)(t + 1) * per));
+        });
         e = hipMemcpyAsync(dst + at, stage, fill, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipEventRecord(ctx->stage_done[buf], st);
         if (e != hipSuccess) return e;
@@ -936,14 +942,12 @@ static int32_t db_offsets(ish_hip_ctx *ctx, const uint64_t *offsets, int64_t n,
         r.bad = dec ? 1 : m > (uint64_t)INT32_MAX / 2 ? 2 : 0;
         r.maxlen = m;
     };
-    const int nt = n >= ((int64_t)1 << 20) ? (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency())) : 1;
+    const int nt = n >= ((int64_t)1 << 20) ? host_pool(ctx).size() : 1;
     std::vector<Part> parts((size_t)nt);
-    std::vector<std::thread> pool;
     const int64_t per = (n + nt - 1) / nt;
-    for (int k = 1; k < nt; ++k)
-        pool.emplace_back(scan, std::min(n, k * per), std::min(n, (k + 1) * per), std::ref(parts[(size_t)k]));
-    scan(0, std::min(n, per), parts[0]);
-    for (auto &t : pool) t.join();
+    host_pool(ctx).run(nt, [&](int k) {
+        scan(std::min(n, k * per), std::min(n, (k + 1) * per), parts[(size_t)k]);
+    });
     uint64_t m = 0;
     for (const Part &r : parts) {
         if (r.bad == 1) return fail(ctx, ISH_E_INVALID, "offsets must be non-decreasing");
@@ -1477,11 +1481,7 @@ static int32_t fetch_results(ish_hip_ctx *ctx, ish_hip_db *db, hipStream_t st, i
     if (n < ((int64_t)1 << 16)) {
         for (const Out &o : outs) memcpy(o.dst, o.src, o.bytes);
     } else {
-        std::vector<std::thread> pool;
-        for (size_t k = 1; k < outs.size(); ++k)
-            pool.emplace_back([o = outs[k]] { memcpy(o.dst, o.src, o.bytes); });
-        memcpy(outs[0].dst, outs[0].src, outs[0].bytes);
-        for (auto &t : pool) t.join();
+        host_pool(ctx).run((int)outs.size(), [&](int k) { memcpy(outs[k].dst, outs[k].src, outs[k].bytes); });
     }
     return ISH_OK;
 }
