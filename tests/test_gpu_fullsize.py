@@ -146,3 +146,34 @@ def test_longread_500k_full(eng):
     res, off = _sample(w.targets, idx)
     ref = O.batch(1, w.query, res, off, O.Matrix("actgn0"), 3, 1, 0, nthreads=THREADS, striped=True)[:3]
     _same("pipeline vs oracle", tuple(a[idx] for a in pipe), ref)
+
+
+def test_file_runner_protein_2M_default_batches(eng, tmp_path):
+    """The 2M protein targets as a 0.89 GB FASTA file through the file runner
+    with its default batching (batches grow with the longest record, each
+    loaded while the one before is scored): the output equals, byte for
+    byte, the records the in-memory pass emits, in input order."""
+    w = workloads.generate(2, 2_000_000, seed=1)
+    path = tmp_path / "targets.fa"
+    workloads.write_fastx(str(path), w.targets, "blosum62", fastq=False, width=60)
+    dec = np.asarray(ish_amd.decoding("blosum62"), np.uint8)
+    pattern = bytes(dec[w.query])
+    out = tmp_path / "out.fa"
+    fd = os.open(str(out), os.O_WRONLY | os.O_CREAT | os.O_TRUNC)
+    try:
+        st = eng.search_file(str(path), pattern, "blosum62", "striped-semi-global", 3, 1, "FFTT", 0.25,
+                             "fastx", fd)
+    finally:
+        os.close(fd)
+    assert st["batches"] >= 2 and st["records"] == w.targets.n
+    eng.set_query(w.query, "blosum62", 3, 1, "FFTT")
+    db = eng.load_db(w.targets)
+    s, er, eq = _scores(eng, db)
+    db.close()
+    ms = int(ish_amd.matrix("blosum62")[w.query, w.query].sum())
+    emit = ish_amd.emit("striped-semi-global", s, er, eq, ms, 0.25)
+    assert emit.sum() > 100
+    off = w.targets.offsets
+    expected = b"".join(b">t%d synthetic\n" % t + bytes(dec[w.targets.residues[int(off[t]):int(off[t + 1])]]) + b"\n"
+                        for t in np.flatnonzero(emit))
+    assert out.read_bytes() == expected
