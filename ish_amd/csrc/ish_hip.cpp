@@ -27,6 +27,7 @@ constexpr int kLenBuckets = 4;
 constexpr int32_t kLenBucketLo[kLenBuckets] = {4096, 1024, 256, 0};
 struct TuneChoice {
     int8_t chunk, schedule;
+    int8_t waves; // waves per workgroup of pipelined semi-global groups (0: the default rule)
 };
 struct TuneKey {
     int32_t kind, algo, qbucket, dbucket;
@@ -1114,10 +1115,11 @@ static int32_t ensure_pairs(ish_hip_ctx *ctx, const ish_hip_db *db) {
 // Plans the launches that score groups [g_lo, g_hi) of db (int32 groups
 // first), sizes the boundary scratch for all of them, then fills in every
 // launch's parameters. `forced` >= 0 fixes (chunk, schedule) for the int16
-// groups (autotuning); otherwise the tuning table or the defaults decide.
+// groups (autotuning), and forced_waves > 0 the width of pipelined
+// semi-global groups; otherwise the tuning table or the defaults decide.
 static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, uint32_t flags,
                        int32_t g_lo, int32_t g_hi, int forced_chunk, int forced_sched,
-                       std::vector<DbLaunch> &out) {
+                       std::vector<DbLaunch> &out, int forced_waves = 0) {
     out.clear();
     if (g_hi <= g_lo) return ISH_OK;
     HIP_TRY(ctx, ctx->counter.ensure(256));
@@ -1154,7 +1156,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         pc = ensure_pairs(ctx, db);
         if (pc < 0) return pc;
     }
-    auto add16 = [&](int32_t g0, int32_t g1, int chunk, int sched) {
+    auto add16 = [&](int32_t g0, int32_t g1, int chunk, int sched, int waves) {
         if (g1 <= g0) return;
         DbLaunch L{};
         bool pipe = false;
@@ -1174,11 +1176,18 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         L.waves = 4;
         if (L.kind == kSgHybrid && n_long > 0) {
             // a pipelined group runs its chunks round by round over the
-            // workgroup's waves: two waves leave fewer idle chunk slots in
-            // the last round when the chunk count is 1 or 2 modulo 4
-            // (ten chunks: 10 of 10 slots busy against 10 of 12)
+            // workgroup's waves. Two waves leave fewer idle chunk slots in
+            // the last round when the chunk count is 1 or 2 modulo 4 (five
+            // chunks: 5 of 6 slots busy against 5 of 8). From 8 chunks on,
+            // four waves win anyway: a group then takes fewer rounds, so the
+            // longest groups, which end the launch, finish sooner (a 150
+            // row query over 200 k targets of 100-300 residues, 1,563
+            // groups: ten 16-row chunks took 0.90 ms over four waves and
+            // 0.99 ms over two; five 32-row chunks 1.02 ms over two and
+            // 1.18 ms over four).
             const int nch = (ctx->Q + chunk - 1) / chunk;
-            if (2 * ((nch + 1) / 2) < 4 * ((nch + 3) / 4)) L.waves = 2;
+            if (nch < 8 && 2 * ((nch + 1) / 2) < 4 * ((nch + 3) / 4)) L.waves = 2;
+            if (waves == 2 || waves == 4) L.waves = waves;
         }
         L.chunk = chunk;
         L.g_begin = g0;
@@ -1187,16 +1196,17 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         out.push_back(L);
     };
     if (forced_chunk > 0) {
-        add16(m16, g_hi, forced_chunk, forced_sched);
+        add16(m16, g_hi, forced_chunk, forced_sched, forced_waves);
     } else {
         auto it = ctx->tuned ? ctx->tune.find(tune_key(ctx, db)) : ctx->tune.end();
         if (it == ctx->tune.end()) {
-            add16(m16, g_hi, ctx->chunk, ctx->schedule);
+            add16(m16, g_hi, ctx->chunk, ctx->schedule, 0);
         } else {
             // the tuned configuration for the whole pass (one launch: a
             // launch per length range would add a tail per launch)
             const TuneChoice &c = it->second;
-            add16(m16, g_hi, c.chunk, ctx->schedule != ISH_SCHEDULE_AUTO ? ctx->schedule : c.schedule);
+            add16(m16, g_hi, c.chunk, ctx->schedule != ISH_SCHEDULE_AUTO ? ctx->schedule : c.schedule,
+                  c.waves);
         }
     }
     // grids and scratch for every launch before any is enqueued (growing the
@@ -1344,10 +1354,12 @@ extern "C" int32_t ish_hip_set_graphs(ish_hip_ctx *ctx, int32_t on) {
     return ISH_OK;
 }
 
-// Times each candidate (tile height, and for local alignment the
-// schedule) on a sample of every target-length bucket of db for the
-// context's query, and keeps the fastest per bucket for the query's
-// (matrix, algorithm, query-length bucket).
+// Times whole passes over db for each candidate plan: tile height 32 or
+// 16; schedule (one wave per group, automatic, one workgroup per group);
+// and, for pipelined semi-global groups, two or four waves per workgroup.
+// Plans that come out identical are timed once. The fastest is kept under
+// (matrix kind, algorithm, query-length bucket, length bucket of db's
+// longest group).
 extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t reps) {
     if (!ctx || !db) return ISH_E_INVALID;
     if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
@@ -1357,16 +1369,20 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const int32_t chunks[2] = {32, 16};
     const int scheds[3] = {ISH_SCHEDULE_WAVE, ISH_SCHEDULE_AUTO, ISH_SCHEDULE_WORKGROUP};
-    TuneChoice best{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO};
+    TuneChoice best{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO, 0};
     float best_ms = 1e30f;
     const int saved_launches = ctx->launches;
     std::vector<std::vector<DbLaunch>> seen;
+    // pipelined semi-global groups also try both workgroup widths (a plan
+    // that has none comes out the same for each and is timed once)
+    const int widths[3] = {0, 2, 4};
     for (int32_t ch : chunks) {
         for (int sched : scheds) {
+          for (int wv : widths) {
             // whole passes: the launch tail is part of what is being tuned
             std::vector<DbLaunch> plan;
             int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, 0, db->ngroups,
-                                 ch, sched, plan);
+                                 ch, sched, plan, wv);
             if (rc) return rc;
             bool dup = false; // a schedule that plans the same launches as one already timed
             for (const auto &q : seen)
@@ -1384,8 +1400,9 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
             }
             if (total < best_ms) {
                 best_ms = total;
-                best = TuneChoice{(int8_t)ch, (int8_t)sched};
+                best = TuneChoice{(int8_t)ch, (int8_t)sched, (int8_t)wv};
             }
+          }
         }
     }
     ctx->launches = saved_launches;

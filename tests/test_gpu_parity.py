@@ -146,9 +146,10 @@ def test_hybrid_long_groups_among_short(eng, flags):
 
 @pytest.mark.parametrize("tile", [16, 32])
 def test_two_wave_pipelines(eng, tile):
-    """Chunk counts of 1 or 2 modulo 4 run pipelined groups over two-wave
-    workgroups (the workgroup schedule pipelines every group); each must
-    equal the oracle under every end-free setting the kernel distinguishes."""
+    """Chunk counts below 8 that are 1 or 2 modulo 4 run pipelined groups
+    over two-wave workgroups, 9 and 10 over four (the workgroup schedule
+    pipelines every group); each must equal the oracle under every end-free
+    setting the kernel distinguishes."""
     rng = np.random.default_rng(31 + tile)
     seqs = [rng.integers(0, 5, int(rng.integers(1, 700))).astype(np.uint8) for _ in range(700)]
     targets = Targets.from_list(seqs)
@@ -646,8 +647,9 @@ def test_multi_query_paired_class_counts(eng, tile, ncodes):
 
 # ---- autotuner and HIP graphs ------------------------------------------------
 def test_autotune_keeps_results(eng):
-    """The tuned launches (per length bucket) give the same results as the
-    defaults, for semi-global and local queries."""
+    """The tuned launch plan gives the same results as the defaults: a
+    protein and a nucleotide semi-global query (the latter with pipelined
+    groups, whose workgroup width is tuned too) and a local query."""
     w = workloads.generate(2, 30000, seed=8)
     eng.set_engine("batched")
     eng.set_tile(32)
@@ -674,6 +676,16 @@ def test_autotune_keeps_results(eng):
         os_, oer, oeq, _ = O.batch(1, lr.query, lr.targets.residues, lr.targets.offsets, m, 3, 1, 0)
         assert np.array_equal(got[0], os_) and np.array_equal(got[1], oer) and np.array_equal(got[2], oeq)
         db2.close()
+        nt = workloads.generate(1, 20000, seed=10)
+        db3 = eng.load_db(nt.targets)
+        eng.set_tile(32)
+        eng.set_query(nt.query, "actgn", 3, 1, "FFTT")
+        ref = eng.align_db(db3)
+        assert eng.autotune(db3, 1) is not None
+        got = eng.align_db(db3)
+        db3.close()
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
     finally:
         eng.set_tile(32)  # switches the tuning table off for later tests
         db.close()
