@@ -172,9 +172,12 @@ int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
                             int32_t *end_ref, int32_t *end_query,
                             uint8_t *status);
 /* Slice size of ish_hip_align_batch in residue bytes: 0 (the default)
- * sizes slices automatically (at least 64 MiB, and enough work per slice
- * that its longest target does not set the launch's length), a positive
- * value is a fixed size, a negative value turns slicing off. No
+ * sizes slices automatically, a positive value is a fixed size, a
+ * negative value turns slicing off. Automatic slices hold at least 64 MiB
+ * and enough work that their longest target does not set the launch's
+ * length (the minimum m); a batch of B bytes gets about sqrt(3 B / m) of
+ * them, which balances the exposed first upload against each slice's
+ * tail. No
  * counterpart in the reference, whose batches are sized by the host
  * (searcher_settings.mojo:44-50). */
 int32_t ish_hip_set_slice_bytes(ish_hip_ctx *ctx, int64_t bytes);
