@@ -105,6 +105,11 @@ struct ish_hip_ctx {
     uint8_t *tab_host = nullptr;
     size_t tab_bytes = 0;
     hipEvent_t tab_done = nullptr;
+    // pinned landing of a build's residue-code mask (a copy into pageable
+    // memory would hold the host until the upload before it ends);
+    // mask_done marks its arrival and is recorded on `stream` only
+    uint32_t *mask_host = nullptr;
+    hipEvent_t mask_done = nullptr;
     // buffers of one-shot databases (sliced batches, file batches): slot k
     // serves every other database, so one is loaded while the other's
     // pass runs; `mask` receives a build's residue-code mask
@@ -301,6 +306,8 @@ extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
                       &ctx->out_st})
         b->release();
     if (ctx->res_host) (void)hipHostFree(ctx->res_host);
+    if (ctx->mask_host) (void)hipHostFree(ctx->mask_host);
+    if (ctx->mask_done) (void)hipEventDestroy(ctx->mask_done);
     if (ctx->tab_done) (void)hipEventSynchronize(ctx->tab_done);
     if (ctx->tab_host) (void)hipHostFree(ctx->tab_host);
     if (ctx->tab_done) (void)hipEventDestroy(ctx->tab_done);
@@ -737,7 +744,10 @@ static hipError_t db_alloc(ish::DevBuf *b, T **p, size_t bytes) {
 // device (d_raw, offsets d_off rebased to 0) and the sort plan of h_off,
 // on stream st, and returns when it is done. Takes ownership of both
 // buffers (or borrows them from reuse), which the database keeps for
-// start-coordinate passes.
+// start-coordinate passes. A one-shot database built on the context's
+// stream (a batch of one slice, whose pass is queued on that stream next)
+// returns once its code mask is on the host, while the table copies and
+// the packing still run.
 static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64_t *d_off,
                         std::vector<uint64_t> h_off, SortPlan &sp, ish::DbBuffers *reuse,
                         ish_hip_db **out, bool copy_off = false) {
@@ -782,9 +792,19 @@ static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64
     // the packer writes every block of every group; the spare block is zero
     DB_TRY(hipMemsetAsync(db->d_res + off, 0, sizeof(uint2) * 64, st));
     // residue codes present (paired profiles are built over their classes)
+    const bool lazy = reuse && !ctx->copy && st == ctx->stream;
     DB_TRY(db_alloc(&ctx->mask, &d_mask, 8 * sizeof(uint32_t)));
     DB_TRY(launch_code_mask(db->d_raw, (uint64_t)db->nres, d_mask, st));
-    DB_TRY(hipMemcpyAsync(db->code_mask, d_mask, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (!ctx->mask_host) {
+        DB_TRY(hipHostMalloc((void **)&ctx->mask_host, 8 * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    DB_TRY(hipMemcpyAsync(ctx->mask_host, d_mask, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (lazy) {
+        if (!ctx->mask_done) {
+            DB_TRY(hipEventCreateWithFlags(&ctx->mask_done, hipEventDisableTiming));
+        }
+        DB_TRY(hipEventRecord(ctx->mask_done, st));
+    }
     {
         const TableCopy tc[4] = {
             {db->d_groups, groups.data(), sizeof(GroupDesc) * (size_t)ng},
@@ -796,7 +816,12 @@ static int32_t db_build(ish_hip_ctx *ctx, hipStream_t st, uint8_t *d_raw, uint64
     if (ng > 0)
         DB_TRY(launch_pack(db->d_raw, db->d_off, db->d_orig, db->d_groups, ng, n, off / 64, db->d_res,
                            st));
-    DB_TRY(hipStreamSynchronize(st));
+    if (lazy) {
+        DB_TRY(hipEventSynchronize(ctx->mask_done));
+    } else {
+        DB_TRY(hipStreamSynchronize(st));
+    }
+    memcpy(db->code_mask, ctx->mask_host, sizeof db->code_mask);
 #undef DB_TRY
     *out = db;
     return ISH_OK;
