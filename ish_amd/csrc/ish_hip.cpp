@@ -1402,33 +1402,37 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     // pipelined semi-global groups also try both workgroup widths (a plan
     // that has none comes out the same for each and is timed once)
     const int widths[3] = {0, 2, 4};
-    for (int32_t ch : chunks) {
-        for (int sched : scheds) {
-          for (int wv : widths) {
-            // whole passes: the launch tail is part of what is being tuned
-            std::vector<DbLaunch> plan;
-            int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, 0, db->ngroups,
-                                 ch, sched, plan, wv);
-            if (rc) return rc;
-            bool dup = false; // a schedule that plans the same launches as one already timed
-            for (const auto &q : seen)
-                dup = dup || (q.size() == plan.size() &&
-                              memcmp(q.data(), plan.data(), plan.size() * sizeof(DbLaunch)) == 0);
-            if (dup) continue;
-            seen.push_back(plan);
-            float total = 0.f;
-            for (int rep = 0; rep <= reps; ++rep) {
-                ctx->launches = 0;
-                for (const DbLaunch &L : plan) HIP_TRY(ctx, enqueue_launch(ctx, L, true));
-                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                collect_timing(ctx);
-                if (rep > 0) total += ctx->last_ms; // the first run warms up
-            }
-            if (total < best_ms) {
-                best_ms = total;
-                best = TuneChoice{(int8_t)ch, (int8_t)sched, (int8_t)wv};
-            }
-          }
+    struct Candidate {
+        int32_t chunk;
+        int sched, waves;
+    };
+    std::vector<Candidate> cands;
+    for (int32_t ch : chunks)
+        for (int sched : scheds)
+            for (int wv : widths) cands.push_back({ch, sched, wv});
+    for (const Candidate &c : cands) {
+        // whole passes: the launch tail is part of what is being tuned
+        std::vector<DbLaunch> plan;
+        int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, 0, db->ngroups,
+                             c.chunk, c.sched, plan, c.waves);
+        if (rc) return rc;
+        bool dup = false; // a candidate that plans the same launches as one already timed
+        for (const auto &q : seen)
+            dup = dup || (q.size() == plan.size() &&
+                          memcmp(q.data(), plan.data(), plan.size() * sizeof(DbLaunch)) == 0);
+        if (dup) continue;
+        seen.push_back(plan);
+        float total = 0.f;
+        for (int rep = 0; rep <= reps; ++rep) {
+            ctx->launches = 0;
+            for (const DbLaunch &L : plan) HIP_TRY(ctx, enqueue_launch(ctx, L, true));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            collect_timing(ctx);
+            if (rep > 0) total += ctx->last_ms; // the first run warms up
+        }
+        if (total < best_ms) {
+            best_ms = total;
+            best = TuneChoice{(int8_t)c.chunk, (int8_t)c.sched, (int8_t)c.waves};
         }
     }
     ctx->launches = saved_launches;
