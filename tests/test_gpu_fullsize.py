@@ -103,9 +103,34 @@ def test_protein_2M_full(eng):
     finally:
         eng.set_engine("batched")
     _same("batched vs plain", got, plain)
+    # the one-shot call cuts this 0.84 GB batch into two automatic slices
+    _same("one-shot call vs resident pass", eng.align(w.targets)[:3], got)
     ref = O.batch(0, w.query, w.targets.residues, w.targets.offsets, O.Matrix("blosum62"), 3, 1,
                   O.flags_from_str("FFTT"), nthreads=THREADS, striped=True)[:3]
     _same("batched vs oracle", got, ref)
+
+
+def test_protein_10M_one_shot_slices(eng):
+    """A 4.19 GB protein batch (10 M targets, near ish's 4 GiB default
+    batch) through one-shot calls: six automatic slices, five fixed 860 MiB
+    slices and no slicing give the same results; a sample of 20 k targets
+    matches the oracle."""
+    w = workloads.generate(2, 10_000_000, seed=1)
+    eng.set_engine("batched")
+    eng.set_query(w.query, "blosum62", 3, 1, "FFTT")
+    try:
+        auto = eng.align(w.targets)[:3]
+        eng.set_slice_bytes(860 << 20)
+        _same("860 MiB slices vs automatic", eng.align(w.targets)[:3], auto)
+        eng.set_slice_bytes(-1)
+        _same("unsliced vs automatic", eng.align(w.targets)[:3], auto)
+    finally:
+        eng.set_slice_bytes(0)
+    idx = np.linspace(0, w.targets.n - 1, 20_000).astype(np.int64)
+    res, off = _sample(w.targets, idx)
+    ref = O.batch(0, w.query, res, off, O.Matrix("blosum62"), 3, 1, O.flags_from_str("FFTT"),
+                  nthreads=THREADS, striped=True)[:3]
+    _same("automatic slices vs oracle on a sample", [a[idx] for a in auto], ref)
 
 
 def test_multiquery_64x4M_full(eng):
