@@ -10,6 +10,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -27,7 +28,8 @@ HIPCC_FLAGS = [
 ]
 
 HIP_SOURCES = ["ish_hip.cpp", "server.cpp", "search.cpp", "sg_kernels.hip", "sw_kernels.hip",
-               "serve_kernels.hip"]
+               "serve_kernels.hip",
+               "noalloc.hip"]
 
 
 def _run(cmd):
@@ -44,11 +46,13 @@ def hipcc() -> str:
 
 def build(verbose: bool = True) -> None:
     os.makedirs(LIB, exist_ok=True)
-    objs = []
-    for src in HIP_SOURCES:
-        obj = os.path.join(LIB, os.path.splitext(src)[0] + ".o")
-        _run([hipcc(), *HIP_FLAGS(), "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    objs = [os.path.join(LIB, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
+    # one compiler process per source file, at most 8 at a time
+    cmds = [[hipcc(), *HIP_FLAGS(), "-c", os.path.join(CSRC, src), "-o", obj]
+            for src, obj in zip(HIP_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     out = os.path.join(LIB, "libish_hip.so")
     tmp = out + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-lz", "-o", tmp])
