@@ -210,6 +210,27 @@ int32_t ish_hip_db_fetch(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *score,
                          int32_t *end_ref, int32_t *end_query,
                          uint8_t *status);
 
+/* No counterpart in the reference: this entry point is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). It is ish_hip_align_db with the
+ * results written to caller-owned DEVICE arrays (n_targets entries each,
+ * original order; d_status may not be NULL) and with the kernels' scratch,
+ * the work counter and the boundary rows between query chunks, in a static
+ * 512 MiB device array that the library holds on every device it runs on
+ * until the process ends. It allocates nothing, builds no paired profile
+ * (the pass reads the int16 profile of ish_hip_set_query), creates no event
+ * and copies nothing between host and device; it returns when the pass is
+ * done. Full occupancy holds for targets up to 512 residues, longer ones
+ * run on fewer workgroups, and a database whose longest target needs more
+ * boundary rows than the array holds for one workgroup (over 262,143
+ * residues) is rejected with ISH_E_RANGE. All calls on one device share the
+ * array and keep the last call's contents: calls that overlap on one device
+ * (two contexts of one device used at once) race. ish_hip_last_timing is
+ * not updated. */
+int32_t ish_hip_align_db_noalloc(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *d_score,
+                                 int32_t *d_end_ref, int32_t *d_end_query,
+                                 uint8_t *d_status);
+
 /* Many queries against one database (semi-global): every target of db is
  * scored against each of nq queries, query k being
  * queries[qoffsets[k] .. qoffsets[k+1]), with one shared matrix, gap

@@ -33,7 +33,7 @@ ABI_SYMBOLS = [
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
     "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_autotune", "ish_hip_tuning",
     "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_set_slice_bytes", "ish_hip_db_load", "ish_hip_db_load_ascii", "ish_hip_db_free", "ish_hip_db_size",
-    "ish_hip_align_db", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
+    "ish_hip_align_db", "ish_hip_align_db_noalloc", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
     "ish_hip_server_clock", "ish_hip_server_submit", "ish_hip_server_wait", "ish_hip_server_hits",
@@ -127,6 +127,7 @@ def lib():
         "ish_hip_db_free": ([vp], None),
         "ish_hip_db_size": ([vp, P(i64), P(i64)], i32),
         "ish_hip_align_db": ([vp, vp, i32p, i32p, i32p, u8p], i32),
+        "ish_hip_align_db_noalloc": ([vp, vp, vp, vp, vp, vp], i32),
         "ish_hip_align_db_async": ([vp, vp], i32),
         "ish_hip_sync": ([vp], i32),
         "ish_hip_db_fetch": ([vp, vp, i32p, i32p, i32p, u8p], i32),
@@ -370,6 +371,14 @@ class Engine:
     def align_db(self, db: "Database"):
         self.align_db_async(db)
         return self.fetch(db)
+
+    def align_db_noalloc(self, db: "Database", d_score: int, d_end_ref: int, d_end_query: int,
+                         d_status: int):
+        """ish_hip_align_db into caller-owned device arrays (raw device
+        addresses, n entries each), with the kernels' scratch in the
+        library's static device array: allocates nothing."""
+        self._check(self._L.ish_hip_align_db_noalloc(self._h, db._h, d_score, d_end_ref, d_end_query,
+                                                     d_status))
 
     def align_db_async(self, db: "Database"):
         self._check(self._L.ish_hip_align_db_async(self._h, db._h))

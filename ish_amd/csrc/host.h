@@ -14,6 +14,7 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/ish_hip.h"
 #include "engine.h"
 
 namespace ish {
@@ -205,6 +206,15 @@ struct ish_hip_db {
 };
 
 namespace ish {
+
+// The no-allocation pass (ish_hip_align_db_noalloc): the device address of
+// its static scratch array is looked up once per device, at context
+// creation with that device current (fixed_scratch_init); align_db_fixed
+// scores db into caller-owned device arrays with the counter and boundary
+// rows in that scratch, and waits for the pass.
+hipError_t fixed_scratch_init(int device);
+int32_t align_db_fixed(ish_hip_ctx *ctx, ish_hip_db *db, void *scratch, size_t scratch_bytes,
+                       int32_t *d_score, int32_t *d_end_ref, int32_t *d_end_q, uint8_t *d_status);
 
 // One query of a multi-query pass over a database.
 struct MultiItem {

@@ -262,6 +262,9 @@ extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
     if (hipGetDeviceProperties(&pr, device) != hipSuccess) return ISH_E_HIP;
     if (strncmp(pr.gcnArchName, "gfx950", 6) != 0) return ISH_E_NO_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return ISH_E_HIP;
+    // the device address of the no-allocation pass's static scratch, once
+    // per device
+    if (ish::fixed_scratch_init(device) != hipSuccess) return ISH_E_HIP;
     ish_hip_ctx *c = new ish_hip_ctx();
     c->device = device;
     c->cus = pr.multiProcessorCount;
@@ -1070,6 +1073,28 @@ struct DbLaunch {
 };
 enum { kSg16 = 0, kSg32 = 1, kSw16 = 2, kSw32 = 3, kSwPipe = 4, kSgHybrid = 5, kSgLock = 6 };
 
+// The launches of one db pass: at most one over the int32 groups and one
+// over the int16 groups. A fixed array, so that planning a pass allocates
+// nothing.
+struct DbPlan {
+    DbLaunch l[2];
+    int n = 0;
+    DbLaunch *begin() { return l; }
+    DbLaunch *end() { return l + n; }
+    const DbLaunch *begin() const { return l; }
+    const DbLaunch *end() const { return l + n; }
+    bool same(const DbPlan &o) const { return n == o.n && memcmp(l, o.l, sizeof(DbLaunch) * (size_t)n) == 0; }
+};
+
+// Caller-owned device scratch of a pass that must not allocate: the work
+// counter in its first 256 bytes, the boundary rows after it. The grid of
+// each launch shrinks to the wave slots the rest holds.
+struct FixedScratch {
+    uint8_t *base;
+    size_t bytes;
+};
+constexpr size_t kFixedCounterBytes = 256;
+
 static int32_t len_bucket(int32_t maxlen) {
     for (int32_t b = 0; b < kLenBuckets - 1; ++b)
         if (maxlen >= kLenBucketLo[b]) return b;
@@ -1143,12 +1168,15 @@ static int32_t ensure_pairs(ish_hip_ctx *ctx, const ish_hip_db *db) {
 // launch's parameters. `forced` >= 0 fixes (chunk, schedule) for the int16
 // groups (autotuning), and forced_waves > 0 the width of pipelined
 // semi-global groups; otherwise the tuning table or the defaults decide.
+// With `fixed`, the pass allocates nothing: no paired profile is built (the
+// launches read the int16 profile of set_query), and the counter and the
+// boundary rows live in the fixed scratch.
 static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, uint32_t flags,
                        int32_t g_lo, int32_t g_hi, int forced_chunk, int forced_sched,
-                       std::vector<DbLaunch> &out, int forced_waves = 0) {
-    out.clear();
+                       DbPlan &out, int forced_waves = 0, const FixedScratch *fixed = nullptr) {
+    out.n = 0;
     if (g_hi <= g_lo) return ISH_OK;
-    HIP_TRY(ctx, ctx->counter.ensure(256));
+    if (!fixed) HIP_TRY(ctx, ctx->counter.ensure(256));
     const bool local = ctx->algo == ISH_ALGO_LOCAL;
     // groups whose longest target exceeds the int16 bound come first
     // (descending lengths); they run in the int32 kernel.
@@ -1169,7 +1197,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         L.chunk = ctx->chunk;
         L.g_begin = a;
         L.g_end = m16;
-        out.push_back(L);
+        out.l[out.n++] = L;
     }
     const size_t pipe_lds = ctx->lds + sw_pipe_extra_lds();
     const size_t hybrid_lds = ctx->lds + sg_pipe_extra_lds();
@@ -1178,7 +1206,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     // hybrid semi-global launches, which then also serve the one-wave
     // schedule with no pipelined groups)
     int32_t pc = 0;
-    if (prof == ctx->prof_t.p) {
+    if (prof == ctx->prof_t.p && !fixed) {
         pc = ensure_pairs(ctx, db);
         if (pc < 0) return pc;
     }
@@ -1219,7 +1247,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         L.g_begin = g0;
         L.g_end = g1;
         L.n_long = n_long;
-        out.push_back(L);
+        out.l[out.n++] = L;
     };
     if (forced_chunk > 0) {
         add16(m16, g_hi, forced_chunk, forced_sched, forced_waves);
@@ -1263,9 +1291,24 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         }
         int grid;
         int64_t ss;
-        int32_t rc = size_grid(ctx, occ, db->group_maxlen[L.g_begin] + 3, &grid, &ss,
-                               L.kind == kSgHybrid ? L.waves : kBlock / 64);
-        if (rc) return rc;
+        const int wpb = L.kind == kSgHybrid ? L.waves : kBlock / 64;
+        const int64_t cols = db->group_maxlen[L.g_begin] + 3;
+        if (fixed) {
+            // as many workgroups as the fixed scratch holds boundary rows for
+            const int64_t per_block = (int64_t)wpb * std::max<int64_t>(cols, 4) * 64 * 8;
+            const int64_t fit = (int64_t)(fixed->bytes - kFixedCounterBytes) / per_block;
+            if (fit < 1)
+                return fail(ctx, ISH_E_RANGE,
+                            "a target of %lld residues needs %lld bytes of boundary rows per "
+                            "workgroup; the fixed scratch holds %zu",
+                            (long long)db->group_maxlen[L.g_begin], (long long)per_block,
+                            fixed->bytes - kFixedCounterBytes);
+            grid = (int)std::min<int64_t>((int64_t)occ * ctx->cus, fit);
+            ss = std::max<int64_t>(cols, 4) * 64;
+        } else {
+            int32_t rc = size_grid(ctx, occ, cols, &grid, &ss, wpb);
+            if (rc) return rc;
+        }
         L.grid = grid;
         L.slot_stride = ss;
     }
@@ -1285,12 +1328,12 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         p.res = db->d_res;
         p.lens = db->d_lens;
         p.orig = db->d_orig;
-        p.counter = (uint32_t *)ctx->counter.p;
+        p.counter = fixed ? (uint32_t *)fixed->base : (uint32_t *)ctx->counter.p;
         p.score = db->d_score;
         p.end_ref = db->d_end_ref;
         p.end_q = db->d_end_q;
         p.status = db->d_status;
-        p.bnd = (uint2 *)ctx->scratch.p;
+        p.bnd = fixed ? (uint2 *)(fixed->base + kFixedCounterBytes) : (uint2 *)ctx->scratch.p;
         p.slot_stride = L.slot_stride;
         p.g_begin = L.g_begin;
         p.g_end = L.g_end;
@@ -1337,15 +1380,15 @@ static int32_t launch_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, 
                          bool allow_graph = true) {
     ctx->launches = 0;
     if (db->ngroups == 0) return ISH_OK;
-    std::vector<DbLaunch> plan;
+    DbPlan plan;
     int32_t rc = plan_db(ctx, db, prof, flags, 0, db->ngroups, 0, 0, plan);
     if (rc) return rc;
     if (!(ctx->graphs && allow_graph)) {
         for (const DbLaunch &L : plan) HIP_TRY(ctx, enqueue_launch(ctx, L, true));
         return ISH_OK;
     }
-    std::vector<uint8_t> sig(plan.size() * sizeof(DbLaunch));
-    if (!plan.empty()) memcpy(sig.data(), plan.data(), sig.size());
+    std::vector<uint8_t> sig((size_t)plan.n * sizeof(DbLaunch));
+    if (plan.n) memcpy(sig.data(), plan.l, sig.size());
     if (!ctx->graph_exec || sig != ctx->graph_sig) {
         drop_graph(ctx);
         HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -1398,7 +1441,7 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     TuneChoice best{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO, 0};
     float best_ms = 1e30f;
     const int saved_launches = ctx->launches;
-    std::vector<std::vector<DbLaunch>> seen;
+    std::vector<DbPlan> seen;
     // pipelined semi-global groups also try both workgroup widths (a plan
     // that has none comes out the same for each and is timed once)
     const int widths[3] = {0, 2, 4};
@@ -1412,14 +1455,12 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
             for (int wv : widths) cands.push_back({ch, sched, wv});
     for (const Candidate &c : cands) {
         // whole passes: the launch tail is part of what is being tuned
-        std::vector<DbLaunch> plan;
+        DbPlan plan;
         int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, 0, db->ngroups,
                              c.chunk, c.sched, plan, c.waves);
         if (rc) return rc;
         bool dup = false; // a candidate that plans the same launches as one already timed
-        for (const auto &q : seen)
-            dup = dup || (q.size() == plan.size() &&
-                          memcmp(q.data(), plan.data(), plan.size() * sizeof(DbLaunch)) == 0);
+        for (const auto &q : seen) dup = dup || q.same(plan);
         if (dup) continue;
         seen.push_back(plan);
         float total = 0.f;
@@ -1461,6 +1502,7 @@ extern "C" int32_t ish_hip_align_db_async(ish_hip_ctx *ctx, ish_hip_db *db) {
 }
 
 int32_t ish::ctx_cus(const ish_hip_ctx *ctx) { return ctx ? ctx->cus : 0; }
+int32_t ish::ctx_device(const ish_hip_ctx *ctx) { return ctx ? ctx->device : -1; }
 
 int32_t ish::one_shot_begin(ish_hip_ctx *ctx) {
     if (ctx->copy) return ISH_OK;
@@ -1557,6 +1599,41 @@ extern "C" int32_t ish_hip_align_db(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *s
     int32_t rc = ish_hip_align_db_async(ctx, db);
     if (rc) return rc;
     return ish_hip_db_fetch(ctx, db, score, end_ref, end_q, status);
+}
+
+// ish_hip_align_db with caller-owned device outputs and the boundary rows
+// and work counter in fixed device scratch: no device memory is allocated,
+// no paired profile is built, no event is created and nothing is copied
+// between host and device. Waits for the pass.
+int32_t ish::align_db_fixed(ish_hip_ctx *ctx, ish_hip_db *db, void *scratch, size_t scratch_bytes,
+                            int32_t *d_score, int32_t *d_end_ref, int32_t *d_end_q,
+                            uint8_t *d_status) {
+    if (!ctx || !db) return ISH_E_INVALID;
+    if (!ctx->have_query) return fail(ctx, ISH_E_NO_QUERY, "no query set");
+    if (db->device != ctx->device) return fail(ctx, ISH_E_INVALID, "db belongs to another device");
+    if (db->n > 0 && (!d_score || !d_end_ref || !d_end_q || !d_status))
+        return fail(ctx, ISH_E_INVALID, "null device outputs");
+    if (ctx->engine != ISH_ENGINE_BATCHED)
+        return fail(ctx, ISH_E_INVALID, "the fixed-scratch pass runs the batched engine only");
+    if (!scratch || scratch_bytes <= kFixedCounterBytes)
+        return fail(ctx, ISH_E_INVALID, "no fixed scratch on device %d", ctx->device);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx->launches = 0;
+    if (db->ngroups == 0) return ISH_OK;
+    const FixedScratch fs{(uint8_t *)scratch, scratch_bytes};
+    DbPlan plan;
+    int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, 0, db->ngroups, 0, 0,
+                         plan, 0, &fs);
+    if (rc) return rc;
+    for (DbLaunch &L : plan) {
+        L.p.score = d_score;
+        L.p.end_ref = d_end_ref;
+        L.p.end_q = d_end_q;
+        L.p.status = d_status;
+        HIP_TRY(ctx, enqueue_launch(ctx, L, false));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return ISH_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -25,8 +25,9 @@ int32_t db_load_ascii_pieces(ish_hip_ctx *ctx, const HostPiece *pieces, size_t n
                              const uint64_t *offsets, int64_t n, int32_t matrix_kind,
                              ish_hip_db **out, int slot = -1);
 
-// Compute units of the context's device.
+// Compute units of the context's device, and the device.
 int32_t ctx_cus(const ish_hip_ctx *ctx);
+int32_t ctx_device(const ish_hip_ctx *ctx);
 
 // Bracket a one-shot call (slot loads, db_score_once_async,
 // db_fetch_after): begin creates the context's copy stream, end waits for
