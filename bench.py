@@ -29,15 +29,69 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS on 1 MI355X (bit-exact scores), batched vs plain; p50/p99 query latency"
 PEAK_INT16_TOPS = 78.6  # 256 CU x 2.4 GHz x 128 packed int16 lane-ops/clk
-# ops_per_cell is the textbook recurrence's count (8 semi-global, 9 local):
-# the kernels issue fewer per cell (offset forms, and no pairing
-# instruction with a paired profile), so `frac` measures algorithmic
-# throughput against the packed-int16 issue peak and can exceed 1
-ROOFLINE_NOTE = ("achieved = cells x textbook ops per cell / kernel time; the kernels issue fewer "
-                 "VALU ops per cell than the textbook recurrence, so frac can exceed 1")
 PEAK_INT32_TOPS = 39.3  # 256 CU x 2.4 GHz x 64 int32 lane-ops/clk
-OPS_PER_CELL_SG = 8     # semi-global Gotoh: 1 add + 2 max (H), 2 sub + 1 max (E), 1 sub + 1 max (F)
-OPS_PER_CELL_SW = 9     # local: the same plus the max with 0
+# Integer ops per cell that the kernels' recurrences need, which `frac`
+# is computed on. Semi-global, offset form H^ = H + (i+j)ge: Hdiag + S',
+# max E, max F, G = H + (ge-go), E = max(E, G), F = max(F, G): 6. Local,
+# column-offset form: the 8 ops of its recurrence (the extension and the
+# zero floor are one scalar per column) plus the column maximum the end
+# tracking needs: 9.
+OPS_PER_CELL_SG = 6
+OPS_PER_CELL_SW = 9
+# The textbook Gotoh count (1 add + 2 max for H; 2 sub + 1 max for E; 1 sub
+# + 1 max for F; +1 max with 0 for local), reported as frac_textbook.
+OPS_TEXTBOOK_SG = 8
+OPS_TEXTBOOK_SW = 9
+
+
+def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, **extra):
+    """The integer-VALU roofline of a pass: achieved = cells x ops per cell
+    of the kernel's recurrence / kernel time, against the packed-int16
+    issue peak; valu_busy (from the rocprof counters of the same workload)
+    is the share of the dispatch the VALU issued."""
+    ops = OPS_PER_CELL_SW if local else OPS_PER_CELL_SG
+    tb = OPS_TEXTBOOK_SW if local else OPS_TEXTBOOK_SG
+    achieved = cells * ops / kernel_s / 1e12
+    out = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
+           "frac": round(achieved / PEAK_INT16_TOPS, 4),
+           "frac_textbook": round(cells * tb / kernel_s / 1e12 / PEAK_INT16_TOPS, 4),
+           "ops_per_cell": ops, "ops_per_cell_textbook": tb, "traffic": traffic,
+           "valu_busy": valu_busy, "kernel_ms": round(kernel_s * 1e3, 3)}
+    out.update(extra)
+    return out
+
+
+def host_cpu():
+    """CPU model, threads per core and the CPUs this process may use."""
+    model, tpc = None, None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Thread(s) per core:"):
+                tpc = int(line.split(":", 1)[1])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "threads_per_core": tpc, "cpus_available": avail,
+            "cpus_on_host": os.cpu_count()}
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: the CPUs this process may use, at most
+    16 (the host share of one GPU on the measured boxes)."""
+    return min(16, host_cpu()["cpus_available"])
+
 
 # name -> (generator kind, default targets, matrix, algo, ends-free, description)
 WORKLOADS = {
@@ -58,7 +112,7 @@ WORKLOADS = {
                 "serving loop, blosum62, go 3 ge 1, threshold 0.8: device-resident DB of 10M protein "
                 "targets (lengths U[100,300], background frequencies); Poisson arrivals at 200 QPS, "
                 "query lengths log-normal (median 300, sigma 0.5, [20,2000]), ends-free FFTT/TTTT 50/50; "
-                "continuous batching (max 16 queries per batch, 2 batches in flight)"),
+                "continuous batching (shortest first with aging, up to 16 queries per batch, 2 batches in flight)"),
 }
 
 
@@ -74,20 +128,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(content_hash, tile, schedule):
-    """HBM bytes per dispatch of the dominant kernel, as rocprofv3's
-    FETCH_SIZE / WRITE_SIZE counters measured them for this exact workload,
-    tile and schedule (profiles/traffic.json, scaled by the copy-kernel
-    calibration); None when that combination was not profiled."""
+def measured_counters(content_hash, tile, schedule):
+    """(HBM bytes per dispatch, VALU busy share) of the dominant kernel, as
+    rocprofv3's counters measured them for this exact workload, tile and
+    schedule (profiles/traffic.json: FETCH_SIZE / WRITE_SIZE scaled by the
+    copy-kernel calibration, SQ_INSTS_VALU x 4 cycles over 1,024 SIMDs at
+    2.4 GHz against the dispatch time); (None, None) when that combination
+    was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         entries = json.load(open(path)).get("entries", [])
     except (OSError, ValueError):
-        return None
+        return None, None
     for e in entries:
         if (e.get("content_hash"), e.get("tile_rows"), e.get("schedule")) == (content_hash, tile, schedule):
-            return e["bytes"]
-    return None
+            return e.get("bytes"), e.get("valu_busy")
+    return None, None
 
 
 def serve_stream(ish_amd, srv, reqs):
@@ -167,22 +223,28 @@ def serving_phases(ish_amd, workloads, db, targets, args, count):
     return sat, sus
 
 
-def run_serving(args, ish_amd, workloads):
-    """Serving loop over a 10M-target resident protein DB."""
+def serving_block(args, ish_amd, workloads, with_cpu):
+    """The serving workload: a resident DB of 10M protein targets
+    (generator kind 5) answering Poisson arrivals; returns the serving
+    record, the sustained rate, and the CPU baseline when asked for."""
     kind, n_default, mkind, algo, flags, desc = WORKLOADS["serving"]
-    n_targets = args.targets or n_default
+    n_targets = args.serving_targets or n_default
     t0 = time.perf_counter()
     w = workloads.generate(kind, n_targets, seed=args.seed)
     nres = int(w.targets.offsets[-1])
-    log(f"workload: {w.targets.n} targets, {nres} residues, {time.perf_counter() - t0:.1f}s")
+    log(f"serving workload: {w.targets.n} targets, {nres} residues, {time.perf_counter() - t0:.1f}s")
     eng = ish_amd.Engine(0)
     t0 = time.perf_counter()
     db = eng.load_db(w.targets)
     load_s = time.perf_counter() - t0
     count = args.requests or 64
-    sat, sus = serving_phases(ish_amd, workloads, db, w.targets, args, count)
+    try:
+        sat, sus = serving_phases(ish_amd, workloads, db, w.targets, args, count)
+    finally:
+        db.close()
+        eng.close()
     cpu = None
-    if not args.no_cpu:
+    if with_cpu:
         from oracle import oracle as O
         reqs = workloads.serving_requests(count, seed=args.seed, rate=200.0)
         r0 = reqs[0]
@@ -195,34 +257,38 @@ def run_serving(args, ish_amd, workloads):
         res = np.concatenate([w.targets.residues[int(w.targets.offsets[i]):int(w.targets.offsets[i + 1])] for i in idx])
         t0 = time.perf_counter()
         _, _, _, used = O.batch(0, r0.query, res, off, m, 3, 1, O.flags_from_str(r0.ends_free),
-                                nthreads=min(16, os.cpu_count() or 1), striped=True)
+                                nthreads=cpu_threads(), striped=True)
         cpu_s = time.perf_counter() - t0
         sc = len(r0.query) * int(off[-1])
         cpu = {"value": round(sc / cpu_s / 1e9, 3), "unit": "GCUPS", "cores": int(used), "kind": "port",
                "sample": f"first request ({len(r0.query)} aa, {r0.ends_free}) x {n_s} of the {w.targets.n} "
-                         f"targets, {sc / 1e9:.2f} Gcells, {CPU_KIND}"}
+                         f"targets, {sc / 1e9:.2f} Gcells, {CPU_KIND}", **host_cpu()}
+    rec = {"workload": desc, "targets": w.targets.n, "residues": nres, "content_hash": w.content_hash(),
+           "db_load_ms": round(load_s * 1e3, 1),
+           "requests": f"{SERVING_HIT_FRACTION:.0%} windows of DB targets, the rest background-frequency queries",
+           "at_offered_200qps": sat, "at_70pct_capacity": sus}
+    return rec, sat, cpu
+
+
+def run_serving(args, ish_amd, workloads):
+    """Serving loop over a 10M-target resident protein DB."""
+    rec, sat, cpu = serving_block(args, ish_amd, workloads, not args.no_cpu)
+    count = args.requests or 64
     out = {
         "metric": METRIC, "value": sat["gcups_sustained"], "unit": "GCUPS", "n_gpus": 1,
         "steps": count, "warmup": 4, "ms_per_step": round(1e3 / sat["achieved_qps"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16",
         "data": "synthetic",
-        "config": {"workload": desc, "name": "serving", "targets": w.targets.n, "residues": nres,
-                   "seed": args.seed, "content_hash": w.content_hash()},
-        "db_load_ms": round(load_s * 1e3, 1),
-        "serving": {"requests": f"{SERVING_HIT_FRACTION:.0%} windows of DB targets, the rest background-frequency queries",
-                    "at_offered_200qps": sat, "at_70pct_capacity": sus},
-        "p50_ms": sus["p50_ms"], "p99_ms": sus["p99_ms"],
-        # batches on two streams overlap on the device, so per-launch event
-        # times double-count; the sustained wall-clock rate is the basis
-        "roofline": {"bound": "valu", "note": ROOFLINE_NOTE,
-                     "achieved": round(sat["gcups_sustained"] * OPS_PER_CELL_SG / 1e3, 3),
-                     "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
-                     "frac": round(sat["gcups_sustained"] * OPS_PER_CELL_SG / 1e3 / PEAK_INT16_TOPS, 4),
-                     "traffic": None, "ops_per_cell": OPS_PER_CELL_SG, "basis": "sustained wall clock"},
+        "config": {"workload": rec["workload"], "name": "serving", "targets": rec["targets"],
+                   "residues": rec["residues"], "seed": args.seed, "content_hash": rec["content_hash"]},
+        "db_load_ms": rec["db_load_ms"],
+        "serving": rec,
+        "p50_ms": rec["at_70pct_capacity"]["p50_ms"], "p99_ms": rec["at_70pct_capacity"]["p99_ms"],
+        # batches on several streams overlap on the device, so per-launch
+        # event times double-count; the sustained wall-clock rate is the basis
+        "roofline": roofline(sat["gcups_sustained"] * 1e9, 1.0, False, basis="sustained wall clock"),
         "cpu_baseline": cpu,
     }
-    db.close()
-    eng.close()
     print(json.dumps(out), flush=True)
 
 
@@ -329,16 +395,15 @@ def run_multiquery(args, ish_amd, workloads):
         for k in sel_q:
             t0 = time.perf_counter()
             os_, oer, oeq, used = O.batch(0, queries[k], res, off, m, 3, 1, O.flags_from_str(flags),
-                                         nthreads=min(16, os.cpu_count() or 1), striped=True)
+                                         nthreads=cpu_threads(), striped=True)
             cpu_s += time.perf_counter() - t0
             parity = parity and bool(np.array_equal(os_, s[k][idx]) and np.array_equal(oer, er[k][idx])
                                      and np.array_equal(oeq, eq[k][idx]))
         sample_cells = sum(int(qlens[k]) for k in sel_q) * int(off[-1])
         cpu = {"value": round(sample_cells / cpu_s / 1e9, 3), "unit": "GCUPS", "cores": int(used), "kind": "port",
                "sample": f"queries {sel_q} x {n_s} of the {w.targets.n} targets, {sample_cells / 1e9:.2f} Gcells, "
-                         f"{CPU_KIND}"}
+                         f"{CPU_KIND}", **host_cpu()}
         log(f"cpu oracle: {cpu['value']} GCUPS on {used} threads; parity on sample: {parity}")
-    achieved = cells * OPS_PER_CELL_SG / kern_s / 1e12
     out = {
         "metric": METRIC, "value": round(cells / step_s / 1e9, 2), "unit": "GCUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3),
@@ -349,10 +414,7 @@ def run_multiquery(args, ish_amd, workloads):
                    "content_hash": w.content_hash(), "tile_rows": args.tile or 32},
         "gcups_kernel": round(cells / kern_s / 1e9, 2), "db_load_ms": round(load_s * 1e3, 1),
         "parity_sample": parity,
-        "roofline": {"bound": "valu", "note": ROOFLINE_NOTE, "achieved": round(achieved, 3),
-                     "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
-                     "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": None,
-                     "ops_per_cell": OPS_PER_CELL_SG, "kernel_ms": round(kern_s * 1e3, 3)},
+        "roofline": roofline(cells, kern_s, False, *measured_counters(w.content_hash(), args.tile or 32, "multi")),
         "cpu_baseline": cpu,
     }
     db.close()
@@ -378,6 +440,8 @@ def main():
                     help="launch schedule used with --tile (ignored when the autotuner runs)")
     ap.add_argument("--requests", type=int, default=0,
                     help="serving requests per phase (0 = 64)")
+    ap.add_argument("--serving-targets", type=int, default=0,
+                    help="targets of the serving database (0 = 10M)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving phases of the default workload")
     args = ap.parse_args()
@@ -401,7 +465,6 @@ def main():
     kind, n_default, mkind, algo, flags, desc = WORKLOADS[args.workload]
     n_targets = args.targets or n_default
     local = algo == "striped-local"
-    ops_per_cell = OPS_PER_CELL_SW if local else OPS_PER_CELL_SG
     t0 = time.perf_counter()
     w = workloads.generate(kind, n_targets, seed=args.seed)
     log(f"workload: {w.targets.n} targets, {int(w.targets.offsets[-1])} residues, "
@@ -497,15 +560,14 @@ def main():
         m = O.Matrix(mkind)
         t0 = time.perf_counter()
         os_, oer, oeq, used = O.batch(1 if local else 0, w.query, res, off, m, 3, 1,
-                                     O.flags_from_str(flags), nthreads=min(16, os.cpu_count() or 1),
+                                     O.flags_from_str(flags), nthreads=cpu_threads(),
                                      striped=True)
         cpu_s = time.perf_counter() - t0
         sample_cells = Q * int(off[-1])
         cpu = {"value": round(sample_cells / cpu_s / 1e9, 3), "unit": "GCUPS", "cores": int(used),
                "kind": "port",
                "sample": f"{n_s} of the {w.targets.n} targets, evenly spaced in input order, "
-                         f"{sample_cells / 1e9:.2f} Gcells, {CPU_KIND}, "
-                         f"{os.cpu_count()} host threads"}
+                         f"{sample_cells / 1e9:.2f} Gcells, {CPU_KIND}", **host_cpu()}
         parity = bool(np.array_equal(os_, s[idx]) and np.array_equal(oer, er[idx]) and np.array_equal(oeq, eq[idx]))
         ms_ = int(m.values.reshape(m.A, m.A)[w.query, w.query].sum())
         emit_gpu = ish_amd.emit(algo, s[idx], er[idx], eq[idx], ms_, 0.8)
@@ -519,14 +581,15 @@ def main():
         parity = parity and bool(np.array_equal(emit_gpu, emit_ref))
         log(f"cpu oracle: {cpu['value']} GCUPS on {used} threads; parity on sample: {parity}")
 
+    db.close()
+    eng.close()
     serving = None
     if args.workload == "protein" and not args.no_serving:
-        sat, sus = serving_phases(ish_amd, workloads, db, w.targets, args, args.requests or 64)
-        serving = {"db": f"the {w.targets.n} resident targets above",
-                   "requests": f"{SERVING_HIT_FRACTION:.0%} windows of DB targets, the rest background-frequency queries",
-                   "at_offered_200qps": sat, "at_70pct_capacity": sus}
+        # the serving workload's own database (10M targets), in place of
+        # the 2M above
+        serving, _, _ = serving_block(args, ish_amd, workloads, False)
 
-    achieved = cells * ops_per_cell / kern_s / 1e12
+    traffic, valu_busy = measured_counters(w.content_hash(), best_tile, schedule)
     out = {
         "metric": METRIC,
         "value": round(gcups, 2),
@@ -557,15 +620,9 @@ def main():
         "p50_ms": serving["at_70pct_capacity"]["p50_ms"] if serving else None,
         "p99_ms": serving["at_70pct_capacity"]["p99_ms"] if serving else None,
         "parity_sample": parity,
-        "roofline": {"bound": "valu", "note": ROOFLINE_NOTE, "achieved": round(achieved, 3),
-                     "peak": PEAK_INT16_TOPS,
-                     "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                     "traffic": measured_traffic(w.content_hash(), best_tile, schedule),
-                     "ops_per_cell": ops_per_cell, "kernel_ms": round(kern_s * 1e3, 3)},
+        "roofline": roofline(cells, kern_s, local, traffic, valu_busy),
         "cpu_baseline": cpu,
     }
-    db.close()
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

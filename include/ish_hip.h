@@ -110,12 +110,17 @@ int64_t ish_hip_encode_and_score(int32_t matrix_kind, const uint8_t *ascii,
 
 /* Query setup: StripedSemiGlobalMatcher.__init__ /
  * StripedLocalMatcher.__init__ (striped_semi_global_matcher.mojo:41-75,
- * striped_local_matcher.mojo:33-59). `matrix` is the A*A int8 matrix of
- * matrix_kind (NULL selects the built-in one). gap_open / gap_extend are
- * the absolute values of the CLI flags (searcher_settings.mojo:218-219).
+ * striped_local_matcher.mojo:33-59). `matrix` / `matrix_len` are what
+ * GpuMatcher.matrix_bytes() / matrix_len() return (ishlib/matcher/
+ * __init__.mojo:79-86; striped_semi_global_matcher.mojo:139-144): the
+ * row-major int8 values of matrix_kind's ScoringMatrix and their count,
+ * which must be A*A for the kind's alphabet A (65536 ascii, 576 blosum62,
+ * 25 actgn and actgn0), else ISH_E_INVALID. NULL selects the built-in
+ * matrix of matrix_kind (matrix_len is then ignored). gap_open / gap_extend
+ * are the absolute values of the CLI flags (searcher_settings.mojo:218-219).
  * ends_free is only used by ISH_ALGO_SEMI_GLOBAL. */
 int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *enc_query,
-                          int32_t qlen, const int8_t *matrix, int32_t alphabet,
+                          int32_t qlen, const int8_t *matrix, int32_t matrix_len,
                           int32_t matrix_kind, int32_t gap_open,
                           int32_t gap_extend, uint32_t ends_free,
                           int32_t algo);
@@ -234,13 +239,14 @@ int32_t ish_hip_align_db_noalloc(ish_hip_ctx *ctx, ish_hip_db *db, int32_t *d_sc
 /* Many queries against one database (semi-global): every target of db is
  * scored against each of nq queries, query k being
  * queries[qoffsets[k] .. qoffsets[k+1]), with one shared matrix, gap
- * penalties and free ends. One persistent launch per cell width drains a
+ * penalties and free ends (matrix / matrix_len as in ish_hip_set_query).
+ * One persistent launch per cell width drains a
  * work queue of (query, group) tiles. Outputs are [nq][n_targets]
  * row-major. This is the batched multi-query mode of the engine; the
  * reference scores one pattern per run (do_search.mojo:24-274). */
 int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
                                const uint8_t *queries, const uint64_t *qoffsets,
-                               int32_t nq, const int8_t *matrix, int32_t alphabet,
+                               int32_t nq, const int8_t *matrix, int32_t matrix_len,
                                int32_t matrix_kind, int32_t gap_open,
                                int32_t gap_extend, uint32_t ends_free,
                                int32_t *score, int32_t *end_ref,
@@ -376,11 +382,16 @@ int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path,
  * the emitted targets.
  *
  * A worker thread owns `slots` launch slots. Whenever a slot is free and
- * requests are waiting, it takes them FIFO (at most max_batch queries and,
- * beyond the first, at most batch_cells cells) and enqueues one multi-query
- * pass plus the --score emission on the GPU; a batch never waits for the
- * batches already in flight. A request's results do not depend on the
- * other requests of its batch.
+ * requests are waiting, it orders them shortest first with aging (the
+ * request's estimated scoring time, from its cells and the measured rate,
+ * minus `aging` times the seconds it has waited) and takes them in that
+ * order while they fit: at most max_batch queries and, beyond the first,
+ * at most batch_cells cells, or by default as many as make the launch
+ * fill the device and then the slot's share of the waiting cells, up to
+ * batch_seconds of scoring. It enqueues one multi-query pass plus the
+ * --score emission on the GPU; a batch never waits for the batches already
+ * in flight. A request's results do not depend on the other requests of
+ * its batch.
  *
  * The database must stay loaded until ish_hip_server_destroy returns.
  * Submit / wait / hits / release may be called from any thread.
@@ -389,20 +400,26 @@ typedef struct ish_hip_server ish_hip_server;
 
 typedef struct ish_hip_server_config {
     int32_t matrix_kind;      /* ISH_MATRIX_* */
-    const int8_t *matrix;     /* alphabet*alphabet, NULL = built-in */
-    int32_t alphabet;
+    const int8_t *matrix;     /* as ish_hip_set_query: NULL = built-in */
+    int32_t matrix_len;       /* entries of matrix, A*A */
     int32_t gap_open;         /* absolute values, as ish_hip_set_query */
     int32_t gap_extend;
     float threshold;          /* --score (float32 rule of ish_hip_emit) */
     int32_t max_batch;        /* queries per batch, 1..256 (0 = 16) */
     int32_t slots;            /* batches in flight, 1..4 (0 = 2) */
     int64_t batch_cells;      /* cells per batch beyond its first query
-                                 (0 = 4e11) */
+                                 (0 = from the database and the queue) */
     int64_t hits_per_query;   /* device hit-list capacity per query; a query
                                  with more hits is emitted on the host from
                                  its full results (0 = 65536) */
     int32_t keep_scores;      /* 1: keep every request's full per-target
                                  score / end_ref / end_query */
+    double aging;             /* admission order: estimated seconds of
+                                 scoring minus aging x seconds waited,
+                                 smallest first (0 = 1.0; < 0 = FIFO) */
+    double batch_seconds;     /* automatic budget: at most this much
+                                 scoring per batch beyond its first query
+                                 (0 = 0.5) */
 } ish_hip_server_config;
 
 typedef struct ish_hip_request_info {
@@ -429,6 +446,8 @@ int32_t ish_hip_server_create(ish_hip_db *db, const ish_hip_server_config *cfg,
                               ish_hip_server **out);
 /* Finishes every submitted request, then frees the server. */
 void ish_hip_server_destroy(ish_hip_server *srv);
+/* The server's last error message, copied for the calling thread: the
+ * pointer stays valid until this thread calls it again. */
 const char *ish_hip_server_last_error(ish_hip_server *srv);
 /* Monotonic clock of the request timestamps, in seconds. */
 double ish_hip_server_clock(void);

@@ -60,11 +60,12 @@ class SearchStats(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("matrix_kind", ctypes.c_int32), ("matrix", ctypes.POINTER(ctypes.c_int8)),
-                ("alphabet", ctypes.c_int32), ("gap_open", ctypes.c_int32),
+                ("matrix_len", ctypes.c_int32), ("gap_open", ctypes.c_int32),
                 ("gap_extend", ctypes.c_int32), ("threshold", ctypes.c_float),
                 ("max_batch", ctypes.c_int32), ("slots", ctypes.c_int32),
                 ("batch_cells", ctypes.c_int64), ("hits_per_query", ctypes.c_int64),
-                ("keep_scores", ctypes.c_int32)]
+                ("keep_scores", ctypes.c_int32), ("aging", ctypes.c_double),
+                ("batch_seconds", ctypes.c_double)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -165,6 +166,16 @@ def lib():
 
 def _p(a, ct):
     return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _matrix_arg(values):
+    """(pointer, matrix_len) of a caller's scoring matrix, passed the way
+    GpuMatcher.matrix_bytes() / matrix_len() hand it over: the flat int8
+    values and their count; (None, 0) selects the built-in matrix."""
+    if values is None:
+        return None, 0, None
+    flat = np.ascontiguousarray(np.asarray(values, np.int8).reshape(-1))
+    return _p(flat, ctypes.c_int8), len(flat), flat
 
 
 def device_count() -> int:
@@ -333,12 +344,16 @@ class Engine:
         return int(t.value), inv.get(int(sc.value), "auto")
 
     def set_query(self, query: np.ndarray, matrix_kind: str, gap_open: int = 3,
-                  gap_extend: int = 1, ends_free="FFTT", algo="striped-semi-global"):
+                  gap_extend: int = 1, ends_free="FFTT", algo="striped-semi-global",
+                  matrix=None):
+        """matrix: the kind's score values (A x A or flat, int8), passed as
+        pointer and entry count; None selects the built-in matrix."""
         q = np.ascontiguousarray(query, np.uint8)
         qb = q if len(q) else np.zeros(1, np.uint8)
         fl = flags_from_str(ends_free) if isinstance(ends_free, str) else int(ends_free)
         self.algo = ALGO[algo] if isinstance(algo, str) else int(algo)
-        self._check(self._L.ish_hip_set_query(self._h, _p(qb, ctypes.c_uint8), len(q), None, 0,
+        mp, mlen, _keep = _matrix_arg(matrix)
+        self._check(self._L.ish_hip_set_query(self._h, _p(qb, ctypes.c_uint8), len(q), mp, mlen,
                                               MATRIX[matrix_kind], gap_open, gap_extend, fl,
                                               self.algo))
 
@@ -412,7 +427,7 @@ class Engine:
         return ts[:n], qs[:n]
 
     def align_db_multi(self, db: "Database", queries, matrix_kind: str, gap_open: int = 3,
-                       gap_extend: int = 1, ends_free="FFTT"):
+                       gap_extend: int = 1, ends_free="FFTT", matrix=None):
         """Scores every target of db against every query (semi-global) in one
         persistent launch per cell width; returns [nq, n] arrays."""
         qs = [np.ascontiguousarray(q, np.uint8) for q in queries]
@@ -423,8 +438,9 @@ class Engine:
         nq, n = len(qs), db.n
         out = [np.zeros(max(nq * n, 1), np.int32) for _ in range(3)]
         st = np.zeros(max(nq * n, 1), np.uint8)
+        mp, mlen, _keep = _matrix_arg(matrix)
         self._check(self._L.ish_hip_align_db_multi(
-            self._h, db._h, _p(qbuf, ctypes.c_uint8), _p(qoff, ctypes.c_uint64), nq, None, 0,
+            self._h, db._h, _p(qbuf, ctypes.c_uint8), _p(qoff, ctypes.c_uint64), nq, mp, mlen,
             MATRIX[matrix_kind], gap_open, gap_extend, fl, _p(out[0], ctypes.c_int32),
             _p(out[1], ctypes.c_int32), _p(out[2], ctypes.c_int32), _p(st, ctypes.c_uint8)))
         return tuple(a[: nq * n].reshape(nq, n) for a in (*out, st))
@@ -476,11 +492,14 @@ class Server:
 
     def __init__(self, db: Database, matrix_kind: str, gap_open: int = 3, gap_extend: int = 1,
                  threshold: float = 0.8, max_batch: int = 16, slots: int = 2,
-                 batch_cells: int = 0, hits_per_query: int = 0, keep_scores: bool = False):
+                 batch_cells: int = 0, hits_per_query: int = 0, keep_scores: bool = False,
+                 matrix=None, aging: float = 0.0, batch_seconds: float = 0.0):
         self._L = lib()
         self._db = db  # keeps the database alive while the server runs
-        cfg = ServerConfig(MATRIX[matrix_kind], None, 0, gap_open, gap_extend, threshold,
-                           max_batch, slots, batch_cells, hits_per_query, int(keep_scores))
+        mp, mlen, _keep = _matrix_arg(matrix)
+        cfg = ServerConfig(MATRIX[matrix_kind], mp, mlen, gap_open, gap_extend, threshold,
+                           max_batch, slots, batch_cells, hits_per_query, int(keep_scores),
+                           aging, batch_seconds)
         h = ctypes.c_void_p()
         rc = self._L.ish_hip_server_create(db._h, ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

@@ -86,10 +86,11 @@ bool build_pairs(const Profile &pf, const uint32_t present[8], int32_t max_class
                  PairProfile &out);
 // Alphabet size of a built-in matrix kind (-1 when unknown).
 int32_t alphabet_of(int32_t kind);
-// The A*A matrix to use: the caller's, or the built-in one of `kind` when
-// matrix is null. Returns 0 or an ISH_E_* code with a message in err.
-int32_t matrix_of(const int8_t *matrix, int32_t &A, int32_t kind, std::vector<int8_t> &mat,
-                  std::string &err);
+// The A*A matrix to use (A the alphabet of `kind`): the caller's, whose
+// matrix_len entries must be A*A, or the built-in one of `kind` when matrix
+// is null. Returns 0 or an ISH_E_* code with a message in err.
+int32_t matrix_of(const int8_t *matrix, int64_t matrix_len, int32_t kind, int32_t &A,
+                  std::vector<int8_t> &mat, std::string &err);
 // Longest target the packed int16 semi-global kernels may score for a
 // query (-1: none).
 int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s);

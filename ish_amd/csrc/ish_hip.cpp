@@ -468,30 +468,35 @@ bool ish::build_pairs(const Profile &pf, const uint32_t present[8], int32_t max_
     return true;
 }
 
-int32_t ish::matrix_of(const int8_t *matrix, int32_t &A, int32_t kind, std::vector<int8_t> &mat,
-                       std::string &err) {
+// matrix_len is the number of int8 entries the caller's matrix holds: the
+// length of ScoringMatrix.values that GpuMatcher.matrix_len() returns
+// (striped_semi_global_matcher.mojo:143-144), A*A for the kind's alphabet A.
+int32_t ish::matrix_of(const int8_t *matrix, int64_t matrix_len, int32_t kind, int32_t &A,
+                       std::vector<int8_t> &mat, std::string &err) {
+    A = alphabet_of(kind);
+    if (A < 0) {
+        err = "unknown matrix kind " + std::to_string(kind);
+        return ISH_E_INVALID;
+    }
     if (!matrix) {
-        A = alphabet_of(kind);
-        if (A < 0) {
-            err = "unknown matrix kind " + std::to_string(kind);
-            return ISH_E_INVALID;
-        }
         mat.resize((size_t)A * A);
         ish_hip_matrix(kind, mat.data(), A * A);
-    } else {
-        if (A <= 0 || A > 256) {
-            err = "alphabet must be 1..256";
-            return ISH_E_INVALID;
-        }
-        mat.assign(matrix, matrix + (size_t)A * A);
+        return ISH_OK;
     }
+    if (matrix_len != (int64_t)A * A) {
+        err = "matrix_len " + std::to_string(matrix_len) + " is not " + std::to_string(A) + "*" +
+              std::to_string(A) + " = " + std::to_string(A * A) + ", the entries of matrix kind " +
+              std::to_string(kind);
+        return ISH_E_INVALID;
+    }
+    mat.assign(matrix, matrix + (size_t)A * A);
     return ISH_OK;
 }
 
-static int32_t resolve_matrix(ish_hip_ctx *ctx, const int8_t *matrix, int32_t &A, int32_t kind,
-                              std::vector<int8_t> &mat) {
+static int32_t resolve_matrix(ish_hip_ctx *ctx, const int8_t *matrix, int32_t matrix_len, int32_t kind,
+                              int32_t &A, std::vector<int8_t> &mat) {
     std::string err;
-    const int32_t rc = matrix_of(matrix, A, kind, mat, err);
+    const int32_t rc = matrix_of(matrix, matrix_len, kind, A, mat, err);
     if (rc) return fail(ctx, rc, "%s", err.c_str());
     return ISH_OK;
 }
@@ -504,7 +509,7 @@ int32_t ish::neg_limit_of(int32_t go, int32_t mn) {
 }
 
 extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t Q,
-                                     const int8_t *matrix, int32_t A, int32_t kind, int32_t go,
+                                     const int8_t *matrix, int32_t matrix_len, int32_t kind, int32_t go,
                                      int32_t ge, uint32_t ends_free, int32_t algo) {
     if (!ctx) return ISH_E_INVALID;
     if (Q < 0 || (Q > 0 && !q)) return fail(ctx, ISH_E_INVALID, "bad query");
@@ -512,7 +517,8 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
         return fail(ctx, ISH_E_INVALID, "unknown algo %d", algo);
     if (go < 0 || ge < 0) return fail(ctx, ISH_E_INVALID, "gap penalties must be >= 0");
     std::vector<int8_t> mat;
-    int32_t rc = resolve_matrix(ctx, matrix, A, kind, mat);
+    int32_t A = 0;
+    int32_t rc = resolve_matrix(ctx, matrix, matrix_len, kind, A, mat);
     if (rc) return rc;
     for (int32_t j = 0; j < Q; ++j)
         if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query code %d at %d >= alphabet %d", q[j], j, A);
@@ -2012,7 +2018,7 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
 
 extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
                                           const uint8_t *queries, const uint64_t *qoffsets,
-                                          int32_t nq, const int8_t *matrix, int32_t A,
+                                          int32_t nq, const int8_t *matrix, int32_t matrix_len,
                                           int32_t kind, int32_t go, int32_t ge,
                                           uint32_t ends_free, int32_t *score, int32_t *end_ref,
                                           int32_t *end_q, uint8_t *status) {
@@ -2024,7 +2030,8 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
     if (nq == 0 || n == 0) return ISH_OK;
     if (!score || !end_ref || !end_q) return fail(ctx, ISH_E_INVALID, "null outputs");
     std::vector<int8_t> mat;
-    int32_t rc = resolve_matrix(ctx, matrix, A, kind, mat);
+    int32_t A = 0;
+    int32_t rc = resolve_matrix(ctx, matrix, matrix_len, kind, A, mat);
     if (rc) return rc;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int32_t mn = 0, mx = 0;

@@ -1,15 +1,27 @@
 // Serving loop of the ish_hip engine: a device-resident target database
 // answers a stream of semi-global queries with continuous batching.
 //
-// Requests are admitted FIFO into launch slots. A slot owns a stream, its
-// own work counter, boundary scratch, result buffers for up to max_batch
+// Requests are admitted into launch slots. A slot owns a stream, its own
+// work counter, boundary scratch, result buffers for up to max_batch
 // queries and a device hit list. Whenever a slot is free and requests are
-// waiting, the worker thread takes as many waiting requests as fit the
-// slot (max_batch queries, batch_cells cells) and enqueues one multi-query
-// pass for them plus the device-side --score emission; it does not wait
-// for earlier slots, so a new batch is queued on the GPU behind the one
-// running and its workgroups start as soon as the running launch's
-// workgroups retire. Only the emitted hits are copied back.
+// waiting, the worker thread orders the waiting requests shortest first
+// with aging (estimated scoring time minus `aging` times the time waited,
+// so that a long request is not passed over for ever), takes them in that
+// order while they fit the slot, and enqueues one multi-query pass for
+// them plus the device-side --score emission. It does not wait for earlier
+// slots, so a new batch is queued on the GPU behind the one running and its
+// workgroups start as soon as the running launch's workgroups retire. Only
+// the emitted hits are copied back.
+//
+// What fits a batch: at most max_batch queries; and, beyond its first
+// query, either a fixed cell budget (batch_cells) or, by default, a budget
+// from the database and the queue. A batch always grows until its work
+// items (groups of 128 targets per query) give every resident wave four of
+// them, since a smaller launch leaves the device idle; past that, it takes
+// its share of the waiting cells (the waiting cells over the slots), up to
+// batch_seconds of scoring at the measured rate. On a database large
+// enough that one query fills the device, batches therefore hold one query
+// while the queue is short, and grow when a queue builds up.
 //
 // Per-request results are independent of which other requests share the
 // batch: every (query, target) cell is computed by the same kernels with
@@ -25,7 +37,6 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
-#include <deque>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -86,12 +97,16 @@ struct ish_hip_server {
     float threshold = 0.8f;
     int32_t max_batch = 16, nslots = 2;
     int64_t batch_cells = 0, cap = 0;
+    double aging = 1.0, batch_seconds = 0.5;
+    bool fifo = false;
+    // scoring rate in cells per second, measured over finished batches
+    double rate = 1e13;
     bool keep = false;
     std::vector<int8_t> diag;
     // state
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<std::shared_ptr<Request>> pending;
+    std::vector<std::shared_ptr<Request>> pending;
     std::unordered_map<int64_t, std::shared_ptr<Request>> reqs;
     std::vector<Slot> slots;
     int64_t next_id = 1;
@@ -256,6 +271,49 @@ static int32_t collect_slot(ish_hip_server *s, Slot &sl, std::string &err, doubl
     return ISH_OK;
 }
 
+// Moves the waiting requests that form the next batch into the free slot
+// sl (lock held): shortest estimated scoring time first, less `aging`
+// times the time waited (FIFO when aging < 0), while they fit.
+static void admit(ish_hip_server *s, Slot &sl) {
+    const double now = now_s();
+    auto &P = s->pending;
+    std::vector<std::pair<double, size_t>> order(P.size());
+    int64_t backlog = 0;
+    for (size_t k = 0; k < P.size(); ++k) {
+        const Request &r = *P[k];
+        const double key = s->fifo ? r.t_submit : (double)r.cells / s->rate - s->aging * (now - r.t_submit);
+        order[k] = {key, k};
+        backlog += r.cells;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<double, size_t> &a, const std::pair<double, size_t> &b) {
+                         return a.first < b.first;
+                     });
+    // work items that keep every resident wave (8 per CU) busy with four
+    const int64_t fill_items = (int64_t)s->cus * 8 * 4;
+    const int64_t per_query_items = std::max<int64_t>(s->db->ngroups, 1);
+    const int64_t cap = s->batch_cells > 0
+                            ? s->batch_cells
+                            : (int64_t)std::min((double)backlog / s->nslots, s->batch_seconds * s->rate);
+    int64_t cells = 0; // beyond the first query
+    std::vector<bool> taken(P.size(), false);
+    for (const auto &o : order) {
+        if ((int32_t)sl.reqs.size() >= s->max_batch) break;
+        const auto &r = P[o.second];
+        if (!sl.reqs.empty()) {
+            const bool fills = (int64_t)sl.reqs.size() * per_query_items >= fill_items;
+            if ((fills || s->batch_cells > 0) && cells + r->cells > cap) break;
+            cells += r->cells;
+        }
+        sl.reqs.push_back(r);
+        taken[o.second] = true;
+    }
+    size_t w = 0;
+    for (size_t k = 0; k < P.size(); ++k)
+        if (!taken[k]) P[w++] = P[k];
+    P.resize(w);
+}
+
 static void worker_main(ish_hip_server *s) {
     (void)hipSetDevice(s->device);
     std::unique_lock<std::mutex> lk(s->mu);
@@ -280,6 +338,11 @@ static void worker_main(ish_hip_server *s) {
                 s->st_cells += r->cells;
             }
             s->st_kernel_ms += kms;
+            if (rc == ISH_OK && kms > 0) {
+                int64_t bc = 0;
+                for (auto &r : sl.reqs) bc += r->cells;
+                s->rate = 0.7 * s->rate + 0.3 * ((double)bc / (kms * 1e-3));
+            }
             sl.reqs.clear();
             sl.busy = false;
             s->cv_done.notify_all();
@@ -287,14 +350,7 @@ static void worker_main(ish_hip_server *s) {
         // 2. admission into free slots
         for (Slot &sl : s->slots) {
             if (sl.busy || s->pending.empty()) continue;
-            int64_t cells = 0;
-            while (!s->pending.empty() && (int32_t)sl.reqs.size() < s->max_batch) {
-                const auto &r = s->pending.front();
-                if (!sl.reqs.empty() && cells + r->cells > s->batch_cells) break;
-                cells += r->cells;
-                sl.reqs.push_back(r);
-                s->pending.pop_front();
-            }
+            admit(s, sl);
             sl.batch_id = s->next_batch++;
             const double t = now_s();
             for (auto &r : sl.reqs) {
@@ -349,9 +405,9 @@ extern "C" int32_t ish_hip_server_create(ish_hip_db *db, const ish_hip_server_co
     auto s = std::make_unique<ish_hip_server>();
     s->db = db;
     s->device = db->device;
-    int32_t A = cfg->alphabet;
+    int32_t A = 0;
     std::string err;
-    int32_t rc = matrix_of(cfg->matrix, A, cfg->matrix_kind, s->mat, err);
+    int32_t rc = matrix_of(cfg->matrix, cfg->matrix_len, cfg->matrix_kind, A, s->mat, err);
     if (rc) return rc;
     s->A = A;
     if (cfg->gap_open < 0 || cfg->gap_extend < 0) return ISH_E_INVALID;
@@ -367,7 +423,10 @@ extern "C" int32_t ish_hip_server_create(ish_hip_db *db, const ish_hip_server_co
     s->max_batch = cfg->max_batch > 0 ? cfg->max_batch : 16;
     s->nslots = cfg->slots > 0 ? cfg->slots : 2;
     if (s->max_batch > 256 || s->nslots > 4) return ISH_E_RANGE;
-    s->batch_cells = cfg->batch_cells > 0 ? cfg->batch_cells : (int64_t)4e11;
+    s->batch_cells = cfg->batch_cells > 0 ? cfg->batch_cells : 0; // 0: from the database and the queue
+    s->aging = cfg->aging > 0 ? cfg->aging : 1.0;
+    s->fifo = cfg->aging < 0;
+    s->batch_seconds = cfg->batch_seconds > 0 ? cfg->batch_seconds : 0.5;
     s->cap = cfg->hits_per_query > 0 ? cfg->hits_per_query : 65536;
     s->keep = cfg->keep_scores != 0;
     if (hipSetDevice(s->device) != hipSuccess) return ISH_E_HIP;
@@ -422,10 +481,17 @@ extern "C" void ish_hip_server_destroy(ish_hip_server *s) {
     delete s;
 }
 
+// The message is copied under the lock into a string of the calling thread:
+// other threads may set a new error at any time, and the pointer stays
+// valid until this thread's next call.
 extern "C" const char *ish_hip_server_last_error(ish_hip_server *s) {
     if (!s) return "null server";
-    std::lock_guard<std::mutex> g(s->mu);
-    return s->err.c_str();
+    thread_local std::string msg;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        msg = s->err;
+    }
+    return msg.c_str();
 }
 
 extern "C" double ish_hip_server_clock(void) { return now_s(); }
