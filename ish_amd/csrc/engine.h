@@ -87,7 +87,8 @@ struct ScoreParams {
     // pprof[pair * pqpad + j] = (S[A][j] + ge) | (S[B][j] + ge) << 16 for
     // the profile classes A, B of pair = A * pnrows + B (0 past the query);
     // plut = byte offsets of a class's pair rows in a wave's LDS copy, for
-    // tile 16 then tile 32, each as [A: 256][B: 256] uint16
+    // tiles 16, 32 and 40 (pair_table_index), each as [A: 256][B: 256]
+    // uint16
     const int32_t *pprof;
     const uint16_t *plut;
     int32_t pnrows, pqpad;
@@ -216,11 +217,18 @@ int occupancy_sg_batched_i32(int chunk, int block, size_t lds, uint32_t flags);
 int occupancy_sg_plain(int block, size_t lds, uint32_t flags);
 
 // Rows of the query held in registers per lane (chunk height). The batched
-// kernels are compiled for 16 and 32; the profile is padded to a multiple
-// of kChunkAlign so that either fits.
+// kernels are compiled for 16 and 32, the hybrid semi-global kernel also
+// for 40 (a 150-row query is then four chunks, one round of a four-wave
+// pipeline). Profiles are padded to a multiple of both kChunkAlign and
+// kChunkAlign40, so that every tile height's last chunk lies inside them.
 constexpr int kChunkAlign = 32;
+constexpr int kChunkAlign40 = 40;
 constexpr int kChunkDefault = 32;
 constexpr int kChunkPlain = 32;
+// Pair tables are stored per tile height, 512 uint16 each, in this order.
+constexpr int kPairTables = 3;
+constexpr int pair_table_index(int K) { return K == 16 ? 0 : K == 32 ? 1 : 2; }
+constexpr int pair_table_rows(int index) { return index == 0 ? 16 : index == 1 ? 32 : 40; }
 
 // Smallest slice of a one-shot batch cut into input-order slices (each
 // uploaded while the slice before is scored).

@@ -328,8 +328,8 @@ extern "C" const char *ish_hip_last_error(const ish_hip_ctx *ctx) {
 
 extern "C" int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane) {
     if (!ctx) return ISH_E_INVALID;
-    if (query_rows_per_lane != 16 && query_rows_per_lane != 32)
-        return fail(ctx, ISH_E_INVALID, "tile height must be 16 or 32 (got %d)", query_rows_per_lane);
+    if (query_rows_per_lane != 16 && query_rows_per_lane != 32 && query_rows_per_lane != 40)
+        return fail(ctx, ISH_E_INVALID, "tile height must be 16, 32 or 40 (got %d)", query_rows_per_lane);
     ctx->chunk = query_rows_per_lane;
     ctx->tuned = false; // an explicit tile overrides the tuning table
     return ISH_OK;
@@ -403,7 +403,8 @@ void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &
         }
         pf.lut[c] = (uint16_t)it->second;
     }
-    pf.qpad = (Q + kChunkAlign - 1) / kChunkAlign * kChunkAlign;
+    pf.qpad = std::max((Q + kChunkAlign - 1) / kChunkAlign * kChunkAlign,
+                       (Q + kChunkAlign40 - 1) / kChunkAlign40 * kChunkAlign40);
     // rows start 16 bytes apart modulo 128: 16-byte slices of different rows
     // spread over the LDS banks and every slice read is 16-byte aligned
     pf.stride = pf.qpad + 8;
@@ -455,10 +456,14 @@ bool ish::build_pairs(const Profile &pf, const uint32_t present[8], int32_t max_
             for (int32_t j = 0; j < pf.qpad; ++j)
                 w[j] = (int32_t)((uint32_t)(uint16_t)ra[j] | ((uint32_t)(uint16_t)rb[j] << 16));
         }
-    // codes the database does not hold take class 0's rows (never read)
-    out.table.assign(1024, 0);
-    for (int ti = 0; ti < 2; ++ti) {
-        const int32_t rowb = pair_row_bytes(ti ? 32 : 16);
+    // codes the database does not hold take class 0's rows (never read);
+    // a tile height whose last pair row lies beyond the uint16 offsets
+    // keeps a zero table (its paired launches take at most kPairMaxRows
+    // classes, whose rows fit)
+    out.table.assign((size_t)kPairTables * 512, 0);
+    for (int ti = 0; ti < kPairTables; ++ti) {
+        const int32_t rowb = pair_row_bytes(pair_table_rows(ti));
+        if ((int64_t)(C * C - 1) * rowb > 65535) continue;
         for (int c = 0; c < 256; ++c) {
             const int32_t r = std::max(remap[(size_t)(pf.lut[c] / rw)], 0);
             out.table[(size_t)ti * 512 + c] = (uint16_t)(r * C * rowb);
@@ -1200,7 +1205,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     if (m16 > a) {
         DbLaunch L{};
         L.kind = local ? kSw32 : kSg32;
-        L.chunk = ctx->chunk;
+        L.chunk = ctx->chunk == 40 ? 32 : ctx->chunk; // the int32 kernels have 16 and 32
         L.g_begin = a;
         L.g_end = m16;
         out.l[out.n++] = L;
@@ -1218,6 +1223,12 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     }
     auto add16 = [&](int32_t g0, int32_t g1, int chunk, int sched, int waves) {
         if (g1 <= g0) return;
+        // the 40-row tile exists for the hybrid semi-global kernel only
+        // (per-wave paired or over the int16 profile); the others take 32
+        if (chunk == 40 &&
+            (local || pc > kPairMaxRows ||
+             !(hybrid_lds <= kLdsLimit || (pc > 0 && sg_hybrid_paired_lds(40, pc, 4) <= kLdsLimit))))
+            chunk = 32;
         DbLaunch L{};
         bool pipe = false;
         int32_t n_long = 0;
@@ -1233,6 +1244,11 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
             else if (sched == ISH_SCHEDULE_AUTO) n_long = long_groups(ctx, db, g0, g1, chunk, flags);
         }
         L.kind = pipe ? kSwPipe : local ? kSw16 : sg_lock ? kSgLock : (n_long > 0 || sg_paired) ? kSgHybrid : kSg16;
+        if (chunk == 40 && L.kind != kSgHybrid) {
+            // no 40-row variant of these launches: the hybrid kernel with no
+            // pipelined group is the one-wave schedule
+            L.kind = kSgHybrid;
+        }
         L.waves = 4;
         if (L.kind == kSgHybrid && n_long > 0) {
             // a pipelined group runs its chunks round by round over the
@@ -1442,7 +1458,7 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     if (ctx->engine != ISH_ENGINE_BATCHED) return fail(ctx, ISH_E_INVALID, "autotuning needs the batched engine");
     if (reps <= 0) reps = 2;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const int32_t chunks[2] = {32, 16};
+    const int32_t chunks[3] = {32, 16, 40};
     const int scheds[3] = {ISH_SCHEDULE_WAVE, ISH_SCHEDULE_AUTO, ISH_SCHEDULE_WORKGROUP};
     TuneChoice best{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO, 0};
     float best_ms = 1e30f;
@@ -2074,7 +2090,8 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
     mb.host_desc = &h_desc;
     ctx->launches = 0;
     std::string err;
-    rc = enqueue_multi(db, items, go, ge, mn, mx, ctx->chunk, ctx->cus, o_score, o_er, o_eq, o_st,
+    rc = enqueue_multi(db, items, go, ge, mn, mx, ctx->chunk == 40 ? 32 : ctx->chunk, ctx->cus, o_score,
+                       o_er, o_eq, o_st,
                        mb, ctx->stream, err,
                        [ctx](bool begin) { return begin ? timed_begin(ctx) : timed_end(ctx); });
     if (rc) {

@@ -108,7 +108,7 @@ __device__ __forceinline__ const short8 *pair_row(const uint8_t *pbuf, const uin
 // and waits for the workgroup.
 template <int K>
 __device__ __forceinline__ void stage_pair_table(const uint16_t *plut, uint8_t *smem) {
-    const uint32_t *src = (const uint32_t *)(plut + (K == 32 ? 512 : 0));
+    const uint32_t *src = (const uint32_t *)(plut + pair_table_index(K) * 512);
     for (int x = threadIdx.x; x < 256; x += blockDim.x) ((uint32_t *)smem)[x] = src[x];
     __syncthreads();
 }

@@ -1222,10 +1222,12 @@ static hipError_t launch_hybrid_w(const ScoreParams &p, int waves, int grid, siz
 hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int waves, int grid,
                                 size_t lds, hipStream_t st) {
     if (paired)
-        return chunk == 16 ? launch_hybrid_w<16, true>(p, waves, grid, lds, st)
-                           : launch_hybrid_w<32, true>(p, waves, grid, lds, st);
-    return chunk == 16 ? launch_hybrid_w<16, false>(p, waves, grid, lds, st)
-                       : launch_hybrid_w<32, false>(p, waves, grid, lds, st);
+        return chunk == 16   ? launch_hybrid_w<16, true>(p, waves, grid, lds, st)
+               : chunk == 40 ? launch_hybrid_w<40, true>(p, waves, grid, lds, st)
+                             : launch_hybrid_w<32, true>(p, waves, grid, lds, st);
+    return chunk == 16   ? launch_hybrid_w<16, false>(p, waves, grid, lds, st)
+           : chunk == 40 ? launch_hybrid_w<40, false>(p, waves, grid, lds, st)
+                         : launch_hybrid_w<32, false>(p, waves, grid, lds, st);
 }
 
 template <int K>
@@ -1343,8 +1345,12 @@ template <int K, bool PAIRED> static int occ_hybrid_w(int waves, size_t lds, uin
 
 int occupancy_sg_hybrid_i16(int chunk, bool paired, int waves, size_t lds, uint32_t flags) {
     if (paired)
-        return chunk == 16 ? occ_hybrid_w<16, true>(waves, lds, flags) : occ_hybrid_w<32, true>(waves, lds, flags);
-    return chunk == 16 ? occ_hybrid_w<16, false>(waves, lds, flags) : occ_hybrid_w<32, false>(waves, lds, flags);
+        return chunk == 16   ? occ_hybrid_w<16, true>(waves, lds, flags)
+               : chunk == 40 ? occ_hybrid_w<40, true>(waves, lds, flags)
+                             : occ_hybrid_w<32, true>(waves, lds, flags);
+    return chunk == 16   ? occ_hybrid_w<16, false>(waves, lds, flags)
+           : chunk == 40 ? occ_hybrid_w<40, false>(waves, lds, flags)
+                         : occ_hybrid_w<32, false>(waves, lds, flags);
 }
 
 template <int K> static int occ_lock(size_t lds, uint32_t flags) {

@@ -5,6 +5,7 @@ the oracle on every target or on a large even sample.
 
 - 200k nucleotide line records, 150 nt query, actgn, FFTT.
   - Every target's score and end coordinates against the oracle.
+  - Tiles of 40 and 16 rows equal to tile 32 on every target.
   - The plain int32 engine equal to the batched one.
   - The emitted set at 0.8 equal to the oracle's emission rule.
 - 2M protein targets, 350 aa query, blosum62, FFTT.
@@ -70,7 +71,11 @@ def test_nucleotide_200k_full(eng):
     db = eng.load_db(w.targets)
     try:
         got = _scores(eng, db)
+        for tile in (40, 16):  # four 40-row chunks: one round of the four-wave pipeline
+            eng.set_tile(tile)
+            _same(f"tile {tile} vs tile 32", _scores(eng, db), got)
     finally:
+        eng.set_tile(32)
         db.close()
     eng.set_engine("plain")
     try:

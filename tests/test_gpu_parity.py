@@ -76,7 +76,9 @@ def test_golden_vectors(eng, engine):
 
 @pytest.mark.parametrize("engine,tile,schedule", [("batched", 32, "auto"), ("batched", 16, "auto"),
                                                   ("batched", 32, "wave"), ("batched", 32, "workgroup"),
-                                                  ("batched", 16, "workgroup"), ("plain", 32, "auto")])
+                                                  ("batched", 16, "workgroup"), ("batched", 40, "auto"),
+                                                  ("batched", 40, "wave"), ("batched", 40, "workgroup"),
+                                                  ("plain", 32, "auto")])
 @pytest.mark.parametrize("kind", ["actgn", "actgn0", "blosum62", "ascii"])
 def test_random_all_flags(eng, engine, tile, kind, schedule):
     rng = np.random.default_rng(zlib.crc32(f"{engine}-{kind}".encode()))
@@ -144,7 +146,7 @@ def test_hybrid_long_groups_among_short(eng, flags):
           "workgroup")
 
 
-@pytest.mark.parametrize("tile", [16, 32])
+@pytest.mark.parametrize("tile", [16, 32, 40])
 def test_two_wave_pipelines(eng, tile):
     """Chunk counts below 8 that are 1 or 2 modulo 4 run pipelined groups
     over two-wave workgroups, 9 and 10 over four (the workgroup schedule
@@ -217,7 +219,7 @@ def test_nucleotide_workload(eng, engine):
     assert emitted.sum() > 0
 
 
-@pytest.mark.parametrize("engine,tile", [("batched", 32), ("batched", 16), ("plain", 32)])
+@pytest.mark.parametrize("engine,tile", [("batched", 32), ("batched", 16), ("batched", 40), ("plain", 32)])
 def test_protein_workload(eng, engine, tile):
     w = workloads.generate(2, 6000, seed=6)
     for flags in ("FFTT", "TTTT"):
