@@ -178,6 +178,40 @@ int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
                             const uint64_t *offsets, int64_t n, int32_t *score,
                             int32_t *end_ref, int32_t *end_query,
                             uint8_t *status);
+/* One batch with start coordinates: the per-batch step of the reference's
+ * GPU path in one call, i.e. batch_match_coarse, then the threshold and
+ * find_start for every target that passes (ishlib/gpu/process_batch.mojo:
+ * 129-160 calling cpu/process_batch.mojo:36-68). Scores as
+ * ish_hip_align_batch; then emit[t] = 1 for the targets that pass the
+ * matchers' emission rule at `threshold` for `max_score` (ish_hip_emit, the
+ * CPU matchers' float32 rule), and target_start / query_start as
+ * ish_hip_find_starts for them, -1 for the others. The reverse passes run
+ * on the GPU over the emitted targets' prefixes, gathered on the device
+ * from the batch already uploaded (slice by slice), so no residue is
+ * uploaded twice. Batched engine only. */
+int32_t ish_hip_align_batch_starts(ish_hip_ctx *ctx, const uint8_t *residues,
+                                   const uint64_t *offsets, int64_t n,
+                                   int64_t max_score, float threshold,
+                                   int32_t *score, int32_t *end_ref,
+                                   int32_t *end_query, uint8_t *status,
+                                   uint8_t *emit, int32_t *target_start,
+                                   int32_t *query_start);
+/* The same asynchronously, for overlapping host work with the GPU as the
+ * reference's runners do while a launch is in flight (ishlib/gpu/
+ * process_batch.mojo:124-128): submit starts the batch on a host thread of
+ * the context and returns; with emit, target_start and query_start NULL it
+ * is ish_hip_align_batch, else ish_hip_align_batch_starts. wait returns
+ * when it is done, with its return code. One batch per context at a time;
+ * until wait returns, the caller keeps every array alive and calls nothing
+ * else on the context. */
+int32_t ish_hip_align_batch_submit(ish_hip_ctx *ctx, const uint8_t *residues,
+                                   const uint64_t *offsets, int64_t n,
+                                   int64_t max_score, float threshold,
+                                   int32_t *score, int32_t *end_ref,
+                                   int32_t *end_query, uint8_t *status,
+                                   uint8_t *emit, int32_t *target_start,
+                                   int32_t *query_start);
+int32_t ish_hip_align_batch_wait(ish_hip_ctx *ctx);
 /* Slice size of ish_hip_align_batch in residue bytes: 0 (the default)
  * sizes slices automatically, a positive value is a fixed size, a
  * negative value turns slicing off. Automatic slices hold at least 64 MiB

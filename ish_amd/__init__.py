@@ -32,7 +32,8 @@ ABI_SYMBOLS = [
     "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
     "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_autotune", "ish_hip_tuning",
-    "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_set_slice_bytes", "ish_hip_db_load", "ish_hip_db_load_ascii", "ish_hip_db_free", "ish_hip_db_size",
+    "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_align_batch_starts",
+    "ish_hip_align_batch_submit", "ish_hip_align_batch_wait", "ish_hip_set_slice_bytes", "ish_hip_db_load", "ish_hip_db_load_ascii", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_noalloc", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
     "ish_hip_last_timing", "ish_hip_emit", "ish_hip_find_starts", "ish_hip_align_db_multi",
     "ish_hip_server_create", "ish_hip_server_destroy", "ish_hip_server_last_error",
@@ -123,6 +124,11 @@ def lib():
         "ish_hip_set_graphs": ([vp, i32], i32),
         "ish_hip_set_slice_bytes": ([vp, ctypes.c_int64], i32),
         "ish_hip_align_batch": ([vp, u8p, u64p, i64, i32p, i32p, i32p, u8p], i32),
+        "ish_hip_align_batch_starts": ([vp, u8p, u64p, i64, i64, ctypes.c_float, i32p, i32p, i32p, u8p,
+                                        u8p, i32p, i32p], i32),
+        "ish_hip_align_batch_submit": ([vp, u8p, u64p, i64, i64, ctypes.c_float, i32p, i32p, i32p, u8p,
+                                        u8p, i32p, i32p], i32),
+        "ish_hip_align_batch_wait": ([vp], i32),
         "ish_hip_db_load": ([vp, u8p, u64p, i64, P(vp)], i32),
         "ish_hip_db_load_ascii": ([vp, u8p, u64p, i64, i32, P(vp)], i32),
         "ish_hip_db_free": ([vp], None),
@@ -293,6 +299,7 @@ class Engine:
             raise IshError(f"ish_hip_create failed: {ERRORS.get(rc, rc)}")
         self._h = h
         self.algo = 0
+        self._pending = None
 
     def close(self):
         if getattr(self, "_h", None):
@@ -366,6 +373,55 @@ class Engine:
             _p(out[0], ctypes.c_int32), _p(out[1], ctypes.c_int32), _p(out[2], ctypes.c_int32),
             _p(st, ctypes.c_uint8)))
         return out[0][:n], out[1][:n], out[2][:n], st[:n]
+
+    def align_starts(self, targets: Targets, max_score: int, threshold: float):
+        """One batch with start coordinates (ish_hip_align_batch_starts):
+        returns score, end_ref, end_query, status, emit (bool), target_start,
+        query_start (-1 where not emitted)."""
+        n = targets.n
+        out = [np.zeros(max(n, 1), np.int32) for _ in range(5)]
+        st = np.zeros(max(n, 1), np.uint8)
+        em = np.zeros(max(n, 1), np.uint8)
+        self._check(self._L.ish_hip_align_batch_starts(
+            self._h, _p(targets.residues, ctypes.c_uint8), _p(targets.offsets, ctypes.c_uint64), n,
+            int(max_score), ctypes.c_float(threshold), *(_p(o, ctypes.c_int32) for o in out[:3]),
+            _p(st, ctypes.c_uint8), _p(em, ctypes.c_uint8), _p(out[3], ctypes.c_int32),
+            _p(out[4], ctypes.c_int32)))
+        return (out[0][:n], out[1][:n], out[2][:n], st[:n], em[:n].astype(bool), out[3][:n], out[4][:n])
+
+    def submit(self, targets: Targets, max_score=None, threshold: float = 0.8):
+        """Starts one batch on the context's host thread and returns a handle
+        for wait(); with max_score the batch also emits and finds starts. The
+        targets and the result arrays stay referenced until wait()."""
+        n = targets.n
+        res = {"targets": targets, "n": n, "out": [np.zeros(max(n, 1), np.int32) for _ in range(3)],
+               "st": np.zeros(max(n, 1), np.uint8), "starts": max_score is not None}
+        P = ctypes.POINTER
+        nul32, nul8 = ctypes.cast(None, P(ctypes.c_int32)), ctypes.cast(None, P(ctypes.c_uint8))
+        if res["starts"]:
+            res["em"] = np.zeros(max(n, 1), np.uint8)
+            res["ts"] = np.zeros(max(n, 1), np.int32)
+            res["qs"] = np.zeros(max(n, 1), np.int32)
+            extra = (_p(res["em"], ctypes.c_uint8), _p(res["ts"], ctypes.c_int32), _p(res["qs"], ctypes.c_int32))
+        else:
+            extra = (nul8, nul32, nul32)
+        self._check(self._L.ish_hip_align_batch_submit(
+            self._h, _p(targets.residues, ctypes.c_uint8), _p(targets.offsets, ctypes.c_uint64), n,
+            int(max_score or 0), ctypes.c_float(threshold), *(_p(o, ctypes.c_int32) for o in res["out"]),
+            _p(res["st"], ctypes.c_uint8), *extra))
+        self._pending = res
+        return res
+
+    def wait(self):
+        """Waits for the submitted batch; returns align()'s tuple, or
+        align_starts()'s when it was submitted with max_score."""
+        res, self._pending = self._pending, None
+        self._check(self._L.ish_hip_align_batch_wait(self._h))
+        n = res["n"]
+        base = (res["out"][0][:n], res["out"][1][:n], res["out"][2][:n], res["st"][:n])
+        if not res["starts"]:
+            return base
+        return base + (res["em"][:n].astype(bool), res["ts"][:n], res["qs"][:n])
 
     def load_db(self, targets: Targets) -> "Database":
         h = ctypes.c_void_p()

@@ -123,6 +123,10 @@ struct ish_hip_ctx {
     std::vector<hipEvent_t> ev;
     int launches = 0;
     float last_ms = 0.f;
+    // a batch submitted with ish_hip_align_batch_submit, running on its own
+    // host thread until ish_hip_align_batch_wait
+    std::thread bg;
+    int32_t bg_rc = 0;
 };
 
 // The context's host worker pool: up to 8 threads in all, with the caller.
@@ -291,6 +295,7 @@ extern "C" int32_t ish_hip_create(int32_t device, ish_hip_ctx **out) {
 
 extern "C" void ish_hip_destroy(ish_hip_ctx *ctx) {
     if (!ctx) return;
+    if (ctx->bg.joinable()) ctx->bg.join();
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ish::one_shot_end(ctx);
@@ -1722,9 +1727,138 @@ static int32_t align_plain(ish_hip_ctx *ctx, const uint8_t *residues, const uint
     return ISH_OK;
 }
 
-extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
-                                       const uint64_t *offsets, int64_t n, int32_t *score,
-                                       int32_t *end_ref, int32_t *end_q, uint8_t *status) {
+// Emission and start coordinates collected slice by slice during a one-shot
+// batch (ish_hip_align_batch_starts): once a slice's results are on the
+// host, its emitted targets' reversed prefixes rev(r[0..end_ref]) are
+// gathered on the device from the slice's residues, into one buffer that
+// the reverse pass scores after the last slice.
+struct BatchStarts {
+    int64_t max_score = 0;
+    float threshold = 0.f;
+    uint8_t *emit = nullptr;
+    int32_t *tstart = nullptr, *qstart = nullptr;
+    std::vector<int64_t> hit;       // emitted targets, batch order
+    std::vector<uint64_t> off{0};   // their prefixes' offsets in acc
+    uint8_t *acc = nullptr;         // gathered prefixes (device)
+    size_t acc_cap = 0;
+    std::vector<void *> to_free;    // device buffers freed at the end
+    std::vector<std::vector<uint64_t>> host_u64; // kept until the copies are done
+    std::vector<std::vector<int64_t>> host_i64;
+    void release() {
+        for (void *p : to_free) (void)hipFree(p);
+        to_free.clear();
+        if (acc) (void)hipFree(acc);
+        acc = nullptr;
+    }
+};
+
+// Emission of targets [t0, t1) of the batch (db holds them, rebased), and
+// the device gather of the emitted targets' reversed prefixes on stream st:
+// the stream that later reuses db's buffers, so that the gather reads them
+// first.
+static int32_t collect_starts(ish_hip_ctx *ctx, BatchStarts &bs, const ish_hip_db *db, int64_t t0,
+                              int64_t t1, hipStream_t st, const int32_t *score, const int32_t *end_ref,
+                              const int32_t *end_q) {
+    const int64_t m = t1 - t0;
+    if (m <= 0) return ISH_OK;
+    if (ish_hip_emit(ctx->algo, score + t0, end_ref + t0, end_q + t0, m, bs.max_score, bs.threshold,
+                     bs.emit + t0) != ISH_OK)
+        return fail(ctx, ISH_E_INVALID, "emission failed");
+    std::vector<int64_t> sel;
+    std::vector<uint64_t> dst{0};
+    for (int64_t t = t0; t < t1; ++t) {
+        if (!bs.emit[t]) continue;
+        const int64_t R = (int64_t)(db->h_off[(size_t)(t - t0) + 1] - db->h_off[(size_t)(t - t0)]);
+        const int64_t L = std::min<int64_t>(std::max<int64_t>((int64_t)end_ref[t] + 1, 0), R);
+        sel.push_back(t - t0);
+        dst.push_back(dst.back() + (uint64_t)L);
+        bs.hit.push_back(t);
+    }
+    if (sel.empty()) return ISH_OK;
+    const size_t base = (size_t)bs.off.back(), add = (size_t)dst.back();
+    for (size_t k = 1; k < dst.size(); ++k) bs.off.push_back(base + dst[k]);
+    hipError_t e = hipSuccess;
+    if (base + add + kRawPad > bs.acc_cap) {
+        // grow: a new buffer, the prefixes so far copied over in stream order
+        const size_t cap = std::max(base + add + kRawPad, 2 * bs.acc_cap);
+        uint8_t *nb = nullptr;
+        e = hipMalloc((void **)&nb, cap);
+        if (e == hipSuccess && base > 0) e = hipMemcpyAsync(nb, bs.acc, base, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) {
+            if (nb) (void)hipFree(nb);
+            return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "start prefixes: %s",
+                        hipGetErrorString(e));
+        }
+        if (bs.acc) bs.to_free.push_back(bs.acc);
+        bs.acc = nb;
+        bs.acc_cap = cap;
+    }
+    if (add == 0) return ISH_OK;
+    int64_t *d_sel = nullptr;
+    uint64_t *d_dst = nullptr;
+    e = hipMalloc((void **)&d_sel, sizeof(int64_t) * sel.size());
+    if (e == hipSuccess) bs.to_free.push_back(d_sel);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_dst, sizeof(uint64_t) * dst.size());
+    if (e == hipSuccess) bs.to_free.push_back(d_dst);
+    bs.host_i64.push_back(std::move(sel));
+    bs.host_u64.push_back(std::move(dst));
+    const auto &hs = bs.host_i64.back();
+    const auto &hd = bs.host_u64.back();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_sel, hs.data(), sizeof(int64_t) * hs.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_dst, hd.data(), sizeof(uint64_t) * hd.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_gather_reversed(db->d_raw, db->d_off, d_sel, d_dst, (int64_t)hs.size(), bs.acc + base, st);
+    if (e != hipSuccess)
+        return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "start prefixes: %s",
+                    hipGetErrorString(e));
+    return ISH_OK;
+}
+
+static int32_t reverse_starts(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off, std::vector<uint64_t> off,
+                              const int32_t *end_ref, const int32_t *end_query, int64_t n_sel,
+                              int32_t *target_start, int32_t *query_start);
+
+// After the last slice (every gather done): the reverse pass over the
+// gathered prefixes, and the start coordinates of the emitted targets
+// (-1 for the others).
+static int32_t finish_starts(ish_hip_ctx *ctx, BatchStarts &bs, int64_t n, const int32_t *end_ref,
+                             const int32_t *end_q) {
+    for (int64_t t = 0; t < n; ++t) bs.tstart[t] = bs.qstart[t] = -1;
+    const int64_t nh = (int64_t)bs.hit.size();
+    if (nh == 0) return ISH_OK;
+    std::vector<int32_t> er((size_t)nh), eq((size_t)nh), ts((size_t)nh), qs((size_t)nh);
+    for (int64_t k = 0; k < nh; ++k) {
+        er[(size_t)k] = end_ref[bs.hit[(size_t)k]];
+        eq[(size_t)k] = end_q[bs.hit[(size_t)k]];
+    }
+    uint64_t *d_off = nullptr;
+    HIP_TRY(ctx, hipMalloc((void **)&d_off, sizeof(uint64_t) * (size_t)(nh + 1)));
+    hipError_t e = hipMemcpy(d_off, bs.off.data(), sizeof(uint64_t) * (size_t)(nh + 1), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d_off);
+        return fail(ctx, ISH_E_HIP, "start offsets: %s", hipGetErrorString(e));
+    }
+    uint8_t *acc = bs.acc;
+    bs.acc = nullptr; // reverse_starts takes it over
+    // the forward pass's timing stays the call's timing
+    const float ms = ctx->last_ms;
+    const int launches = ctx->launches;
+    const int32_t rc = reverse_starts(ctx, acc, d_off, bs.off, er.data(), eq.data(), nh, ts.data(), qs.data());
+    ctx->last_ms = ms;
+    ctx->launches = launches;
+    if (rc) return rc;
+    for (int64_t k = 0; k < nh; ++k) {
+        bs.tstart[bs.hit[(size_t)k]] = ts[(size_t)k];
+        bs.qstart[bs.hit[(size_t)k]] = qs[(size_t)k];
+    }
+    return ISH_OK;
+}
+
+static int32_t align_batch_impl(ish_hip_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
+                                int64_t n, int32_t *score, int32_t *end_ref, int32_t *end_q,
+                                uint8_t *status, BatchStarts *bs) {
     if (!ctx) return ISH_E_INVALID;
     if (n < 0 || (n > 0 && (!offsets || !residues || !score || !end_ref || !end_q)))
         return fail(ctx, ISH_E_INVALID, "bad batch arguments");
@@ -1734,6 +1868,7 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
     if (ctx->engine == ISH_ENGINE_PLAIN) {
         if (ctx->algo != ISH_ALGO_SEMI_GLOBAL)
             return fail(ctx, ISH_E_INVALID, "the plain engine scores semi-global alignments only");
+        if (bs) return fail(ctx, ISH_E_INVALID, "start coordinates need the batched engine");
         return align_plain(ctx, residues, offsets, n, score, end_ref, end_q, status);
     }
     // slices of about equal residues, cut at target boundaries
@@ -1787,13 +1922,19 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
         } else {
             (void)hipStreamSynchronize(ctx->stream);
         }
+        if (rc == ISH_OK && bs) {
+            rc = collect_starts(ctx, *bs, db, 0, n, ctx->stream, score, end_ref, end_q);
+            const hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (rc == ISH_OK && e != hipSuccess) rc = fail(ctx, ISH_E_HIP, "start prefixes: %s", hipGetErrorString(e));
+        }
         if (db) ish_hip_db_free(db);
+        if (rc == ISH_OK && bs) rc = finish_starts(ctx, *bs, n, end_ref, end_q);
         return rc;
     }
     int32_t rc = ish::one_shot_begin(ctx);
     int32_t launches = 0;
     ish_hip_db *prev = nullptr;
-    int64_t prev_t0 = 0;
+    int64_t prev_t0 = 0, prev_t1 = 0;
     for (size_t s = 0; s + 1 < cut.size() && rc == ISH_OK; ++s) {
         const int64_t t0 = cut[s], t1 = cut[s + 1];
         const int slot = (int)(s & 1);
@@ -1808,13 +1949,17 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
             launches += ctx->launches;
         }
         if (prev) {
-            const int32_t r = ish::db_fetch_after(ctx, prev, slot ^ 1, score + prev_t0, end_ref + prev_t0,
-                                                  end_q + prev_t0, status ? status + prev_t0 : nullptr);
+            int32_t r = ish::db_fetch_after(ctx, prev, slot ^ 1, score + prev_t0, end_ref + prev_t0,
+                                            end_q + prev_t0, status ? status + prev_t0 : nullptr);
+            // the gather runs on the copy stream, before the next slice's
+            // upload overwrites prev's buffers
+            if (r == ISH_OK && bs) r = collect_starts(ctx, *bs, prev, prev_t0, prev_t1, ctx->copy, score, end_ref, end_q);
             if (rc == ISH_OK) rc = r;
             ish_hip_db_free(prev);
         }
         prev = db;
         prev_t0 = t0;
+        prev_t1 = t1;
     }
     if (prev && rc == ISH_OK) {
         const int slot = (int)((cut.size() - 2) & 1);
@@ -1823,6 +1968,8 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
         if (rc == ISH_OK)
             rc = ish::db_fetch_after(ctx, prev, slot, score + prev_t0, end_ref + prev_t0, end_q + prev_t0,
                                      status ? status + prev_t0 : nullptr);
+        if (rc == ISH_OK && bs)
+            rc = collect_starts(ctx, *bs, prev, prev_t0, prev_t1, ctx->copy, score, end_ref, end_q);
         float ms = 0.f;
         if (rc == ISH_OK && hipEventSynchronize(ctx->span[1]) == hipSuccess &&
             hipEventElapsedTime(&ms, ctx->span[0], ctx->span[1]) == hipSuccess)
@@ -1833,9 +1980,71 @@ extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues
         if (ctx->copy) (void)hipStreamSynchronize(ctx->copy);
     }
     if (prev) ish_hip_db_free(prev);
-    ish::one_shot_end(ctx);
+    ish::one_shot_end(ctx); // waits for the copy stream: every gather is done
     ctx->launches = launches;
+    if (rc == ISH_OK && bs) rc = finish_starts(ctx, *bs, n, end_ref, end_q);
     return rc;
+}
+
+extern "C" int32_t ish_hip_align_batch(ish_hip_ctx *ctx, const uint8_t *residues,
+                                       const uint64_t *offsets, int64_t n, int32_t *score,
+                                       int32_t *end_ref, int32_t *end_q, uint8_t *status) {
+    return align_batch_impl(ctx, residues, offsets, n, score, end_ref, end_q, status, nullptr);
+}
+
+extern "C" int32_t ish_hip_align_batch_starts(ish_hip_ctx *ctx, const uint8_t *residues,
+                                              const uint64_t *offsets, int64_t n, int64_t max_score,
+                                              float threshold, int32_t *score, int32_t *end_ref,
+                                              int32_t *end_q, uint8_t *status, uint8_t *emit,
+                                              int32_t *target_start, int32_t *query_start) {
+    if (!ctx) return ISH_E_INVALID;
+    if (n > 0 && (!emit || !target_start || !query_start))
+        return fail(ctx, ISH_E_INVALID, "bad batch arguments");
+    BatchStarts bs;
+    bs.max_score = max_score;
+    bs.threshold = threshold;
+    bs.emit = emit;
+    bs.tstart = target_start;
+    bs.qstart = query_start;
+    const int32_t rc = align_batch_impl(ctx, residues, offsets, n, score, end_ref, end_q, status, &bs);
+    if (rc) {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->copy) (void)hipStreamSynchronize(ctx->copy);
+    }
+    bs.release();
+    return rc;
+}
+
+extern "C" int32_t ish_hip_align_batch_submit(ish_hip_ctx *ctx, const uint8_t *residues,
+                                              const uint64_t *offsets, int64_t n, int64_t max_score,
+                                              float threshold, int32_t *score, int32_t *end_ref,
+                                              int32_t *end_q, uint8_t *status, uint8_t *emit,
+                                              int32_t *target_start, int32_t *query_start) {
+    if (!ctx) return ISH_E_INVALID;
+    if (ctx->bg.joinable()) return fail(ctx, ISH_E_INVALID, "a submitted batch is still in flight");
+    const bool starts = emit || target_start || query_start;
+    if (starts && n > 0 && (!emit || !target_start || !query_start))
+        return fail(ctx, ISH_E_INVALID, "emit, target_start and query_start go together");
+    ctx->bg_rc = ISH_OK;
+    try {
+        ctx->bg = std::thread([=] {
+            ctx->bg_rc = starts ? ish_hip_align_batch_starts(ctx, residues, offsets, n, max_score, threshold,
+                                                             score, end_ref, end_q, status, emit,
+                                                             target_start, query_start)
+                                : ish_hip_align_batch(ctx, residues, offsets, n, score, end_ref, end_q,
+                                                      status);
+        });
+    } catch (...) {
+        return fail(ctx, ISH_E_NOMEM, "could not start the batch thread");
+    }
+    return ISH_OK;
+}
+
+extern "C" int32_t ish_hip_align_batch_wait(ish_hip_ctx *ctx) {
+    if (!ctx) return ISH_E_INVALID;
+    if (!ctx->bg.joinable()) return fail(ctx, ISH_E_INVALID, "no batch was submitted");
+    ctx->bg.join();
+    return ctx->bg_rc;
 }
 
 extern "C" int32_t ish_hip_set_slice_bytes(ish_hip_ctx *ctx, int64_t bytes) {
@@ -2115,6 +2324,47 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
 // ---------------------------------------------------------------------------
 // start coordinates
 // ---------------------------------------------------------------------------
+// The reverse passes of the start coordinates: d_raw holds rev(r[0..L_k))
+// of n_sel targets at offsets off (n_sel + 1 entries, device copy d_off);
+// both device buffers are taken over (and freed). Target k's forward end
+// coordinates are end_ref[k], end_query[k].
+static int32_t reverse_starts(ish_hip_ctx *ctx, uint8_t *d_raw, uint64_t *d_off, std::vector<uint64_t> off,
+                              const int32_t *end_ref, const int32_t *end_query, int64_t n_sel,
+                              int32_t *target_start, int32_t *query_start) {
+    const bool local = ctx->algo == ISH_ALGO_LOCAL;
+    ish_hip_db *rdb = nullptr;
+    SortPlan sp;
+    plan_sort(off, sp);
+    int32_t rc = db_build(ctx, ctx->stream, d_raw, d_off, std::move(off), sp, nullptr, &rdb);
+    if (rc) return rc;
+    // free ends swap for the reverse pass (striped.mojo:165-183)
+    const uint32_t f = ctx->flags;
+    const uint32_t rflags = ((f & 1u) << 1) | ((f >> 1) & 1u) | ((f & 4u) << 1) | ((f >> 1) & 4u);
+    rc = launch_db(ctx, rdb, (const int16_t *)ctx->prof_rt.p, local ? 0u : rflags, false);
+    std::vector<int32_t> rs((size_t)n_sel), rer((size_t)n_sel), req((size_t)n_sel);
+    if (!rc) rc = ish_hip_db_fetch(ctx, rdb, rs.data(), rer.data(), req.data(), nullptr);
+    ish_hip_db_free(rdb);
+    if (rc) return rc;
+    for (int64_t k = 0; k < n_sel; ++k) {
+        if (local) {
+            if (end_ref[k] <= 0 || end_query[k] <= 0) {
+                target_start[k] = -1;
+                query_start[k] = -1;
+            } else {
+                // begin = position where the reverse running maximum last
+                // rose, in original coordinates (local_aln/striped.mojo:232-262)
+                target_start[k] = rer[(size_t)k] < 0 ? -1 : end_ref[k] - rer[(size_t)k];
+                query_start[k] = end_query[k] - req[(size_t)k];
+            }
+        } else {
+            // semi_global_aln_start_end (striped.mojo:185-191)
+            target_start[k] = end_ref[k] - rer[(size_t)k];
+            query_start[k] = ctx->Q - req[(size_t)k] - 1;
+        }
+    }
+    return ISH_OK;
+}
+
 extern "C" int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db, const int64_t *sel,
                                        const int32_t *end_ref, const int32_t *end_query,
                                        int64_t n_sel, int32_t *target_start,
@@ -2160,37 +2410,8 @@ extern "C" int32_t ish_hip_find_starts(ish_hip_ctx *ctx, ish_hip_db *db, const i
         if (d_off) (void)hipFree(d_off);
         return fail(ctx, e == hipErrorOutOfMemory ? ISH_E_NOMEM : ISH_E_HIP, "reverse gather: %s", hipGetErrorString(e));
     }
-    ish_hip_db *rdb = nullptr;
-    SortPlan sp;
-    plan_sort(off, sp);
-    int32_t rc = db_build(ctx, ctx->stream, d_raw, d_off, std::move(off), sp, nullptr, &rdb);
-    if (rc) return rc;
-    // free ends swap for the reverse pass (striped.mojo:165-183)
-    const uint32_t f = ctx->flags;
-    const uint32_t rflags = ((f & 1u) << 1) | ((f >> 1) & 1u) | ((f & 4u) << 1) | ((f >> 1) & 4u);
-    rc = launch_db(ctx, rdb, (const int16_t *)ctx->prof_rt.p, local ? 0u : rflags, false);
-    std::vector<int32_t> rs((size_t)n_sel), rer((size_t)n_sel), req((size_t)n_sel);
-    if (!rc) rc = ish_hip_db_fetch(ctx, rdb, rs.data(), rer.data(), req.data(), nullptr);
-    ish_hip_db_free(rdb);
-    if (rc) return rc;
-    for (int64_t k = 0; k < n_sel; ++k) {
-        if (local) {
-            if (end_ref[k] <= 0 || end_query[k] <= 0) {
-                target_start[k] = -1;
-                query_start[k] = -1;
-            } else {
-                // begin = position where the reverse running maximum last
-                // rose, in original coordinates (local_aln/striped.mojo:232-262)
-                target_start[k] = rer[(size_t)k] < 0 ? -1 : end_ref[k] - rer[(size_t)k];
-                query_start[k] = end_query[k] - req[(size_t)k];
-            }
-        } else {
-            // semi_global_aln_start_end (striped.mojo:185-191)
-            target_start[k] = end_ref[k] - rer[(size_t)k];
-            query_start[k] = ctx->Q - req[(size_t)k] - 1;
-        }
-    }
-    return ISH_OK;
+    return reverse_starts(ctx, d_raw, d_off, std::move(off), end_ref, end_query, n_sel, target_start,
+                          query_start);
 }
 
 // ---------------------------------------------------------------------------
