@@ -127,6 +127,18 @@ int32_t oracle_batch_striped(int32_t algo, const uint8_t *q, int32_t Q,
                              int32_t *end_ref, int32_t *end_query,
                              int32_t nthreads);
 
+/* Test hooks of the striped restatement (striped.cpp). oracle_sat_lanes:
+ * r = a + b (op 0) or a - b (op 1) lane by lane with the saturating lanes of
+ * `type` (0 uint8, 1 int16, 2 uint16) at `width` lanes (8, 16, 32 or 64);
+ * returns 0, or -1 for an unknown pair. oracle_local_profile: the biased
+ * profile ssw_align builds (local_aln/striped.mojo:110-139) for `type`
+ * (0 uint8, 2 uint16) at `width` lanes, written vector by vector, lane by
+ * lane (A * seg * width entries); returns seg, or -1. */
+int32_t oracle_sat_lanes(int32_t type, int32_t width, int32_t op, const int32_t *a,
+                         const int32_t *b, int32_t *r);
+int32_t oracle_local_profile(const uint8_t *q, int32_t Q, const int8_t *S, int32_t A,
+                             int32_t bias, int32_t type, int32_t width, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,6 +197,42 @@ def sw_emit(score, end_ref, end_q, max_score, thr) -> bool:
                                      ctypes.c_float(thr)))
 
 
+def sat_lanes(type_: str, width: int, op: str, a, b):
+    """The striped restatement's saturating add / sub on `width` lanes of
+    type "u8", "i16" or "u16" (the overloads its kernels use)."""
+    t = {"u8": 0, "i16": 1, "u16": 2}[type_]
+    a = np.ascontiguousarray(a, np.int32)
+    b = np.ascontiguousarray(b, np.int32)
+    r = np.zeros(width, np.int32)
+    P = ctypes.POINTER(ctypes.c_int32)
+    L = lib()
+    L.oracle_sat_lanes.argtypes = [ctypes.c_int32] * 3 + [P, P, P]
+    L.oracle_sat_lanes.restype = ctypes.c_int32
+    rc = L.oracle_sat_lanes(t, width, 0 if op == "add" else 1, a.ctypes.data_as(P), b.ctypes.data_as(P),
+                            r.ctypes.data_as(P))
+    if rc != 0:
+        raise ValueError(f"no {type_} lanes at width {width}")
+    return r
+
+
+def local_profile(q, values, A, bias, type_: str, width: int):
+    """The biased ssw profile of the striped restatement: (A * seg, width)."""
+    q, qp = _u8(q)
+    S = np.ascontiguousarray(values, np.int8)
+    seg = (len(q) + width - 1) // width
+    out = np.zeros(A * seg * width, np.int32)
+    L = lib()
+    L.oracle_local_profile.argtypes = [ctypes.POINTER(ctypes.c_uint8), ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_int8), ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.oracle_local_profile.restype = ctypes.c_int32
+    got = L.oracle_local_profile(qp, len(q), S.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), A, bias,
+                                 {"u8": 0, "u16": 2}[type_], width, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    if got != seg:
+        raise ValueError(f"no {type_} profile at width {width}")
+    return out.reshape(A * seg, width)
+
+
 def batch(algo, q, residues, offsets, m: Matrix, go, ge, flags, nthreads=0, striped=False):
     """Scores every target; returns (score, end_ref, end_query, threads_used)."""
     q, qp = _u8(q)

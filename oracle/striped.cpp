@@ -591,3 +591,64 @@ extern "C" oracle_end_t oracle_sw_striped(const uint8_t *q, int32_t Q,
     default: return sw_striped<kNativeW>(q, Q, r, R, S, A, go, ge);
     }
 }
+
+/* Test hooks: the lane operations and the profile layout above, on their
+ * own, so that tests can check them against the reference's vectors for
+ * saturating_add / saturating_sub (tests/matcher/alignment/
+ * test_striped_utils.mojo:9-66) and for the ssw profile
+ * (tests/test_local_and_profile.mojo:19-152). The overloads called are the
+ * ones the kernels above resolve to at each lane count (AVX2 / AVX-512 at
+ * their native widths, the generic loops elsewhere). */
+namespace {
+template <typename T, int W>
+int32_t sat_lanes(int32_t op, const int32_t *a, const int32_t *b, int32_t *r) {
+    Vec<T, W> x = vset<T, W>(0), y = vset<T, W>(0);
+    for (int k = 0; k < W; ++k) {
+        lset(x, k, (T)a[k]);
+        lset(y, k, (T)b[k]);
+    }
+    const Vec<T, W> z = op == 0 ? vadds(x, y) : vsubs(x, y);
+    for (int k = 0; k < W; ++k) r[k] = (int32_t)lget(z, k);
+    return 0;
+}
+template <typename T>
+int32_t sat_lanes_w(int32_t width, int32_t op, const int32_t *a, const int32_t *b, int32_t *r) {
+    switch (width) {
+    case 8: return sat_lanes<T, 8>(op, a, b, r);
+    case 16: return sat_lanes<T, 16>(op, a, b, r);
+    case 32: return sat_lanes<T, 32>(op, a, b, r);
+    case 64: return sat_lanes<T, 64>(op, a, b, r);
+    default: return -1;
+    }
+}
+template <typename T, int W>
+int32_t profile_lanes(const uint8_t *q, int32_t Q, const int8_t *S, int32_t A, int32_t bias, int32_t *out) {
+    std::vector<Vec<T, W>> P;
+    int32_t seg = 0;
+    make_profile<T, W>(q, Q, S, A, bias, P, seg);
+    for (size_t v = 0; v < P.size(); ++v)
+        for (int k = 0; k < W; ++k) out[v * W + (size_t)k] = (int32_t)lget(P[v], k);
+    return seg;
+}
+} // namespace
+
+extern "C" int32_t oracle_sat_lanes(int32_t type, int32_t width, int32_t op, const int32_t *a,
+                                    const int32_t *b, int32_t *r) {
+    switch (type) {
+    case 0: return sat_lanes_w<uint8_t>(width, op, a, b, r);
+    case 1: return sat_lanes_w<int16_t>(width, op, a, b, r);
+    case 2: return sat_lanes_w<uint16_t>(width, op, a, b, r);
+    default: return -1;
+    }
+}
+
+extern "C" int32_t oracle_local_profile(const uint8_t *q, int32_t Q, const int8_t *S, int32_t A,
+                                        int32_t bias, int32_t type, int32_t width, int32_t *out) {
+    if (type == 0 && width == 16) return profile_lanes<uint8_t, 16>(q, Q, S, A, bias, out);
+    if (type == 0 && width == 64) return profile_lanes<uint8_t, 64>(q, Q, S, A, bias, out);
+    if (type == 0 && width == 32) return profile_lanes<uint8_t, 32>(q, Q, S, A, bias, out);
+    if (type == 2 && width == 8) return profile_lanes<uint16_t, 8>(q, Q, S, A, bias, out);
+    if (type == 2 && width == 16) return profile_lanes<uint16_t, 16>(q, Q, S, A, bias, out);
+    if (type == 2 && width == 32) return profile_lanes<uint16_t, 32>(q, Q, S, A, bias, out);
+    return -1;
+}

@@ -6,7 +6,12 @@
   equal the maximum over paths, for every free-end flag combination);
 - the scalar textbook DP against the step-by-step striped restatement
   (Farrar + lazy-F) at 8, 16 and 32 lanes, for both algorithms;
-- closed forms (exact copy = diagonal sum, empty input = (0,0,0)).
+- closed forms (exact copy = diagonal sum, empty input = (0,0,0));
+- the striped restatement's saturating uint8 / int16 / uint16 lanes against
+  the saturating_add / saturating_sub vectors of
+  tests/matcher/alignment/test_striped_utils.mojo:9-66 (and against clipped
+  integer arithmetic on random pairs), and its biased ssw profile against
+  the layout vectors of tests/test_local_and_profile.mojo:19-152.
 """
 import json
 import os
@@ -275,3 +280,83 @@ def test_emission_float32():
     assert O.sw_emit(241, 5, 5, 300, 0.8) is True
     assert O.sw_emit(241, 0, 5, 300, 0.8) is False  # no begin at ref 0
     assert O.sw_emit(241, 5, 0, 300, 0.8) is False
+
+
+# ---- the striped restatement's lanes and profile, against the reference's vectors
+SAT = VEC["saturating"]
+LIMITS = {"u8": (0, 255), "i16": (-32768, 32767), "u16": (0, 65535)}
+
+
+@pytest.mark.parametrize("type_", ["u8", "i16", "u16"])
+@pytest.mark.parametrize("width", [8, 16, 32, 64])
+def test_saturating_lanes_reference_vectors(type_, width):
+    """saturating_add / saturating_sub (test_striped_utils.mojo:9-66) on the
+    lane types the striped restatement uses, at every lane count it can be
+    built with (the AVX2 / AVX-512 overloads at their native widths)."""
+    lo, hi = LIMITS[type_]
+    word = {"MAX": hi, "MIN": lo}
+    for c in SAT["cases"]:
+        a = np.full(width, word.get(c["lhs"], c["lhs"]), np.int32)
+        b = np.full(width, c["rhs"], np.int32)
+        r = O.sat_lanes(type_, width, c["op"], a, b)
+        assert (r == word.get(c["expected"], c["expected"])).all(), (type_, width, c)
+
+
+@pytest.mark.parametrize("type_", ["u8", "i16", "u16"])
+def test_saturating_lanes_exhaustive_pairs(type_):
+    """Beyond the reference's four vectors: every lane clamps exactly like
+    integer arithmetic clipped to the type, on random pairs with the limits
+    mixed in (a dropped clamp or a wrap fails here)."""
+    lo, hi = LIMITS[type_]
+    rng = np.random.default_rng(5)
+    for width in (8, 16, 32, 64):
+        for _ in range(50):
+            a = rng.integers(lo, hi + 1, width)
+            b = rng.integers(lo, hi + 1, width)
+            a[:4] = [lo, hi, lo, hi]
+            b[:4] = [hi, hi, lo, lo]
+            for op, f in (("add", np.add), ("sub", np.subtract)):
+                r = O.sat_lanes(type_, width, op, a, b)
+                assert np.array_equal(r, np.clip(f(a, b), lo, hi)), (type_, width, op)
+
+
+def test_ssw_profile_reference_layout():
+    """The biased u8 (16 lanes) and u16 (8 lanes) profiles of
+    test_local_and_profile.mojo:19-152."""
+    P = VEC["ssw_profile"]
+    A = P["alphabet"]
+    q = np.array(P["query"], np.uint8)
+    S = np.full((A, A), P["mismatch"], np.int8)
+    np.fill_diagonal(S, P["match"])
+    for type_, lanes, nvec in (("u8", P["u8_lanes"], P["u8_vectors"]), ("u16", P["u16_lanes"], P["u16_vectors"])):
+        prof = O.local_profile(q, S.reshape(-1), A, P["bias"], type_, lanes)
+        assert prof.shape == (nvec, lanes)
+        seg = nvec // A
+        for c in range(A):
+            want = np.zeros((seg, lanes), np.int32)
+            want[c % seg, c // seg] = 3
+            assert np.array_equal(prof[c * seg:(c + 1) * seg], want), (type_, c)
+        if type_ == "u8":
+            for c, vecs in enumerate(P["u8_first_chars"]):
+                assert np.array_equal(prof[c * seg:(c + 1) * seg], np.array(vecs)), c
+
+
+@pytest.mark.parametrize("type_,lanes", [("u8", 64), ("u8", 32), ("u16", 32), ("u16", 16)])
+def test_ssw_profile_native_widths(type_, lanes):
+    """The same layout rule at the lane counts the timed baseline uses:
+    lane k of vector s of char c holds S[c][q[s + k*seg]] + bias, and bias
+    past the query."""
+    rng = np.random.default_rng(lanes)
+    A = 5
+    S = rng.integers(-3, 4, (A, A)).astype(np.int8)
+    bias = int(-S.min())
+    for Q in (1, lanes - 1, lanes, 3 * lanes + 7):
+        q = rng.integers(0, A, Q).astype(np.uint8)
+        prof = O.local_profile(q, S.reshape(-1), A, bias, type_, lanes)
+        seg = (Q + lanes - 1) // lanes
+        for c in range(A):
+            for s in range(seg):
+                for k in range(lanes):
+                    j = s + k * seg
+                    want = bias if j >= Q else int(S[c, q[j]]) + bias
+                    assert prof[c * seg + s, k] == want, (Q, c, s, k)
