@@ -360,6 +360,16 @@ typedef struct ish_hip_search_config {
     int32_t reader_segment_bytes; /* bytes per parallel parse segment
                                      (0 = 8 MiB); output never depends on
                                      either */
+    int32_t tty;          /* 1: the runners' terminal format, which the
+                             reference uses when stdout is a tty
+                             (parallel_line_search_runner.mojo:123-150,
+                             parallel_fastx_search_runner.mojo:150-167):
+                             line: PURPLE path RESET ":" GREEN line number
+                             RESET ": " then the line with the match span
+                             [target_start, end_ref+1) in RED..RESET;
+                             fastx: the sequence with that span in
+                             "\x1b[1;31m" RED..RESET. Starts come from the
+                             GPU reverse pass (ish_hip_find_starts). */
 } ish_hip_search_config;
 
 typedef struct ish_hip_search_stats {

@@ -49,7 +49,8 @@ class SearchConfig(ctypes.Structure):
     _fields_ = [("matrix_kind", ctypes.c_int32), ("algo", ctypes.c_int32), ("gap_open", ctypes.c_int32),
                 ("gap_extend", ctypes.c_int32), ("ends_free", ctypes.c_uint32), ("threshold", ctypes.c_float),
                 ("record_type", ctypes.c_int32), ("batch_bytes", ctypes.c_int64),
-                ("reader_threads", ctypes.c_int32), ("reader_segment_bytes", ctypes.c_int32)]
+                ("reader_threads", ctypes.c_int32), ("reader_segment_bytes", ctypes.c_int32),
+                ("tty", ctypes.c_int32)]
 
 
 class SearchStats(ctypes.Structure):
@@ -504,7 +505,7 @@ class Engine:
     def search_file(self, path, pattern: bytes, matrix_kind: str, algo="striped-semi-global",
                     gap_open: int = 3, gap_extend: int = 1, ends_free="FFTT", threshold: float = 0.8,
                     record_type: str = "line", out_fd: int = 1, batch_bytes: int = 0,
-                    reader_threads: int = 0, reader_segment_bytes: int = 0):
+                    reader_threads: int = 0, reader_segment_bytes: int = 0, tty: bool = False):
         """Searches one file (plain or gzip) and writes the emitted records to
         out_fd in the reference runners' format; returns the run's stats."""
         pat = np.frombuffer(bytes(pattern), np.uint8).copy()
@@ -512,7 +513,7 @@ class Engine:
         fl = flags_from_str(ends_free) if isinstance(ends_free, str) else int(ends_free)
         cfg = SearchConfig(MATRIX[matrix_kind], ALGO[algo] if isinstance(algo, str) else int(algo), gap_open,
                            gap_extend, fl, threshold, RECORD[record_type], batch_bytes, reader_threads,
-                           reader_segment_bytes)
+                           reader_segment_bytes, int(bool(tty)))
         st = SearchStats()
         self._check(self._L.ish_hip_search_file(self._h, os.fsencode(path), _p(pb, ctypes.c_uint8), len(pat),
                                                 ctypes.byref(cfg), out_fd, ctypes.byref(st)))

@@ -1,7 +1,8 @@
 // File search runner: reads line or FASTA/FASTQ records (plain or gzip),
 // encodes them, scores each batch on the GPU and writes the emitted records
 // in input order, with the output bytes of the reference's parallel runners
-// in non-tty mode:
+// in non-tty mode (and, with ish_hip_search_config.tty, in their terminal
+// format with the match span coloured):
 //   line  (ishlib/parallel_line_search_runner.mojo:75-161):
 //         decode(encode(line)) "\n"
 //   fastx (ishlib/parallel_fastx_search_runner.mojo:79-182):
@@ -42,6 +43,7 @@
 #include <chrono>
 #include <climits>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -675,10 +677,16 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
     std::unique_ptr<Batch> cur; // the batch whose database is being scored
     ish_hip_db *cur_db = nullptr;
     int cur_slot = 0, next_slot = 0;
+    int64_t cur_first = 0; // records of the file before cur
     std::vector<int32_t> score, er, eq;
     std::vector<uint8_t> emit;
-    // fetches cur's results and drops its database; false when no matches
-    // are to be written
+    // tty output: start coordinates of cur's emitted records (per record;
+    // -1 elsewhere)
+    const bool tty = cfg->tty != 0;
+    std::vector<int32_t> tstart;
+    // fetches cur's results (and, for tty output, the start coordinates of
+    // its emitted records, from the residues still on the device) and drops
+    // its database; false when no matches are to be written
     auto fetch = [&]() {
         if (!cur_db) return false;
         int64_t n = 0;
@@ -689,15 +697,34 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
         emit.resize((size_t)n);
         const double tg = now_s();
         int32_t r = ish::db_fetch_after(ctx, cur_db, cur_slot, score.data(), er.data(), eq.data(), nullptr);
-        ish_hip_db_free(cur_db);
-        cur_db = nullptr;
         if (r == ISH_OK)
             r = ish_hip_emit(cfg->algo, score.data(), er.data(), eq.data(), n, max_score,
                              cfg->threshold, emit.data());
+        if (r == ISH_OK && tty) {
+            std::vector<int64_t> sel;
+            std::vector<int32_t> ser, seq;
+            for (int64_t t = 0; t < n; ++t)
+                if (emit[(size_t)t]) {
+                    sel.push_back(t);
+                    ser.push_back(er[(size_t)t]);
+                    seq.push_back(eq[(size_t)t]);
+                }
+            std::vector<int32_t> ts(sel.size()), qs(sel.size());
+            r = ish_hip_find_starts(ctx, cur_db, sel.data(), ser.data(), seq.data(), (int64_t)sel.size(),
+                                    ts.data(), qs.data());
+            tstart.assign((size_t)n, -1);
+            for (size_t k = 0; k < sel.size(); ++k) tstart[(size_t)sel[k]] = ts[k];
+        }
+        ish_hip_db_free(cur_db);
+        cur_db = nullptr;
         st.seconds_gpu += now_s() - tg;
         if (r != ISH_OK && rc == ISH_OK) rc = r;
         return r == ISH_OK;
     };
+    // the runners' tty colours (ishlib/__init__.mojo:5-10)
+    static const char kPurple[] = "\x1b[35m", kGreen[] = "\x1b[32m", kRed[] = "\x1b[31m",
+                      kReset[] = "\x1b[0m", kBoldRed[] = "\x1b[1;31m";
+    const size_t path_len = strlen(path);
     auto write = [&](const Batch &b) {
         const double tw = now_s();
         std::vector<uint8_t> tmp;
@@ -710,14 +737,51 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
                 const uint64_t a = r.off[x], e = r.off[x + 1];
                 tmp.resize(e - a);
                 for (uint64_t y = a; y < e; ++y) tmp[y - a] = dec[enc[r.res[y]]];
+                // the match span [start, end_ref + 1) of MatchResult
+                size_t m0 = 0, m1 = 0;
+                if (tty) {
+                    m1 = (size_t)std::min<int64_t>(std::max<int64_t>((int64_t)er[k] + 1, 0), (int64_t)tmp.size());
+                    m0 = (size_t)std::min<int64_t>(std::max<int64_t>(tstart[k], 0), (int64_t)m1);
+                }
                 if (cfg->record_type == ISH_RECORD_LINE) {
-                    out.put(tmp.data(), tmp.size());
+                    if (tty) {
+                        // PURPLE file RESET ":" GREEN line RESET ": " pre RED match RESET post
+                        // (parallel_line_search_runner.mojo:123-150)
+                        char num[32];
+                        snprintf(num, sizeof num, "%lld", (long long)(cur_first + (int64_t)k + 1));
+                        out.put(kPurple);
+                        out.put((const uint8_t *)path, path_len);
+                        out.put(kReset);
+                        out.put(":");
+                        out.put(kGreen);
+                        out.put(num);
+                        out.put(kReset);
+                        out.put(": ");
+                        out.put(tmp.data(), m0);
+                        out.put(kRed);
+                        out.put(tmp.data() + m0, m1 - m0);
+                        out.put(kReset);
+                        out.put(tmp.data() + m1, tmp.size() - m1);
+                    } else {
+                        out.put(tmp.data(), tmp.size());
+                    }
                     out.put("\n");
                 } else {
                     out.put(">");
                     out.put(r.names.data() + r.noff[x], r.noff[x + 1] - r.noff[x]);
                     out.put("\n");
-                    out.put(tmp.data(), tmp.size());
+                    if (tty) {
+                        // pre BOLD-RED RED match RESET post
+                        // (parallel_fastx_search_runner.mojo:150-167)
+                        out.put(tmp.data(), m0);
+                        out.put(kBoldRed);
+                        out.put(kRed);
+                        out.put(tmp.data() + m0, m1 - m0);
+                        out.put(kReset);
+                        out.put(tmp.data() + m1, tmp.size() - m1);
+                    } else {
+                        out.put(tmp.data(), tmp.size());
+                    }
                     if (fastq) {
                         out.put("\n+\n");
                         out.put(r.quals.data() + r.qoff[x], r.qoff[x + 1] - r.qoff[x]);
@@ -766,6 +830,7 @@ extern "C" int32_t ish_hip_search_file(ish_hip_ctx *ctx, const char *path, const
             db = nullptr;
         }
         if (have && rc == ISH_OK) write(*cur);
+        if (cur) cur_first += (int64_t)cur->records();
         cur = std::move(b);
         cur_db = db;
         if (db) {

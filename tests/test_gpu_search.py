@@ -69,7 +69,14 @@ def test_searcher_fixtures_from_files(eng, tmp_path, gz):
                     (c["input"], algo, ends, g[:200])
 
 
-def _expected_output(data, rtype, pattern, kind, algo, ends, thr):
+PURPLE, GREEN, RED, RESET, BOLD_RED = b"\x1b[35m", b"\x1b[32m", b"\x1b[31m", b"\x1b[0m", b"\x1b[1;31m"
+
+
+def _expected_output(data, rtype, pattern, kind, algo, ends, thr, tty_path=None):
+    """The runners' output for the oracle's results; with tty_path, their
+    terminal format (ishlib/parallel_line_search_runner.mojo:123-150,
+    parallel_fastx_search_runner.mojo:150-167) with the match span
+    [start, end_ref + 1) from the oracle's start/end functions."""
     recs, flags, _ = expected_records(data, rtype)
     if rtype == "line" and flags & 1:
         return b""
@@ -89,6 +96,21 @@ def _expected_output(data, rtype, pattern, kind, algo, ends, thr):
         if not emit[k]:
             continue
         body = bytes(dec[seqs[k]])
+        if tty_path is not None:
+            if algo == "striped-local":
+                a = O.sw_align(q, seqs[k], m, 3, 1, cutoff=-1.0)
+                m0, m1 = a["ref_begin1"], a["ref_end1"] + 1
+            else:
+                se = O.sg_start_end(q, seqs[k], m, 3, 1, fl)
+                m0, m1 = se["target_start"], se["target_end"] + 1
+            span = (body[:m0], body[m0:m1], body[m1:])
+            if rtype == "line":
+                out.append(PURPLE + tty_path + RESET + b":" + GREEN + str(k + 1).encode() + RESET + b": " +
+                           span[0] + RED + span[1] + RESET + span[2] + b"\n")
+            else:
+                out.append(b">" + name + b"\n" + span[0] + BOLD_RED + RED + span[1] + RESET + span[2] +
+                           ((b"\n+\n" + qual) if fastq else b"") + b"\n")
+            continue
         if rtype == "line":
             out.append(body + b"\n")
         else:
@@ -135,6 +157,39 @@ def test_random_files_match_oracle(eng, tmp_path, kind, rtype):
             assert st["batches"] > 5 and st["records"] == 3000
             assert got == exp, (kind, gz, algo, ends, len(got), len(exp))
             assert st["matches"] > 0
+
+
+@pytest.mark.parametrize("kind,rtype", [("fastq", "fastx"), ("fasta", "fastx"), ("line", "line")])
+def test_tty_format_matches_oracle_spans(eng, tmp_path, kind, rtype):
+    """The terminal format: line numbers across small batches, and the match
+    span of every emitted record from the GPU start pass equal to the
+    oracle's start/end coordinates."""
+    rng = np.random.default_rng(zlib.crc32(b"tty" + kind.encode()))
+    pattern = _b(rng, b"ACGT", 40)
+    data = _random_records(rng, kind, 1500, pattern)
+    path = str(tmp_path / "in.txt")
+    _write(path, data, False)
+    for algo, ends in (("striped-semi-global", "FFTT"), ("striped-local", "FFFF"), ("striped-semi-global", "TTTT")):
+        got, st = _search(eng, tmp_path, path, pattern=pattern, matrix_kind="actgn", algo=algo, ends_free=ends,
+                          threshold=0.7, record_type=rtype, batch_bytes=40000, tty=True)
+        exp = _expected_output(data, rtype, pattern, "actgn", algo, ends, 0.7, tty_path=path.encode())
+        assert st["batches"] > 3 and st["matches"] > 0
+        assert got == exp, (kind, algo, ends, got[:300], exp[:300])
+
+
+def test_tty_format_searcher_fixtures(eng, tmp_path):
+    cases = json.load(open(os.path.join(HERE, "golden", "searcher_cases.json")))
+    for c in cases["cases"]:
+        path = str(tmp_path / "in")
+        data = cases["inputs"][c["input"]].encode()
+        _write(path, data, False)
+        for algo in ("striped-local", "striped-semi-global"):
+            for ends in ("TTTT", "FFTT"):
+                got, _ = _search(eng, tmp_path, path, pattern=c["pattern"].encode(), matrix_kind=c["matrix"],
+                                 algo=algo, ends_free=ends, threshold=0.8, record_type=c["record_type"], tty=True)
+                exp = _expected_output(data, c["record_type"], c["pattern"].encode(), c["matrix"], algo, ends, 0.8,
+                                       tty_path=path.encode())
+                assert got == exp, (c["input"], algo, ends, got[:300], exp[:300])
 
 
 def test_binary_file_skipped_in_line_mode(eng, tmp_path):
