@@ -1095,6 +1095,10 @@ enum { kSg16 = 0, kSg32 = 1, kSw16 = 2, kSw32 = 3, kSwPipe = 4, kSgHybrid = 5, k
 struct DbPlan {
     DbLaunch l[2];
     int n = 0;
+    // plans are compared bytewise (graph replay, the autotuner's
+    // duplicates): the padding between fields must be zero too
+    DbPlan() { memset((void *)l, 0, sizeof l); }
+    void push(const DbLaunch &L) { memcpy((void *)&l[n++], (const void *)&L, sizeof L); }
     DbLaunch *begin() { return l; }
     DbLaunch *end() { return l + n; }
     const DbLaunch *begin() const { return l; }
@@ -1208,12 +1212,13 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     }
     const int32_t a = g_lo, m16 = std::max(g_lo, std::min(g16, g_hi));
     if (m16 > a) {
-        DbLaunch L{};
+        DbLaunch L;
+        memset((void *)&L, 0, sizeof L);
         L.kind = local ? kSw32 : kSg32;
         L.chunk = ctx->chunk == 40 ? 32 : ctx->chunk; // the int32 kernels have 16 and 32
         L.g_begin = a;
         L.g_end = m16;
-        out.l[out.n++] = L;
+        out.push(L);
     }
     const size_t pipe_lds = ctx->lds + sw_pipe_extra_lds();
     const size_t hybrid_lds = ctx->lds + sg_pipe_extra_lds();
@@ -1234,7 +1239,8 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
             (local || pc > kPairMaxRows ||
              !(hybrid_lds <= kLdsLimit || (pc > 0 && sg_hybrid_paired_lds(40, pc, 4) <= kLdsLimit))))
             chunk = 32;
-        DbLaunch L{};
+        DbLaunch L;
+        memset((void *)&L, 0, sizeof L);
         bool pipe = false;
         int32_t n_long = 0;
         const bool sg_paired = !local && pc > 0 && pc <= kPairMaxRows &&
@@ -1274,7 +1280,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         L.g_begin = g0;
         L.g_end = g1;
         L.n_long = n_long;
-        out.l[out.n++] = L;
+        out.push(L);
     };
     if (forced_chunk > 0) {
         add16(m16, g_hi, forced_chunk, forced_sched, forced_waves);
