@@ -237,7 +237,7 @@ def serving_block(args, ish_amd, workloads, with_cpu):
     t0 = time.perf_counter()
     db = eng.load_db(w.targets)
     load_s = time.perf_counter() - t0
-    count = args.requests or 64
+    count = args.requests or 128
     try:
         sat, sus = serving_phases(ish_amd, workloads, db, w.targets, args, count)
     finally:
@@ -273,7 +273,7 @@ def serving_block(args, ish_amd, workloads, with_cpu):
 def run_serving(args, ish_amd, workloads):
     """Serving loop over a 10M-target resident protein DB."""
     rec, sat, cpu = serving_block(args, ish_amd, workloads, not args.no_cpu)
-    count = args.requests or 64
+    count = args.requests or 128
     out = {
         "metric": METRIC, "value": sat["gcups_sustained"], "unit": "GCUPS", "n_gpus": 1,
         "steps": count, "warmup": 4, "ms_per_step": round(1e3 / sat["achieved_qps"], 3),
@@ -439,7 +439,7 @@ def main():
     ap.add_argument("--schedule", choices=["auto", "wave", "workgroup"], default="auto",
                     help="launch schedule used with --tile (ignored when the autotuner runs)")
     ap.add_argument("--requests", type=int, default=0,
-                    help="serving requests per phase (0 = 64)")
+                    help="serving requests per phase (0 = 128)")
     ap.add_argument("--serving-targets", type=int, default=0,
                     help="targets of the serving database (0 = 10M)")
     ap.add_argument("--no-serving", action="store_true",
