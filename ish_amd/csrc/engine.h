@@ -31,7 +31,8 @@ struct QueryDesc {
     int32_t Q, nchunks, neg_limit;
     int32_t g_begin, g_end; // groups of this launch for this query
     uint32_t flags;         // free ends of this query: qs|qe<<1|ts<<2|te<<3
-    int32_t reserved;
+    int32_t pad, front;     // top padding rows and profile front (ScoreParams)
+    int32_t spare;
     int64_t tile_base;      // first work tile of this query (prefix sum)
     int64_t out_off;        // offset of this query's results in the outputs
     // paired profile of this query (paired launches; see ScoreParams)
@@ -51,6 +52,10 @@ struct ScoreParams {
     int32_t stride;     // int16 elements per profile row (16-byte multiple + 16)
     int32_t Q;          // query length
     int32_t nchunks;    // ceil(Q / K)
+    // semi-global top padding (see top_pad): chunk c holds query rows
+    // c*K - pad .. c*K - pad + K - 1, rows below 0 being padding rows;
+    // query row j of profile row r is at prof[r * stride + front + j]
+    int32_t pad, front;
     int32_t go, ge;     // gap open / extend (non-negative)
     uint32_t flags;     // free ends: qs|qe<<1|ts<<2|te<<3
     int32_t neg_limit;  // int16 floor that seeds the last-row maximum
@@ -225,6 +230,20 @@ constexpr int kChunkAlign = 32;
 constexpr int kChunkAlign40 = 40;
 constexpr int kChunkDefault = 32;
 constexpr int kChunkPlain = 32;
+// Top padding of semi-global queries (used when go >= ge): pad rows above
+// query row 0 make the query a whole number of K-row chunks, so that row Q-1
+// is the last register row of the last chunk and the kernels read it
+// without a select. The profile holds the padding rows' entries in front of
+// row 0: front = 40 + (-Q mod 8) entries, at least the 39 rows a 40-row
+// tile pads and such that every chunk start (front + c*K - pad) is a
+// multiple of 8 entries, i.e. 16-byte aligned, for every tile height. Their
+// value is ge when the target start is free and 0 otherwise: with go >= ge
+// the padding rows then repeat the top boundary row (H^ = (i-1)*ge, resp.
+// -go-ge), so the rows below see exactly the boundary they would without
+// them.
+constexpr int kTopFront = 40;
+constexpr int top_front(int Q) { return kTopFront + (8 - Q % 8) % 8; }
+constexpr int top_pad(int Q, int K) { return (K - Q % K) % K; }
 // Pair tables are stored per tile height, 512 uint16 each, in this order.
 constexpr int kPairTables = 3;
 constexpr int pair_table_index(int K) { return K == 16 ? 0 : K == 32 ? 1 : 2; }

@@ -58,22 +58,34 @@ struct DevBuf {
 // is identical (for the ascii matrix this collapses 256 codes to the
 // distinct query bytes plus one "absent" row); lut maps a code to the dword
 // offset of its row.
+// With top rows (semi-global, go >= ge), fwd and rev rows hold front =
+// top_front(Q) padding-row entries before query row 0 (plain rows start at
+// row 0 regardless).
 struct Profile {
-    int32_t stride = 0, nrows = 0, qpad = 0;
+    int32_t stride = 0, nrows = 0, qpad = 0, front = 0;
+    bool top = false; // top padding rows (front > 0)
     std::vector<int16_t> fwd, rev, plain; // offset form, reversed query, S
     uint16_t lut[256];
     size_t lds() const { return (size_t)nrows * stride * 2 + 512; }
     int32_t words() const { return nrows * stride / 2; }
+    // entries of a row the kernels may read: the front and the padded query
+    int32_t width() const { return front + qpad; }
 };
 
 // Profile entries are S + bias (the kernels' offset form); entries past the
-// query are 0.
+// query are 0. top_rows: fwd rows start with top_front(Q) entries of value
+// top_fwd and rev rows with entries of value top_rev (the padding rows'
+// score: ge where that pass's target start is free, else 0).
 void build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
-                   int32_t bias, bool want_rev_plain, Profile &pf);
+                   int32_t bias, bool want_rev_plain, bool top_rows, int32_t top_fwd,
+                   int32_t top_rev, Profile &pf);
+// Padding-row score of a semi-global pass whose free ends are `flags`.
+inline int32_t top_row_score(uint32_t flags, int32_t ge) { return (flags & 4u) ? ge : 0; }
 // Paired profile of a query over the profile classes of the residue codes
 // a database holds (the layout is described with ScoreParams.pprof):
-// `classes` classes, words[(a * classes + b) * qpad + j], and the pair
-// tables of tile heights 16 then 32, each [A: 256][B: 256].
+// `classes` classes, words[(a * classes + b) * qpad + e] for the profile's
+// row entries e (front included; qpad = width rounded up to 8), and the pair
+// tables of tile heights 16, 32 and 40, each [A: 256][B: 256].
 struct PairProfile {
     int32_t classes = 0, qpad = 0;
     std::vector<int32_t> words;
@@ -92,7 +104,8 @@ int32_t alphabet_of(int32_t kind);
 int32_t matrix_of(const int8_t *matrix, int64_t matrix_len, int32_t kind, int32_t &A,
                   std::vector<int8_t> &mat, std::string &err);
 // Longest target the packed int16 semi-global kernels may score for a
-// query (-1: none).
+// query (-1: none; always so when go < ge, since the packed kernels read
+// row Q-1 from the last register row, which takes top padding rows).
 int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s);
 // int16 floor seeding the last-row maximum.
 int32_t neg_limit_of(int32_t go, int32_t mn);

@@ -48,6 +48,8 @@ struct ish_hip_ctx {
     // query
     bool have_query = false;
     int32_t Q = 0, algo = 0, go = 0, ge = 0, kind = 0, nchunks = 0, stride = 0, rows = 0;
+    int32_t front = 0; // profile entries before query row 0 (top padding)
+    bool top = false;  // semi-global top padding rows
     uint32_t flags = 0;
     int32_t neg_limit = 0, min_score = 0, max_score = 0;
     int64_t r16max = -1; // longest target routed to the int16 kernel
@@ -365,6 +367,7 @@ extern "C" int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine) {
 //   H^ = H + (i+j)*ge in [-2go, Q*M + (R+Q)*ge]
 // plus one profile term (<= M + 2ge) and one gap term (>= -go - m).
 int64_t ish::int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s) {
+    if (go < ge) return -1; // no top padding rows (engine.h, top_pad)
     const int64_t M = std::max(max_s, 0), m = std::max(-min_s, 0);
     const int64_t lim = 32767 - 16;
     if (3LL * go + 2LL * ge + m > lim) return -1;
@@ -394,7 +397,8 @@ static bool local_fits_int16(int32_t qpad, int32_t go, int32_t ge, int32_t max_s
 }
 
 void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &mat, int32_t A,
-                        int32_t bias, bool want_rev_plain, Profile &pf) {
+                        int32_t bias, bool want_rev_plain, bool top_rows, int32_t top_fwd,
+                        int32_t top_rev, Profile &pf) {
     std::map<std::vector<int8_t>, int> row_of;
     std::vector<std::vector<int8_t>> rows;
     for (int c = 0; c < 256; ++c) {
@@ -410,9 +414,11 @@ void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &
     }
     pf.qpad = std::max((Q + kChunkAlign - 1) / kChunkAlign * kChunkAlign,
                        (Q + kChunkAlign40 - 1) / kChunkAlign40 * kChunkAlign40);
+    pf.top = top_rows;
+    pf.front = top_rows ? top_front(Q) : 0;
     // rows start 16 bytes apart modulo 128: 16-byte slices of different rows
     // spread over the LDS banks and every slice read is 16-byte aligned
-    pf.stride = pf.qpad + 8;
+    pf.stride = (pf.width() + 7) / 8 * 8 + 8;
     pf.nrows = (int32_t)rows.size();
     for (int c = 0; c < 256; ++c) pf.lut[c] = (uint16_t)(pf.lut[c] * (pf.stride / 2));
     const size_t elems = (size_t)pf.nrows * pf.stride;
@@ -421,17 +427,22 @@ void ish::build_profile(const uint8_t *q, int32_t Q, const std::vector<int8_t> &
         pf.rev.assign(elems, 0);
         pf.plain.assign(elems, 0);
     }
-    for (int32_t r = 0; r < pf.nrows; ++r)
+    for (int32_t r = 0; r < pf.nrows; ++r) {
+        int16_t *fw = pf.fwd.data() + (size_t)r * pf.stride;
+        for (int32_t e = 0; e < pf.front; ++e) fw[e] = (int16_t)top_fwd;
+        if (want_rev_plain)
+            for (int32_t e = 0; e < pf.front; ++e) pf.rev[(size_t)r * pf.stride + e] = (int16_t)top_rev;
         for (int32_t j = 0; j < Q; ++j) {
             const int16_t v = (int16_t)(rows[r][j] + bias);
-            pf.fwd[(size_t)r * pf.stride + j] = v;
+            fw[pf.front + j] = v;
             if (want_rev_plain) {
                 pf.plain[(size_t)r * pf.stride + j] = rows[r][j];
                 // reversed query: the profile of the reverse passes
                 // (Profile of rev_pattern, striped_semi_global_matcher.mojo:68-75)
-                pf.rev[(size_t)r * pf.stride + (Q - 1 - j)] = v;
+                pf.rev[(size_t)r * pf.stride + pf.front + (Q - 1 - j)] = v;
             }
         }
+    }
 }
 
 bool ish::build_pairs(const Profile &pf, const uint32_t present[8], int32_t max_classes,
@@ -451,15 +462,15 @@ bool ish::build_pairs(const Profile &pf, const uint32_t present[8], int32_t max_
     // the table's uint16 offsets must reach the last pair row (tile 32)
     if ((int64_t)(C * C - 1) * pair_row_bytes(32) > 65535) return false;
     out.classes = C;
-    out.qpad = pf.qpad;
-    out.words.assign((size_t)C * C * pf.qpad, 0);
+    out.qpad = (pf.width() + 7) / 8 * 8; // <= stride: every entry lies in the row
+    out.words.assign((size_t)C * C * out.qpad, 0);
     for (int32_t a = 0; a < C; ++a)
         for (int32_t b = 0; b < C; ++b) {
             const int16_t *ra = pf.fwd.data() + (size_t)cls_of[(size_t)a] * pf.stride;
             const int16_t *rb = pf.fwd.data() + (size_t)cls_of[(size_t)b] * pf.stride;
-            int32_t *w = out.words.data() + ((size_t)a * C + b) * pf.qpad;
-            for (int32_t j = 0; j < pf.qpad; ++j)
-                w[j] = (int32_t)((uint32_t)(uint16_t)ra[j] | ((uint32_t)(uint16_t)rb[j] << 16));
+            int32_t *w = out.words.data() + ((size_t)a * C + b) * out.qpad;
+            for (int32_t e = 0; e < out.qpad; ++e)
+                w[e] = (int32_t)((uint32_t)(uint16_t)ra[e] | ((uint32_t)(uint16_t)rb[e] << 16));
         }
     // codes the database does not hold take class 0's rows (never read);
     // a tile height whose last pair row lies beyond the uint16 offsets
@@ -542,7 +553,11 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     Profile pf;
     // offset-form scores: S + 2ge for the semi-global kernels, S + ge for
     // the local ones (their offsets are explained in the kernel sources)
-    build_profile(q, Q, mat, A, algo == ISH_ALGO_LOCAL ? ge : 2 * ge, true, pf);
+    // semi-global with go >= ge: top padding rows (engine.h, top_pad); the
+    // reverse passes swap the free ends, so their target start is te
+    const bool top = algo != ISH_ALGO_LOCAL && go >= ge;
+    build_profile(q, Q, mat, A, algo == ISH_ALGO_LOCAL ? ge : 2 * ge, true, top,
+                  top_row_score(ends_free, ge), top_row_score((ends_free >> 1) & 4u, ge), pf);
     const size_t lds = pf.lds();
     if (lds > kLdsLimit)
         return fail(ctx, ISH_E_RANGE, "query profile needs %zu bytes of LDS (limit %zu)", lds,
@@ -566,6 +581,8 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     ctx->flags = ends_free & 15u;
     ctx->nchunks = 0; // set per launch from the tile height
     ctx->stride = pf.stride;
+    ctx->front = pf.front;
+    ctx->top = pf.top;
     ctx->rows = pf.nrows;
     ctx->neg_limit = neg_limit_of(go, mn);
     ctx->min_score = mn;
@@ -1353,6 +1370,8 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         p.stride = ctx->stride;
         p.Q = ctx->Q;
         p.nchunks = (ctx->Q + L.chunk - 1) / L.chunk;
+        p.pad = ctx->top ? top_pad(ctx->Q, L.chunk) : 0;
+        p.front = ctx->front;
         p.go = ctx->go;
         p.ge = ctx->ge;
         p.flags = flags;
@@ -2169,6 +2188,8 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
                 d.prof_words = it.pf->words();
                 d.Q = it.Q;
                 d.nchunks = (it.Q + chunk - 1) / chunk;
+                d.pad = it.pf->top ? top_pad(it.Q, chunk) : 0;
+                d.front = it.pf->front;
                 d.neg_limit = neg_limit_of(go, mn);
                 d.g_begin = packed ? lo : 0;
                 d.g_end = packed ? db->ngroups : lo;
@@ -2278,7 +2299,8 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
         const uint8_t *q = queries + qoffsets[k];
         for (int32_t j = 0; j < Q; ++j)
             if (q[j] >= A) return fail(ctx, ISH_E_INVALID, "query %d code %d >= alphabet %d", k, q[j], A);
-        build_profile(q, Q, mat, A, 2 * ge, false, pfs[(size_t)k]);
+        build_profile(q, Q, mat, A, 2 * ge, false, go >= ge, top_row_score(ends_free, ge), 0,
+                      pfs[(size_t)k]);
         items[(size_t)k] = MultiItem{&pfs[(size_t)k], Q, ends_free & 15u, (int64_t)k * n};
     }
     const size_t tot = (size_t)nq * (size_t)n;

@@ -511,7 +511,8 @@ extern "C" int32_t ish_hip_server_submit(ish_hip_server *s, const uint8_t *enc_q
     }
     r->max_score = ms;
     r->cells = (int64_t)qlen * s->db->nres;
-    build_profile(enc_query, qlen, s->mat, s->A, 2 * s->ge, false, r->pf);
+    build_profile(enc_query, qlen, s->mat, s->A, 2 * s->ge, false, s->go >= s->ge,
+                  top_row_score(r->flags, s->ge), 0, r->pf);
     if (r->pf.lds() > kLdsLimit)
         return sfail(s, ISH_E_RANGE, "query profile needs " + std::to_string(r->pf.lds()) + " bytes of LDS");
     {

@@ -107,6 +107,7 @@ __device__ __forceinline__ void finalize(int R, int Q, bool te, bool qe, bool ha
 // Per-query values a group pass needs (one query, or one of many).
 struct QueryView {
     int Q, nchunks, neg_limit;
+    int pad, front;  // top padding rows, profile entries before row 0
     uint32_t flags;  // free ends of this query (qs|qe<<1|ts<<2|te<<3)
     int64_t out_off; // offset of this query's results in the output arrays
 };
@@ -118,17 +119,29 @@ template <typename V, int K> struct SgChunk {
     V diag_top; // H^ of the row above the chunk, previous column
 };
 
+// H^(j,-1), the left boundary of row j. Rows j < 0 are the top padding
+// rows of the first chunk: with go >= ge and a profile entry of ge (target
+// start free) or 0 per column, row j < 0 holds H^(j,i) = (i-1)*ge, resp.
+// -go-ge, in every column i >= -1 (E^ and F^ never exceed it), which is the
+// top boundary H^(-1,i) the query rows need; row -1's own left entry is the
+// corner H^(-1,-1) = -2ge.
+__device__ __forceinline__ int left_boundary(int j, bool qs, bool ts, int go, int ge) {
+    if (j >= 0) return qs ? (j - 1) * ge : -go - ge;
+    if (j == -1) return -2 * ge;
+    return ts ? -2 * ge : -go - ge;
+}
+
 template <typename V, int K>
-__device__ __forceinline__ void sg_chunk_init(SgChunk<V, K> &st, int j0, bool qs, int go, int ge) {
+__device__ __forceinline__ void sg_chunk_init(SgChunk<V, K> &st, int j0, bool qs, bool ts, int go,
+                                              int ge) {
     const int cst = ge - go;
 #pragma unroll
     for (int m = 0; m < K; ++m) {
-        const int j = j0 + m;
-        const int hl = qs ? (j - 1) * ge : -go - ge; // H^(j,-1)
+        const int hl = left_boundary(j0 + m, qs, ts, go, ge); // H^(j,-1)
         st.H[m] = splat<V>(hl);
         st.E[m] = splat<V>(hl + cst); // E^(j,0)
     }
-    st.diag_top = splat<V>(j0 == 0 ? -2 * ge : (qs ? (j0 - 2) * ge : -go - ge));
+    st.diag_top = splat<V>(left_boundary(j0 - 1, qs, ts, go, ge));
 }
 
 __device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
@@ -141,12 +154,14 @@ __device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
     t32.corner = 0;
 }
 
-// Scores columns 4b .. 4b+3 of one chunk (rows j0 .. j0+K-1).
+// Scores columns 4b .. 4b+3 of one chunk (rows j0 .. j0+K-1; rows below 0
+// are top padding rows).
 //   rr / rr_next: residue words of this block and the next (the profile
 //   slices of the next column are fetched while a column is scored);
 //   bn: the chunk-top rows (H^, F^ of the row above) unless `first`;
 //   out: where the chunk-bottom rows go (t*64 + lane), null for the last
-//   chunk, which tracks row Q-1 (register mstar) instead.
+//   chunk, which tracks row Q-1 instead: register mstar, which the packed
+//   variants take to be K-1 (their queries are top padded).
 // TE / QE say which end rules the variant can apply; qe_rt is the query's.
 // PAIRED (packed lanes): chunk_base is the wave's paired chunk profile and
 // slut the pair table (see PairView).
@@ -159,6 +174,7 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                                          int ge, TrackI16 &t16, TrackI32 &t32, uint2 *out,
                                          int lane) {
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    constexpr bool FIXED_LAST = PACKED;
     constexpr int NS = K / 8;
     const int cst = ge - go;
     const V vcst = splat<V>(cst);
@@ -235,10 +251,13 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             out[t * 64 + lane] = o;
         } else {
             // row Q-1 lives in register mstar of the last chunk
-            V hl = st.H[0];
+            V hl = st.H[K - 1];
+            if constexpr (!FIXED_LAST) {
+                hl = st.H[0];
 #pragma unroll
-            for (int m = 1; m < K; ++m)
-                if (m == mstar) hl = st.H[m];
+                for (int m = 1; m < K; ++m)
+                    if (m == mstar) hl = st.H[m];
+            }
             // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
             const V u = hl - splat<V>(i * ge);
             if constexpr (PACKED) {
@@ -270,8 +289,8 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             if (__builtin_amdgcn_ballot_w64(cap_a || cap_b) != 0) {
 #pragma unroll
                 for (int m = 0; m < K; ++m) {
-                    if (last && m > mstar) continue;
                     const int j = j0 + m;
+                    if ((last && m > mstar) || j < 0) continue; // past Q-1, padding
                     // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
                     const V wv = st.H[m] - splat<V>(j * ge);
                     if constexpr (PACKED) {
@@ -335,10 +354,10 @@ __device__ __forceinline__ void sg_finish(const ScoreParams &p, const QueryView 
     }
 }
 
-// One query chunk c (rows j0 .. j0+K-1) of a group: every column of the
-// lane's target(s). chunk_base / slut: the chunk's profile rows and their
-// table (int16 profile: smem + 2*j0 and the residue lut; PAIRED: the
-// chunk's pair rows and the pair table).
+// One query chunk c (rows j0 = c*K - pad .. j0+K-1) of a group: every
+// column of the lane's target(s). chunk_base / slut: the chunk's profile
+// rows and their table (int16 profile: smem + 2*(front + j0) and the residue
+// lut; PAIRED: the chunk's pair rows and the pair table).
 template <typename V, int K, bool TE, bool QE, bool PAIRED>
 __device__ __forceinline__ void sg_chunk(const ScoreParams &p, const QueryView &qv,
                                          const uint8_t *chunk_base, const uint16_t *slut,
@@ -349,12 +368,12 @@ __device__ __forceinline__ void sg_chunk(const ScoreParams &p, const QueryView &
     const int go = p.go, ge = p.ge, Q = qv.Q;
     const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
     const bool qe_rt = QE && (qv.flags & 2u);
-    const int j0 = c * K;
+    const int j0 = c * K - qv.pad;
     const bool first = c == 0;
     const bool last = c == qv.nchunks - 1;
     const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
     SgChunk<V, K> st;
-    sg_chunk_init(st, j0, qs, go, ge);
+    sg_chunk_init(st, j0, qs, ts, go, ge);
     uint2 rr_next = res[lane];
     uint2 bn_next[4];
 #pragma unroll
@@ -425,10 +444,10 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
     TrackI32 t32;
     track_init(t16, t32);
     for (int c = 0; c < qv.nchunks; ++c) {
-        const int j0 = c * K;
+        const int e0 = qv.front + c * K - qv.pad; // profile entry of the chunk's first row
         // LDS address of this chunk's rows in profile row `code`
-        const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * j0;
-        if constexpr (PAIRED) stage_pairs<K>(*pv, j0, lane);
+        const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * e0;
+        if constexpr (PAIRED) stage_pairs<K>(*pv, e0, lane);
         sg_chunk<V, K, TE, QE, PAIRED>(p, qv, chunk_base, slut, bnd, res, nblk, c, half, lane, RA,
                                        RB, t16, t32);
     }
@@ -467,11 +486,11 @@ __device__ __forceinline__ void sg_tile_lockstep(const ScoreParams &p, const Que
     TrackI32 t32;
     track_init(t16, t32);
     for (int c = 0; c < qv.nchunks; ++c) {
-        const int j0 = c * K;
+        const int e0 = qv.front + c * K - qv.pad; // profile entry of the chunk's first row
         __syncthreads(); // every wave is done with the previous chunk's rows
         for (int x = threadIdx.x; x < npairs * parts; x += blockDim.x) {
             const int pr = x / parts, part = x % parts;
-            const uint4 v = *(const uint4 *)(pprof + (size_t)pr * pqpad + j0 + part * 4);
+            const uint4 v = *(const uint4 *)(pprof + (size_t)pr * pqpad + e0 + part * 4);
             *(uint4 *)(pbuf + pr * pair_row_bytes(K) + part * 16) = v;
         }
         __syncthreads();
@@ -537,7 +556,7 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
     TrackI32 t32;
     track_init(t16, t32);
     SgChunk<V, K> st;
-    sg_chunk_init(st, 0, qs, go, ge);
+    sg_chunk_init(st, 0, qs, ts, go, ge);
     uint2 rr_next = make_uint2(0u, 0u);
     uint2 bn_next[4];
     short8 sa[NS], sb[NS];
@@ -550,21 +569,21 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
     for (int s = 0; s < steps; ++s) {
         const int c = W * r + w;
         if (b >= 0 && b < nblk && r < rounds && c < nch) {
-            const int j0 = c * K;
+            const int j0 = c * K - qv.pad;
             const bool first = c == 0;
             const bool last = c == nch - 1;
             const int mstar = Q - 1 - j0;
             const int par = (r * P + b) & 1; // ring parity of this (round, block)
-            const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * j0;
+            const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * (qv.front + j0);
             if (b == 0) {
-                sg_chunk_init(st, j0, qs, go, ge);
+                sg_chunk_init(st, j0, qs, ts, go, ge);
                 rr_next = res[lane];
                 if (!first && w == 0) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
                 }
                 if constexpr (PAIRED) {
-                    stage_pairs<K>(*pv, j0, lane);
+                    stage_pairs<K>(*pv, qv.front + j0, lane);
                     const short8 *np = pair_row(chunk_base, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
 #pragma unroll
                     for (int k = 0; k < NS; ++k) {
@@ -655,6 +674,8 @@ __global__ void __launch_bounds__(256) sg_batched(ScoreParams p) {
     qv.Q = p.Q;
     qv.nchunks = p.nchunks;
     qv.neg_limit = p.neg_limit;
+    qv.pad = p.pad;
+    qv.front = p.front;
     qv.flags = p.flags;
     qv.out_off = 0;
 
@@ -708,6 +729,8 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
     qv.Q = p.Q;
     qv.nchunks = p.nchunks;
     qv.neg_limit = p.neg_limit;
+    qv.pad = p.pad;
+    qv.front = p.front;
     qv.flags = p.flags;
     qv.out_off = 0;
     const int ng = p.g_end - p.g_begin;
@@ -765,6 +788,8 @@ __global__ void __launch_bounds__(256) sg_hybrid_lock(ScoreParams p) {
     qv.Q = p.Q;
     qv.nchunks = p.nchunks;
     qv.neg_limit = p.neg_limit;
+    qv.pad = p.pad;
+    qv.front = p.front;
     qv.flags = p.flags;
     qv.out_off = 0;
     const int ng = p.g_end - p.g_begin;
@@ -854,6 +879,8 @@ __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
             qv.Q = qd.Q;
             qv.nchunks = qd.nchunks;
             qv.neg_limit = qd.neg_limit;
+            qv.pad = qd.pad;
+            qv.front = qd.front;
             qv.flags = qd.flags;
             qv.out_off = qd.out_off;
             const int g = qd.g_begin + (PACKED ? item : (item >> 1));
@@ -921,6 +948,8 @@ __global__ void __launch_bounds__(256) sg_multi_lock(ScoreParams p) {
         qv.Q = qd.Q;
         qv.nchunks = qd.nchunks;
         qv.neg_limit = qd.neg_limit;
+        qv.pad = qd.pad;
+        qv.front = qd.front;
         qv.flags = qd.flags;
         qv.out_off = qd.out_off;
         const int item = s_tile[1] + wave;

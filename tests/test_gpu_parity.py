@@ -174,6 +174,34 @@ def test_gap_open_not_above_extend(eng, engine):
             check(eng, engine, q, Targets.from_list(seqs), "actgn", go, ge, flags, f"go={go} ge={ge}")
 
 
+@pytest.mark.parametrize("tile", [16, 32, 40])
+@pytest.mark.parametrize("schedule", ["wave", "workgroup"])
+def test_top_padding_every_row_offset(eng, tile, schedule):
+    """With go >= ge the kernels put padding rows above query row 0 so that
+    the query fills whole chunks (0 .. tile-1 of them). Query lengths at
+    every offset modulo the tile height, and a few multi-chunk ones, rotate
+    through target start free and not (the padding rows' two forms), go == ge
+    and go > ge, the other free ends, and nucleotide and protein codes (per
+    wave and workgroup-shared paired profiles): every result equals the
+    oracle."""
+    rng = np.random.default_rng(zlib.crc32(f"toppad-{tile}-{schedule}".encode()))
+    nuc = Targets.from_list([rng.integers(0, 5, int(rng.integers(0, 260))).astype(np.uint8) for _ in range(300)])
+    prot = Targets.from_list([rng.integers(0, 24, int(rng.integers(0, 260))).astype(np.uint8) for _ in range(300)])
+    combos = [(ts, gaps, fl, kind) for ts in (True, False) for gaps in ((3, 1), (2, 2))
+              for fl in ("TT", "FF", "TF", "FT") for kind in ("actgn", "blosum62")]
+    lengths = list(range(1, tile + 2)) + [3 * tile - 5, 3 * tile, 3 * tile + 3, 5 * tile + 7]
+    for n, Q in enumerate(lengths):
+        ts, (go, ge), fl, kind = combos[(n * 5) % len(combos)]
+        # flags string: qs qe ts te
+        flags = fl[0] + fl[1][0] + ("T" if ts else "F") + fl[1][0]
+        if n % 3 == 1:
+            flags = flags[:3] + ("F" if flags[3] == "T" else "T")
+        A = 5 if kind == "actgn" else 24
+        q = rng.integers(0, A, Q).astype(np.uint8)
+        check(eng, "batched", q, nuc if kind == "actgn" else prot, kind, go, ge, O.flags_from_str(flags),
+              f"Q={Q} {flags} go={go} ge={ge} {kind}", tile, schedule)
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("n", [1, 63, 127, 128, 129, 1000])
 def test_batch_sizes_and_empty_targets(eng, engine, n):
@@ -202,9 +230,23 @@ def test_int32_widening(eng):
     seqs = [rng.integers(0, 24, L).astype(np.uint8) for L in lens]
     for flags in (12, 0, 15, 2):
         for tile in (16, 32):
-            s, er, eq, st = check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 3, 4, flags, "", tile)
+            s, er, eq, st = check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 5, 4, flags, "", tile)
             assert (st[:3] & ish_amd.STATUS_WIDENED).all()
             assert (st == ish_amd.STATUS_I16).sum() > 100
+
+
+def test_gap_open_below_extend_runs_int32(eng):
+    """go < ge: the padding rows above the query would not repeat the top
+    boundary, so every group runs in the int32 kernels (which read row Q-1
+    from any register row) and stays exact."""
+    rng = np.random.default_rng(12)
+    q = rng.integers(0, 24, 150).astype(np.uint8)
+    seqs = [rng.integers(0, 24, int(L)).astype(np.uint8) for L in rng.integers(1, 500, 600)]
+    for flags in (0, 4, 10, 15):
+        for tile in (16, 32, 40):
+            s, er, eq, st = check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 2, 3, flags,
+                                  f"flags={flags}", tile)
+            assert (st & ish_amd.STATUS_WIDENED).all()
 
 
 @pytest.mark.parametrize("engine", ENGINES)
