@@ -154,6 +154,33 @@ __device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
     t32.corner = 0;
 }
 
+// Makes v be computed at this point: an empty asm that reads and writes
+// it, past which the compiler cannot sink the computation.
+template <typename V> __device__ __forceinline__ void pin(V &v) {
+    uint32_t u = as_u32(v);
+    asm volatile("" : "+v"(u));
+    v = from_u32<V>(u);
+}
+
+// Profile value of row mm (0..7) of an 8-row slice: PAIRED, the packed
+// word of rows mm (sa: rows 0..3, sb: rows 4..7); packed int16, the halves
+// of the lane's two targets' slices; int32, the one slice's entry.
+template <typename V, bool PAIRED, bool PACKED>
+__device__ __forceinline__ V slice_val(const short8 &sa, const short8 &sb, int mm) {
+    V s;
+    if constexpr (PAIRED) {
+        const short8 &src = mm < 4 ? sa : sb;
+        s.x = src[2 * (mm & 3)];
+        s.y = src[2 * (mm & 3) + 1];
+    } else if constexpr (PACKED) {
+        s.x = sa[mm];
+        s.y = sb[mm];
+    } else {
+        s = (int)sa[mm];
+    }
+    return s;
+}
+
 // Scores columns 4b .. 4b+3 of one chunk (rows j0 .. j0+K-1; rows below 0
 // are top padding rows).
 //   rr / rr_next: residue words of this block and the next (the profile
@@ -205,42 +232,48 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             htop = from_u32<V>(bn[t].x);
             f = from_u32<V>(bn[t].y);
         }
-        V diag = st.diag_top;
+        // Rows run software-pipelined: row m's F chain (h = max(t2, f),
+        // gg = h + c, f = max(f, gg)) alternates with work that does not
+        // depend on it (t1 and t2 of row m+2, the E update of row m), so no
+        // packed operation reads the result of the one just before it,
+        // which would cost a wait state. The scheduling barriers keep this
+        // order.
+        V t2[K]; // max(H^diag + S', E^) of the rows ahead
+        t2[0] = vmax(st.diag_top + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 0), st.E[0]);
+        t2[1] = vmax(st.H[0] + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 1), st.E[1]);
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-#pragma unroll
-            for (int mm = 0; mm < 8; ++mm) {
-                const int m = k * 8 + mm;
-                V s;
-                if constexpr (PAIRED) {
-                    // rows 8k..8k+3 in sa[k], 8k+4..8k+7 in sb[k]
-                    const short8 &src = mm < 4 ? sa[k] : sb[k];
-                    s.x = src[2 * (mm & 3)];
-                    s.y = src[2 * (mm & 3) + 1];
-                } else if constexpr (PACKED) {
-                    s.x = sa[k][mm];
-                    s.y = sb[k][mm];
-                } else {
-                    s = (int)sa[k][mm];
-                }
-                V h = vmax(vmax(diag + s, st.E[m]), f);
-                diag = st.H[m];
-                st.H[m] = h;
-                const V gg = h + vcst;
-                st.E[m] = vmax(st.E[m], gg);
-                f = vmax(f, gg);
-            }
-            // slice k is consumed: fetch it for the next column. The
-            // scheduling barrier keeps the fetch from being hoisted above
-            // the rows that still read the old slice, so only one extra
-            // slice is live at a time.
+        for (int m = 0; m < K; ++m) {
+            const V h = vmax(t2[m], f);
             __builtin_amdgcn_sched_barrier(0);
-            if constexpr (PAIRED) {
-                sa[k] = na[2 * k];
-                sb[k] = na[2 * k + 1];
-            } else {
-                sa[k] = na[k];
-                if (PACKED) sb[k] = nb[k];
+            // row m-1's E update stays in its slot (not sunk into the next
+            // column); pinned one operation after it is written
+            if (m > 0) pin(st.E[m - 1]);
+            V t1;
+            if (m + 2 < K) // H^diag of row m+2 is row m+1's value in the previous column
+                t1 = st.H[m + 1] + slice_val<V, PAIRED, PACKED>(sa[(m + 2) / 8], sb[(m + 2) / 8], (m + 2) % 8);
+            __builtin_amdgcn_sched_barrier(0);
+            const V gg = h + vcst;
+            __builtin_amdgcn_sched_barrier(0);
+            if (m + 2 < K) t2[m + 2] = vmax(t1, st.E[m + 2]);
+            __builtin_amdgcn_sched_barrier(0);
+            f = vmax(f, gg);
+            __builtin_amdgcn_sched_barrier(0);
+            st.E[m] = vmax(st.E[m], gg);
+            st.H[m] = h;
+            __builtin_amdgcn_sched_barrier(0);
+            if (m % 8 == 7) {
+                // slice m/8 is consumed (its last row's t1 was taken two rows
+                // ago): fetch it for the next column, after the rows that
+                // read it, so only one extra slice is live at a time
+                const int k = m / 8;
+                if constexpr (PAIRED) {
+                    sa[k] = na[2 * k];
+                    sb[k] = na[2 * k + 1];
+                } else {
+                    sa[k] = na[k];
+                    if (PACKED) sb[k] = nb[k];
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         st.diag_top = htop;
