@@ -68,6 +68,33 @@ __device__ __forceinline__ void stage_profile(const int16_t *prof, const uint16_
 
 typedef short short8 __attribute__((ext_vector_type(8)));
 
+// Makes v be computed at this point: an empty asm that reads and writes
+// it, past which the compiler cannot sink the computation.
+template <typename V> __device__ __forceinline__ void pin(V &v) {
+    uint32_t u = as_u32(v);
+    asm volatile("" : "+v"(u));
+    v = from_u32<V>(u);
+}
+
+// Profile value of row mm (0..7) of an 8-row slice: PAIRED, the packed
+// word of rows mm (sa: rows 0..3, sb: rows 4..7); packed int16, the halves
+// of the lane's two targets' slices; int32, the one slice's entry.
+template <typename V, bool PAIRED, bool PACKED>
+__device__ __forceinline__ V slice_val(const short8 &sa, const short8 &sb, int mm) {
+    V s;
+    if constexpr (PAIRED) {
+        const short8 &src = mm < 4 ? sa : sb;
+        s.x = src[2 * (mm & 3)];
+        s.y = src[2 * (mm & 3) + 1];
+    } else if constexpr (PACKED) {
+        s.x = sa[mm];
+        s.y = sb[mm];
+    } else {
+        s = (int)sa[mm];
+    }
+    return s;
+}
+
 // Paired profile (small alphabets): one row per pair of profile classes
 // (class of a lane's target A, class of its target B) holding, per query
 // row j, the int32 word (S_A[j] + bias) | (S_B[j] + bias) << 16, so that a
