@@ -161,11 +161,14 @@ __device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
 //   bn: the chunk-top rows (H^, F^ of the row above) unless `first`;
 //   out: where the chunk-bottom rows go (t*64 + lane), null for the last
 //   chunk, which tracks row Q-1 instead: register mstar, which the packed
-//   variants take to be K-1 (their queries are top padded).
+//   variants take to be K-1 (their queries are top padded). DEFER (out in
+//   global memory): the four columns' rows are stored together after the
+//   block, so that the wait for the next block's residues (in the last
+//   column) finds the previous block's stores long done.
 // TE / QE say which end rules the variant can apply; qe_rt is the query's.
 // PAIRED (packed lanes): chunk_base is the wave's paired chunk profile and
 // slut the pair table (see PairView).
-template <typename V, int K, bool TE, bool QE, bool PAIRED = false>
+template <typename V, int K, bool TE, bool QE, bool PAIRED = false, bool DEFER = false>
 __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool first, bool last,
                                          int mstar, uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
                                          short8 (&sa)[K / 8], short8 (&sb)[K / 8],
@@ -181,6 +184,7 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
     auto row_ptr = [&](uint32_t code) -> const short8 * {
         return (const short8 *)(chunk_base + 4u * (uint32_t)slut[code]);
     };
+    uint2 bottom[4]; // chunk-bottom rows (H^, F^) of the block's columns
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int i = b * 4 + t;
@@ -251,10 +255,9 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
         }
         st.diag_top = htop;
         if (!last) {
-            uint2 o;
-            o.x = as_u32(st.H[K - 1]);
-            o.y = as_u32(f);
-            out[t * 64 + lane] = o;
+            bottom[t].x = as_u32(st.H[K - 1]);
+            bottom[t].y = as_u32(f);
+            if constexpr (!DEFER) out[t * 64 + lane] = bottom[t];
         } else {
             // row Q-1 lives in register mstar of the last chunk
             V hl = st.H[K - 1];
@@ -313,6 +316,10 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                 }
             }
         }
+    }
+    if (DEFER && !last) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) out[t * 64 + lane] = bottom[t];
     }
 }
 
@@ -418,9 +425,10 @@ __device__ __forceinline__ void sg_chunk(const ScoreParams &p, const QueryView &
                 for (int t = 0; t < 4; ++t) bn_next[t] = bnd[(size_t)((b + 1) * 4 + t) * 64 + lane];
             }
         }
-        sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
-                                       chunk_base, slut, half, RA, RB, ts, qe_rt, go, ge, t16, t32,
-                                       last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
+        sg_block<V, K, TE, QE, PAIRED, true>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa,
+                                             sb, chunk_base, slut, half, RA, RB, ts, qe_rt, go, ge,
+                                             t16, t32, last ? nullptr : bnd + (size_t)b * 4 * 64,
+                                             lane);
     }
 }
 
