@@ -178,7 +178,6 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                                          int lane) {
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
     constexpr bool FIXED_LAST = PACKED;
-    constexpr int NS = K / 8;
     const int cst = ge - go;
     const V vcst = splat<V>(cst);
     auto row_ptr = [&](uint32_t code) -> const short8 * {
