@@ -27,8 +27,9 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["ish_hip.cpp", "server.cpp", "search.cpp", "sg_kernels.hip", "sw_kernels.hip",
-               "serve_kernels.hip",
+# the kernel files that take longest to compile come first
+HIP_SOURCES = ["sg_hybrid.hip", "sg_multi.hip", "sg_lock.hip", "sg_batched.hip", "sw_kernels.hip",
+               "sg_misc.hip", "ish_hip.cpp", "server.cpp", "search.cpp", "serve_kernels.hip",
                "noalloc.hip"]
 
 

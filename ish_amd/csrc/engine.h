@@ -116,7 +116,7 @@ struct PlainParams {
     uint8_t *status;
 };
 
-// Launchers (sg_kernels.hip). Return hipError_t of the launch.
+// Launchers (sg_*.hip, sw_kernels.hip). Return hipError_t of the launch.
 hipError_t launch_reset(uint32_t *counter, hipStream_t st);
 hipError_t launch_sg_batched_i16(const ScoreParams &p, int chunk, int grid,
                                  int block, size_t lds, hipStream_t st);
@@ -131,6 +131,12 @@ int occupancy_sw_batched_i16(int chunk, int block, size_t lds);
 // LDS copy of a chunk (K int32 words plus 16 bytes), and the largest class
 // count that gets one.
 constexpr int pair_row_bytes(int K) { return 4 * K + 16; }
+// LDS bytes of the lookup tables, held as uint32 entries so that a lookup
+// is one zero-extended load: the residue-code -> profile-row table (256
+// dwords) after the int16 profile, and the pair table (2 x 256 dwords)
+// at the start of a paired launch's LDS.
+constexpr size_t kLutLds = 256 * 4;
+constexpr size_t kPairTableLds = 512 * 4;
 constexpr int kPairMaxRows = 8;
 // Hybrid semi-global launch (packed int16, `waves` = 2 or 4 waves per
 // workgroup): the first n_long groups are pipelined across a workgroup's

@@ -66,7 +66,7 @@ struct Profile {
     bool top = false; // top padding rows (front > 0)
     std::vector<int16_t> fwd, rev, plain; // offset form, reversed query, S
     uint16_t lut[256];
-    size_t lds() const { return (size_t)nrows * stride * 2 + 512; }
+    size_t lds() const { return (size_t)nrows * stride * 2 + kLutLds; }
     int32_t words() const { return nrows * stride / 2; }
     // entries of a row the kernels may read: the front and the padded query
     int32_t width() const { return front + qpad; }

@@ -1493,7 +1493,18 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     TuneChoice best{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO, 0};
     float best_ms = 1e30f;
     const int saved_launches = ctx->launches;
-    std::vector<DbPlan> seen;
+    // what a plan launches, without the buffers it points at (which a
+    // grown scratch changes): candidates that plan the same launches are
+    // timed once
+    auto launch_key = [](const DbPlan &plan) {
+        std::vector<int32_t> k;
+        for (const DbLaunch &L : plan) {
+            for (int32_t v : {L.kind, L.chunk, L.g_begin, L.g_end, L.grid, L.n_long, L.paired, L.waves})
+                k.push_back(v);
+        }
+        return k;
+    };
+    std::vector<std::vector<int32_t>> seen;
     // pipelined semi-global groups also try both workgroup widths (a plan
     // that has none comes out the same for each and is timed once)
     const int widths[3] = {0, 2, 4};
@@ -1511,10 +1522,16 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
         int32_t rc = plan_db(ctx, db, (const int16_t *)ctx->prof_t.p, ctx->flags, 0, db->ngroups,
                              c.chunk, c.sched, plan, c.waves);
         if (rc) return rc;
+        std::vector<int32_t> key = launch_key(plan);
         bool dup = false; // a candidate that plans the same launches as one already timed
-        for (const auto &q : seen) dup = dup || q.same(plan);
+        for (const auto &q : seen) dup = dup || q == key;
         if (dup) continue;
-        seen.push_back(plan);
+        seen.push_back(std::move(key));
+        // the tile height the int16 launch runs with (a candidate height
+        // that launch lacks falls back to another)
+        int32_t eff = c.chunk;
+        for (const DbLaunch &L : plan)
+            if (L.kind != kSg32 && L.kind != kSw32) eff = L.chunk;
         float total = 0.f;
         for (int rep = 0; rep <= reps; ++rep) {
             ctx->launches = 0;
@@ -1525,7 +1542,7 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
         }
         if (total < best_ms) {
             best_ms = total;
-            best = TuneChoice{(int8_t)c.chunk, (int8_t)c.sched, (int8_t)c.waves};
+            best = TuneChoice{(int8_t)eff, (int8_t)c.sched, (int8_t)c.waves};
         }
     }
     ctx->launches = saved_launches;
@@ -2116,8 +2133,8 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
         paired = build_pairs(*items[k].pf, db->code_mask, kPairMaxShared, pairs[k]);
         max_classes = std::max(max_classes, pairs[k].classes);
     }
-    const size_t wave_pair_lds = 1024 + (size_t)(kBlock / 64) * max_classes * max_classes * pair_row_bytes(chunk);
-    const size_t lock_lds = 1024 + (size_t)max_classes * max_classes * pair_row_bytes(chunk);
+    const size_t wave_pair_lds = kPairTableLds + (size_t)(kBlock / 64) * max_classes * max_classes * pair_row_bytes(chunk);
+    const size_t lock_lds = kPairTableLds + (size_t)max_classes * max_classes * pair_row_bytes(chunk);
     enum { kUnpaired = 0, kPerWave = 1, kShared = 2 };
     int pmode = kUnpaired;
     if (paired && max_classes <= kPairMaxRows && wave_pair_lds <= kLdsLimit) pmode = kPerWave;

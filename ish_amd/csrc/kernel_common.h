@@ -53,16 +53,16 @@ template <> __device__ __forceinline__ s16x2 pack2<s16x2>(int a, int b) {
 template <> __device__ __forceinline__ int pack2<int>(int a, int) { return a; }
 
 // Stage the query profile and the residue -> profile-row table into LDS.
-// The table holds each row's offset in dwords (256 x uint16), so a residue
-// code turns into an LDS address with one shift-add.
+// The table holds each row's offset in dwords (256 x uint16 in memory,
+// widened to uint32 in LDS), so a residue code turns into an LDS address
+// with one load and one shift-add.
 __device__ __forceinline__ void stage_profile(const int16_t *prof, const uint16_t *lut,
                                               int prof_words, uint8_t *smem) {
     const int *src = (const int *)prof;
     int *dst = (int *)smem;
     for (int x = threadIdx.x; x < prof_words; x += blockDim.x) dst[x] = src[x];
-    const int *lsrc = (const int *)lut;
-    int *ldst = (int *)(smem + (size_t)prof_words * 4);
-    for (int x = threadIdx.x; x < 128; x += blockDim.x) ldst[x] = lsrc[x];
+    uint32_t *ldst = (uint32_t *)(smem + (size_t)prof_words * 4);
+    for (int x = threadIdx.x; x < 256; x += blockDim.x) ldst[x] = lut[x];
     __syncthreads();
 }
 
@@ -126,17 +126,18 @@ __device__ __forceinline__ void stage_pairs(const PairView &pv, int j0, int lane
 }
 
 // Pair row of the residue codes (a, b) in a wave's buffer.
-__device__ __forceinline__ const short8 *pair_row(const uint8_t *pbuf, const uint16_t *plut,
+__device__ __forceinline__ const short8 *pair_row(const uint8_t *pbuf, const uint32_t *plut,
                                                   uint32_t a, uint32_t b) {
-    return (const short8 *)(pbuf + (uint32_t)plut[a] + (uint32_t)plut[256 + b]);
+    return (const short8 *)(pbuf + plut[a] + plut[256 + b]);
 }
 
-// Stages the pair table of tile height K (2 x 256 uint16, 1 KB) into smem
-// and waits for the workgroup.
+// Stages the pair table of tile height K (2 x 256 uint16 in memory) into
+// smem as 512 uint32 entries (kPairTableLds bytes) and waits for the
+// workgroup.
 template <int K>
 __device__ __forceinline__ void stage_pair_table(const uint16_t *plut, uint8_t *smem) {
-    const uint32_t *src = (const uint32_t *)(plut + pair_table_index(K) * 512);
-    for (int x = threadIdx.x; x < 256; x += blockDim.x) ((uint32_t *)smem)[x] = src[x];
+    const uint16_t *src = plut + pair_table_index(K) * 512;
+    for (int x = threadIdx.x; x < 512; x += blockDim.x) ((uint32_t *)smem)[x] = src[x];
     __syncthreads();
 }
 

@@ -100,7 +100,7 @@ template <typename V, int K, bool PAIRED = false>
 __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool first,
                                          uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
                                          short8 (&sa)[K / 8], short8 (&sb)[K / 8],
-                                         const uint8_t *chunk_base, const uint16_t *slut, int half,
+                                         const uint8_t *chunk_base, const uint32_t *slut, int half,
                                          int RA, int RB, int rmin, V lbm1, int go, int ge,
                                          uint2 *out, int lane) {
     constexpr bool PACKED = !__is_same(V, int);
@@ -145,42 +145,59 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             htop = from_u32<V>(bn[t].x);
             f = from_u32<V>(bn[t].y);
         }
-        V diag = st.diag_top;
+        // Rows run software-pipelined: row m's F chain (h = max(t3, f),
+        // g = h + c, f = max(f, g) - ge) alternates with work that does not
+        // depend on it (t1, t2, t3 of row m+2: the diagonal, E and floor
+        // terms; the E update of row m), so no packed operation reads the
+        // result of the one just before it, which would cost a wait state.
+        // The scheduling barriers keep this order.
+        V t3[K];  // max(H~diag + S', E~, floor) of the rows ahead
+        V cms[NS]; // column maximum of each 8-row slice (every h includes the floor vz)
+        t3[0] = vmax(vmax(st.diag_top + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 0), st.E[0]), vz);
+        t3[1] = vmax(vmax(st.H[0] + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 1), st.E[1]), vz);
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-#pragma unroll
-            for (int mm = 0; mm < 8; ++mm) {
-                const int m = k * 8 + mm;
-                V sv;
-                if constexpr (PAIRED) {
-                    // rows 8k..8k+3 in sa[k], 8k+4..8k+7 in sb[k]
-                    const short8 &src = mm < 4 ? sa[k] : sb[k];
-                    sv.x = src[2 * (mm & 3)];
-                    sv.y = src[2 * (mm & 3) + 1];
-                } else if constexpr (PACKED) {
-                    sv.x = sa[k][mm];
-                    sv.y = sb[k][mm];
-                } else {
-                    sv = (int)sa[k][mm];
-                }
-                // the floor and E come from the previous column: only the
-                // last max waits on the F chain
-                const V h = vmax(vmax(vmax(diag + sv, st.E[m]), vz), f);
-                diag = st.H[m];
-                st.H[m] = h;
-                const V g = h + vcst;
-                st.E[m] = vmax(st.E[m], g);
-                f = vmax(f, g) - vge;
-            }
-            // slice k is consumed: fetch it for the next column (the
-            // scheduling barrier keeps only one extra slice live)
+        for (int m = 0; m < K; ++m) {
+            const V h = vmax(t3[m], f);
             __builtin_amdgcn_sched_barrier(0);
-            if constexpr (PAIRED) {
-                sa[k] = na[2 * k];
-                sb[k] = na[2 * k + 1];
-            } else {
-                sa[k] = na[k];
-                if (PACKED) sb[k] = nb[k];
+            // row m-1's E update and slice-maximum step stay in their slot
+            // (not sunk past the loop); pinned one operation after they are
+            // written
+            if (m > 0) {
+                pin(st.E[m - 1]);
+                pin(cms[(m - 1) / 8]);
+            }
+            V t1, t2;
+            if (m + 2 < K) // H~diag of row m+2 is row m+1's value in the previous column
+                t1 = st.H[m + 1] + slice_val<V, PAIRED, PACKED>(sa[(m + 2) / 8], sb[(m + 2) / 8], (m + 2) % 8);
+            __builtin_amdgcn_sched_barrier(0);
+            const V g = h + vcst;
+            __builtin_amdgcn_sched_barrier(0);
+            if (m + 2 < K) t2 = vmax(t1, st.E[m + 2]);
+            __builtin_amdgcn_sched_barrier(0);
+            const V f1 = vmax(f, g);
+            __builtin_amdgcn_sched_barrier(0);
+            if (m + 2 < K) t3[m + 2] = vmax(t2, vz);
+            __builtin_amdgcn_sched_barrier(0);
+            f = f1 - vge;
+            __builtin_amdgcn_sched_barrier(0);
+            st.E[m] = vmax(st.E[m], g);
+            st.H[m] = h;
+            // the slice maxima run as chains one row apart
+            cms[m / 8] = m % 8 == 0 ? h : vmax(cms[m / 8], h);
+            __builtin_amdgcn_sched_barrier(0);
+            if (m % 8 == 7) {
+                // slice m/8 is consumed (its last row's t1 was taken two rows
+                // ago): fetch it for the next column, after the rows that
+                // read it, so only one extra slice is live at a time
+                const int k = m / 8;
+                if constexpr (PAIRED) {
+                    sa[k] = na[2 * k];
+                    sb[k] = na[2 * k + 1];
+                } else {
+                    sa[k] = na[k];
+                    if (PACKED) sb[k] = nb[k];
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         st.diag_top = htop;
@@ -190,15 +207,7 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             o.y = as_u32(f);
             out[t * 64 + lane] = o;
         }
-        // column maximum, one partial per 8-row slice, each a tree over its
-        // rows (every h already includes the floor vz)
-        V cms[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const V *hk = st.H + 8 * k;
-            cms[k] = vmax(vmax(vmax(hk[0], hk[1]), vmax(hk[2], hk[3])),
-                          vmax(vmax(hk[4], hk[5]), vmax(hk[6], hk[7])));
-        }
+        // column maximum over the slices' maxima
         V cm = cms[0];
 #pragma unroll
         for (int k = 1; k < NS; ++k) cm = vmax(cm, cms[k]);
@@ -294,7 +303,7 @@ template <typename V, int K>
 __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     stage_profile(p.prof, p.lut, p.prof_words, smem);
-    const uint16_t *slut = (const uint16_t *)(smem + (size_t)p.prof_words * 4);
+    const uint32_t *slut = (const uint32_t *)(smem + (size_t)p.prof_words * 4);
 
     constexpr bool PACKED = !__is_same(V, int);
     constexpr int NS = K / 8;
@@ -403,10 +412,10 @@ constexpr size_t kPipePubBytes = (size_t)kPipeWaves * 64 * sizeof(uint32_t);
 constexpr size_t kPipeExtraBytes = kPipeRingBytes + kPipeCombBytes + kPipePubBytes + 16;
 size_t sw_pipe_extra_lds() { return kPipeExtraBytes; }
 
-// Paired variant: the LDS a launch needs is the pair table (1 KB), the
+// Paired variant: the LDS a launch needs is the pair table (2 KB), the
 // pipeline's own buffers, and one paired chunk profile per wave.
 size_t sw_pipe_paired_lds(int chunk, int nrows) {
-    return 1024 + sw_pipe_extra_lds() +
+    return kPairTableLds + sw_pipe_extra_lds() +
            (size_t)kPipeWaves * nrows * nrows * pair_row_bytes(chunk);
 }
 
@@ -418,12 +427,12 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     size_t head;
     if constexpr (PAIRED) {
         stage_pair_table<K>(p.plut, smem);
-        head = 1024;
+        head = kPairTableLds;
     } else {
         stage_profile(p.prof, p.lut, p.prof_words, smem);
-        head = (size_t)p.prof_words * 4 + 512;
+        head = (size_t)p.prof_words * 4 + kLutLds;
     }
-    const uint16_t *slut = (const uint16_t *)(PAIRED ? smem : smem + (size_t)p.prof_words * 4);
+    const uint32_t *slut = (const uint32_t *)(PAIRED ? smem : smem + (size_t)p.prof_words * 4);
     uint8_t *tail = smem + ((head + 15) & ~(size_t)15);
     uint2 *ring = (uint2 *)tail;                // [wave 0..2][parity][t][lane]
     int *comb = (int *)(tail + kPipeRingBytes); // [wave][lane][6]
@@ -450,7 +459,7 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
         __syncthreads();
-        const int item = s_item;
+        const int item = (int)wave_broadcast0((uint32_t)s_item); // scalar from here on
         pub[w * 64 + lane] = 0u;
         __syncthreads(); // s_item is rewritten at the top of the next group;
                          // every pub entry is reset before anyone reads it
