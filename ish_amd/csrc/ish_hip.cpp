@@ -1292,6 +1292,9 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
             const int nch = (ctx->Q + chunk - 1) / chunk;
             if (nch < 8 && 2 * ((nch + 1) / 2) < 4 * ((nch + 3) / 4)) L.waves = 2;
             if (waves == 2 || waves == 4) L.waves = waves;
+            // the 40-row tile runs on four waves only (its two-wave
+            // variants would spill registers)
+            if (chunk == 40) L.waves = 4;
         }
         L.chunk = chunk;
         L.g_begin = g0;

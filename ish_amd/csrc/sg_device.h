@@ -278,14 +278,13 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             if constexpr (PACKED) {
                 const uint32_t uu = as_u32(u);
                 if (TE || QE) {
-                    const int ka = key_lo(uu, i), kb = key_hi(uu, i);
-                    if (i < rmin) {
-                        t16.te_a = max(t16.te_a, ka);
-                        t16.te_b = max(t16.te_b, kb);
-                    } else {
-                        if (i < RA) t16.te_a = max(t16.te_a, ka);
-                        if (i < RB) t16.te_b = max(t16.te_b, kb);
+                    int ka = key_lo(uu, i), kb = key_hi(uu, i);
+                    if (i >= rmin) { // past some lane's target end
+                        if (i >= RA) ka = INT_MIN;
+                        if (i >= RB) kb = INT_MIN;
                     }
+                    t16.te_a = max(t16.te_a, ka);
+                    t16.te_b = max(t16.te_b, kb);
                 } else if (i >= rmin - 1) {
                     if (i == RA - 1) t16.corner_a = (int)(short)(uu & 0xffffu);
                     if (i == RB - 1) t16.corner_b = (int)(short)(uu >> 16);
@@ -302,7 +301,7 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                 }
             }
         }
-        if (qe_rt && i >= rmin - 1) { // the last column of a target is at or past rmin - 1
+        if (qe_rt) {
             const bool cap_a = i == RA - 1;
             const bool cap_b = PACKED && i == RB - 1;
             // wave-uniform test keeps this once-per-target path a real
@@ -344,6 +343,9 @@ __device__ __forceinline__ void sg_finish(const ScoreParams &p, const QueryView 
     const int Q = qv.Q, ge = p.ge;
     const int qoff = (Q - 1) * ge;
     if constexpr (PACKED) {
+        // once per group: kept rolled, so that it adds few registers where
+        // the chunk state is live
+#pragma unroll 1
         for (int which = 0; which < 2; ++which) {
             const int t = which ? tB : tA;
             const int R = which ? RB : RA;
@@ -567,7 +569,10 @@ constexpr int kSgPipeWaves = 4;
 constexpr size_t kSgRingBytes = (size_t)(kSgPipeWaves - 1) * 2 * 4 * 64 * sizeof(uint2);
 constexpr size_t kSgCombBytes = (size_t)kSgPipeWaves * 64 * 6 * sizeof(int);
 
-constexpr size_t kSgExtraBytes = kSgRingBytes + kSgCombBytes + 16;
+// LDS after the ring: two end-key buffers (one for sg_pipe_group; the
+// group parity's for sg_pipe_stream) and the stream's control words
+constexpr size_t kSgCtlBytes = 32;
+constexpr size_t kSgExtraBytes = kSgRingBytes + 2 * kSgCombBytes + kSgCtlBytes;
 
 // W: the waves of the workgroup, which share the group's query chunks.
 template <int K, bool TE, bool QE, bool PAIRED = false, int W = kSgPipeWaves>
@@ -699,6 +704,184 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
         sg_finish<V>(p, qv, tA, tB, RA, RB, te_rt, qe_rt, t16, t32);
     }
     __syncthreads(); // ring and comb are reused by the next item
+}
+
+// ---------------------------------------------------------------------------
+// Streamed workgroup pipeline for queries of at most W chunks (one round):
+// wave w always scores chunk w, and the workgroup runs through consecutive
+// long groups of the queue without draining between them. Group k holds
+// steps [start_k + w, start_k + w + nblk_k) of wave w, with
+// start_{k+1} = start_k + max(nblk_k, W). Chunk-bottom rows pass to the
+// next wave through the LDS ring at the parity of the step, so a wave that
+// starts a group while the next wave still finishes the previous one does
+// not overwrite what that wave reads. Wave w's end keys of group k go to
+// comb buffer k & 1; the wave of the last chunk combines them and writes
+// the group's results after its last block (the other waves wrote theirs
+// at earlier steps, and the buffer is written again no sooner than 2W
+// steps later). The paired chunk profile of a wave is staged once.
+// Thread 0 fetches the next queue item when a group starts on wave 0; an
+// item that is not a long group ends the stream after that group and is
+// returned to the caller. The query needs at least one chunk: wave 0 must
+// run, or no fetch ever ends the stream. ctl: [0..3] items of groups k (k & 3), [4] the
+// stream's step count once its last group is known, [5] the returned
+// item, [6] the number of groups in the stream.
+// ---------------------------------------------------------------------------
+// LAST: this wave scores the query's last chunk (its tracking and the
+// other waves' ring stores are separate code, so that neither adds its
+// registers to the other).
+template <int K, bool TE, bool QE, bool PAIRED, int W, bool LAST>
+__device__ __forceinline__ int sg_pipe_stream_role(const ScoreParams &p, const QueryView &qv,
+                                                   const uint8_t *smem, const uint32_t *slut,
+                                                   uint2 *ring, int *comb, int *ctl, int item0,
+                                                   int nlong, int w, int lane,
+                                                   const PairView *pv) {
+    using V = s16x2;
+    constexpr int NS = K / 8;
+    const int go = p.go, ge = p.ge, Q = qv.Q;
+    const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
+    const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
+    const int last_w = qv.nchunks - 1; // nchunks <= W
+    const bool active = LAST || w < last_w; // wave-uniform
+    const bool first = w == 0;
+    constexpr bool last = LAST;
+    const int j0 = w * K - qv.pad;
+    const int mstar = Q - 1 - j0;
+    const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * (qv.front + j0);
+    if (threadIdx.x == 0) {
+        ctl[0] = item0;
+        ctl[4] = INT_MAX;
+        ctl[6] = INT_MAX;
+    }
+    if constexpr (PAIRED) {
+        if (active) stage_pairs<K>(*pv, qv.front + j0, lane);
+    }
+    __syncthreads();
+
+    TrackI16 t16;
+    TrackI32 t32;
+    track_init(t16, t32);
+    SgChunk<V, K> st;
+    sg_chunk_init(st, j0, qs, ts, go, ge);
+    short8 sa[NS], sb[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sa[k] = sb[k] = (short8)0;
+    uint2 rr_next = make_uint2(0u, 0u);
+    // this wave's group k: first step of the group (on wave 0), blocks,
+    // steps it holds, targets
+    int k = 0, start = 0, nblk = 0, P = 0;
+    int tA = 0, tB = 0, RA = 0, RB = 0, rmin = 0;
+    const uint2 *res = p.res;
+    int ngroups = INT_MAX; // groups of the stream, once known
+    // group k is done on this wave: its end keys, and from the wave of the
+    // last chunk the group's results
+    auto group_done = [&]() {
+        int *cb = comb + (size_t)(k & 1) * W * 64 * 6;
+        if (!last) {
+            int *cw = cb + (size_t)(w * 64 + lane) * 6;
+            cw[0] = t16.te_a;
+            cw[1] = t16.te_b;
+            cw[2] = t16.qe_a;
+            cw[3] = t16.qe_b;
+        } else {
+            for (int v = 0; v < last_w; ++v) {
+                const int *cv = cb + (size_t)(v * 64 + lane) * 6;
+                t16.te_a = max(t16.te_a, cv[0]);
+                t16.te_b = max(t16.te_b, cv[1]);
+                t16.qe_a = max(t16.qe_a, cv[2]);
+                t16.qe_b = max(t16.qe_b, cv[3]);
+            }
+            sg_finish<V>(p, qv, tA, tB, RA, RB, te_rt, qe_rt, t16, t32);
+        }
+        start += P;
+        ++k;
+    };
+    for (int s = 0;; ++s) {
+        const int b = s - start - w; // block of group k on this wave
+        if (active && b == 0 && k < ngroups) {
+            // group k begins on this wave
+            const int item = (int)wave_broadcast0((uint32_t)ctl[k & 3]);
+            const GroupDesc gd = p.groups[p.g_begin + item];
+            nblk = gd.ncols >> 2;
+            P = nblk > W ? nblk : W;
+            tA = gd.first + lane;
+            tB = gd.first + 64 + lane;
+            RA = p.lens[tA];
+            RB = p.lens[tB];
+            rmin = wave_min(min(RA, RB));
+            res = p.res + gd.res_off;
+            if (w == 0 && lane == 0) {
+                // the next queue item; one that is not a long group ends
+                // the stream after this group
+                const int nx = (int)atomicAdd(p.counter, 1u);
+                ctl[(k + 1) & 3] = nx;
+                if (nx >= nlong) {
+                    ctl[4] = start + last_w + (nblk > 0 ? nblk : 1);
+                    ctl[5] = nx;
+                    ctl[6] = k + 1;
+                }
+            }
+            sg_chunk_init(st, j0, qs, ts, go, ge);
+            track_init(t16, t32);
+            if (nblk == 0) {
+                group_done(); // no column: results from the boundary alone
+            } else {
+                rr_next = res[lane];
+                if constexpr (PAIRED) {
+                    const short8 *np = pair_row(chunk_base, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        sa[q] = np[2 * q];
+                        sb[q] = np[2 * q + 1];
+                    }
+                } else {
+                    const short8 *na = (const short8 *)(chunk_base + 4u * slut[rr_next.x & 0xffu]);
+                    const short8 *nb = (const short8 *)(chunk_base + 4u * slut[rr_next.y & 0xffu]);
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        sa[q] = na[q];
+                        sb[q] = nb[q];
+                    }
+                }
+            }
+        }
+        if (active && b >= 0 && b < nblk) {
+            const uint2 rr = rr_next;
+            uint2 bn[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bn[t] = make_uint2(0u, 0u);
+            if (!first) {
+                const uint2 *src = ring + (size_t)((w - 1) * 2 + ((s - 1) & 1)) * 256;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bn[t] = src[t * 64 + lane];
+            }
+            if (b + 1 < nblk) rr_next = res[(size_t)(b + 1) * 64 + lane];
+            uint2 *out = last ? nullptr : ring + (size_t)(w * 2 + (s & 1)) * 256;
+            sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
+                                           chunk_base, slut, 0, RA, RB, rmin, ts, qe_rt, go, ge, t16,
+                                           t32, out, lane);
+            if (b == nblk - 1) group_done();
+        }
+        __syncthreads();
+        if (s + 1 >= (int)wave_broadcast0((uint32_t)ctl[4])) break;
+        ngroups = (int)wave_broadcast0((uint32_t)ctl[6]);
+    }
+    const int nx = (int)wave_broadcast0((uint32_t)ctl[5]);
+    __syncthreads(); // ring, comb and ctl are reused by the next item
+    return nx;
+}
+
+template <int K, bool TE, bool QE, bool PAIRED, int W>
+__device__ __forceinline__ int sg_pipe_stream(const ScoreParams &p, const QueryView &qv,
+                                              const uint8_t *smem, const uint32_t *slut,
+                                              uint2 *ring, int *comb, int *ctl, int item0,
+                                              int nlong, int w, int lane,
+                                              const PairView *pv = nullptr) {
+    // every wave runs the same steps and barriers in either role
+    if (w == qv.nchunks - 1)
+        return sg_pipe_stream_role<K, TE, QE, PAIRED, W, true>(p, qv, smem, slut, ring, comb, ctl,
+                                                               item0, nlong, w, lane, pv);
+    return sg_pipe_stream_role<K, TE, QE, PAIRED, W, false>(p, qv, smem, slut, ring, comb, ctl,
+                                                            item0, nlong, w, lane, pv);
 }
 
 // Resident workgroups per CU of kernel k at `block` threads and `lds` bytes

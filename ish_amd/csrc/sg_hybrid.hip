@@ -22,7 +22,7 @@ size_t sg_hybrid_paired_lds(int chunk, int nrows, int waves) {
 // the launch.
 // ---------------------------------------------------------------------------
 template <int K, bool TE, bool QE, bool PAIRED, int W>
-__global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) sg_hybrid(ScoreParams p) {
     using V = s16x2;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_item;
@@ -38,6 +38,7 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
     uint8_t *tail = smem + ((head + 15) & ~(size_t)15);
     uint2 *ring = (uint2 *)tail;
     int *comb = (int *)(tail + kSgRingBytes);
+    int *ctl = (int *)(tail + kSgRingBytes + 2 * kSgCombBytes);
     const int lane = threadIdx.x & 63;
     const int w = (int)wave_broadcast0(threadIdx.x >> 6); // wave-uniform
     constexpr int waves = W; // the launch has 64 * W threads
@@ -59,19 +60,35 @@ __global__ void __launch_bounds__(256) sg_hybrid(ScoreParams p) {
     const int ng = p.g_end - p.g_begin;
     const int nlong = p.n_long < ng ? p.n_long : ng;
     const int items = nlong + (ng - nlong + waves - 1) / waves;
-    for (;;) {
+    auto fetch = [&]() {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(p.counter, 1u);
         __syncthreads();
-        const int item = (int)wave_broadcast0((uint32_t)s_item); // scalar from here on
+        const int it = (int)wave_broadcast0((uint32_t)s_item); // scalar from here on
         __syncthreads(); // s_item is rewritten by the next fetch
-        if (item >= items) break;
+        return it;
+    };
+    int item = fetch();
+    while (item < items) {
         if (item < nlong) {
+            // one round per group: stream the long groups through the
+            // pipeline; the item that ends the stream is scored next. (Over
+            // the int16 profile, or with the query-end rule's per-row keys,
+            // the streamed code would push the kernel past 2 waves per SIMD.)
+            if constexpr (PAIRED && !QE) {
+                // (an empty query has no chunk and no wave to run the stream)
+                if (qv.nchunks >= 1 && qv.nchunks <= W) {
+                    item = sg_pipe_stream<K, TE, QE, PAIRED, W>(p, qv, smem, slut, ring, comb, ctl,
+                                                               item, nlong, w, lane, &pv);
+                    continue;
+                }
+            }
             sg_pipe_group<K, TE, QE, PAIRED, W>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item,
                                                 w, lane, &pv);
         } else {
             const int g = p.g_begin + nlong + (item - nlong) * waves + w;
             if (g < p.g_end) sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, 0, lane, &pv);
         }
+        item = fetch();
     }
 }
 
@@ -90,9 +107,17 @@ static hipError_t launch_hybrid(const ScoreParams &p, int grid, size_t lds, hipS
     return hipGetLastError();
 }
 
+// The 40-row tile has four-wave variants only (two waves would spill
+// registers); the planner never asks for two.
 template <int K, bool PAIRED>
 static hipError_t launch_hybrid_w(const ScoreParams &p, int waves, int grid, size_t lds, hipStream_t st) {
-    return waves == 2 ? launch_hybrid<K, PAIRED, 2>(p, grid, lds, st) : launch_hybrid<K, PAIRED, 4>(p, grid, lds, st);
+    if constexpr (K == 40) {
+        if (waves != 4) return hipErrorInvalidValue;
+        return launch_hybrid<K, PAIRED, 4>(p, grid, lds, st);
+    } else {
+        return waves == 2 ? launch_hybrid<K, PAIRED, 2>(p, grid, lds, st)
+                          : launch_hybrid<K, PAIRED, 4>(p, grid, lds, st);
+    }
 }
 
 hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int waves, int grid,
@@ -115,7 +140,11 @@ template <int K, bool PAIRED, int W> static int occ_hybrid(size_t lds, uint32_t 
 }
 
 template <int K, bool PAIRED> static int occ_hybrid_w(int waves, size_t lds, uint32_t flags) {
-    return waves == 2 ? occ_hybrid<K, PAIRED, 2>(lds, flags) : occ_hybrid<K, PAIRED, 4>(lds, flags);
+    if constexpr (K == 40) {
+        return occ_hybrid<K, PAIRED, 4>(lds, flags);
+    } else {
+        return waves == 2 ? occ_hybrid<K, PAIRED, 2>(lds, flags) : occ_hybrid<K, PAIRED, 4>(lds, flags);
+    }
 }
 
 int occupancy_sg_hybrid_i16(int chunk, bool paired, int waves, size_t lds, uint32_t flags) {

@@ -149,8 +149,10 @@ def test_hybrid_long_groups_among_short(eng, flags):
 @pytest.mark.parametrize("tile", [16, 32, 40])
 def test_two_wave_pipelines(eng, tile):
     """Chunk counts below 8 that are 1 or 2 modulo 4 run pipelined groups
-    over two-wave workgroups, 9 and 10 over four (the workgroup schedule
-    pipelines every group); each must equal the oracle under every end-free
+    over two-wave workgroups (the 40-row tile over four), 9 and 10 over four
+    (the workgroup schedule pipelines every group); one or two chunks stream
+    the groups through the pipeline without draining (nucleotide codes and
+    no query-end rule). Each must equal the oracle under every end-free
     setting the kernel distinguishes."""
     rng = np.random.default_rng(31 + tile)
     seqs = [rng.integers(0, 5, int(rng.integers(1, 700))).astype(np.uint8) for _ in range(700)]
