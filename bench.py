@@ -38,25 +38,38 @@ PEAK_INT32_TOPS = 39.3  # 256 CU x 2.4 GHz x 64 int32 lane-ops/clk
 # tracking needs: 9.
 OPS_PER_CELL_SG = 6
 OPS_PER_CELL_SW = 9
+# Semi-global in binary16 lanes: the packed three-input maximum takes
+# H = max(Hdiag + S', E, F) in one op, so the recurrence is 5 ops. A pass
+# runs most groups in binary16 and the rest in int16; its count is the
+# cell-weighted mix of 5 and 6.
+OPS_PER_CELL_SG_F16 = 5
 # The textbook Gotoh count (1 add + 2 max for H; 2 sub + 1 max for E; 1 sub
 # + 1 max for F; +1 max with 0 for local), reported as frac_textbook.
 OPS_TEXTBOOK_SG = 8
 OPS_TEXTBOOK_SW = 9
 
 
-def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, **extra):
-    """The integer-VALU roofline of a pass: achieved = cells x ops per cell
-    of the kernel's recurrence / kernel time, against the packed-int16
-    issue peak; valu_busy (from the rocprof counters of the same workload)
-    is the share of the dispatch the VALU issued."""
-    ops = OPS_PER_CELL_SW if local else OPS_PER_CELL_SG
+def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, f16_share=0.0, **extra):
+    """The packed-VALU roofline of a pass: achieved = cells x ops per cell
+    of the kernel's recurrence / kernel time, against the packed 16-bit
+    issue peak (int16 and binary16 issue at the same rate); f16_share is
+    the share of cells that ran in binary16 lanes (5 ops instead of 6);
+    valu_busy (from the rocprof counters of the same workload) is the share
+    of the dispatch the VALU issued."""
+    if local:
+        ops = OPS_PER_CELL_SW
+    else:
+        ops = f16_share * OPS_PER_CELL_SG_F16 + (1.0 - f16_share) * OPS_PER_CELL_SG
     tb = OPS_TEXTBOOK_SW if local else OPS_TEXTBOOK_SG
     achieved = cells * ops / kernel_s / 1e12
     out = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
            "frac": round(achieved / PEAK_INT16_TOPS, 4),
            "frac_textbook": round(cells * tb / kernel_s / 1e12 / PEAK_INT16_TOPS, 4),
-           "ops_per_cell": ops, "ops_per_cell_textbook": tb, "traffic": traffic,
+           "ops_per_cell": round(ops, 3), "ops_per_cell_textbook": tb, "traffic": traffic,
            "valu_busy": valu_busy, "kernel_ms": round(kernel_s * 1e3, 3)}
+    if not local:
+        out["frac_int16_form"] = round(cells * OPS_PER_CELL_SG / kernel_s / 1e12 / PEAK_INT16_TOPS, 4)
+        out["binary16_share"] = round(f16_share, 4)
     out.update(extra)
     return out
 
@@ -128,20 +141,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_counters(content_hash, tile, schedule):
+def measured_counters(content_hash, tile, schedule, lanes="int16"):
     """(HBM bytes per dispatch, VALU busy share) of the dominant kernel, as
     rocprofv3's counters measured them for this exact workload, tile and
     schedule (profiles/traffic.json: FETCH_SIZE / WRITE_SIZE scaled by the
     copy-kernel calibration, SQ_INSTS_VALU x 4 cycles over 1,024 SIMDs at
     2.4 GHz against the dispatch time); (None, None) when that combination
-    was not profiled."""
+    was not profiled. lanes: "f16" when most cells ran in binary16 lanes
+    (another kernel variant), else "int16"."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         entries = json.load(open(path)).get("entries", [])
     except (OSError, ValueError):
         return None, None
     for e in entries:
-        if (e.get("content_hash"), e.get("tile_rows"), e.get("schedule")) == (content_hash, tile, schedule):
+        if (e.get("content_hash"), e.get("tile_rows"), e.get("schedule"), e.get("lanes", "int16")) == \
+                (content_hash, tile, schedule, lanes):
             return e.get("bytes"), e.get("valu_busy")
     return None, None
 
@@ -444,6 +459,8 @@ def main():
                     help="targets of the serving database (0 = 10M)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving phases of the default workload")
+    ap.add_argument("--no-f16", action="store_true",
+                    help="keep every group in int16 lanes (no binary16 lanes)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -474,6 +491,7 @@ def main():
 
     eng = ish_amd.Engine(0)
     eng.set_engine("batched")
+    eng.set_f16(not args.no_f16)
     eng.set_query(w.query, mkind, 3, 1, flags, algo)
     t0 = time.perf_counter()
     db = eng.load_db(w.targets)
@@ -513,8 +531,10 @@ def main():
     kern_s = float(np.mean(kernel_ms)) / 1e3
     gcups_kernel = cells / kern_s / 1e9
     widened = int((st & ish_amd.STATUS_WIDENED != 0).sum())
+    lens = np.diff(w.targets.offsets).astype(np.float64)
+    f16_share = float(lens[(st & ish_amd.STATUS_F16) != 0].sum() / max(lens.sum(), 1.0))
     log(f"batched: {step_s * 1e3:.2f} ms/step, {gcups:.1f} GCUPS (kernel {gcups_kernel:.1f}), "
-        f"widened targets {widened}")
+        f"widened targets {widened}, binary16 share of cells {f16_share:.4f}")
 
     eng.set_graphs(False)
     # end-to-end: one synchronous batch call from host buffers (upload, sort
@@ -589,7 +609,8 @@ def main():
         # the 2M above
         serving, _, _ = serving_block(args, ish_amd, workloads, False)
 
-    traffic, valu_busy = measured_counters(w.content_hash(), best_tile, schedule)
+    traffic, valu_busy = measured_counters(w.content_hash(), best_tile, schedule,
+                                           "f16" if f16_share >= 0.5 else "int16")
     out = {
         "metric": METRIC,
         "value": round(gcups, 2),
@@ -601,12 +622,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int16",
+        "dtype": "f16" if f16_share >= 0.5 else "int16",
         "data": "synthetic",
         "config": {"workload": desc, "name": args.workload,
                    "targets": w.targets.n, "residues": int(w.targets.offsets[-1]), "query_len": Q,
                    "cells": cells, "seed": args.seed, "content_hash": w.content_hash(),
-                   "engine": "batched persistent, packed int16 (int32 for wide groups), HIP graph replay",
+                   "engine": "batched persistent, packed 16-bit lanes: binary16 (exact integers within the "
+                             "query's range bound) or int16; int32 for wide groups; HIP graph replay",
+                   "binary16_share_of_cells": round(f16_share, 4),
                    "tile_rows": best_tile or None,
                    "schedule": schedule,
                    "tuning": {"tile": tuning[0], "schedule": tuning[1]} if tuning else None},
@@ -620,7 +643,7 @@ def main():
         "p50_ms": serving["at_70pct_capacity"]["p50_ms"] if serving else None,
         "p99_ms": serving["at_70pct_capacity"]["p99_ms"] if serving else None,
         "parity_sample": parity,
-        "roofline": roofline(cells, kern_s, local, traffic, valu_busy),
+        "roofline": roofline(cells, kern_s, local, traffic, valu_busy, f16_share),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -76,9 +76,11 @@ extern "C" {
                                 order, one target per lane */
 
 /* per-target status byte: cell width used, plus a flag when the target was
- * routed to int32 because its proven score range exceeds int16. */
+ * routed to int32 because its proven score range exceeds int16, and a flag
+ * when its 16-bit cells were binary16 halves (ish_hip_set_f16). */
 #define ISH_STATUS_I16 16u
 #define ISH_STATUS_I32 32u
+#define ISH_STATUS_F16 0x40u
 #define ISH_STATUS_WIDENED 0x80u
 
 typedef struct ish_hip_ctx ish_hip_ctx;
@@ -146,6 +148,15 @@ int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane);
 #define ISH_SCHEDULE_WAVE 1
 #define ISH_SCHEDULE_WORKGROUP 2
 int32_t ish_hip_set_schedule(ish_hip_ctx *ctx, int32_t schedule);
+
+/* Binary16 lanes (default on). A semi-global pass over the shared paired
+ * profile runs the groups whose every value the query's range bound keeps
+ * within [-2040, 2040] in binary16 halves, where integer arithmetic is
+ * exact and one packed three-input maximum takes H = max(Hdiag+S, E, F):
+ * five operations per cell pair instead of six. Other groups stay in
+ * int16. 0 keeps every group in int16. A tuning knob only; results never
+ * depend on it. No counterpart in the reference. */
+int32_t ish_hip_set_f16(ish_hip_ctx *ctx, int32_t enable);
 
 /* Autotuner: times whole passes over db for the context's current query
  * with every launch configuration (tile height 32, 16 or 40; schedule WAVE,

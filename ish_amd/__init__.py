@@ -24,14 +24,14 @@ ALGO = {"striped-semi-global": 0, "semi-global": 0, "striped-local": 1, "local":
 ENGINE = {"batched": 0, "plain": 1}
 SCHEDULE = {"auto": 0, "wave": 1, "workgroup": 2}
 FREE_QUERY_START, FREE_QUERY_END, FREE_TARGET_START, FREE_TARGET_END = 1, 2, 4, 8
-STATUS_I16, STATUS_I32, STATUS_WIDENED = 16, 32, 0x80
+STATUS_I16, STATUS_I32, STATUS_F16, STATUS_WIDENED = 16, 32, 0x40, 0x80
 
 # every symbol the public header declares
 ABI_SYMBOLS = [
     "ish_hip_version", "ish_hip_device_count", "ish_hip_create", "ish_hip_destroy",
     "ish_hip_last_error", "ish_hip_matrix", "ish_hip_encoding", "ish_hip_decoding",
     "ish_hip_encode_and_score", "ish_hip_set_query", "ish_hip_set_engine",
-    "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_autotune", "ish_hip_tuning",
+    "ish_hip_set_tile", "ish_hip_set_schedule", "ish_hip_set_f16", "ish_hip_autotune", "ish_hip_tuning",
     "ish_hip_set_graphs", "ish_hip_align_batch", "ish_hip_align_batch_starts",
     "ish_hip_align_batch_submit", "ish_hip_align_batch_wait", "ish_hip_set_slice_bytes", "ish_hip_db_load", "ish_hip_db_load_ascii", "ish_hip_db_free", "ish_hip_db_size",
     "ish_hip_align_db", "ish_hip_align_db_noalloc", "ish_hip_align_db_async", "ish_hip_sync", "ish_hip_db_fetch",
@@ -120,6 +120,7 @@ def lib():
         "ish_hip_set_engine": ([vp, i32], i32),
         "ish_hip_set_tile": ([vp, i32], i32),
         "ish_hip_set_schedule": ([vp, i32], i32),
+        "ish_hip_set_f16": ([vp, i32], i32),
         "ish_hip_autotune": ([vp, vp, i32], i32),
         "ish_hip_tuning": ([vp, vp, i32p, i32p], i32),
         "ish_hip_set_graphs": ([vp, i32], i32),
@@ -323,6 +324,11 @@ class Engine:
 
     def set_schedule(self, schedule: str):
         self._check(self._L.ish_hip_set_schedule(self._h, SCHEDULE[schedule]))
+
+    def set_f16(self, on: bool):
+        """Binary16 lanes for the groups within the query's range bound
+        (default on); off keeps every group in int16. Results are the same."""
+        self._check(self._L.ish_hip_set_f16(self._h, int(bool(on))))
 
     def set_graphs(self, on: bool):
         self._check(self._L.ish_hip_set_graphs(self._h, int(bool(on))))

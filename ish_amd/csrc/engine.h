@@ -59,6 +59,7 @@ struct ScoreParams {
     int32_t go, ge;     // gap open / extend (non-negative)
     uint32_t flags;     // free ends: qs|qe<<1|ts<<2|te<<3
     int32_t neg_limit;  // int16 floor that seeds the last-row maximum
+    int32_t fshift;     // binary16 lanes: every value is held minus fshift
 
     // device-resident targets
     const GroupDesc *groups;
@@ -155,8 +156,9 @@ int occupancy_sg_hybrid_i16(int chunk, bool paired, int waves, size_t lds, uint3
 // and K = 32 the last pair row starts at (21 * 21 - 1) * 144 = 63,360 bytes.
 constexpr int kPairMaxShared = 21;
 size_t sg_hybrid_lock_lds(int chunk, int nrows, size_t profile_lds);
-hipError_t launch_sg_hybrid_lock(const ScoreParams &p, int chunk, int grid, size_t lds, hipStream_t st);
-int occupancy_sg_hybrid_lock(int chunk, size_t lds, uint32_t flags);
+hipError_t launch_sg_hybrid_lock(const ScoreParams &p, int chunk, bool f16, int grid, size_t lds,
+                                 hipStream_t st);
+int occupancy_sg_hybrid_lock(int chunk, bool f16, size_t lds, uint32_t flags);
 // Workgroup-pipelined local kernel (packed int16, 4 waves share a group):
 // extra LDS beyond the profile, launcher (block of 256) and occupancy.
 // `paired` selects the variant that reads the paired profile (p.pprof,

@@ -53,6 +53,12 @@ struct ish_hip_ctx {
     uint32_t flags = 0;
     int32_t neg_limit = 0, min_score = 0, max_score = 0;
     int64_t r16max = -1; // longest target routed to the int16 kernel
+    // binary16 lanes (semi-global lockstep tiles): groups of at most w16
+    // columns may run in them, every value held minus fshift; f16 = 0
+    // keeps every group in int16
+    int64_t w16 = -1;
+    int32_t fshift = 0;
+    bool f16 = true;
     int32_t prof_words = 0;
     size_t lds = 0;
     DevBuf prof_t, prof_rt, prof_p, lut; // forward, reversed-query, plain profiles
@@ -350,6 +356,12 @@ extern "C" int32_t ish_hip_set_schedule(ish_hip_ctx *ctx, int32_t schedule) {
     return ISH_OK;
 }
 
+extern "C" int32_t ish_hip_set_f16(ish_hip_ctx *ctx, int32_t enable) {
+    if (!ctx) return ISH_E_INVALID;
+    ctx->f16 = enable != 0;
+    return ISH_OK;
+}
+
 extern "C" int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine) {
     if (!ctx) return ISH_E_INVALID;
     if (engine != ISH_ENGINE_BATCHED && engine != ISH_ENGINE_PLAIN)
@@ -383,6 +395,37 @@ int64_t ish::int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int
     if (ge > 0) best = std::min(best, lo / ge - Q);
     if ((int64_t)Q > 65534) return -1;
     return best < 0 ? -1 : best;
+}
+
+// Columns a group may have for the semi-global offset form to run in
+// binary16 lanes (-1: none), and the shift C of the values they hold.
+// Every value such a pass produces is an integer in [L, U]:
+//   H(j,i) <= P_j = sum over rows k <= j of max(0, max_r S(r, k)), since an
+//     alignment scores at most its best match per query row and boundary
+//     and gap terms are <= 0; E, F and Hdiag + S are bounded the same way;
+//     with the offset (i+j)*ge and i <= cols-1, j <= Q-1:
+//     U = P_{Q-1} + (cols + Q - 2)*ge (the top padding rows hold (i-1)*ge);
+//   E^ starts at the left boundary plus ge-go >= -2go and only rises, H^ is
+//     at least E^, F^ starts at >= -2go, G = H^ + ge - go >= -3go + ge, and
+//     Hdiag^ + S' >= -2go + min S': L = -3go - 2ge - m - 2, m = max(0, -min S).
+// Held minus C = L + 2040, they lie in [-2040, 2040] when U <= L + 4080,
+// where binary16 arithmetic on integers is exact.
+// rowmax: max over the profile rows of each query row's score S.
+static int64_t f16_max_cols(const std::vector<int32_t> &rowmax, int32_t go, int32_t ge,
+                            int32_t min_s, int32_t *shift) {
+    const int64_t Q = (int64_t)rowmax.size();
+    if (go < ge || Q == 0) return -1; // no top padding rows (engine.h, top_pad)
+    int64_t P = 0;
+    for (int32_t v : rowmax) P += std::max(v, 0);
+    const int64_t m = std::max(-min_s, 0);
+    const int64_t L = -3LL * go - 2LL * ge - m - 2;
+    if (L < -2040) return -1;
+    *shift = (int32_t)(L + 2040);
+    const int64_t room = L + 4080 - P; // for (cols + Q - 2) * ge
+    if (room < 0) return -1;
+    if (ge == 0) return INT64_MAX;
+    const int64_t cols = room / ge - Q + 2;
+    return cols < 4 ? -1 : cols;
 }
 
 // Local alignment in packed int16: with the state shifted down every 256
@@ -591,6 +634,18 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     ctx->lds = lds;
     ctx->r16max = algo == ISH_ALGO_LOCAL ? (local_fits_int16(pf.qpad, go, ge, mx, mn) ? INT64_MAX : -1)
                                          : int16_max_len(Q, go, ge, mx, mn);
+    ctx->w16 = -1;
+    ctx->fshift = 0;
+    if (algo != ISH_ALGO_LOCAL && pf.top) {
+        // each query row's best score over every profile row a residue
+        // code can select (entries are S + 2ge)
+        std::vector<int32_t> rowmax((size_t)Q, INT32_MIN);
+        for (int32_t r = 0; r < pf.nrows; ++r)
+            for (int32_t j = 0; j < Q; ++j)
+                rowmax[(size_t)j] = std::max<int32_t>(rowmax[(size_t)j],
+                                                      pf.fwd[(size_t)r * pf.stride + pf.front + j] - 2 * ge);
+        ctx->w16 = f16_max_cols(rowmax, go, ge, mn, &ctx->fshift);
+    }
     ctx->hprof = std::move(pf);
     ctx->query_gen++;
     ctx->have_query = true;
@@ -1100,6 +1155,7 @@ struct DbLaunch {
     int32_t n_long; // kSgHybrid / kSgLock: leading groups scored by whole workgroups
     int32_t paired; // kSwPipe / kSgHybrid: the paired-profile variant (kSgLock: always)
     int32_t waves;  // waves per workgroup (kSgHybrid: 2 or 4; the others: 4)
+    int32_t f16;    // kSgLock: the tiles run in binary16 lanes
     int64_t slot_stride;
     uint64_t lds;
     ScoreParams p;
@@ -1296,6 +1352,17 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
             // variants would spill registers)
             if (chunk == 40) L.waves = 4;
         }
+        if (L.kind == kSgLock && ctx->f16 && ctx->w16 >= 4) {
+            // binary16 tiles: the groups wider than the range bound join the
+            // pipelined long groups (a prefix, groups being sorted by
+            // length), if they are few
+            int32_t n = 0;
+            while (g0 + n < g1 && (((int64_t)db->group_maxlen[g0 + n] + 3) & ~3LL) > ctx->w16) ++n;
+            if (n <= std::max<int32_t>(n_long, (g1 - g0) / 32)) {
+                L.f16 = 1;
+                n_long = std::max(n_long, n);
+            }
+        }
         L.chunk = chunk;
         L.g_begin = g0;
         L.g_end = g1;
@@ -1339,7 +1406,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
         case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
         case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.waves, L.lds, flags); break;
-        case kSgLock: occ = occupancy_sg_hybrid_lock(L.chunk, L.lds, flags); break;
+        case kSgLock: occ = occupancy_sg_hybrid_lock(L.chunk, L.f16 != 0, L.lds, flags); break;
         default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.lds); break;
         }
         int grid;
@@ -1379,6 +1446,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         p.ge = ctx->ge;
         p.flags = flags;
         p.neg_limit = ctx->neg_limit;
+        p.fshift = ctx->fshift;
         p.groups = db->d_groups;
         p.res = db->d_res;
         p.lens = db->d_lens;
@@ -1413,7 +1481,7 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
     case kSgHybrid:
         e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.waves, L.grid, L.lds, ctx->stream);
         break;
-    case kSgLock: e = launch_sg_hybrid_lock(L.p, L.chunk, L.grid, L.lds, ctx->stream); break;
+    case kSgLock: e = launch_sg_hybrid_lock(L.p, L.chunk, L.f16 != 0, L.grid, L.lds, ctx->stream); break;
     default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
     }
     if (e == hipSuccess && timed) e = timed_end(ctx);

@@ -8,11 +8,26 @@ namespace ish {
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+// Two cells in binary16 halves. Integers of magnitude <= 2048 are exact in
+// binary16 and so are sums and maxima whose result stays in that range, so
+// a recurrence whose values are proven to stay inside it computes exactly
+// what the int16 form computes. gfx950's packed three-input maximum
+// (v_pk_maximum3_f16) then takes H = max(Hdiag + S, E, F) in one operation.
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) {
     return __builtin_elementwise_max(a, b);
 }
 __device__ __forceinline__ int vmax(int a, int b) { return a > b ? a : b; }
+// IEEE maximum (the values are never NaN); nested, it selects the
+// three-input form.
+__device__ __forceinline__ h16x2 vmax(h16x2 a, h16x2 b) {
+    return __builtin_elementwise_maximum(a, b);
+}
+__device__ __forceinline__ h16x2 vmax3(h16x2 a, h16x2 b, h16x2 c) {
+    return __builtin_elementwise_maximum(__builtin_elementwise_maximum(a, b), c);
+}
 
 template <typename V> __device__ __forceinline__ V splat(int v);
 template <> __device__ __forceinline__ s16x2 splat<s16x2>(int v) {
@@ -22,14 +37,45 @@ template <> __device__ __forceinline__ s16x2 splat<s16x2>(int v) {
     return r;
 }
 template <> __device__ __forceinline__ int splat<int>(int v) { return v; }
+template <> __device__ __forceinline__ h16x2 splat<h16x2>(int v) {
+    const _Float16 h = (_Float16)v; // exact: |v| <= 2048 wherever it is used
+    h16x2 r;
+    r.x = h;
+    r.y = h;
+    return r;
+}
 
 __device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ uint32_t as_u32(int v) { return (uint32_t)v; }
+__device__ __forceinline__ uint32_t as_u32(h16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 template <typename V> __device__ __forceinline__ V from_u32(uint32_t v);
 template <> __device__ __forceinline__ s16x2 from_u32<s16x2>(uint32_t v) {
     return __builtin_bit_cast(s16x2, v);
 }
 template <> __device__ __forceinline__ int from_u32<int>(uint32_t v) { return (int)v; }
+template <> __device__ __forceinline__ h16x2 from_u32<h16x2>(uint32_t v) {
+    return __builtin_bit_cast(h16x2, v);
+}
+
+// The int16 pair of a cell pair: binary16 halves hold value - shift (exact
+// integers), the int16 form the value itself.
+__device__ __forceinline__ s16x2 to_s16x2(s16x2 v, int) { return v; }
+__device__ __forceinline__ s16x2 to_s16x2(h16x2 v, int shift) {
+    s16x2 r;
+    r.x = (short)((int)v.x + shift);
+    r.y = (short)((int)v.y + shift);
+    return r;
+}
+
+// A word of two int16 profile scores as the binary16 pair of the same
+// integers (|score| <= 2048).
+__device__ __forceinline__ uint32_t i16x2_to_h16x2(uint32_t w) {
+    const s16x2 s = from_u32<s16x2>(w);
+    h16x2 h;
+    h.x = (_Float16)s.x;
+    h.y = (_Float16)s.y;
+    return as_u32(h);
+}
 
 __device__ __forceinline__ uint32_t wave_broadcast0(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
@@ -82,7 +128,11 @@ template <typename V> __device__ __forceinline__ void pin(V &v) {
 template <typename V, bool PAIRED, bool PACKED>
 __device__ __forceinline__ V slice_val(const short8 &sa, const short8 &sb, int mm) {
     V s;
-    if constexpr (PAIRED) {
+    if constexpr (__is_same(V, h16x2)) {
+        // PAIRED only: the slice words hold binary16 pairs
+        const short8 &src = mm < 4 ? sa : sb;
+        s = from_u32<V>(__builtin_bit_cast(u32x4, src)[mm & 3]);
+    } else if constexpr (PAIRED) {
         const short8 &src = mm < 4 ? sa : sb;
         s.x = src[2 * (mm & 3)];
         s.y = src[2 * (mm & 3) + 1];

@@ -40,6 +40,8 @@
 #include <limits.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "engine.h"
 #include "kernel_common.h"
 
@@ -114,6 +116,7 @@ struct QueryView {
     int pad, front;  // top padding rows, profile entries before row 0
     uint32_t flags;  // free ends of this query (qs|qe<<1|ts<<2|te<<3)
     int64_t out_off; // offset of this query's results in the output arrays
+    int fshift;      // binary16 lanes: the shift of every value (read by those only)
 };
 
 // State of one lane while it scores one query chunk (K rows) of its
@@ -135,17 +138,19 @@ __device__ __forceinline__ int left_boundary(int j, bool qs, bool ts, int go, in
     return ts ? -2 * ge : -go - ge;
 }
 
+// shift: binary16 lanes hold every value minus shift (0 for the integer
+// lanes).
 template <typename V, int K>
 __device__ __forceinline__ void sg_chunk_init(SgChunk<V, K> &st, int j0, bool qs, bool ts, int go,
-                                              int ge) {
+                                              int ge, int shift = 0) {
     const int cst = ge - go;
 #pragma unroll
     for (int m = 0; m < K; ++m) {
-        const int hl = left_boundary(j0 + m, qs, ts, go, ge); // H^(j,-1)
+        const int hl = left_boundary(j0 + m, qs, ts, go, ge) - shift; // H^(j,-1)
         st.H[m] = splat<V>(hl);
         st.E[m] = splat<V>(hl + cst); // E^(j,0)
     }
-    st.diag_top = splat<V>(left_boundary(j0 - 1, qs, ts, go, ge));
+    st.diag_top = splat<V>(left_boundary(j0 - 1, qs, ts, go, ge) - shift);
 }
 
 __device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
@@ -182,8 +187,10 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                                          const uint8_t *chunk_base, const uint32_t *slut,
                                          int half, int RA, int RB, int rmin, bool ts, bool qe_rt, int go,
                                          int ge, TrackI16 &t16, TrackI32 &t32, uint2 *out,
-                                         int lane) {
+                                         int lane, int shift = 0) {
     constexpr bool PACKED = sizeof(V) == 4 && !__is_same(V, int);
+    constexpr bool F16 = __is_same(V, h16x2);
+    static_assert(!F16 || PAIRED, "binary16 lanes read the paired profile");
     constexpr bool FIXED_LAST = PACKED;
     const int cst = ge - go;
     const V vcst = splat<V>(cst);
@@ -208,7 +215,7 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
         }
         V htop, f;
         if (first) {
-            const int ht = ts ? (i - 1) * ge : -go - ge; // H^(-1,i)
+            const int ht = (ts ? (i - 1) * ge : -go - ge) - shift; // H^(-1,i)
             htop = splat<V>(ht);
             f = splat<V>(ht + cst); // F^(0,i)
         } else {
@@ -221,6 +228,38 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
         // packed operation reads the result of the one just before it,
         // which would cost a wait state. The scheduling barriers keep this
         // order.
+        if constexpr (F16) {
+            // Binary16 lanes: H = max3(H^diag + S', E^, F^) is one
+            // operation, so a row is five: t1 = H^diag + S' (two rows
+            // ahead), h = max3(t1, E, f), gg = h + c, f = max(f, gg),
+            // E = max(E, gg). The F chain (max3, add, max) alternates with
+            // t1 of row m+2 and the E update of row m.
+            V t1[K];
+            t1[0] = st.diag_top + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 0);
+            t1[1] = st.H[0] + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 1);
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const V h = vmax3(t1[m], st.E[m], f);
+                __builtin_amdgcn_sched_barrier(0);
+                if (m + 2 < K)
+                    t1[m + 2] = st.H[m + 1] + slice_val<V, PAIRED, PACKED>(sa[(m + 2) / 8], sb[(m + 2) / 8],
+                                                                           (m + 2) % 8);
+                __builtin_amdgcn_sched_barrier(0);
+                const V gg = h + vcst;
+                __builtin_amdgcn_sched_barrier(0);
+                f = vmax(f, gg);
+                __builtin_amdgcn_sched_barrier(0);
+                st.E[m] = vmax(st.E[m], gg);
+                st.H[m] = h;
+                __builtin_amdgcn_sched_barrier(0);
+                if (m % 8 == 7) {
+                    const int k = m / 8;
+                    sa[k] = na[2 * k];
+                    sb[k] = na[2 * k + 1];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        } else {
         V t2[K]; // max(H^diag + S', E^) of the rows ahead
         t2[0] = vmax(st.diag_top + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 0), st.E[0]);
         t2[1] = vmax(st.H[0] + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 1), st.E[1]);
@@ -259,6 +298,7 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+        }
         st.diag_top = htop;
         if (!last) {
             bottom[t].x = as_u32(st.H[K - 1]);
@@ -276,7 +316,9 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
             const V u = hl - splat<V>(i * ge);
             if constexpr (PACKED) {
-                const uint32_t uu = as_u32(u);
+                // binary16 lanes: back to int16 before the subtraction
+                // (i*ge alone may leave the exact range)
+                const uint32_t uu = F16 ? as_u32(to_s16x2(hl, shift) - splat<s16x2>(i * ge)) : as_u32(u);
                 if (TE || QE) {
                     int ka = key_lo(uu, i), kb = key_hi(uu, i);
                     if (i >= rmin) { // past some lane's target end
@@ -314,7 +356,8 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                     // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
                     const V wv = st.H[m] - splat<V>(j * ge);
                     if constexpr (PACKED) {
-                        const uint32_t ww = as_u32(wv);
+                        const uint32_t ww =
+                            F16 ? as_u32(to_s16x2(st.H[m], shift) - splat<s16x2>(j * ge)) : as_u32(wv);
                         if (cap_a) t16.qe_a = max(t16.qe_a, key_lo(ww, j));
                         if (cap_b) t16.qe_b = max(t16.qe_b, key_hi(ww, j));
                     } else {
@@ -364,7 +407,7 @@ __device__ __forceinline__ void sg_finish(const ScoreParams &p, const QueryView 
             p.score[qv.out_off + o] = sc;
             p.end_ref[qv.out_off + o] = er;
             p.end_q[qv.out_off + o] = eq;
-            p.status[qv.out_off + o] = 16;
+            p.status[qv.out_off + o] = __is_same(V, h16x2) ? 16 | 0x40 : 16;
         }
     } else {
         const int o = p.orig[tA];
@@ -403,8 +446,9 @@ __device__ __forceinline__ void sg_chunk_v(const ScoreParams &p, const QueryView
     const bool last = !MID && c == qv.nchunks - 1;
     const int mstar = Q - 1 - j0; // row Q-1 inside the last chunk
     const int rmin = wave_min(PACKED ? min(RA, RB) : RA);
+    const int shift = __is_same(V, h16x2) ? qv.fshift : 0;
     SgChunk<V, K> st;
-    sg_chunk_init(st, j0, qs, ts, go, ge);
+    sg_chunk_init(st, j0, qs, ts, go, ge, shift);
     uint2 rr_next = res[lane];
     uint2 bn_next[4];
 #pragma unroll
@@ -448,7 +492,7 @@ __device__ __forceinline__ void sg_chunk_v(const ScoreParams &p, const QueryView
         sg_block<V, K, TE, QE, PAIRED, true>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa,
                                              sb, chunk_base, slut, half, RA, RB, rmin, ts, qe_rt, go, ge,
                                              t16, t32, last ? nullptr : bnd + (size_t)b * 4 * 64,
-                                             lane);
+                                             lane, shift);
     }
 }
 
@@ -512,12 +556,12 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
 // whose group is past the range only joins the barriers and the copies.
 // pprof / npairs / pqpad: the query's paired profile; valid: the wave has
 // a group (wave-uniform).
-template <int K, bool TE, bool QE>
+template <int K, bool TE, bool QE, bool F16 = false>
 __device__ __forceinline__ void sg_tile_lockstep(const ScoreParams &p, const QueryView &qv,
                                                  const uint32_t *ptab, uint8_t *pbuf,
                                                  const int32_t *pprof, int npairs, int pqpad,
                                                  uint2 *bnd, int g, bool valid, int lane) {
-    using V = s16x2;
+    using V = typename std::conditional<F16, h16x2, s16x2>::type;
     const bool te_rt = TE && (qv.flags & 8u), qe_rt = QE && (qv.flags & 2u);
     GroupDesc gd;
     gd.res_off = 0;
@@ -540,7 +584,13 @@ __device__ __forceinline__ void sg_tile_lockstep(const ScoreParams &p, const Que
         __syncthreads(); // every wave is done with the previous chunk's rows
         for (int x = threadIdx.x; x < npairs * parts; x += blockDim.x) {
             const int pr = x / parts, part = x % parts;
-            const uint4 v = *(const uint4 *)(pprof + (size_t)pr * pqpad + e0 + part * 4);
+            uint4 v = *(const uint4 *)(pprof + (size_t)pr * pqpad + e0 + part * 4);
+            if constexpr (F16) {
+                v.x = i16x2_to_h16x2(v.x);
+                v.y = i16x2_to_h16x2(v.y);
+                v.z = i16x2_to_h16x2(v.z);
+                v.w = i16x2_to_h16x2(v.w);
+            }
             *(uint4 *)(pbuf + pr * pair_row_bytes(K) + part * 16) = v;
         }
         __syncthreads();

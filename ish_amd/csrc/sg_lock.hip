@@ -21,7 +21,9 @@ size_t sg_hybrid_lock_lds(int chunk, int nrows, size_t profile_lds) {
     return kPairTableLds + (pairs > prof ? pairs : prof);
 }
 
-template <int K, bool TE, bool QE>
+// F16: the tiles run in binary16 lanes (every group past the first n_long
+// is within the query's binary16 range bound; the long groups keep int16).
+template <int K, bool TE, bool QE, bool F16>
 __global__ void __launch_bounds__(256) sg_hybrid_lock(ScoreParams p) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_item;
@@ -45,6 +47,7 @@ __global__ void __launch_bounds__(256) sg_hybrid_lock(ScoreParams p) {
     qv.front = p.front;
     qv.flags = p.flags;
     qv.out_off = 0;
+    qv.fshift = p.fshift;
     const int ng = p.g_end - p.g_begin;
     const int nlong = p.n_long < ng ? p.n_long : ng;
     const int items = nlong + (ng - nlong + waves - 1) / waves;
@@ -65,40 +68,43 @@ __global__ void __launch_bounds__(256) sg_hybrid_lock(ScoreParams p) {
         } else {
             have_profile = false; // the tile's pair rows overwrite it
             const int g = p.g_begin + nlong + (item - nlong) * waves + w;
-            sg_tile_lockstep<K, TE, QE>(p, qv, ptab, area, p.pprof, p.pnrows * p.pnrows, p.pqpad, bnd,
-                                        g, g < p.g_end, lane);
+            sg_tile_lockstep<K, TE, QE, F16>(p, qv, ptab, area, p.pprof, p.pnrows * p.pnrows, p.pqpad,
+                                             bnd, g, g < p.g_end, lane);
         }
     }
 }
 
-template <int K>
+template <int K, bool F16>
 static hipError_t launch_lock(const ScoreParams &p, int grid, size_t lds, hipStream_t st) {
     const bool te = p.flags & 8u, qe = p.flags & 2u;
     if (te && qe)
-        hipLaunchKernelGGL((sg_hybrid_lock<K, true, true>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid_lock<K, true, true, F16>), dim3(grid), dim3(256), lds, st, p);
     else if (te)
-        hipLaunchKernelGGL((sg_hybrid_lock<K, true, false>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid_lock<K, true, false, F16>), dim3(grid), dim3(256), lds, st, p);
     else if (qe)
-        hipLaunchKernelGGL((sg_hybrid_lock<K, false, true>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid_lock<K, false, true, F16>), dim3(grid), dim3(256), lds, st, p);
     else
-        hipLaunchKernelGGL((sg_hybrid_lock<K, false, false>), dim3(grid), dim3(256), lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid_lock<K, false, false, F16>), dim3(grid), dim3(256), lds, st, p);
     return hipGetLastError();
 }
 
-hipError_t launch_sg_hybrid_lock(const ScoreParams &p, int chunk, int grid, size_t lds, hipStream_t st) {
-    return chunk == 16 ? launch_lock<16>(p, grid, lds, st) : launch_lock<32>(p, grid, lds, st);
+hipError_t launch_sg_hybrid_lock(const ScoreParams &p, int chunk, bool f16, int grid, size_t lds,
+                                 hipStream_t st) {
+    if (f16) return chunk == 16 ? launch_lock<16, true>(p, grid, lds, st) : launch_lock<32, true>(p, grid, lds, st);
+    return chunk == 16 ? launch_lock<16, false>(p, grid, lds, st) : launch_lock<32, false>(p, grid, lds, st);
 }
 
-template <int K> static int occ_lock(size_t lds, uint32_t flags) {
+template <int K, bool F16> static int occ_lock(size_t lds, uint32_t flags) {
     const bool te = flags & 8u, qe = flags & 2u;
-    if (te && qe) return occ(sg_hybrid_lock<K, true, true>, 256, lds);
-    if (te) return occ(sg_hybrid_lock<K, true, false>, 256, lds);
-    if (qe) return occ(sg_hybrid_lock<K, false, true>, 256, lds);
-    return occ(sg_hybrid_lock<K, false, false>, 256, lds);
+    if (te && qe) return occ(sg_hybrid_lock<K, true, true, F16>, 256, lds);
+    if (te) return occ(sg_hybrid_lock<K, true, false, F16>, 256, lds);
+    if (qe) return occ(sg_hybrid_lock<K, false, true, F16>, 256, lds);
+    return occ(sg_hybrid_lock<K, false, false, F16>, 256, lds);
 }
 
-int occupancy_sg_hybrid_lock(int chunk, size_t lds, uint32_t flags) {
-    return chunk == 16 ? occ_lock<16>(lds, flags) : occ_lock<32>(lds, flags);
+int occupancy_sg_hybrid_lock(int chunk, bool f16, size_t lds, uint32_t flags) {
+    if (f16) return chunk == 16 ? occ_lock<16, true>(lds, flags) : occ_lock<32, true>(lds, flags);
+    return chunk == 16 ? occ_lock<16, false>(lds, flags) : occ_lock<32, false>(lds, flags);
 }
 
 } // namespace ish

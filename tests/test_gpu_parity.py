@@ -234,7 +234,7 @@ def test_int32_widening(eng):
         for tile in (16, 32):
             s, er, eq, st = check(eng, "batched", q, Targets.from_list(seqs), "blosum62", 5, 4, flags, "", tile)
             assert (st[:3] & ish_amd.STATUS_WIDENED).all()
-            assert (st == ish_amd.STATUS_I16).sum() > 100
+            assert ((st & (0xFF ^ ish_amd.STATUS_F16)) == ish_amd.STATUS_I16).sum() > 100
 
 
 def test_gap_open_below_extend_runs_int32(eng):
@@ -290,8 +290,12 @@ def test_sliced_batches(eng, slice_bytes):
             whole = eng.align(targets)
             eng.set_slice_bytes(slice_bytes)
             sliced = eng.align(targets)
-            for a, b in zip(whole, sliced):
+            for a, b in zip(whole[:3], sliced[:3]):
                 assert np.array_equal(a, b), (algo, flags)
+            # the lanes of a launch follow its own groups, so a slice may run
+            # in binary16 where the whole batch did not: same cell width
+            keep = 0xFF ^ ish_amd.STATUS_F16
+            assert np.array_equal(whole[3] & keep, sliced[3] & keep), (algo, flags)
             m = O.Matrix("blosum62")
             if algo == "striped-local":
                 ref = O.batch(1, w.query, targets.residues, targets.offsets, m, 3, 1, 0)
@@ -360,7 +364,7 @@ def test_database_reuse_across_queries(eng):
         os_, oer, oeq, _ = O.batch(0, q, w.targets.residues, w.targets.offsets, m, 3, 1,
                                    O.flags_from_str(flags))
         assert np.array_equal(s, os_) and np.array_equal(er, oer) and np.array_equal(eq, oeq), (qlen, flags)
-        assert (st == ish_amd.STATUS_I16).all()
+        assert ((st & (0xFF ^ ish_amd.STATUS_F16)) == ish_amd.STATUS_I16).all()
     db.close()
 
 
@@ -706,8 +710,11 @@ def test_autotune_keeps_results(eng):
         tuned = eng.autotune(db)
         assert tuned[0] in (16, 32) and tuned[1] in ("wave", "auto", "workgroup")
         got = eng.align_db(db)
-        for a, b in zip(ref, got):
+        for a, b in zip(ref[:3], got[:3]):
             assert np.array_equal(a, b)
+        # the tuned plan may run other groups in binary16 lanes
+        keep = 0xFF ^ ish_amd.STATUS_F16
+        assert np.array_equal(ref[3] & keep, got[3] & keep)
         lr = workloads.generate(4, 400, seed=9)
         db2 = eng.load_db(lr.targets)
         eng.set_tile(32)
