@@ -378,6 +378,7 @@ def run_multiquery(args, ish_amd, workloads):
         f"{cells:.3e} cells, {time.perf_counter() - t0:.1f}s")
     eng = ish_amd.Engine(0)
     eng.set_tile(args.tile or 32)
+    eng.set_f16(not args.no_f16)
     t0 = time.perf_counter()
     db = eng.load_db(w.targets)
     load_s = time.perf_counter() - t0
@@ -391,8 +392,12 @@ def run_multiquery(args, ish_amd, workloads):
     wall = time.perf_counter() - t0
     step_s = wall / args.steps
     kern_s = float(np.mean(kernel_ms)) / 1e3
+    qcells = qlens.astype(np.float64)[:, None] * np.diff(w.targets.offsets).astype(np.float64)[None, :]
+    f16_share = float(qcells[(st & ish_amd.STATUS_F16) != 0].sum() / max(qcells.sum(), 1.0))
+    del qcells
     log(f"multi: {step_s * 1e3:.1f} ms/step (kernels {kern_s * 1e3:.1f} ms), "
-        f"{cells / kern_s / 1e9:.1f} GCUPS kernel, widened {int((st & ish_amd.STATUS_WIDENED != 0).sum())}")
+        f"{cells / kern_s / 1e9:.1f} GCUPS kernel, widened {int((st & ish_amd.STATUS_WIDENED != 0).sum())}, "
+        f"binary16 share of cells {f16_share:.4f}")
     cpu = parity = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -422,14 +427,18 @@ def run_multiquery(args, ish_amd, workloads):
     out = {
         "metric": METRIC, "value": round(cells / step_s / 1e9, 2), "unit": "GCUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16" if f16_share >= 0.5 else "int16",
         "data": "synthetic",
         "config": {"workload": desc, "name": "multiquery", "targets": w.targets.n, "residues": nres,
                    "queries": 64, "query_residues": int(qlens.sum()), "cells": cells, "seed": args.seed,
                    "content_hash": w.content_hash(), "tile_rows": args.tile or 32},
         "gcups_kernel": round(cells / kern_s / 1e9, 2), "db_load_ms": round(load_s * 1e3, 1),
         "parity_sample": parity,
-        "roofline": roofline(cells, kern_s, False, *measured_counters(w.content_hash(), args.tile or 32, "multi")),
+        "roofline": roofline(cells, kern_s, False,
+                             *measured_counters(w.content_hash(), args.tile or 32, "multi",
+                                                "f16" if f16_share >= 0.5 else "int16"),
+                             f16_share=f16_share),
         "cpu_baseline": cpu,
     }
     db.close()

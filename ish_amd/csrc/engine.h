@@ -32,7 +32,7 @@ struct QueryDesc {
     int32_t g_begin, g_end; // groups of this launch for this query
     uint32_t flags;         // free ends of this query: qs|qe<<1|ts<<2|te<<3
     int32_t pad, front;     // top padding rows and profile front (ScoreParams)
-    int32_t spare;
+    int32_t f16, fshift;    // lockstep tiles: binary16 lanes, the shift of their values
     int64_t tile_base;      // first work tile of this query (prefix sum)
     int64_t out_off;        // offset of this query's results in the outputs
     // paired profile of this query (paired launches; see ScoreParams)
@@ -146,9 +146,10 @@ constexpr int kPairMaxRows = 8;
 // size is sg_hybrid_paired_lds(chunk, classes, waves).
 size_t sg_pipe_extra_lds();
 size_t sg_hybrid_paired_lds(int chunk, int nrows, int waves);
-hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int waves, int grid,
+// f16 (paired only): binary16 lanes for every group of the launch.
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, bool f16, int waves, int grid,
                                 size_t lds, hipStream_t st);
-int occupancy_sg_hybrid_i16(int chunk, bool paired, int waves, size_t lds, uint32_t flags);
+int occupancy_sg_hybrid_i16(int chunk, bool paired, bool f16, int waves, size_t lds, uint32_t flags);
 // Hybrid launch over a workgroup-shared paired profile (up to
 // kPairMaxShared classes; tiles of four groups in lockstep, long groups over
 // the int16 profile). Its LDS is sg_hybrid_lock_lds(chunk, classes, the int16

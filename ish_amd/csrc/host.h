@@ -107,6 +107,11 @@ int32_t matrix_of(const int8_t *matrix, int64_t matrix_len, int32_t kind, int32_
 // query (-1: none; always so when go < ge, since the packed kernels read
 // row Q-1 from the last register row, which takes top padding rows).
 int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t min_s);
+// Widest group (columns) whose semi-global pass may run in binary16 lanes
+// for this profile over the residue codes in present[8] (null: any code),
+// and the shift of the values those lanes hold; -1 when none.
+int64_t f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
+                  const uint32_t *present, int32_t *shift);
 // int16 floor seeding the last-row maximum.
 int32_t neg_limit_of(int32_t go, int32_t mn);
 // Grid of a persistent launch (occ_blocks per CU on `cus` CUs) and the
@@ -253,11 +258,13 @@ struct MultiBuffers {
 // (semi-global): int32 groups first, then packed int16, one persistent
 // launch per (cell width, end-rule variant) that has work. `on_launch` is
 // called with true before and false after each kernel launch (timing).
+// f16: the lockstep tiles of a query whose groups all lie within its
+// binary16 range bound run in binary16 lanes (f16_bound).
 // Returns 0 or an ISH_E_* code with a message in err.
 int32_t enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &items, int32_t go,
                       int32_t ge, int32_t mn, int32_t mx, int chunk, int cus, int32_t *score,
                       int32_t *end_ref, int32_t *end_q, uint8_t *status, MultiBuffers &mb,
                       hipStream_t st, std::string &err,
-                      const std::function<hipError_t(bool)> &on_launch);
+                      const std::function<hipError_t(bool)> &on_launch, bool f16 = true);
 
 } // namespace ish

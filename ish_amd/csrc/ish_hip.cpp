@@ -428,6 +428,27 @@ static int64_t f16_max_cols(const std::vector<int32_t> &rowmax, int32_t go, int3
     return cols < 4 ? -1 : cols;
 }
 
+// f16_max_cols for a semi-global query profile over the rows a pass can
+// read: the classes of the residue codes set in present[8] and of code 0,
+// which lanes past their target's end read (every row when present is
+// null). Entries are S + 2ge.
+int64_t ish::f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
+                       const uint32_t *present, int32_t *shift) {
+    if (!pf.top || Q <= 0) return -1;
+    const int32_t rw = pf.stride / 2;
+    std::vector<char> use((size_t)pf.nrows, present ? 0 : 1);
+    if (present)
+        for (int c = 0; c < 256; ++c)
+            if (c == 0 || ((present[c >> 5] >> (c & 31)) & 1u)) use[(size_t)(pf.lut[c] / rw)] = 1;
+    std::vector<int32_t> rowmax((size_t)Q, INT32_MIN);
+    for (int32_t r = 0; r < pf.nrows; ++r) {
+        if (!use[(size_t)r]) continue;
+        const int16_t *row = pf.fwd.data() + (size_t)r * pf.stride + pf.front;
+        for (int32_t j = 0; j < Q; ++j) rowmax[(size_t)j] = std::max<int32_t>(rowmax[(size_t)j], row[j] - 2 * ge);
+    }
+    return f16_max_cols(rowmax, go, ge, min_s, shift);
+}
+
 // Local alignment in packed int16: with the state shifted down every 256
 // columns, every value lies in [-(go+ge+|min S|), Q*M + (256+Q+8)*ge + M],
 // whatever the target length.
@@ -634,18 +655,8 @@ extern "C" int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *q, int32_t
     ctx->lds = lds;
     ctx->r16max = algo == ISH_ALGO_LOCAL ? (local_fits_int16(pf.qpad, go, ge, mx, mn) ? INT64_MAX : -1)
                                          : int16_max_len(Q, go, ge, mx, mn);
-    ctx->w16 = -1;
+    ctx->w16 = -1; // set with the paired profile, per database code set
     ctx->fshift = 0;
-    if (algo != ISH_ALGO_LOCAL && pf.top) {
-        // each query row's best score over every profile row a residue
-        // code can select (entries are S + 2ge)
-        std::vector<int32_t> rowmax((size_t)Q, INT32_MIN);
-        for (int32_t r = 0; r < pf.nrows; ++r)
-            for (int32_t j = 0; j < Q; ++j)
-                rowmax[(size_t)j] = std::max<int32_t>(rowmax[(size_t)j],
-                                                      pf.fwd[(size_t)r * pf.stride + pf.front + j] - 2 * ge);
-        ctx->w16 = f16_max_cols(rowmax, go, ge, mn, &ctx->fshift);
-    }
     ctx->hprof = std::move(pf);
     ctx->query_gen++;
     ctx->have_query = true;
@@ -1155,7 +1166,7 @@ struct DbLaunch {
     int32_t n_long; // kSgHybrid / kSgLock: leading groups scored by whole workgroups
     int32_t paired; // kSwPipe / kSgHybrid: the paired-profile variant (kSgLock: always)
     int32_t waves;  // waves per workgroup (kSgHybrid: 2 or 4; the others: 4)
-    int32_t f16;    // kSgLock: the tiles run in binary16 lanes
+    int32_t f16;    // kSgLock: the tiles run in binary16 lanes; paired kSgHybrid: every group
     int64_t slot_stride;
     uint64_t lds;
     ScoreParams p;
@@ -1224,7 +1235,7 @@ static int32_t long_groups(const ish_hip_ctx *ctx, const ish_hip_db *db, int32_t
                            int32_t chunk, uint32_t flags) {
     double cols = 0;
     for (int32_t g = g0; g < g1; ++g) cols += db->group_maxlen[g] + 3;
-    const int occ = occupancy_sg_hybrid_i16(chunk, false, 4, ctx->lds + sg_pipe_extra_lds(), flags);
+    const int occ = occupancy_sg_hybrid_i16(chunk, false, false, 4, ctx->lds + sg_pipe_extra_lds(), flags);
     const double per_wave = cols / ((double)occ * ctx->cus * (kBlock / 64));
     int32_t n = 0;
     while (g0 + n < g1 && db->group_maxlen[g0 + n] > 0.5 * per_wave) ++n;
@@ -1253,6 +1264,9 @@ static int32_t ensure_pairs(ish_hip_ctx *ctx, const ish_hip_db *db) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pair_rows = ctx->pairs.classes;
     ctx->pqpad = ctx->pairs.qpad;
+    ctx->w16 = ctx->algo == ISH_ALGO_LOCAL
+                   ? -1
+                   : f16_bound(ctx->hprof, ctx->Q, ctx->go, ctx->ge, ctx->min_score, db->code_mask, &ctx->fshift);
     return ctx->pair_rows;
 }
 
@@ -1398,6 +1412,9 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
                    sg_hybrid_paired_lds(L.chunk, pc, 4) <= kLdsLimit) {
             L.paired = 1;
             L.lds = sg_hybrid_paired_lds(L.chunk, pc, L.waves);
+            // binary16 lanes when every group of the launch lies within
+            // the query's range bound
+            if (ctx->f16 && ctx->w16 >= (((int64_t)db->group_maxlen[L.g_begin] + 3) & ~3LL)) L.f16 = 1;
         }
         int occ = 1;
         switch (L.kind) {
@@ -1405,7 +1422,9 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         case kSg16: occ = occupancy_sg_batched_i16(L.chunk, kBlock, L.lds, flags); break;
         case kSw32: occ = occupancy_sw_batched_i32(L.chunk, kBlock, L.lds); break;
         case kSw16: occ = occupancy_sw_batched_i16(L.chunk, kBlock, L.lds); break;
-        case kSgHybrid: occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.waves, L.lds, flags); break;
+        case kSgHybrid:
+            occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.f16 != 0, L.waves, L.lds, flags);
+            break;
         case kSgLock: occ = occupancy_sg_hybrid_lock(L.chunk, L.f16 != 0, L.lds, flags); break;
         default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.lds); break;
         }
@@ -1479,7 +1498,7 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
     case kSw32: e = launch_sw_batched_i32(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSw16: e = launch_sw_batched_i16(L.p, L.chunk, L.grid, kBlock, L.lds, ctx->stream); break;
     case kSgHybrid:
-        e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.waves, L.grid, L.lds, ctx->stream);
+        e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.f16 != 0, L.waves, L.grid, L.lds, ctx->stream);
         break;
     case kSgLock: e = launch_sg_hybrid_lock(L.p, L.chunk, L.f16 != 0, L.grid, L.lds, ctx->stream); break;
     default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
@@ -2173,7 +2192,7 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
                            int32_t ge, int32_t mn, int32_t mx, int chunk, int cus, int32_t *score,
                            int32_t *end_ref, int32_t *end_q, uint8_t *status, MultiBuffers &mb,
                            hipStream_t st, std::string &err,
-                           const std::function<hipError_t(bool)> &on_launch) {
+                           const std::function<hipError_t(bool)> &on_launch, bool f16) {
     const size_t nq = items.size();
     if (nq == 0 || db->n == 0) return ISH_OK;
     auto hip_fail = [&](hipError_t e, const char *what) {
@@ -2291,6 +2310,16 @@ int32_t ish::enqueue_multi(const ish_hip_db *db, const std::vector<MultiItem> &i
                     d.plut = (const uint16_t *)(base + pq.words.size() * 4);
                     d.pnrows = pq.classes;
                     d.pqpad = pq.qpad;
+                    // binary16 lanes when every group of the query lies
+                    // within its range bound
+                    if (f16 && d.g_end > d.g_begin) {
+                        int32_t sh = 0;
+                        const int64_t w16 = f16_bound(*it.pf, it.Q, go, ge, mn, db->code_mask, &sh);
+                        if (w16 >= (((int64_t)db->group_maxlen[d.g_begin] + 3) & ~3LL)) {
+                            d.f16 = 1;
+                            d.fshift = sh;
+                        }
+                    }
                 }
                 const int64_t witems = (int64_t)(d.g_end - d.g_begin) * (packed ? 1 : 2);
                 tiles += (witems + waves - 1) / waves;
@@ -2418,7 +2447,7 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
     rc = enqueue_multi(db, items, go, ge, mn, mx, ctx->chunk == 40 ? 32 : ctx->chunk, ctx->cus, o_score,
                        o_er, o_eq, o_st,
                        mb, ctx->stream, err,
-                       [ctx](bool begin) { return begin ? timed_begin(ctx) : timed_end(ctx); });
+                       [ctx](bool begin) { return begin ? timed_begin(ctx) : timed_end(ctx); }, ctx->f16);
     if (rc) {
         fail(ctx, rc, "%s", err.c_str());
     } else {

@@ -62,9 +62,9 @@ template <> __device__ __forceinline__ h16x2 from_u32<h16x2>(uint32_t v) {
 __device__ __forceinline__ s16x2 to_s16x2(s16x2 v, int) { return v; }
 __device__ __forceinline__ s16x2 to_s16x2(h16x2 v, int shift) {
     s16x2 r;
-    r.x = (short)((int)v.x + shift);
-    r.y = (short)((int)v.y + shift);
-    return r;
+    r.x = (short)v.x; // exact integers of magnitude <= 2048
+    r.y = (short)v.y;
+    return r + splat<s16x2>(shift);
 }
 
 // A word of two int16 profile scores as the binary16 pair of the same
@@ -164,12 +164,19 @@ struct PairView {
 // Copies rows j0 .. j0+K-1 of every pair row into the wave's buffer. The
 // previous chunk's reads of the buffer were issued earlier, and a wave's
 // LDS accesses complete in order.
-template <int K>
+// F16: the copy holds each word's two scores as a binary16 pair.
+template <int K, bool F16 = false>
 __device__ __forceinline__ void stage_pairs(const PairView &pv, int j0, int lane) {
     constexpr int parts = K / 4; // 16-byte parts per row
     for (int x = lane; x < pv.npairs * parts; x += 64) {
         const int pr = x / parts, part = x % parts;
-        const uint4 v = *(const uint4 *)(pv.pprof + (size_t)pr * pv.pqpad + j0 + part * 4);
+        uint4 v = *(const uint4 *)(pv.pprof + (size_t)pr * pv.pqpad + j0 + part * 4);
+        if constexpr (F16) {
+            v.x = i16x2_to_h16x2(v.x);
+            v.y = i16x2_to_h16x2(v.y);
+            v.z = i16x2_to_h16x2(v.z);
+            v.w = i16x2_to_h16x2(v.w);
+        }
         *(uint4 *)(pv.pbuf + pr * pair_row_bytes(K) + part * 16) = v;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

@@ -138,6 +138,15 @@ __device__ __forceinline__ int left_boundary(int j, bool qs, bool ts, int go, in
     return ts ? -2 * ge : -go - ge;
 }
 
+// A wave-uniform value as a cell pair. Binary16 lanes convert on the VALU;
+// the empty asm keeps the conversion where the value is used, since the
+// compiler would otherwise hoist a chunk's loop-invariant boundary values
+// out of the pipeline's step loop and keep them all in vector registers.
+template <typename V> __device__ __forceinline__ V splat_uniform(int v) {
+    if constexpr (__is_same(V, h16x2)) asm volatile("" : "+s"(v));
+    return splat<V>(v);
+}
+
 // shift: binary16 lanes hold every value minus shift (0 for the integer
 // lanes).
 template <typename V, int K>
@@ -147,10 +156,10 @@ __device__ __forceinline__ void sg_chunk_init(SgChunk<V, K> &st, int j0, bool qs
 #pragma unroll
     for (int m = 0; m < K; ++m) {
         const int hl = left_boundary(j0 + m, qs, ts, go, ge) - shift; // H^(j,-1)
-        st.H[m] = splat<V>(hl);
-        st.E[m] = splat<V>(hl + cst); // E^(j,0)
+        st.H[m] = splat_uniform<V>(hl);
+        st.E[m] = splat_uniform<V>(hl + cst); // E^(j,0)
     }
-    st.diag_top = splat<V>(left_boundary(j0 - 1, qs, ts, go, ge) - shift);
+    st.diag_top = splat_uniform<V>(left_boundary(j0 - 1, qs, ts, go, ge) - shift);
 }
 
 __device__ __forceinline__ void track_init(TrackI16 &t16, TrackI32 &t32) {
@@ -241,6 +250,9 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
             for (int m = 0; m < K; ++m) {
                 const V h = vmax3(t1[m], st.E[m], f);
                 __builtin_amdgcn_sched_barrier(0);
+                // row m-1's E update stays in its slot (not sunk into the
+                // next column, where its operands would stay live)
+                if (m > 0) pin(st.E[m - 1]);
                 if (m + 2 < K)
                     t1[m + 2] = st.H[m + 1] + slice_val<V, PAIRED, PACKED>(sa[(m + 2) / 8], sb[(m + 2) / 8],
                                                                            (m + 2) % 8);
@@ -538,7 +550,7 @@ __device__ __forceinline__ void sg_group(const ScoreParams &p, const QueryView &
         const int e0 = qv.front + c * K - qv.pad; // profile entry of the chunk's first row
         // LDS address of this chunk's rows in profile row `code`
         const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * e0;
-        if constexpr (PAIRED) stage_pairs<K>(*pv, e0, lane);
+        if constexpr (PAIRED) stage_pairs<K, __is_same(V, h16x2)>(*pv, e0, lane);
         // one code path for every chunk: a group per wave runs beside
         // pipelined groups (sg_hybrid), whose code shares the instruction
         // cache
@@ -625,12 +637,15 @@ constexpr size_t kSgCtlBytes = 32;
 constexpr size_t kSgExtraBytes = kSgRingBytes + 2 * kSgCombBytes + kSgCtlBytes;
 
 // W: the waves of the workgroup, which share the group's query chunks.
-template <int K, bool TE, bool QE, bool PAIRED = false, int W = kSgPipeWaves>
+// F16 (PAIRED only): binary16 lanes, values held minus qv.fshift.
+template <int K, bool TE, bool QE, bool PAIRED = false, int W = kSgPipeWaves, bool F16 = false>
 __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryView &qv,
                                               const uint8_t *smem, const uint32_t *slut,
                                               uint2 *ring, int *comb, uint2 *gbnd, int g, int w,
                                               int lane, const PairView *pv = nullptr) {
-    using V = s16x2;
+    static_assert(!F16 || PAIRED, "binary16 lanes read the paired profile");
+    using V = typename std::conditional<F16, h16x2, s16x2>::type;
+    const int shift = F16 ? qv.fshift : 0;
     constexpr int NS = K / 8;
     const int go = p.go, ge = p.ge, Q = qv.Q;
     const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
@@ -653,7 +668,7 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
     TrackI32 t32;
     track_init(t16, t32);
     SgChunk<V, K> st;
-    sg_chunk_init(st, 0, qs, ts, go, ge);
+    sg_chunk_init(st, 0, qs, ts, go, ge, shift);
     uint2 rr_next = make_uint2(0u, 0u);
     uint2 bn_next[4];
     short8 sa[NS], sb[NS];
@@ -673,14 +688,14 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
             const int par = (r * P + b) & 1; // ring parity of this (round, block)
             const uint8_t *chunk_base = PAIRED ? pv->pbuf : smem + 2 * (qv.front + j0);
             if (b == 0) {
-                sg_chunk_init(st, j0, qs, ts, go, ge);
+                sg_chunk_init(st, j0, qs, ts, go, ge, shift);
                 rr_next = res[lane];
                 if (!first && w == 0) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
                 }
                 if constexpr (PAIRED) {
-                    stage_pairs<K>(*pv, qv.front + j0, lane);
+                    stage_pairs<K, F16>(*pv, qv.front + j0, lane);
                     const short8 *np = pair_row(chunk_base, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
 #pragma unroll
                     for (int k = 0; k < NS; ++k) {
@@ -723,7 +738,7 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
                                            : gbnd + (size_t)b * 4 * 64;
             sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
                                            chunk_base, slut, 0, RA, RB, rmin, ts, qe_rt, go, ge, t16,
-                                           t32, out, lane);
+                                           t32, out, lane, shift);
         }
         __syncthreads();
         if (++b == P) {
@@ -779,13 +794,15 @@ __device__ __forceinline__ void sg_pipe_group(const ScoreParams &p, const QueryV
 // LAST: this wave scores the query's last chunk (its tracking and the
 // other waves' ring stores are separate code, so that neither adds its
 // registers to the other).
-template <int K, bool TE, bool QE, bool PAIRED, int W, bool LAST>
+template <int K, bool TE, bool QE, bool PAIRED, int W, bool LAST, bool F16>
 __device__ __forceinline__ int sg_pipe_stream_role(const ScoreParams &p, const QueryView &qv,
                                                    const uint8_t *smem, const uint32_t *slut,
                                                    uint2 *ring, int *comb, int *ctl, int item0,
                                                    int nlong, int w, int lane,
                                                    const PairView *pv) {
-    using V = s16x2;
+    static_assert(!F16 || PAIRED, "binary16 lanes read the paired profile");
+    using V = typename std::conditional<F16, h16x2, s16x2>::type;
+    const int shift = F16 ? qv.fshift : 0;
     constexpr int NS = K / 8;
     const int go = p.go, ge = p.ge, Q = qv.Q;
     const bool qs = qv.flags & 1u, ts = qv.flags & 4u;
@@ -803,7 +820,7 @@ __device__ __forceinline__ int sg_pipe_stream_role(const ScoreParams &p, const Q
         ctl[6] = INT_MAX;
     }
     if constexpr (PAIRED) {
-        if (active) stage_pairs<K>(*pv, qv.front + j0, lane);
+        if (active) stage_pairs<K, F16>(*pv, qv.front + j0, lane);
     }
     __syncthreads();
 
@@ -811,7 +828,7 @@ __device__ __forceinline__ int sg_pipe_stream_role(const ScoreParams &p, const Q
     TrackI32 t32;
     track_init(t16, t32);
     SgChunk<V, K> st;
-    sg_chunk_init(st, j0, qs, ts, go, ge);
+    sg_chunk_init(st, j0, qs, ts, go, ge, shift);
     short8 sa[NS], sb[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) sa[k] = sb[k] = (short8)0;
@@ -870,7 +887,7 @@ __device__ __forceinline__ int sg_pipe_stream_role(const ScoreParams &p, const Q
                     ctl[6] = k + 1;
                 }
             }
-            sg_chunk_init(st, j0, qs, ts, go, ge);
+            sg_chunk_init(st, j0, qs, ts, go, ge, shift);
             track_init(t16, t32);
             if (nblk == 0) {
                 group_done(); // no column: results from the boundary alone
@@ -908,7 +925,7 @@ __device__ __forceinline__ int sg_pipe_stream_role(const ScoreParams &p, const Q
             uint2 *out = last ? nullptr : ring + (size_t)(w * 2 + (s & 1)) * 256;
             sg_block<V, K, TE, QE, PAIRED>(st, b, j0, first, last, mstar, rr, rr_next, bn, sa, sb,
                                            chunk_base, slut, 0, RA, RB, rmin, ts, qe_rt, go, ge, t16,
-                                           t32, out, lane);
+                                           t32, out, lane, shift);
             if (b == nblk - 1) group_done();
         }
         __syncthreads();
@@ -920,7 +937,7 @@ __device__ __forceinline__ int sg_pipe_stream_role(const ScoreParams &p, const Q
     return nx;
 }
 
-template <int K, bool TE, bool QE, bool PAIRED, int W>
+template <int K, bool TE, bool QE, bool PAIRED, int W, bool F16 = false>
 __device__ __forceinline__ int sg_pipe_stream(const ScoreParams &p, const QueryView &qv,
                                               const uint8_t *smem, const uint32_t *slut,
                                               uint2 *ring, int *comb, int *ctl, int item0,
@@ -928,10 +945,10 @@ __device__ __forceinline__ int sg_pipe_stream(const ScoreParams &p, const QueryV
                                               const PairView *pv = nullptr) {
     // every wave runs the same steps and barriers in either role
     if (w == qv.nchunks - 1)
-        return sg_pipe_stream_role<K, TE, QE, PAIRED, W, true>(p, qv, smem, slut, ring, comb, ctl,
-                                                               item0, nlong, w, lane, pv);
-    return sg_pipe_stream_role<K, TE, QE, PAIRED, W, false>(p, qv, smem, slut, ring, comb, ctl,
-                                                            item0, nlong, w, lane, pv);
+        return sg_pipe_stream_role<K, TE, QE, PAIRED, W, true, F16>(p, qv, smem, slut, ring, comb, ctl,
+                                                                    item0, nlong, w, lane, pv);
+    return sg_pipe_stream_role<K, TE, QE, PAIRED, W, false, F16>(p, qv, smem, slut, ring, comb, ctl,
+                                                                 item0, nlong, w, lane, pv);
 }
 
 // Resident workgroups per CU of kernel k at `block` threads and `lds` bytes

@@ -21,9 +21,13 @@ size_t sg_hybrid_paired_lds(int chunk, int nrows, int waves) {
 // quarter of the time one wave needs, so they no longer set the length of
 // the launch.
 // ---------------------------------------------------------------------------
-template <int K, bool TE, bool QE, bool PAIRED, int W>
+// F16 (PAIRED only): every group runs in binary16 lanes (the launch's
+// groups all lie within the query's range bound), values held minus
+// p.fshift.
+template <int K, bool TE, bool QE, bool PAIRED, int W, bool F16>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) sg_hybrid(ScoreParams p) {
-    using V = s16x2;
+    static_assert(!F16 || PAIRED, "binary16 lanes read the paired profile");
+    using V = typename std::conditional<F16, h16x2, s16x2>::type;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_item;
     size_t head;
@@ -57,6 +61,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
     qv.front = p.front;
     qv.flags = p.flags;
     qv.out_off = 0;
+    qv.fshift = p.fshift;
     const int ng = p.g_end - p.g_begin;
     const int nlong = p.n_long < ng ? p.n_long : ng;
     const int items = nlong + (ng - nlong + waves - 1) / waves;
@@ -77,13 +82,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
             if constexpr (PAIRED && !QE) {
                 // (an empty query has no chunk and no wave to run the stream)
                 if (qv.nchunks >= 1 && qv.nchunks <= W) {
-                    item = sg_pipe_stream<K, TE, QE, PAIRED, W>(p, qv, smem, slut, ring, comb, ctl,
-                                                               item, nlong, w, lane, &pv);
+                    item = sg_pipe_stream<K, TE, QE, PAIRED, W, F16>(p, qv, smem, slut, ring, comb, ctl,
+                                                                    item, nlong, w, lane, &pv);
                     continue;
                 }
             }
-            sg_pipe_group<K, TE, QE, PAIRED, W>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item,
-                                                w, lane, &pv);
+            sg_pipe_group<K, TE, QE, PAIRED, W, F16>(p, qv, smem, slut, ring, comb, gbnd, p.g_begin + item,
+                                                     w, lane, &pv);
         } else {
             const int g = p.g_begin + nlong + (item - nlong) * waves + w;
             if (g < p.g_end) sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, 0, lane, &pv);
@@ -92,69 +97,75 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
     }
 }
 
-template <int K, bool PAIRED, int W>
+template <int K, bool PAIRED, int W, bool F16>
 static hipError_t launch_hybrid(const ScoreParams &p, int grid, size_t lds, hipStream_t st) {
     const bool te = p.flags & 8u, qe = p.flags & 2u;
     const dim3 block(64 * W);
     if (te && qe)
-        hipLaunchKernelGGL((sg_hybrid<K, true, true, PAIRED, W>), dim3(grid), block, lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, true, true, PAIRED, W, F16>), dim3(grid), block, lds, st, p);
     else if (te)
-        hipLaunchKernelGGL((sg_hybrid<K, true, false, PAIRED, W>), dim3(grid), block, lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, true, false, PAIRED, W, F16>), dim3(grid), block, lds, st, p);
     else if (qe)
-        hipLaunchKernelGGL((sg_hybrid<K, false, true, PAIRED, W>), dim3(grid), block, lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, false, true, PAIRED, W, F16>), dim3(grid), block, lds, st, p);
     else
-        hipLaunchKernelGGL((sg_hybrid<K, false, false, PAIRED, W>), dim3(grid), block, lds, st, p);
+        hipLaunchKernelGGL((sg_hybrid<K, false, false, PAIRED, W, F16>), dim3(grid), block, lds, st, p);
     return hipGetLastError();
 }
 
 // The 40-row tile has four-wave variants only (two waves would spill
 // registers); the planner never asks for two.
-template <int K, bool PAIRED>
+template <int K, bool PAIRED, bool F16>
 static hipError_t launch_hybrid_w(const ScoreParams &p, int waves, int grid, size_t lds, hipStream_t st) {
     if constexpr (K == 40) {
         if (waves != 4) return hipErrorInvalidValue;
-        return launch_hybrid<K, PAIRED, 4>(p, grid, lds, st);
+        return launch_hybrid<K, PAIRED, 4, F16>(p, grid, lds, st);
     } else {
-        return waves == 2 ? launch_hybrid<K, PAIRED, 2>(p, grid, lds, st)
-                          : launch_hybrid<K, PAIRED, 4>(p, grid, lds, st);
+        return waves == 2 ? launch_hybrid<K, PAIRED, 2, F16>(p, grid, lds, st)
+                          : launch_hybrid<K, PAIRED, 4, F16>(p, grid, lds, st);
     }
 }
 
-hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, int waves, int grid,
+template <bool PAIRED, bool F16>
+static hipError_t launch_hybrid_k(const ScoreParams &p, int chunk, int waves, int grid, size_t lds,
+                                  hipStream_t st) {
+    return chunk == 16   ? launch_hybrid_w<16, PAIRED, F16>(p, waves, grid, lds, st)
+           : chunk == 40 ? launch_hybrid_w<40, PAIRED, F16>(p, waves, grid, lds, st)
+                         : launch_hybrid_w<32, PAIRED, F16>(p, waves, grid, lds, st);
+}
+
+hipError_t launch_sg_hybrid_i16(const ScoreParams &p, int chunk, bool paired, bool f16, int waves, int grid,
                                 size_t lds, hipStream_t st) {
-    if (paired)
-        return chunk == 16   ? launch_hybrid_w<16, true>(p, waves, grid, lds, st)
-               : chunk == 40 ? launch_hybrid_w<40, true>(p, waves, grid, lds, st)
-                             : launch_hybrid_w<32, true>(p, waves, grid, lds, st);
-    return chunk == 16   ? launch_hybrid_w<16, false>(p, waves, grid, lds, st)
-           : chunk == 40 ? launch_hybrid_w<40, false>(p, waves, grid, lds, st)
-                         : launch_hybrid_w<32, false>(p, waves, grid, lds, st);
+    if (paired && f16) return launch_hybrid_k<true, true>(p, chunk, waves, grid, lds, st);
+    if (paired) return launch_hybrid_k<true, false>(p, chunk, waves, grid, lds, st);
+    return launch_hybrid_k<false, false>(p, chunk, waves, grid, lds, st);
 }
 
-template <int K, bool PAIRED, int W> static int occ_hybrid(size_t lds, uint32_t flags) {
+template <int K, bool PAIRED, int W, bool F16> static int occ_hybrid(size_t lds, uint32_t flags) {
     const bool te = flags & 8u, qe = flags & 2u;
-    if (te && qe) return occ(sg_hybrid<K, true, true, PAIRED, W>, 64 * W, lds);
-    if (te) return occ(sg_hybrid<K, true, false, PAIRED, W>, 64 * W, lds);
-    if (qe) return occ(sg_hybrid<K, false, true, PAIRED, W>, 64 * W, lds);
-    return occ(sg_hybrid<K, false, false, PAIRED, W>, 64 * W, lds);
+    if (te && qe) return occ(sg_hybrid<K, true, true, PAIRED, W, F16>, 64 * W, lds);
+    if (te) return occ(sg_hybrid<K, true, false, PAIRED, W, F16>, 64 * W, lds);
+    if (qe) return occ(sg_hybrid<K, false, true, PAIRED, W, F16>, 64 * W, lds);
+    return occ(sg_hybrid<K, false, false, PAIRED, W, F16>, 64 * W, lds);
 }
 
-template <int K, bool PAIRED> static int occ_hybrid_w(int waves, size_t lds, uint32_t flags) {
+template <int K, bool PAIRED, bool F16> static int occ_hybrid_w(int waves, size_t lds, uint32_t flags) {
     if constexpr (K == 40) {
-        return occ_hybrid<K, PAIRED, 4>(lds, flags);
+        return occ_hybrid<K, PAIRED, 4, F16>(lds, flags);
     } else {
-        return waves == 2 ? occ_hybrid<K, PAIRED, 2>(lds, flags) : occ_hybrid<K, PAIRED, 4>(lds, flags);
+        return waves == 2 ? occ_hybrid<K, PAIRED, 2, F16>(lds, flags) : occ_hybrid<K, PAIRED, 4, F16>(lds, flags);
     }
 }
 
-int occupancy_sg_hybrid_i16(int chunk, bool paired, int waves, size_t lds, uint32_t flags) {
-    if (paired)
-        return chunk == 16   ? occ_hybrid_w<16, true>(waves, lds, flags)
-               : chunk == 40 ? occ_hybrid_w<40, true>(waves, lds, flags)
-                             : occ_hybrid_w<32, true>(waves, lds, flags);
-    return chunk == 16   ? occ_hybrid_w<16, false>(waves, lds, flags)
-           : chunk == 40 ? occ_hybrid_w<40, false>(waves, lds, flags)
-                         : occ_hybrid_w<32, false>(waves, lds, flags);
+template <bool PAIRED, bool F16> static int occ_hybrid_k(int chunk, int waves, size_t lds, uint32_t flags) {
+    return chunk == 16   ? occ_hybrid_w<16, PAIRED, F16>(waves, lds, flags)
+           : chunk == 40 ? occ_hybrid_w<40, PAIRED, F16>(waves, lds, flags)
+                         : occ_hybrid_w<32, PAIRED, F16>(waves, lds, flags);
+}
+
+int occupancy_sg_hybrid_i16(int chunk, bool paired, bool f16, int waves, size_t lds, uint32_t flags) {
+    if (paired && f16) return occ_hybrid_k<true, true>(chunk, waves, lds, flags);
+    if (paired) return occ_hybrid_k<true, false>(chunk, waves, lds, flags);
+    return occ_hybrid_k<false, false>(chunk, waves, lds, flags);
 }
 
 } // namespace ish

@@ -79,7 +79,17 @@ __global__ void __launch_bounds__(256) sg_multi(ScoreParams p) {
             pv.npairs = qd.pnrows * qd.pnrows;
             pv.pqpad = qd.pqpad;
             pv.pbuf = smem + kPairTableLds + (size_t)wave * p.pnrows * p.pnrows * pair_row_bytes(K);
-            sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, half, lane, &pv);
+            if constexpr (PACKED && PAIRED) {
+                // binary16 lanes when every group of the query lies within
+                // its range bound (a wave-uniform choice per query)
+                qv.fshift = qd.fshift;
+                if (qd.f16)
+                    sg_group<h16x2, K, TE, QE, true>(p, qv, smem, slut, bnd, g, half, lane, &pv);
+                else
+                    sg_group<V, K, TE, QE, true>(p, qv, smem, slut, bnd, g, half, lane, &pv);
+            } else {
+                sg_group<V, K, TE, QE, PAIRED>(p, qv, smem, slut, bnd, g, half, lane, &pv);
+            }
         }
         __syncthreads(); // everyone is done with s_tile and the profile
     }
@@ -139,9 +149,17 @@ __global__ void __launch_bounds__(256) sg_multi_lock(ScoreParams p) {
         qv.front = qd.front;
         qv.flags = qd.flags;
         qv.out_off = qd.out_off;
+        qv.fshift = qd.fshift;
         const int item = (int)wave_broadcast0((uint32_t)s_tile[1]) + wave;
-        sg_tile_lockstep<K, TE, QE>(p, qv, ptab, area, qd.pprof, qd.pnrows * qd.pnrows, qd.pqpad,
-                                    bnd, qd.g_begin + item, item < (int)wave_broadcast0((uint32_t)s_tile[2]), lane);
+        const bool valid = item < (int)wave_broadcast0((uint32_t)s_tile[2]);
+        // the query's groups all lie within its binary16 range bound, or
+        // all run in int16 (a wave-uniform choice per query)
+        if (qd.f16)
+            sg_tile_lockstep<K, TE, QE, true>(p, qv, ptab, area, qd.pprof, qd.pnrows * qd.pnrows, qd.pqpad,
+                                              bnd, qd.g_begin + item, valid, lane);
+        else
+            sg_tile_lockstep<K, TE, QE, false>(p, qv, ptab, area, qd.pprof, qd.pnrows * qd.pnrows, qd.pqpad,
+                                               bnd, qd.g_begin + item, valid, lane);
         __syncthreads(); // everyone is done with s_tile and the pair rows
     }
 }

@@ -171,3 +171,70 @@ def test_f16_protein_workload_reuse(eng):
         eng.set_schedule("auto")
         eng.set_f16(True)
         db.close()
+
+
+@pytest.mark.parametrize("flags", ["FFTT", "TTTT", "TFTF"])
+def test_f16_multi_query_per_query_lanes(eng, flags):
+    """One multi-query launch over the workgroup-shared paired profiles:
+    each query whose bound admits every group of the database runs in
+    binary16 lanes, the high-scoring long query in int16; every query's
+    results equal those with the lanes off and the oracle's on a sample."""
+    rng = np.random.default_rng(11)
+    seqs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(0, 600, 6000)]
+    for k in range(0, len(seqs), 50):  # tandem copies of a query: high scores
+        seqs[k] = np.tile(HIGH, 80)[:len(seqs[k])].copy()
+    T = Targets.from_list(seqs)
+    queries = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in (30, 150, 350, 500)]
+    queries.append(HIGH[rng.integers(0, len(HIGH), 600)])
+    m = O.Matrix("blosum62")
+    widest = (int(np.diff(T.offsets).max()) + 3) & ~3
+    expect = [widest_group(q, m) >= widest for q in queries]
+    assert expect == [True, True, True, True, False], [widest_group(q, m) for q in queries]
+    db = eng.load_db(T)
+    try:
+        eng.set_engine("batched")
+        eng.set_tile(32)
+        eng.set_f16(True)
+        on = eng.align_db_multi(db, queries, "blosum62", GO, GE, flags)
+        eng.set_f16(False)
+        off = eng.align_db_multi(db, queries, "blosum62", GO, GE, flags)
+    finally:
+        eng.set_f16(True)
+        db.close()
+    idx = np.arange(0, T.n, 11)
+    sub = Targets.from_list([T.residues[int(T.offsets[i]):int(T.offsets[i + 1])] for i in idx])
+    for k, q in enumerate(queries):
+        same(f"query {k} {flags} binary16 vs int16", [a[k] for a in on[:3]], [a[k] for a in off[:3]])
+        assert not (off[3][k] & ish_amd.STATUS_F16).any()
+        assert bool((on[3][k] & ish_amd.STATUS_F16).all()) == expect[k], k
+        ref = O.batch(0, q, sub.residues, sub.offsets, m, GO, GE, O.flags_from_str(flags),
+                      nthreads=THREADS, striped=True)[:3]
+        same(f"query {k} {flags} vs oracle", [a[k][idx] for a in on[:3]], ref)
+
+
+def test_f16_multi_query_nucleotide_codes(eng):
+    """The multi-query workload's database holds only A, C, G, T and N codes
+    scored with BLOSUM62: the bound over those codes admits queries of up to
+    1,000 residues, which run in binary16 lanes."""
+    from ish_amd import workloads
+    w = workloads.generate(3, 30_000, seed=2)
+    queries = [workloads.query(3, 300 + k, L) for k, L in enumerate((60, 400, 1000))]
+    db = eng.load_db(w.targets)
+    try:
+        eng.set_engine("batched")
+        eng.set_tile(32)
+        on = eng.align_db_multi(db, queries, "blosum62", GO, GE, "FFTT")
+        eng.set_f16(False)
+        off = eng.align_db_multi(db, queries, "blosum62", GO, GE, "FFTT")
+    finally:
+        eng.set_f16(True)
+        db.close()
+    idx = np.arange(0, w.targets.n, 17)
+    sub = Targets.from_list([w.targets.residues[int(w.targets.offsets[i]):int(w.targets.offsets[i + 1])]
+                             for i in idx])
+    for k, q in enumerate(queries):
+        same(f"query {k} binary16 vs int16", [a[k] for a in on[:3]], [a[k] for a in off[:3]])
+        assert (on[3][k] & ish_amd.STATUS_F16).all(), k
+        ref = O.batch(0, q, sub.residues, sub.offsets, O.Matrix("blosum62"), GO, GE,
+                      O.flags_from_str("FFTT"), nthreads=THREADS, striped=True)[:3]
+        same(f"query {k} vs oracle", [a[k][idx] for a in on[:3]], ref)

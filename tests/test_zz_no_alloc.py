@@ -3,7 +3,9 @@ kernels keep their scratch in the library's static device array and whose
 results go to caller-owned device arrays.
 
 Each call is checked bit-exact (score, end_ref, end_query, status on every
-target) against ish_hip_align_db on the same context, on two calls in a row
+target; the status without ish_hip_align_db's binary16 flag, since this pass
+always runs int16 lanes) against ish_hip_align_db on the same context, on two
+calls in a row
 whose second database is smaller, for semi-global and local queries; the
 second call must leave the device's free memory unchanged; a target longer
 than the array can hold boundary rows for is rejected before any launch.
@@ -81,6 +83,8 @@ def _check_pair(eng, db, n, check_memory=False):
         got = out.read()
     finally:
         out.free()
+    keep = 0xFF ^ ish_amd.STATUS_F16
+    ref = (*ref[:3], ref[3] & keep)
     for name, a, b in zip(("score", "end_ref", "end_query", "status"), got, ref):
         assert np.array_equal(a, b), f"{name} differs on {int((a != b).sum())} of {n} targets"
     if check_memory:
