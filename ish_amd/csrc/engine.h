@@ -167,9 +167,10 @@ int occupancy_sg_hybrid_lock(int chunk, bool f16, size_t lds, uint32_t flags);
 // sw_pipe_paired_lds(chunk, profile classes).
 size_t sw_pipe_extra_lds();
 size_t sw_pipe_paired_lds(int chunk, int nrows);
-hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
+// f16 (paired only): binary16 lanes for every group of the launch.
+hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, bool paired, bool f16, int grid, size_t lds,
                               hipStream_t st);
-int occupancy_sw_pipe_i16(int chunk, bool paired, size_t lds);
+int occupancy_sw_pipe_i16(int chunk, bool paired, bool f16, size_t lds);
 int occupancy_sw_batched_i32(int chunk, int block, size_t lds);
 hipError_t launch_sg_plain(const PlainParams &p, int grid, int block,
                            size_t lds, hipStream_t st);

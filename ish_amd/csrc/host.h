@@ -112,6 +112,10 @@ int64_t int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int32_t 
 // and the shift of the values those lanes hold; -1 when none.
 int64_t f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
                   const uint32_t *present, int32_t *shift);
+// Whether a local pass of this profile over those codes may run in
+// binary16 lanes, and the shift of the state those lanes hold.
+bool sw_f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
+                  const uint32_t *present, int32_t *shift);
 // int16 floor seeding the last-row maximum.
 int32_t neg_limit_of(int32_t go, int32_t mn);
 // Grid of a persistent launch (occ_blocks per CU on `cus` CUs) and the

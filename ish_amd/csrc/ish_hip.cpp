@@ -449,6 +449,42 @@ int64_t ish::f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int
     return f16_max_cols(rowmax, go, ge, min_s, shift);
 }
 
+// Local alignment (column-offset form, state shifted down every 256
+// columns) in binary16 lanes: whether every value stays exact, and the
+// shift C of the state. With P = sum over query rows of max(0, best score
+// of the row over the profile rows a pass can read) (an alignment scores
+// at most its best match per row):
+//   H~ = H + i'*ge, i' < 256, lies in [0, P + 255ge] (H >= 0 and the
+//   floor is i'*ge); E~ >= -go, F~ >= -go and Hdiag~ + S' >= min S (after a
+//   shift the carried state is at least -ge below the floor) give
+//   L = -(go + ge + m) - 2 (m = max(0, -min S)), U = P + 264ge + 2.
+// The state is held minus C = L + 2040: exact when U <= L + 4080. The
+// chunk and column maxima and the target's best are plain values in
+// [-1, P], exact when P <= 2040; the shift of 256ge is exact when
+// 256ge <= 2040.
+bool ish::sw_f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
+                       const uint32_t *present, int32_t *shift) {
+    if (Q <= 0 || 256LL * ge > 2040) return false;
+    const int32_t rw = pf.stride / 2;
+    std::vector<char> use((size_t)pf.nrows, present ? 0 : 1);
+    if (present)
+        for (int c = 0; c < 256; ++c)
+            if (c == 0 || ((present[c >> 5] >> (c & 31)) & 1u)) use[(size_t)(pf.lut[c] / rw)] = 1;
+    int64_t P = 0;
+    for (int32_t j = 0; j < Q; ++j) {
+        int32_t best = INT32_MIN;
+        for (int32_t r = 0; r < pf.nrows; ++r)
+            if (use[(size_t)r]) best = std::max<int32_t>(best, pf.fwd[(size_t)r * pf.stride + pf.front + j] - ge);
+        P += std::max(best, 0);
+    }
+    const int64_t m = std::max(-min_s, 0);
+    const int64_t L = -(int64_t)go - ge - m - 2;
+    const int64_t U = P + 264LL * ge + 2;
+    if (P > 2040 || L < -2040 || U > L + 4080) return false;
+    *shift = (int32_t)(L + 2040);
+    return true;
+}
+
 // Local alignment in packed int16: with the state shifted down every 256
 // columns, every value lies in [-(go+ge+|min S|), Q*M + (256+Q+8)*ge + M],
 // whatever the target length.
@@ -1264,9 +1300,14 @@ static int32_t ensure_pairs(ish_hip_ctx *ctx, const ish_hip_db *db) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pair_rows = ctx->pairs.classes;
     ctx->pqpad = ctx->pairs.qpad;
-    ctx->w16 = ctx->algo == ISH_ALGO_LOCAL
-                   ? -1
-                   : f16_bound(ctx->hprof, ctx->Q, ctx->go, ctx->ge, ctx->min_score, db->code_mask, &ctx->fshift);
+    // binary16 lanes: semi-global, the widest group admitted; local, any
+    // group (INT64_MAX) or none
+    if (ctx->algo == ISH_ALGO_LOCAL)
+        ctx->w16 = sw_f16_bound(ctx->hprof, ctx->Q, ctx->go, ctx->ge, ctx->min_score, db->code_mask, &ctx->fshift)
+                       ? INT64_MAX
+                       : -1;
+    else
+        ctx->w16 = f16_bound(ctx->hprof, ctx->Q, ctx->go, ctx->ge, ctx->min_score, db->code_mask, &ctx->fshift);
     return ctx->pair_rows;
 }
 
@@ -1408,6 +1449,8 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
                    sw_pipe_paired_lds(L.chunk, pc) <= kLdsLimit) {
             L.paired = 1;
             L.lds = sw_pipe_paired_lds(L.chunk, pc);
+            // binary16 lanes: the local bound holds for any target length
+            if (ctx->f16 && ctx->w16 == INT64_MAX) L.f16 = 1;
         } else if (L.kind == kSgHybrid && pc > 0 && pc <= kPairMaxRows &&
                    sg_hybrid_paired_lds(L.chunk, pc, 4) <= kLdsLimit) {
             L.paired = 1;
@@ -1426,7 +1469,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
             occ = occupancy_sg_hybrid_i16(L.chunk, L.paired != 0, L.f16 != 0, L.waves, L.lds, flags);
             break;
         case kSgLock: occ = occupancy_sg_hybrid_lock(L.chunk, L.f16 != 0, L.lds, flags); break;
-        default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.lds); break;
+        default: occ = occupancy_sw_pipe_i16(L.chunk, L.paired != 0, L.f16 != 0, L.lds); break;
         }
         int grid;
         int64_t ss;
@@ -1501,7 +1544,7 @@ static hipError_t enqueue_launch(ish_hip_ctx *ctx, const DbLaunch &L, bool timed
         e = launch_sg_hybrid_i16(L.p, L.chunk, L.paired != 0, L.f16 != 0, L.waves, L.grid, L.lds, ctx->stream);
         break;
     case kSgLock: e = launch_sg_hybrid_lock(L.p, L.chunk, L.f16 != 0, L.grid, L.lds, ctx->stream); break;
-    default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.grid, L.lds, ctx->stream); break;
+    default: e = launch_sw_pipe_i16(L.p, L.chunk, L.paired != 0, L.f16 != 0, L.grid, L.lds, ctx->stream); break;
     }
     if (e == hipSuccess && timed) e = timed_end(ctx);
     return e;

@@ -57,6 +57,15 @@ template <> __device__ __forceinline__ h16x2 from_u32<h16x2>(uint32_t v) {
     return __builtin_bit_cast(h16x2, v);
 }
 
+// A wave-uniform value as a cell pair. Binary16 lanes convert on the VALU;
+// the empty asm keeps the conversion where the value is used, since the
+// compiler would otherwise hoist a chunk's loop-invariant boundary values
+// out of the pipeline's step loop and keep them all in vector registers.
+template <typename V> __device__ __forceinline__ V splat_uniform(int v) {
+    if constexpr (__is_same(V, h16x2)) asm volatile("" : "+s"(v));
+    return splat<V>(v);
+}
+
 // The int16 pair of a cell pair: binary16 halves hold value - shift (exact
 // integers), the int16 form the value itself.
 __device__ __forceinline__ s16x2 to_s16x2(s16x2 v, int) { return v; }
@@ -97,6 +106,12 @@ template <> __device__ __forceinline__ s16x2 pack2<s16x2>(int a, int b) {
     return r;
 }
 template <> __device__ __forceinline__ int pack2<int>(int a, int) { return a; }
+template <> __device__ __forceinline__ h16x2 pack2<h16x2>(int a, int b) {
+    h16x2 r;
+    r.x = (_Float16)(short)a; // exact: |a|, |b| <= 2048 wherever it is used
+    r.y = (_Float16)(short)b;
+    return r;
+}
 
 // Stage the query profile and the residue -> profile-row table into LDS.
 // The table holds each row's offset in dwords (256 x uint16 in memory,

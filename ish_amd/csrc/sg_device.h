@@ -138,15 +138,6 @@ __device__ __forceinline__ int left_boundary(int j, bool qs, bool ts, int go, in
     return ts ? -2 * ge : -go - ge;
 }
 
-// A wave-uniform value as a cell pair. Binary16 lanes convert on the VALU;
-// the empty asm keeps the conversion where the value is used, since the
-// compiler would otherwise hoist a chunk's loop-invariant boundary values
-// out of the pipeline's step loop and keep them all in vector registers.
-template <typename V> __device__ __forceinline__ V splat_uniform(int v) {
-    if constexpr (__is_same(V, h16x2)) asm volatile("" : "+s"(v));
-    return splat<V>(v);
-}
-
 // shift: binary16 lanes hold every value minus shift (0 for the integer
 // lanes).
 template <typename V, int K>

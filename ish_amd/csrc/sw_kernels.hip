@@ -37,6 +37,8 @@
 #include <limits.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "engine.h"
 #include "kernel_common.h"
 
@@ -68,14 +70,16 @@ template <typename V, int K> struct SwChunk {
     int ibase; // first column of the current offset window
 };
 
+// shift: binary16 lanes hold the state (H~, E~, F~) minus shift; the
+// chunk maximum `run` and the column maxima are plain values in either.
 template <typename V, int K>
-__device__ __forceinline__ void sw_chunk_init(SwChunk<V, K> &st, int go, int ge) {
+__device__ __forceinline__ void sw_chunk_init(SwChunk<V, K> &st, int go, int ge, int shift = 0) {
 #pragma unroll
     for (int m = 0; m < K; ++m) {
-        st.H[m] = splat<V>(-ge); // H~(j, -1): H = 0 at column -1, offset -ge
-        st.E[m] = splat<V>(-go); // E~(j, 0) = H(j, -1) - go
+        st.H[m] = splat_uniform<V>(-ge - shift); // H~(j, -1): H = 0 at column -1, offset -ge
+        st.E[m] = splat_uniform<V>(-go - shift); // E~(j, 0) = H(j, -1) - go
     }
-    st.diag_top = splat<V>(-ge);
+    st.diag_top = splat_uniform<V>(-ge - shift);
     st.run = splat<V>(0);
     st.icA = st.rcA = st.icB = st.rcB = 0;
     st.icA = st.icB = -1;
@@ -102,8 +106,10 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
                                          short8 (&sa)[K / 8], short8 (&sb)[K / 8],
                                          const uint8_t *chunk_base, const uint32_t *slut, int half,
                                          int RA, int RB, int rmin, V lbm1, int go, int ge,
-                                         uint2 *out, int lane) {
+                                         uint2 *out, int lane, int shift = 0) {
     constexpr bool PACKED = !__is_same(V, int);
+    constexpr bool F16 = __is_same(V, h16x2);
+    static_assert(!F16 || PAIRED, "binary16 lanes read the paired profile");
     constexpr int NS = K / 8;
     const V vcst = splat<V>(ge - go);
     const V vge = splat<V>(ge);
@@ -136,11 +142,11 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
             nb = row_ptr((wb >> sh) & 0xffu);
         }
         const int zi = (i - st.ibase) * ge; // floor of column i (all rows)
-        const V vz = splat<V>(zi);
+        const V vz = splat<V>(zi - shift);
         V htop, f;
         if (first) {
-            htop = vz;                // H~(-1, i): H = 0 above the query
-            f = splat<V>(zi - go);    // F~(0, i) = H(-1, i) - go
+            htop = vz;                       // H~(-1, i): H = 0 above the query
+            f = splat<V>(zi - go - shift);   // F~(0, i) = H(-1, i) - go
         } else {
             htop = from_u32<V>(bn[t].x);
             f = from_u32<V>(bn[t].y);
@@ -151,8 +157,51 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
         // terms; the E update of row m), so no packed operation reads the
         // result of the one just before it, which would cost a wait state.
         // The scheduling barriers keep this order.
-        V t3[K];  // max(H~diag + S', E~, floor) of the rows ahead
         V cms[NS]; // column maximum of each 8-row slice (every h includes the floor vz)
+        if constexpr (F16) {
+            // Binary16 lanes: h = max3(t3, E~, f) with t3 = max(H~diag + S',
+            // floor) two rows ahead, and the slice maxima take two rows per
+            // three-input maximum: 7.5 packed ops per row instead of 9. The
+            // F chain (max3, add, max, sub) alternates with t1 and t3 of row
+            // m+2, the E update of row m and the slice maximum.
+            V t3[K];
+            t3[0] = vmax(st.diag_top + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 0), vz);
+            t3[1] = vmax(st.H[0] + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 1), vz);
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const V h = vmax3(t3[m], st.E[m], f);
+                __builtin_amdgcn_sched_barrier(0);
+                if (m > 0) {
+                    pin(st.E[m - 1]);
+                    if ((m - 1) % 2 == 1) pin(cms[(m - 1) / 8]);
+                }
+                V t1;
+                if (m + 2 < K)
+                    t1 = st.H[m + 1] + slice_val<V, PAIRED, PACKED>(sa[(m + 2) / 8], sb[(m + 2) / 8], (m + 2) % 8);
+                __builtin_amdgcn_sched_barrier(0);
+                const V g = h + vcst;
+                __builtin_amdgcn_sched_barrier(0);
+                if (m + 2 < K) t3[m + 2] = vmax(t1, vz);
+                __builtin_amdgcn_sched_barrier(0);
+                const V f1 = vmax(f, g);
+                __builtin_amdgcn_sched_barrier(0);
+                st.E[m] = vmax(st.E[m], g);
+                __builtin_amdgcn_sched_barrier(0);
+                f = f1 - vge;
+                __builtin_amdgcn_sched_barrier(0);
+                // rows m-1 and m of the slice in one maximum
+                if (m % 2 == 1) cms[m / 8] = m % 8 == 1 ? vmax(st.H[m - 1], h) : vmax3(cms[m / 8], st.H[m - 1], h);
+                st.H[m] = h;
+                __builtin_amdgcn_sched_barrier(0);
+                if (m % 8 == 7) {
+                    const int k = m / 8;
+                    sa[k] = na[2 * k];
+                    sb[k] = na[2 * k + 1];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        } else {
+        V t3[K];  // max(H~diag + S', E~, floor) of the rows ahead
         t3[0] = vmax(vmax(st.diag_top + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 0), st.E[0]), vz);
         t3[1] = vmax(vmax(st.H[0] + slice_val<V, PAIRED, PACKED>(sa[0], sb[0], 1), st.E[1]), vz);
 #pragma unroll
@@ -199,6 +248,7 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
+        }
         }
         st.diag_top = htop;
         if (out) {
@@ -419,9 +469,13 @@ size_t sw_pipe_paired_lds(int chunk, int nrows) {
            (size_t)kPipeWaves * nrows * nrows * pair_row_bytes(chunk);
 }
 
-template <int K, bool PAIRED>
+// F16 (PAIRED only): binary16 lanes for every group (the query's range
+// bound holds for any target), the state held minus p.fshift.
+template <int K, bool PAIRED, bool F16 = false>
 __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
-    using V = s16x2;
+    static_assert(!F16 || PAIRED, "binary16 lanes read the paired profile");
+    using V = typename std::conditional<F16, h16x2, s16x2>::type;
+    const int shift = F16 ? p.fshift : 0;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     __shared__ int s_item;
     size_t head;
@@ -477,7 +531,7 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
 
         int bestA = 0, erA = -1, eqA = 0, bestB = 0, erB = -1, eqB = 0;
         SwChunk<V, K> st;
-        sw_chunk_init(st, go, ge);
+        sw_chunk_init(st, go, ge, shift);
         uint2 rr_next = make_uint2(0u, 0u);
         uint2 bn_next[4];
         short8 sa[NS], sb[NS];
@@ -496,14 +550,14 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                 const int par = (r * P + b) & 1; // ring parity of this (round, block)
                 const uint8_t *chunk_base = PAIRED ? pbuf : smem + 2 * j0;
                 if (b == 0) {
-                    sw_chunk_init(st, go, ge);
+                    sw_chunk_init(st, go, ge, shift);
                     rr_next = res[lane];
                     if (!first && w == 0) {
 #pragma unroll
                         for (int t = 0; t < 4; ++t) bn_next[t] = gbnd[t * 64 + lane];
                     }
                     if constexpr (PAIRED) {
-                        stage_pairs<K>(pv, j0, lane);
+                        stage_pairs<K, F16>(pv, j0, lane);
                         const short8 *np = pair_row(pbuf, slut, rr_next.x & 0xffu, rr_next.y & 0xffu);
 #pragma unroll
                         for (int k = 0; k < NS; ++k) {
@@ -557,7 +611,7 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                     lb = vmax(lb, from_u32<V>(pub[v * 64 + lane]));
                 }
                 sw_block<V, K, PAIRED>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, 0,
-                                       RA, RB, rmin, lb - splat<V>(1), go, ge, out, lane);
+                                       RA, RB, rmin, lb - splat<V>(1), go, ge, out, lane, shift);
                 pub[w * 64 + lane] = as_u32(vmax(bestv, st.run));
                 if (b == nblk - 1) {
                     sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
@@ -592,17 +646,20 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                 p.score[o] = best;
                 p.end_ref[o] = best > 0 ? (which ? erB : erA) : -1;
                 p.end_q[o] = best > 0 ? (which ? eqB : eqA) : 0;
-                p.status[o] = 16;
+                p.status[o] = F16 ? 16 | 0x40 : 16;
             }
         }
         __syncthreads(); // comb is rewritten by the next group
     }
 }
 
-hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, bool paired, int grid, size_t lds,
+hipError_t launch_sw_pipe_i16(const ScoreParams &p, int chunk, bool paired, bool f16, int grid, size_t lds,
                               hipStream_t st) {
     const dim3 g(grid), b(64 * kPipeWaves);
-    if (paired) {
+    if (paired && f16) {
+        if (chunk == 16) hipLaunchKernelGGL((sw_pipe<16, true, true>), g, b, lds, st, p);
+        else hipLaunchKernelGGL((sw_pipe<32, true, true>), g, b, lds, st, p);
+    } else if (paired) {
         if (chunk == 16) hipLaunchKernelGGL((sw_pipe<16, true>), g, b, lds, st, p);
         else hipLaunchKernelGGL((sw_pipe<32, true>), g, b, lds, st, p);
     } else {
@@ -636,7 +693,10 @@ template <typename KernelT> static int occ_of(KernelT k, int block, size_t lds) 
     return nb > 0 ? nb : 1;
 }
 
-int occupancy_sw_pipe_i16(int chunk, bool paired, size_t lds) {
+int occupancy_sw_pipe_i16(int chunk, bool paired, bool f16, size_t lds) {
+    if (paired && f16)
+        return chunk == 16 ? occ_of(sw_pipe<16, true, true>, 64 * kPipeWaves, lds)
+                           : occ_of(sw_pipe<32, true, true>, 64 * kPipeWaves, lds);
     if (paired)
         return chunk == 16 ? occ_of(sw_pipe<16, true>, 64 * kPipeWaves, lds)
                            : occ_of(sw_pipe<32, true>, 64 * kPipeWaves, lds);

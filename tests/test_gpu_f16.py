@@ -238,3 +238,72 @@ def test_f16_multi_query_nucleotide_codes(eng):
         ref = O.batch(0, q, sub.residues, sub.offsets, O.Matrix("blosum62"), GO, GE,
                       O.flags_from_str("FFTT"), nthreads=THREADS, striped=True)[:3]
         same(f"query {k} vs oracle", [a[k][idx] for a in on[:3]], ref)
+
+
+def run_local(eng, q, T, kind, f16, tile=32, schedule="workgroup"):
+    eng.set_f16(f16)
+    eng.set_engine("batched")
+    eng.set_tile(tile)
+    eng.set_schedule(schedule)
+    eng.set_query(q, kind, GO, GE, 0, "striped-local")
+    try:
+        return eng.align(T)
+    finally:
+        eng.set_schedule("auto")
+        eng.set_f16(True)
+
+
+@pytest.mark.parametrize("qlen,lanes", [(1020, True), (1021, False), (300, True)])
+def test_f16_local_at_the_range_edge(eng, qlen, lanes):
+    """Local alignment of reads in the workgroup pipeline: an actgn0 query
+    of 1,020 nt scores at most 2,040, the largest plain value binary16
+    lanes admit; reads holding exact copies of it among random flanks
+    (several 256-column offset windows) reach that score. One more query
+    residue leaves the bound and the pass stays in int16. Both lane types
+    give the same results, equal to the oracle's."""
+    rng = np.random.default_rng(qlen)
+    q = rng.integers(0, 4, qlen).astype(np.uint8)
+    seqs = []
+    for k in range(1500):
+        L = int(rng.integers(qlen + 50, 4000))
+        r = rng.integers(0, 5, L).astype(np.uint8)
+        if k % 3 == 0:  # an exact copy at a random offset
+            o = int(rng.integers(0, L - qlen + 1))
+            r[o:o + qlen] = q
+        elif k % 3 == 1:  # a copy with a few substitutions and an N run
+            o = int(rng.integers(0, L - qlen + 1))
+            c = q.copy()
+            c[rng.integers(0, qlen, 8)] = rng.integers(0, 5, 8)
+            c[qlen // 2:qlen // 2 + 5] = 4
+            r[o:o + qlen] = c
+        seqs.append(r)
+    T = Targets.from_list(seqs)
+    for tile in (32, 16):
+        on = run_local(eng, q, T, "actgn0", True, tile)
+        off = run_local(eng, q, T, "actgn0", False, tile)
+        same(f"local {qlen} tile {tile} binary16 vs int16", on[:3], off[:3])
+        assert bool((on[3] & ish_amd.STATUS_F16).all()) == lanes, (on[3] & ish_amd.STATUS_F16).mean()
+        assert not (off[3] & ish_amd.STATUS_F16).any()
+    if qlen == 1020:
+        assert on[0].max() == 2040
+    idx = np.arange(0, T.n, 7)
+    S = Targets.from_list([T.residues[int(T.offsets[i]):int(T.offsets[i + 1])] for i in idx])
+    ref = O.batch(1, q, S.residues, S.offsets, O.Matrix("actgn0"), GO, GE, 0, nthreads=THREADS, striped=True)[:3]
+    same(f"local {qlen} vs oracle", [a[idx] for a in on[:3]], ref)
+
+
+def test_f16_local_long_read_workload(eng):
+    """The long-read workload's generator (log-normal read lengths around
+    8 kb, errors and N, planted query copies) through the automatic
+    schedule: binary16 and int16 lanes agree on every read."""
+    from ish_amd import workloads
+    w = workloads.generate(4, 3000, seed=4)
+    on = run_local(eng, w.query, w.targets, "actgn0", True, 32, "auto")
+    off = run_local(eng, w.query, w.targets, "actgn0", False, 32, "auto")
+    same("long reads binary16 vs int16", on[:3], off[:3])
+    assert (on[3] & ish_amd.STATUS_F16).all()
+    idx = np.arange(0, w.targets.n, 19)
+    S = Targets.from_list([w.targets.residues[int(w.targets.offsets[i]):int(w.targets.offsets[i + 1])] for i in idx])
+    ref = O.batch(1, w.query, S.residues, S.offsets, O.Matrix("actgn0"), GO, GE, 0, nthreads=THREADS,
+                  striped=True)[:3]
+    same("long reads vs oracle", [a[idx] for a in on[:3]], ref)
