@@ -459,7 +459,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0,
-                    help="query rows per lane (16 or 32); 0 = let the autotuner pick tile and schedule")
+                    help="query rows per lane (16, 24, 32 or 40); 0 = let the autotuner pick tile and schedule")
     ap.add_argument("--schedule", choices=["auto", "wave", "workgroup"], default="auto",
                     help="launch schedule used with --tile (ignored when the autotuner runs)")
     ap.add_argument("--requests", type=int, default=0,

@@ -128,9 +128,9 @@ int32_t ish_hip_set_query(ish_hip_ctx *ctx, const uint8_t *enc_query,
                           int32_t algo);
 int32_t ish_hip_set_engine(ish_hip_ctx *ctx, int32_t engine);
 /* Tile height of the batched kernels: query rows each lane holds in
- * registers (16, 32, or 40 for the hybrid semi-global kernel; launches
- * that have no 40-row variant use 32). A tuning knob only; results never
- * depend on it.
+ * registers (16, 32; 40 for the hybrid semi-global kernel, 24 for the
+ * workgroup-shared lockstep one; launches that have no variant of the
+ * height use 32). A tuning knob only; results never depend on it.
  * No counterpart in the reference (its GPU kernel keeps whole rows in
  * private memory, ishlib/matcher/alignment/semi_global_aln/basic.mojo:
  * 305-306). */
@@ -159,7 +159,7 @@ int32_t ish_hip_set_schedule(ish_hip_ctx *ctx, int32_t schedule);
 int32_t ish_hip_set_f16(ish_hip_ctx *ctx, int32_t enable);
 
 /* Autotuner: times whole passes over db for the context's current query
- * with every launch configuration (tile height 32, 16 or 40; schedule WAVE,
+ * with every launch configuration (tile height 32, 16, 40 or 24; schedule WAVE,
  * AUTO or WORKGROUP), `reps` timed runs each after a warm-up (0 = 2), and
  * keeps the fastest under the key (matrix kind, algorithm,
  * ceil(log2(query length)), length bucket of the database's longest

@@ -254,10 +254,11 @@ constexpr int kChunkPlain = 32;
 constexpr int kTopFront = 40;
 constexpr int top_front(int Q) { return kTopFront + (8 - Q % 8) % 8; }
 constexpr int top_pad(int Q, int K) { return (K - Q % K) % K; }
-// Pair tables are stored per tile height, 512 uint16 each, in this order.
-constexpr int kPairTables = 3;
-constexpr int pair_table_index(int K) { return K == 16 ? 0 : K == 32 ? 1 : 2; }
-constexpr int pair_table_rows(int index) { return index == 0 ? 16 : index == 1 ? 32 : 40; }
+// Pair tables are stored per tile height, 512 uint16 each, in this order
+// (24 is a height of the workgroup-shared lockstep launch only).
+constexpr int kPairTables = 4;
+constexpr int pair_table_index(int K) { return K == 16 ? 0 : K == 32 ? 1 : K == 40 ? 2 : 3; }
+constexpr int pair_table_rows(int index) { return index == 0 ? 16 : index == 1 ? 32 : index == 2 ? 40 : 24; }
 
 // Smallest slice of a one-shot batch cut into input-order slices (each
 // uploaded while the slice before is scored).

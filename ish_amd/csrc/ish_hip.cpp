@@ -341,8 +341,9 @@ extern "C" const char *ish_hip_last_error(const ish_hip_ctx *ctx) {
 
 extern "C" int32_t ish_hip_set_tile(ish_hip_ctx *ctx, int32_t query_rows_per_lane) {
     if (!ctx) return ISH_E_INVALID;
-    if (query_rows_per_lane != 16 && query_rows_per_lane != 32 && query_rows_per_lane != 40)
-        return fail(ctx, ISH_E_INVALID, "tile height must be 16, 32 or 40 (got %d)", query_rows_per_lane);
+    if (query_rows_per_lane != 16 && query_rows_per_lane != 24 && query_rows_per_lane != 32 &&
+        query_rows_per_lane != 40)
+        return fail(ctx, ISH_E_INVALID, "tile height must be 16, 24, 32 or 40 (got %d)", query_rows_per_lane);
     ctx->chunk = query_rows_per_lane;
     ctx->tuned = false; // an explicit tile overrides the tuning table
     return ISH_OK;
@@ -1343,7 +1344,7 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         DbLaunch L;
         memset((void *)&L, 0, sizeof L);
         L.kind = local ? kSw32 : kSg32;
-        L.chunk = ctx->chunk == 40 ? 32 : ctx->chunk; // the int32 kernels have 16 and 32
+        L.chunk = ctx->chunk == 40 || ctx->chunk == 24 ? 32 : ctx->chunk; // the int32 kernels have 16 and 32
         L.g_begin = a;
         L.g_end = m16;
         out.push(L);
@@ -1361,6 +1362,10 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
     }
     auto add16 = [&](int32_t g0, int32_t g1, int chunk, int sched, int waves) {
         if (g1 <= g0) return;
+        // the 24-row tile exists for the workgroup-shared lockstep launch
+        // only; the others take 32
+        if (chunk == 24 && (local || pc <= kPairMaxRows || sg_hybrid_lock_lds(24, pc, ctx->lds) > kLdsLimit))
+            chunk = 32;
         // the 40-row tile exists for the hybrid semi-global kernel only
         // (per-wave paired or over the int16 profile); the others take 32
         if (chunk == 40 &&
@@ -1621,7 +1626,7 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     if (ctx->engine != ISH_ENGINE_BATCHED) return fail(ctx, ISH_E_INVALID, "autotuning needs the batched engine");
     if (reps <= 0) reps = 2;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const int32_t chunks[3] = {32, 16, 40};
+    const int32_t chunks[4] = {32, 16, 40, 24};
     const int scheds[3] = {ISH_SCHEDULE_WAVE, ISH_SCHEDULE_AUTO, ISH_SCHEDULE_WORKGROUP};
     TuneChoice best{(int8_t)kChunkDefault, (int8_t)ISH_SCHEDULE_AUTO, 0};
     float best_ms = 1e30f;
@@ -1632,7 +1637,7 @@ extern "C" int32_t ish_hip_autotune(ish_hip_ctx *ctx, ish_hip_db *db, int32_t re
     auto launch_key = [](const DbPlan &plan) {
         std::vector<int32_t> k;
         for (const DbLaunch &L : plan) {
-            for (int32_t v : {L.kind, L.chunk, L.g_begin, L.g_end, L.grid, L.n_long, L.paired, L.waves})
+            for (int32_t v : {L.kind, L.chunk, L.g_begin, L.g_end, L.grid, L.n_long, L.paired, L.waves, L.f16})
                 k.push_back(v);
         }
         return k;
@@ -2487,7 +2492,7 @@ extern "C" int32_t ish_hip_align_db_multi(ish_hip_ctx *ctx, ish_hip_db *db,
     mb.host_desc = &h_desc;
     ctx->launches = 0;
     std::string err;
-    rc = enqueue_multi(db, items, go, ge, mn, mx, ctx->chunk == 40 ? 32 : ctx->chunk, ctx->cus, o_score,
+    rc = enqueue_multi(db, items, go, ge, mn, mx, ctx->chunk == 40 || ctx->chunk == 24 ? 32 : ctx->chunk, ctx->cus, o_score,
                        o_er, o_eq, o_st,
                        mb, ctx->stream, err,
                        [ctx](bool begin) { return begin ? timed_begin(ctx) : timed_end(ctx); }, ctx->f16);

@@ -90,8 +90,13 @@ static hipError_t launch_lock(const ScoreParams &p, int grid, size_t lds, hipStr
 
 hipError_t launch_sg_hybrid_lock(const ScoreParams &p, int chunk, bool f16, int grid, size_t lds,
                                  hipStream_t st) {
-    if (f16) return chunk == 16 ? launch_lock<16, true>(p, grid, lds, st) : launch_lock<32, true>(p, grid, lds, st);
-    return chunk == 16 ? launch_lock<16, false>(p, grid, lds, st) : launch_lock<32, false>(p, grid, lds, st);
+    if (f16)
+        return chunk == 16   ? launch_lock<16, true>(p, grid, lds, st)
+               : chunk == 24 ? launch_lock<24, true>(p, grid, lds, st)
+                             : launch_lock<32, true>(p, grid, lds, st);
+    return chunk == 16   ? launch_lock<16, false>(p, grid, lds, st)
+           : chunk == 24 ? launch_lock<24, false>(p, grid, lds, st)
+                         : launch_lock<32, false>(p, grid, lds, st);
 }
 
 template <int K, bool F16> static int occ_lock(size_t lds, uint32_t flags) {
@@ -103,8 +108,13 @@ template <int K, bool F16> static int occ_lock(size_t lds, uint32_t flags) {
 }
 
 int occupancy_sg_hybrid_lock(int chunk, bool f16, size_t lds, uint32_t flags) {
-    if (f16) return chunk == 16 ? occ_lock<16, true>(lds, flags) : occ_lock<32, true>(lds, flags);
-    return chunk == 16 ? occ_lock<16, false>(lds, flags) : occ_lock<32, false>(lds, flags);
+    if (f16)
+        return chunk == 16 ? occ_lock<16, true>(lds, flags)
+               : chunk == 24 ? occ_lock<24, true>(lds, flags)
+                             : occ_lock<32, true>(lds, flags);
+    return chunk == 16 ? occ_lock<16, false>(lds, flags)
+           : chunk == 24 ? occ_lock<24, false>(lds, flags)
+                         : occ_lock<32, false>(lds, flags);
 }
 
 } // namespace ish

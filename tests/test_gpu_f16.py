@@ -90,7 +90,7 @@ def same(label, a, b):
         assert len(bad) == 0, f"{label}: {name} differs on {len(bad)} targets, first {bad[0]}"
 
 
-@pytest.mark.parametrize("tile,schedule", [(32, "wave"), (32, "auto"), (16, "wave")])
+@pytest.mark.parametrize("tile,schedule", [(32, "wave"), (32, "auto"), (16, "wave"), (24, "wave"), (24, "auto")])
 @pytest.mark.parametrize("flags", ["FFTT", "TTTT", "FFFF", "TFTF", "FTFT"])
 def test_f16_equals_int16_at_the_range_edge(eng, flags, tile, schedule):
     rng = np.random.default_rng(7)

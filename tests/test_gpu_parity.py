@@ -103,7 +103,7 @@ def test_random_all_flags(eng, engine, tile, kind, schedule):
 
 
 @pytest.mark.parametrize("schedule", ["auto", "wave", "workgroup"])
-@pytest.mark.parametrize("tile", [32, 16])
+@pytest.mark.parametrize("tile", [32, 24, 16])
 @pytest.mark.parametrize("ncodes", [9, 14, 20, 21, 22])
 def test_shared_paired_profile_class_counts(eng, tile, schedule, ncodes):
     """Databases whose codes fall into 9..21 profile classes run over the
@@ -708,7 +708,7 @@ def test_autotune_keeps_results(eng):
         eng.set_query(w.query, "blosum62", 3, 1, "FFTT")
         ref = eng.align_db(db)
         tuned = eng.autotune(db)
-        assert tuned[0] in (16, 32) and tuned[1] in ("wave", "auto", "workgroup")
+        assert tuned[0] in (16, 24, 32) and tuned[1] in ("wave", "auto", "workgroup")
         got = eng.align_db(db)
         for a, b in zip(ref[:3], got[:3]):
             assert np.array_equal(a, b)
