@@ -43,6 +43,10 @@ OPS_PER_CELL_SW = 9
 # runs most groups in binary16 and the rest in int16; its count is the
 # cell-weighted mix of 5 and 6.
 OPS_PER_CELL_SG_F16 = 5
+# Local in binary16 lanes: H takes one three-input maximum and the floor
+# one more max, and the column maximum takes two rows per three-input
+# maximum: 7.5 ops instead of 9.
+OPS_PER_CELL_SW_F16 = 7.5
 # The textbook Gotoh count (1 add + 2 max for H; 2 sub + 1 max for E; 1 sub
 # + 1 max for F; +1 max with 0 for local), reported as frac_textbook.
 OPS_TEXTBOOK_SG = 8
@@ -57,7 +61,7 @@ def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, f16_share=0.0
     valu_busy (from the rocprof counters of the same workload) is the share
     of the dispatch the VALU issued."""
     if local:
-        ops = OPS_PER_CELL_SW
+        ops = f16_share * OPS_PER_CELL_SW_F16 + (1.0 - f16_share) * OPS_PER_CELL_SW
     else:
         ops = f16_share * OPS_PER_CELL_SG_F16 + (1.0 - f16_share) * OPS_PER_CELL_SG
     tb = OPS_TEXTBOOK_SW if local else OPS_TEXTBOOK_SG
@@ -67,9 +71,9 @@ def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, f16_share=0.0
            "frac_textbook": round(cells * tb / kernel_s / 1e12 / PEAK_INT16_TOPS, 4),
            "ops_per_cell": round(ops, 3), "ops_per_cell_textbook": tb, "traffic": traffic,
            "valu_busy": valu_busy, "kernel_ms": round(kernel_s * 1e3, 3)}
-    if not local:
-        out["frac_int16_form"] = round(cells * OPS_PER_CELL_SG / kernel_s / 1e12 / PEAK_INT16_TOPS, 4)
-        out["binary16_share"] = round(f16_share, 4)
+    out["frac_int16_form"] = round(cells * (OPS_PER_CELL_SW if local else OPS_PER_CELL_SG) / kernel_s / 1e12
+                                   / PEAK_INT16_TOPS, 4)
+    out["binary16_share"] = round(f16_share, 4)
     out.update(extra)
     return out
 
