@@ -1415,10 +1415,11 @@ static int32_t plan_db(ish_hip_ctx *ctx, ish_hip_db *db, const int16_t *prof, ui
         if (L.kind == kSgLock && ctx->f16 && ctx->w16 >= 4) {
             // binary16 tiles: the groups wider than the range bound join the
             // pipelined long groups (a prefix, groups being sorted by
-            // length), if they are few
+            // length), if they are few: at most 4, 1/16 of the groups, or
+            // the long groups the schedule pipelines anyway
             int32_t n = 0;
             while (g0 + n < g1 && (((int64_t)db->group_maxlen[g0 + n] + 3) & ~3LL) > ctx->w16) ++n;
-            if (n <= std::max<int32_t>(n_long, (g1 - g0) / 32)) {
+            if (n <= std::max<int32_t>(std::max<int32_t>(n_long, (g1 - g0) / 16), 4)) {
                 L.f16 = 1;
                 n_long = std::max(n_long, n);
             }

@@ -307,3 +307,45 @@ def test_f16_local_long_read_workload(eng):
     ref = O.batch(1, w.query, S.residues, S.offsets, O.Matrix("actgn0"), GO, GE, 0, nthreads=THREADS,
                   striped=True)[:3]
     same("long reads vs oracle", [a[idx] for a in on[:3]], ref)
+
+
+@pytest.mark.parametrize("go,ge,lanes", [(3, 0, True), (11, 1, True), (700, 1, False), (2, 3, False)])
+def test_f16_gap_penalties(eng, go, ge, lanes):
+    """Gap penalties at the bound's limits: a zero extension admits groups
+    of any width (no column offset), a large open moves the lower bound
+    past the binary16 range, and go < ge has no top padding rows; the
+    last two stay in int16. Every case equals the int16 lanes and the
+    oracle, on targets up to 5,000 residues."""
+    rng = np.random.default_rng(go * 10 + ge)
+    q = rng.integers(0, 20, 120).astype(np.uint8)
+    lens = [int(x) for x in rng.integers(2000, 5000, 40)] + [int(x) for x in rng.integers(0, 400, 1500)]
+    seqs = [rng.integers(0, 20, L).astype(np.uint8) for L in lens]
+    for k in range(0, len(seqs), 4):
+        r = seqs[k]
+        if len(r) > 120:
+            o = int(rng.integers(0, len(r) - 119))
+            r[o:o + 120] = q
+    T = Targets.from_list(seqs)
+    m = O.Matrix("blosum62")
+    for flags in ("FFTT", "TTTT", "FTFT"):
+        fl = O.flags_from_str(flags)
+        res = []
+        for f16 in (True, False):
+            eng.set_f16(f16)
+            eng.set_engine("batched")
+            eng.set_tile(32)
+            eng.set_schedule("wave")
+            eng.set_query(q, "blosum62", go, ge, fl)
+            try:
+                res.append(eng.align(T))
+            finally:
+                eng.set_schedule("auto")
+                eng.set_f16(True)
+        on, off = res
+        same(f"go {go} ge {ge} {flags} binary16 vs int16", on[:3], off[:3])
+        assert bool((on[3] & ish_amd.STATUS_F16).any()) == lanes
+        # (with go <= ge the reference's striped answer depends on its SIMD
+        # width; the engine's contract there is textbook Gotoh, the scalar
+        # oracle)
+        ref = O.batch(0, q, T.residues, T.offsets, m, go, ge, fl, nthreads=THREADS, striped=go > ge)[:3]
+        same(f"go {go} ge {ge} {flags} vs oracle", on[:3], ref)
