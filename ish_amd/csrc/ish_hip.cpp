@@ -412,12 +412,8 @@ int64_t ish::int16_max_len(int32_t Q, int32_t go, int32_t ge, int32_t max_s, int
 // Held minus C = L + 2040, they lie in [-2040, 2040] when U <= L + 4080,
 // where binary16 arithmetic on integers is exact.
 // rowmax: max over the profile rows of each query row's score S.
-static int64_t f16_max_cols(const std::vector<int32_t> &rowmax, int32_t go, int32_t ge,
-                            int32_t min_s, int32_t *shift) {
-    const int64_t Q = (int64_t)rowmax.size();
+static int64_t f16_max_cols(int64_t P, int64_t Q, int32_t go, int32_t ge, int32_t min_s, int32_t *shift) {
     if (go < ge || Q == 0) return -1; // no top padding rows (engine.h, top_pad)
-    int64_t P = 0;
-    for (int32_t v : rowmax) P += std::max(v, 0);
     const int64_t m = std::max(-min_s, 0);
     const int64_t L = -3LL * go - 2LL * ge - m - 2;
     if (L < -2040) return -1;
@@ -429,32 +425,40 @@ static int64_t f16_max_cols(const std::vector<int32_t> &rowmax, int32_t go, int3
     return cols < 4 ? -1 : cols;
 }
 
-// f16_max_cols for a semi-global query profile over the rows a pass can
-// read: the classes of the residue codes set in present[8] and of code 0,
-// which lanes past their target's end read (every row when present is
-// null). Entries are S + 2ge.
-int64_t ish::f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
-                       const uint32_t *present, int32_t *shift) {
-    if (!pf.top || Q <= 0) return -1;
+// P = sum over query rows of max(0, the row's best score over the profile
+// rows a pass can read): the classes of the residue codes set in
+// present[8] and of code 0, which lanes past their target's end read (every
+// row when present is null). Profile entries are S + bias.
+static int64_t best_possible(const Profile &pf, int32_t Q, int32_t bias, const uint32_t *present) {
     const int32_t rw = pf.stride / 2;
     std::vector<char> use((size_t)pf.nrows, present ? 0 : 1);
     if (present)
         for (int c = 0; c < 256; ++c)
             if (c == 0 || ((present[c >> 5] >> (c & 31)) & 1u)) use[(size_t)(pf.lut[c] / rw)] = 1;
-    std::vector<int32_t> rowmax((size_t)Q, INT32_MIN);
+    std::vector<int32_t> rowmax((size_t)Q, 0); // max(0, best) per row
     for (int32_t r = 0; r < pf.nrows; ++r) {
         if (!use[(size_t)r]) continue;
         const int16_t *row = pf.fwd.data() + (size_t)r * pf.stride + pf.front;
-        for (int32_t j = 0; j < Q; ++j) rowmax[(size_t)j] = std::max<int32_t>(rowmax[(size_t)j], row[j] - 2 * ge);
+        for (int32_t j = 0; j < Q; ++j) rowmax[(size_t)j] = std::max<int32_t>(rowmax[(size_t)j], row[j] - bias);
     }
-    return f16_max_cols(rowmax, go, ge, min_s, shift);
+    int64_t P = 0;
+    for (int32_t v : rowmax) P += v;
+    return P;
+}
+
+// f16_max_cols for a semi-global query profile over the rows a pass can
+// read (best_possible).
+int64_t ish::f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
+                       const uint32_t *present, int32_t *shift) {
+    if (!pf.top || Q <= 0) return -1;
+    return f16_max_cols(best_possible(pf, Q, 2 * ge, present), Q, go, ge, min_s, shift);
 }
 
 // Local alignment (column-offset form, state shifted down every 256
 // columns) in binary16 lanes: whether every value stays exact, and the
-// shift C of the state. With P = sum over query rows of max(0, best score
-// of the row over the profile rows a pass can read) (an alignment scores
-// at most its best match per row):
+// shift C of the state. With P the query's best possible score over the
+// rows a pass can read (best_possible: an alignment scores at most its best
+// match per row):
 //   H~ = H + i'*ge, i' < 256, lies in [0, P + 255ge] (H >= 0 and the
 //   floor is i'*ge); E~ >= -go, F~ >= -go and Hdiag~ + S' >= min S (after a
 //   shift the carried state is at least -ge below the floor) give
@@ -466,18 +470,7 @@ int64_t ish::f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int
 bool ish::sw_f16_bound(const Profile &pf, int32_t Q, int32_t go, int32_t ge, int32_t min_s,
                        const uint32_t *present, int32_t *shift) {
     if (Q <= 0 || 256LL * ge > 2040) return false;
-    const int32_t rw = pf.stride / 2;
-    std::vector<char> use((size_t)pf.nrows, present ? 0 : 1);
-    if (present)
-        for (int c = 0; c < 256; ++c)
-            if (c == 0 || ((present[c >> 5] >> (c & 31)) & 1u)) use[(size_t)(pf.lut[c] / rw)] = 1;
-    int64_t P = 0;
-    for (int32_t j = 0; j < Q; ++j) {
-        int32_t best = INT32_MIN;
-        for (int32_t r = 0; r < pf.nrows; ++r)
-            if (use[(size_t)r]) best = std::max<int32_t>(best, pf.fwd[(size_t)r * pf.stride + pf.front + j] - ge);
-        P += std::max(best, 0);
-    }
+    const int64_t P = best_possible(pf, Q, ge, present);
     const int64_t m = std::max(-min_s, 0);
     const int64_t L = -(int64_t)go - ge - m - 2;
     const int64_t U = P + 264LL * ge + 2;

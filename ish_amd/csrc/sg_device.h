@@ -316,11 +316,11 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                 for (int m = 1; m < K; ++m)
                     if (m == mstar) hl = st.H[m];
             }
-            // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge
-            const V u = hl - splat<V>(i * ge);
+            // u = H^(Q-1,i) - i*ge = H(Q-1,i) + (Q-1)*ge; binary16 lanes go
+            // back to int16 first (i*ge alone may leave the exact range)
+            V u{};
+            if constexpr (!F16) u = hl - splat<V>(i * ge);
             if constexpr (PACKED) {
-                // binary16 lanes: back to int16 before the subtraction
-                // (i*ge alone may leave the exact range)
                 const uint32_t uu = F16 ? as_u32(to_s16x2(hl, shift) - splat<s16x2>(i * ge)) : as_u32(u);
                 if (TE || QE) {
                     int ka = key_lo(uu, i), kb = key_hi(uu, i);
@@ -357,7 +357,8 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                     const int j = j0 + m;
                     if ((last && m > mstar) || j < 0) continue; // past Q-1, padding
                     // w = H^(j,R-1) - j*ge = H(j,R-1) + (R-1)*ge
-                    const V wv = st.H[m] - splat<V>(j * ge);
+                    V wv{};
+                    if constexpr (!F16) wv = st.H[m] - splat<V>(j * ge);
                     if constexpr (PACKED) {
                         const uint32_t ww =
                             F16 ? as_u32(to_s16x2(st.H[m], shift) - splat<s16x2>(j * ge)) : as_u32(wv);
