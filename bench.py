@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "GCUPS on 1 MI355X (bit-exact scores), batched vs plain; p50/p99 query latency"
 PEAK_INT16_TOPS = 78.6  # 256 CU x 2.4 GHz x 128 packed int16 lane-ops/clk
 PEAK_INT32_TOPS = 39.3  # 256 CU x 2.4 GHz x 64 int32 lane-ops/clk
+PEAK_SCLK_GHZ = 2.4     # the clock both peaks are quoted at
 # Integer ops per cell that the kernels' recurrences need, which `frac`
 # is computed on. Semi-global, offset form H^ = H + (i+j)ge: Hdiag + S',
 # max E, max F, G = H + (ge-go), E = max(E, G), F = max(F, G): 6. Local,
@@ -53,13 +54,17 @@ OPS_TEXTBOOK_SG = 8
 OPS_TEXTBOOK_SW = 9
 
 
-def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, f16_share=0.0, **extra):
+def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, f16_share=0.0, sclk_ghz=None,
+             **extra):
     """The packed-VALU roofline of a pass: achieved = cells x ops per cell
     of the kernel's recurrence / kernel time, against the packed 16-bit
     issue peak (int16 and binary16 issue at the same rate); f16_share is
     the share of cells that ran in binary16 lanes (5 ops instead of 6);
     valu_busy (from the rocprof counters of the same workload) is the share
-    of the dispatch the VALU issued."""
+    of the dispatch the VALU issued, counted at the 2.4 GHz peak clock;
+    sclk_ghz, the shader clock the same counter run measured (GRBM_COUNT
+    per XCD over the dispatch time), turns it into the share of the
+    cycles the SIMDs actually ran (valu_busy_at_sclk)."""
     if local:
         ops = f16_share * OPS_PER_CELL_SW_F16 + (1.0 - f16_share) * OPS_PER_CELL_SW
     else:
@@ -74,6 +79,9 @@ def roofline(cells, kernel_s, local, traffic=None, valu_busy=None, f16_share=0.0
     out["frac_int16_form"] = round(cells * (OPS_PER_CELL_SW if local else OPS_PER_CELL_SG) / kernel_s / 1e12
                                    / PEAK_INT16_TOPS, 4)
     out["binary16_share"] = round(f16_share, 4)
+    out["sclk_ghz"] = sclk_ghz
+    out["valu_busy_at_sclk"] = (round(valu_busy * PEAK_SCLK_GHZ / sclk_ghz, 3)
+                                if valu_busy is not None and sclk_ghz else None)
     out.update(extra)
     return out
 
@@ -146,23 +154,24 @@ def log(msg):
 
 
 def measured_counters(content_hash, tile, schedule, lanes="int16"):
-    """(HBM bytes per dispatch, VALU busy share) of the dominant kernel, as
-    rocprofv3's counters measured them for this exact workload, tile and
-    schedule (profiles/traffic.json: FETCH_SIZE / WRITE_SIZE scaled by the
-    copy-kernel calibration, SQ_INSTS_VALU x 4 cycles over 1,024 SIMDs at
-    2.4 GHz against the dispatch time); (None, None) when that combination
-    was not profiled. lanes: "f16" when most cells ran in binary16 lanes
-    (another kernel variant), else "int16"."""
+    """(HBM bytes per dispatch, VALU busy share, shader clock in GHz) of the
+    dominant kernel, as rocprofv3's counters measured them for this exact
+    workload, tile and schedule (profiles/traffic.json: FETCH_SIZE /
+    WRITE_SIZE scaled by the copy-kernel calibration, SQ_INSTS_VALU x 4
+    cycles over 1,024 SIMDs at 2.4 GHz against the dispatch time,
+    GRBM_COUNT / 8 XCDs over the dispatch time); None for what that
+    combination's profile does not hold. lanes: "f16" when most cells ran
+    in binary16 lanes (another kernel variant), else "int16"."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         entries = json.load(open(path)).get("entries", [])
     except (OSError, ValueError):
-        return None, None
+        return None, None, None
     for e in entries:
         if (e.get("content_hash"), e.get("tile_rows"), e.get("schedule"), e.get("lanes", "int16")) == \
                 (content_hash, tile, schedule, lanes):
-            return e.get("bytes"), e.get("valu_busy")
-    return None, None
+            return e.get("bytes"), e.get("valu_busy"), e.get("sclk_ghz")
+    return None, None, None
 
 
 def serve_stream(ish_amd, srv, reqs):
@@ -428,6 +437,8 @@ def run_multiquery(args, ish_amd, workloads):
                "sample": f"queries {sel_q} x {n_s} of the {w.targets.n} targets, {sample_cells / 1e9:.2f} Gcells, "
                          f"{CPU_KIND}", **host_cpu()}
         log(f"cpu oracle: {cpu['value']} GCUPS on {used} threads; parity on sample: {parity}")
+    mq_counters = measured_counters(w.content_hash(), args.tile or 32, "multi",
+                                    "f16" if f16_share >= 0.5 else "int16")
     out = {
         "metric": METRIC, "value": round(cells / step_s / 1e9, 2), "unit": "GCUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3),
@@ -439,10 +450,8 @@ def run_multiquery(args, ish_amd, workloads):
                    "content_hash": w.content_hash(), "tile_rows": args.tile or 32},
         "gcups_kernel": round(cells / kern_s / 1e9, 2), "db_load_ms": round(load_s * 1e3, 1),
         "parity_sample": parity,
-        "roofline": roofline(cells, kern_s, False,
-                             *measured_counters(w.content_hash(), args.tile or 32, "multi",
-                                                "f16" if f16_share >= 0.5 else "int16"),
-                             f16_share=f16_share),
+        "roofline": roofline(cells, kern_s, False, *mq_counters[:2], f16_share=f16_share,
+                             sclk_ghz=mq_counters[2]),
         "cpu_baseline": cpu,
     }
     db.close()
@@ -622,8 +631,8 @@ def main():
         # the 2M above
         serving, _, _ = serving_block(args, ish_amd, workloads, False)
 
-    traffic, valu_busy = measured_counters(w.content_hash(), best_tile, schedule,
-                                           "f16" if f16_share >= 0.5 else "int16")
+    traffic, valu_busy, sclk = measured_counters(w.content_hash(), best_tile, schedule,
+                                                 "f16" if f16_share >= 0.5 else "int16")
     out = {
         "metric": METRIC,
         "value": round(gcups, 2),
@@ -656,7 +665,7 @@ def main():
         "p50_ms": serving["at_70pct_capacity"]["p50_ms"] if serving else None,
         "p99_ms": serving["at_70pct_capacity"]["p99_ms"] if serving else None,
         "parity_sample": parity,
-        "roofline": roofline(cells, kern_s, local, traffic, valu_busy, f16_share),
+        "roofline": roofline(cells, kern_s, local, traffic, valu_busy, f16_share, sclk),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
