@@ -28,6 +28,7 @@ __device__ __forceinline__ h16x2 vmax(h16x2 a, h16x2 b) {
 __device__ __forceinline__ h16x2 vmax3(h16x2 a, h16x2 b, h16x2 c) {
     return __builtin_elementwise_maximum(__builtin_elementwise_maximum(a, b), c);
 }
+__device__ __forceinline__ s16x2 vmax3(s16x2 a, s16x2 b, s16x2 c) { return vmax(vmax(a, b), c); }
 
 template <typename V> __device__ __forceinline__ V splat(int v);
 template <> __device__ __forceinline__ s16x2 splat<s16x2>(int v) {

@@ -91,9 +91,10 @@ __device__ __forceinline__ void sw_chunk_init(SwChunk<V, K> &st, int go, int ge,
 //   slices of the next column are fetched while a column is scored);
 //   bn: chunk-top rows (H~, F~ of the row above) when the chunk is not the
 //   first; out: where the chunk-bottom rows go (4 x 64 uint2, t*64 + lane),
-//   null for the last chunk. lbm1: one less than a lower bound of the
-//   maximum over the target's other chunks, per half (filters the row
-//   scans). rmin: the shortest target length of the wave (wave-uniform);
+//   null for the last chunk. lb: a lower bound of the maximum over the
+//   target's other chunks, per half (filters the row scans; it may also
+//   include this chunk's own earlier maximum, which the test's run term
+//   covers anyway). rmin: the shortest target length of the wave (wave-uniform);
 //   columns below it need no end-of-target mask.
 // PAIRED (packed lanes only): chunk_base is the wave's paired profile of
 //   this chunk, one row per pair of profile classes (class of target A,
@@ -105,7 +106,7 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
                                          uint2 rr, uint2 rr_next, const uint2 (&bn)[4],
                                          short8 (&sa)[K / 8], short8 (&sb)[K / 8],
                                          const uint8_t *chunk_base, const uint32_t *slut, int half,
-                                         int RA, int RB, int rmin, V lbm1, int go, int ge,
+                                         int RA, int RB, int rmin, V lb, int go, int ge,
                                          uint2 *out, int lane, int shift = 0) {
     constexpr bool PACKED = !__is_same(V, int);
     constexpr bool F16 = __is_same(V, h16x2);
@@ -127,6 +128,7 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
         st.diag_top = st.diag_top - vshift;
         st.ibase = b * 4;
     }
+    V lbm1;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int i = b * 4 + t;
@@ -278,6 +280,9 @@ __device__ __forceinline__ void sw_block(SwChunk<V, K> &st, int b, int j0, bool 
         // (M_c, i_c, r_c). Per half, "cm > max(run, lb - 1)" is exactly
         // "the chunk maximum strictly rises, to at least lb"; one packed max
         // and one 32-bit compare test both halves.
+        // lb - 1 is formed at the first column's test, so a caller's loads
+        // of lb are waited for behind that column's rows
+        if (t == 0) lbm1 = lb - splat<V>(1);
         const V thr = vmax(st.run, lbm1);
         // wave-uniform tests keep these rare paths real branches
         if (__builtin_amdgcn_ballot_w64(as_u32(vmax(cm, thr)) != as_u32(thr)) != 0) {
@@ -385,7 +390,7 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
             const bool first = c == 0;
             const bool last = c == p.nchunks - 1;
             const uint8_t *chunk_base = smem + 2 * j0;
-            const V lbm1 = pack2<V>(bestA - 1, bestB - 1);
+            const V lb = pack2<V>(bestA, bestB);
             SwChunk<V, K> st;
             sw_chunk_init(st, go, ge);
             uint2 rr_next = nblk > 0 ? res[lane] : make_uint2(0u, 0u);
@@ -415,7 +420,7 @@ __global__ void __launch_bounds__(256) sw_batched(ScoreParams p) {
                     }
                 }
                 sw_block<V, K>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, half, RA,
-                               RB, rmin, lbm1, go, ge, last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
+                               RB, rmin, lb, go, ge, last ? nullptr : bnd + (size_t)b * 4 * 64, lane);
             }
             if constexpr (PACKED) {
                 sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
@@ -602,16 +607,16 @@ __global__ void __launch_bounds__(256) sw_pipe(ScoreParams p) {
                 // target's other chunks. This wave's finished chunks give
                 // bestA / bestB; the other waves publish their best so far
                 // (finished chunks and running chunk maxima, which only
-                // grow, so a stale entry is still a lower bound).
+                // grow, so a stale entry is still a lower bound). This
+                // wave's own entry is read too, so that the four reads go
+                // out together with no branch: it is at most max(best,
+                // run), which the filter's own terms already cover.
+                static_assert(kPipeWaves == 4, "one read per wave's entry");
                 const V bestv = pack2<V>(bestA, bestB);
-                V lb = bestv;
-#pragma unroll
-                for (int v = 0; v < kPipeWaves; ++v) {
-                    if (v == w) continue;
-                    lb = vmax(lb, from_u32<V>(pub[v * 64 + lane]));
-                }
+                const V lb = vmax3(vmax3(bestv, from_u32<V>(pub[lane]), from_u32<V>(pub[64 + lane])),
+                                   from_u32<V>(pub[128 + lane]), from_u32<V>(pub[192 + lane]));
                 sw_block<V, K, PAIRED>(st, b, j0, first, rr, rr_next, bn, sa, sb, chunk_base, slut, 0,
-                                       RA, RB, rmin, lb - splat<V>(1), go, ge, out, lane, shift);
+                                       RA, RB, rmin, lb, go, ge, out, lane, shift);
                 pub[w * 64 + lane] = as_u32(vmax(bestv, st.run));
                 if (b == nblk - 1) {
                     sw_combine((int)st.run.x, st.icA, st.rcA, bestA, erA, eqA);
