@@ -325,6 +325,9 @@ __device__ __forceinline__ void sg_block(SgChunk<V, K> &st, int b, int j0, bool 
                 if (TE || QE) {
                     int ka = key_lo(uu, i), kb = key_hi(uu, i);
                     if (i >= rmin) { // past some lane's target end
+                        // a real (scalar) branch: the empty asm keeps the
+                        // compiler from selecting on every column instead
+                        asm volatile("");
                         if (i >= RA) ka = INT_MIN;
                         if (i >= RB) kb = INT_MIN;
                     }
